@@ -1,0 +1,166 @@
+// Internal declarations of the alt3 library (HIP/CDNA4 three-phonon path).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/alt3.h"
+
+namespace alt3 {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define ALT3_HIP(call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ::alt3::fail(ALT3_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define ALT3_TRY(call)              \
+    do {                            \
+        int rc_ = (call);           \
+        if (rc_ != ALT3_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------- physical constants
+// THERMACOND/modules.f90:4-24; cnst is a REAL*4 literal there, so it carries float rounding.
+constexpr double PI = 3.14159265358979323846;
+constexpr double H_PLANK = 6.62606896e-34;
+constexpr double K_B = 1.3806504e-23;
+constexpr double C_LIGHT = 2.99792458e+8;
+constexpr double HBAR = 1.054571628e-34;
+constexpr double EE = 1.60217653e-19;
+constexpr double UMA = 1.66053878e-27;
+inline double cnst() { return (double)521.1098918f; }
+double const33();
+
+// ---------------------------------------------------------------- device buffer
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        if (count == n && p) return ALT3_OK;
+        release();
+        if (count == 0) return ALT3_OK;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            return fail(ALT3_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+        return ALT3_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Mesh geometry shared by the kernels: mesh point index  idx = c2 + n2*(c1 + n1*c0)
+// with c in [0,n) (THERMACOND's i-outer / k-inner loop), reduced coordinate c/n - 1/2.
+struct MeshDims {
+    int n0, n1, n2;
+};
+
+// ---------------------------------------------------------------- host tables
+struct HostTables {
+    int nk = 0, nibz = 0, nops = 0, npos = 0;
+    double g[9] = {0};           // rows g1,g2,g3 (2 pi included)
+    double volume_r = 0;
+    std::vector<double> kpc_eig; // [nk][3] REAL*4-fraction mesh (eigen solves)
+    std::vector<double> kpc;     // [nk][3] FP64 mesh
+    std::vector<int> mapibz, mapinv, mappos;
+    std::vector<double> kibz, wibz;
+    std::vector<double> opk;     // [nops][9] column-major
+    std::vector<double> opk_inv; // [nops][9] column-major inverse
+    // stars on the mesh (kstarsinfbz1) per IBZ point
+    std::vector<int> star_n, star_op, star_n0;        // [nibz], [nibz][48], [nibz][48]
+    // full stars (getstar) folded onto the mesh (veloc_avergeIBZ_invsyFBZ)
+    std::vector<int> fstar_n, fstar_op, fstar_n0;
+    std::vector<int> tet;        // [6nk][4]
+};
+
+int build_tables(const alt3_system *sys, HostTables &t);
+
+// ---------------------------------------------------------------- FC3 preparation
+struct V3Tables {
+    // reference-order term list (home-cell first index), for the per-element kernel
+    int nterm = 0;
+    std::vector<int> t_abc;        // [nterm][3] row indices a,b,c (alpha + 3*(atom0-1))
+    std::vector<double> t_psi;     // [nterm] fc3/sqrt(mi mj mk)
+    std::vector<double> t_rj, t_rk;// [nterm][3]
+    // phase-GEMM layout: slots (dR, b, c) with term lists
+    int ndr = 0, nslot = 0;
+    std::vector<double> dr;        // [ndr][3]
+    std::vector<int> slot_dr, slot_bc;   // [nslot]
+    std::vector<int> slot_ptr;     // [nslot+1]
+    std::vector<int> sterm_a;      // [nterm] a of each term in slot order
+    std::vector<double> sterm_psi; // [nterm]
+    std::vector<double> sterm_rk;  // [nterm][3]
+};
+int build_v3_tables(const alt3_system *sys, V3Tables &v);
+
+}  // namespace alt3
+
+// ---------------------------------------------------------------- the context
+struct alt3_ctx {
+    int natoms0 = 0, nb = 0, nk = 0, nibz = 0, nops = 0, classical = 0;
+    bool host_only = false;
+    int nc[3] = {0, 0, 0};
+    alt3::HostTables T;
+    alt3::V3Tables V;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    long long peak_bytes = 0;
+
+    // harmonic
+    int nt2h = 0;                                 // FC2 terms with a home-cell first atom
+    alt3::DBuf<int> d_fc2_rc;                      // [nt2h][2] row, col
+    alt3::DBuf<double> d_fc2_amp, d_fc2_rr;        // [nt2h], [nt2h][3]
+    alt3::DBuf<double> d_kpc_eig, d_kpc;           // [nk][3]
+    alt3::DBuf<double> d_eival, d_freq, d_vel;     // [nk][nb], [nk][nb], [nk][nb][3]
+    alt3::DBuf<double> d_evec;                     // [nk][nb][nb] complex (interleaved)
+    alt3::DBuf<int> d_mappos;
+    bool have_eig = false;
+
+    // V3
+    alt3::DBuf<int> d_t_abc;
+    alt3::DBuf<double> d_t_psi, d_t_rj, d_t_rk;
+    alt3::DBuf<double> d_dr;
+    alt3::DBuf<int> d_slot_dr, d_slot_bc, d_slot_ptr, d_sterm_a;
+    alt3::DBuf<double> d_sterm_psi, d_sterm_rk;
+    alt3::DBuf<double> d_gp;                       // per q1: [ndr][nb][nb*nb] complex (l1,b,c)
+    alt3::DBuf<float> d_gpf;                       // FP32 hi/lo copy for the mixed path
+    alt3::DBuf<int> d_mapinv;
+    int ibz_lo = -1, ibz_hi = -1, v3_prec = -1;
+    alt3::DBuf<double> d_v33;                      // [ksub][nb][nb][nb][nk] FP64
+    alt3::DBuf<float> d_v33f;                      // same, FP32 (mixed path)
+    bool have_v3 = false;
+
+    // FGR
+    alt3::DBuf<double> d_p1, d_p2;                 // [ksub][nb][nb][nb][nk]
+    bool have_fgr = false;
+
+    // per temperature
+    double tempk = -1;
+    alt3::DBuf<double> d_dist;                     // [nk][nb]
+    alt3::DBuf<double> d_q, d_qn, d_qu;            // [ksub][nb]
+    alt3::DBuf<double> d_r1, d_r2;                 // [ksub][nb][nb][nk]  sum_l3 P1, P2
+    alt3::DBuf<double> d_qall;                     // [nibz][nb] (whole IBZ, for F_RTA)
+    bool have_tau = false;
+    alt3::DBuf<int> d_star_n, d_star_op, d_star_n0, d_fstar_n, d_fstar_op, d_fstar_n0;
+    alt3::DBuf<double> d_opk, d_opk_inv, d_kibz;
+    alt3::DBuf<double> d_frta, d_kpart, d_kappa, d_taueff, d_veloci;
+    alt3::DBuf<double> d_cm, d_rhs;
+    bool have_cm = false;
+    alt3::DBuf<double> d_cg;                       // CG work vectors
+    alt3::DBuf<double> d_red;                      // reduction scratch
+};

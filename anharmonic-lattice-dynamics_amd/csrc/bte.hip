@@ -1,0 +1,613 @@
+// Temperature-dependent part of the three-phonon path on the GPU (FP64):
+//   distribution           sub.f90:1955-1980, nbe extratools.f90:257-288
+//   calculate_tauRTA_mpi2  sub.f90:7953-8194 (Q, N/U split) + Qandtau :8198-8305
+//   F_RTA + mode_kappa     sub.f90:1238-1289, :1292-1401
+//   CollisionMatrix_mpi    sub.f90:3034-3210, cal_RHS_IBZ :2618-2691
+//   cg_quadratic           cg.f90:19-69
+//   mode_kappa_d           sub.f90:1407-1530 with veloc_avergeIBZ_invsyFBZ :5479-5537
+//
+// Layouts on the device: P1sm/P2sm [s][l1][l2][l3][q2]; the tau kernel also writes
+// R1/R2[s][l1][l2][q2] = sum_l3 P1, P2 (occupations applied), which is all the collision
+// matrix reads: C = -R1(n0) + R1(q2-q1) + R2(n0)/2 + R2(q1-q2)/2 off the diagonal and -Q on it.
+#include "alt3_impl.h"
+
+namespace {
+
+__device__ __forceinline__ void coords(const alt3::MeshDims &d, int k, int &c0, int &c1, int &c2)
+{
+    c2 = k % d.n2;
+    c1 = (k / d.n2) % d.n1;
+    c0 = k / (d.n1 * d.n2);
+}
+__device__ __forceinline__ int idx(const alt3::MeshDims &d, int c0, int c1, int c2) { return c2 + d.n2 * (c1 + d.n1 * c0); }
+__device__ __forceinline__ int wrap(int x, int n)
+{
+    x %= n;
+    return x < 0 ? x + n : x;
+}
+
+__device__ __forceinline__ double nbe(double w, double t, int m)
+{
+    double z = w / t;
+    if (m == 0) {
+        if (z > 60) return 0;
+        if (z < (double)0.0010f) return 1 / (z * (1 + z / 2 * (1 + z / 3)));
+        return 1 / (exp(z) - 1);
+    }
+    return 1 / z;
+}
+
+__global__ void k_dist(int n, const double *__restrict__ freq, double temp, int classical, double *__restrict__ dist,
+                       int *__restrict__ err)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double w = freq[i];
+    if (!(w > 0)) {
+        atomicExch(err, 1);
+        dist[i] = 0;
+        return;
+    }
+    dist[i] = nbe(w, temp, classical);
+}
+
+__device__ double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// one workgroup per (s, l1): Q, QN, QU and the l3-reduced R1/R2
+__global__ void __launch_bounds__(256) k_tau(alt3::MeshDims md, int nk, int nb, int ibz_lo,
+                                             const int *__restrict__ mapinv, const double *__restrict__ dist,
+                                             const double *__restrict__ P1sm, const double *__restrict__ P2sm,
+                                             double *__restrict__ Q, double *__restrict__ QN, double *__restrict__ QU,
+                                             double *__restrict__ R1, double *__restrict__ R2)
+{
+    __shared__ double sh[256];
+    const int s = blockIdx.x / nb, l1 = blockIdx.x % nb;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    coords(md, k1, a0, a1, a2);
+    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    const double d1 = dist[(size_t)k1 * nb + l1];
+    double q = 0, qn = 0, qu = 0;
+    for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+        int c0, c1, c2;
+        coords(md, j, c0, c1, c2);
+        int s0 = a0 + c0, s1 = a1 + c1, s2 = a2 + c2;  // q1+q2 unfolded: u = (c1+c2)/n - 1
+        bool in1 = (s0 >= h0 && s0 < 3 * h0) && (s1 >= h1 && s1 < 3 * h1) && (s2 >= h2 && s2 < 3 * h2);
+        int np = idx(md, wrap(s0 - h0, md.n0), wrap(s1 - h1, md.n1), wrap(s2 - h2, md.n2));
+        int t0 = a0 - c0, t1 = a1 - c1, t2 = a2 - c2;  // q1-q2 unfolded: u = (c1-c2)/n
+        bool in2 = (t0 >= -h0 && t0 < h0) && (t1 >= -h1 && t1 < h1) && (t2 >= -h2 && t2 < h2);
+        int nm = idx(md, wrap(t0 + h0, md.n0), wrap(t1 + h1, md.n1), wrap(t2 + h2, md.n2));
+        for (int l2 = 0; l2 < nb; l2++) {
+            const double d2 = dist[(size_t)j * nb + l2];
+            double r1 = 0, r2 = 0;
+            for (int l3 = 0; l3 < nb; l3++) {
+                size_t x = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk + j;
+                double p1 = P1sm[x] * (d1 * d2 * (dist[(size_t)np * nb + l3] + 1));
+                double p2 = P2sm[x] * (d1 * (d2 + 1) * (dist[(size_t)nm * nb + l3] + 1));
+                q = q + (p1 + 0.5 * p2);
+                if (in1) qn += p1; else qu += p1;
+                if (in2) qn += 0.5 * p2; else qu += 0.5 * p2;
+                r1 += p1;
+                r2 += p2;
+            }
+            size_t o = (((size_t)s * nb + l1) * nb + l2) * (size_t)nk + j;
+            R1[o] = r1;
+            R2[o] = r2;
+        }
+    }
+    q = block_sum(q, sh);
+    qn = block_sum(qn, sh);
+    qu = block_sum(qu, sh);
+    if (threadIdx.x == 0) {
+        Q[(size_t)s * nb + l1] = q;
+        QN[(size_t)s * nb + l1] = qn;
+        QU[(size_t)s * nb + l1] = qu;
+    }
+}
+
+// tauinv = Q/(n(n+1)) c 100 ; outputs Fortran (ksub,nb)
+__global__ void k_qandtau(int ksub, int nb, int ibz_lo, const int *__restrict__ mapinv, const double *__restrict__ dist,
+                          const double *__restrict__ Q, const double *__restrict__ QN, const double *__restrict__ QU,
+                          double *__restrict__ fQ, double *__restrict__ fQN, double *__restrict__ fQU,
+                          double *__restrict__ tN, double *__restrict__ tU, double *__restrict__ tT)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ksub * nb) return;
+    int s = x / nb, l = x % nb;
+    double nb1 = dist[(size_t)mapinv[ibz_lo + s] * nb + l];
+    size_t f = s + (size_t)ksub * l;
+    fQ[f] = Q[x];
+    fQN[f] = QN[x];
+    fQU[f] = QU[x];
+    double a = QN[x] / (nb1 * (nb1 + 1)) * alt3::C_LIGHT * 100;
+    double b = QU[x] / (nb1 * (nb1 + 1)) * alt3::C_LIGHT * 100;
+    tN[f] = a;
+    tU[f] = b;
+    tT[f] = a + b;
+}
+
+__device__ __forceinline__ void opmul(const double *op, const double *f, double *o)
+{
+    for (int r = 0; r < 3; r++) {
+        double acc = 0;
+        for (int c = 0; c < 3; c++) acc = acc + op[3 * c + r] * f[c];
+        o[r] = acc;
+    }
+}
+
+// F_RTA (nibz,nb,3) Fortran layout, then per (i,ii) partial kappa (9) over the mesh star
+__global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ mapinv, const double *__restrict__ freq,
+                            const double *__restrict__ dist, const double *__restrict__ eival,
+                            const double *__restrict__ vel, const double *__restrict__ Q /*[i][nb]*/, double tempk,
+                            double temp, int classical, double const1, double const2, double cnst,
+                            const int *__restrict__ star_n, const int *__restrict__ star_op,
+                            const int *__restrict__ star_n0, const double *__restrict__ opk,
+                            double *__restrict__ FRTA, double *__restrict__ kpart, double *__restrict__ taueff,
+                            int *__restrict__ err)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nibz * nb) return;
+    int i = x / nb, ii = x % nb;
+    int m = mapinv[i];
+    double omega = freq[(size_t)m * nb + ii], nbv = dist[(size_t)m * nb + ii];
+    double F[3], v[3];
+    for (int a = 0; a < 3; a++) {
+        v[a] = vel[((size_t)m * nb + ii) * 3 + a];
+        F[a] = -(alt3::C_LIGHT * v[a]) * alt3::H_PLANK * (omega * alt3::C_LIGHT * 100) * nbv * (nbv + 1) /
+               (alt3::K_B * tempk * tempk) / (Q[(size_t)i * nb + ii] * alt3::C_LIGHT * 100);
+        FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)] = F[a];
+    }
+    double kp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double te = 0;
+    for (int s = 0; s < star_n[i]; s++) {
+        const double *op = opk + 9 * star_op[48 * i + s];
+        double F12[3], vc[3];
+        opmul(op, F, F12);
+        opmul(op, v, vc);
+        int n0 = star_n0[48 * i + s];
+        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
+        if (!(om > 0)) atomicExch(err, 1);
+        double nb0 = nbe(om, temp, classical);
+        for (int j = 0; j < 3; j++) {
+            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
+            if (vc[j] == 0.0)
+                te = te - const2 * F12[j] / om / 0.01;
+            else
+                te = te - const2 * F12[j] / om / vc[j] / (star_n[i] * 3.0);
+        }
+    }
+    for (int e = 0; e < 9; e++) kpart[(size_t)x * 9 + e] = kp[e];
+    // mode_kappa zeroes the whole tauinv_eff array for every (i,ii): only the last survives
+    taueff[i + (size_t)nibz * ii] = (i == nibz - 1 && ii == nb - 1) ? te : 0.0;
+}
+
+// kappa(ii, j, jj) Fortran (nb,3,3) = sum over i of the partials, in IBZ order
+__global__ void k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart, double *__restrict__ kappa)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nb * 9) return;
+    int ii = x / 9, e = x % 9;
+    double acc = 0;
+    for (int i = 0; i < nibz; i++) acc = acc + kpart[((size_t)i * nb + ii) * 9 + e];
+    kappa[ii + (size_t)nb * e] = acc;
+}
+
+// collision matrix, one thread per (row block (s,l1), column block (n2,l2)); CM column-major
+__global__ void k_collision(alt3::MeshDims md, int nk, int nb, int ibz_lo, int ksub, int nibz,
+                            const int *__restrict__ mapinv, const double *__restrict__ Q,
+                            const double *__restrict__ R1, const double *__restrict__ R2,
+                            const int *__restrict__ star_n, const int *__restrict__ star_op,
+                            const int *__restrict__ star_n0, const double *__restrict__ opk, double *__restrict__ CM)
+{
+    int rb = blockIdx.x * blockDim.x + threadIdx.x;  // (s, l1)
+    int cb = blockIdx.y;                              // (n2, l2)
+    if (rb >= ksub * nb) return;
+    int s = rb / nb, l1 = rb % nb, n2 = cb / nb, l2 = cb % nb;
+    int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    coords(md, k1, a0, a1, a2);
+    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    const size_t nrow = (size_t)ksub * nb * 3;
+    const size_t rbase = (((size_t)s * nb + l1) * nb + l2) * (size_t)nk;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int ns = 0; ns < star_n[n2]; ns++) {
+        int n0 = star_n0[48 * n2 + ns];
+        double C;
+        if (n0 == k1 && l1 == l2) {
+            C = -Q[(size_t)s * nb + l1];
+        } else {
+            int c0, c1, c2;
+            coords(md, n0, c0, c1, c2);
+            int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
+            int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
+            C = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+        }
+        const double *op = opk + 9 * star_op[48 * n2 + ns];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (C * op[3 * j + i]);
+    }
+    const size_t d1 = 3 * (size_t)rb, d2 = 3 * (size_t)cb;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) CM[(d1 + i) + nrow * (d2 + j)] = acc[i + 3 * j];
+}
+
+__global__ void k_rhs(int nibz, int nb, double tempk, const int *__restrict__ mapinv, const double *__restrict__ freq,
+                      const double *__restrict__ dist, const double *__restrict__ vel, double *__restrict__ RHS)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nibz * nb) return;
+    int i = x / nb, ii = x % nb, n0 = mapinv[i];
+    double omega = freq[(size_t)n0 * nb + ii], nb0 = dist[(size_t)n0 * nb + ii];
+    for (int a = 0; a < 3; a++)
+        RHS[3 * (size_t)x + a] = (alt3::C_LIGHT * vel[((size_t)n0 * nb + ii) * 3 + a]) * alt3::H_PLANK * (omega * alt3::C_LIGHT * 100) *
+                                 nb0 * (nb0 + 1) / (alt3::K_B * tempk * tempk) / (alt3::C_LIGHT * 100);
+}
+
+// ------------------------------------------------------------------ CG (HBM-bound)
+// y = A x, A column-major n x n: one thread per row, sequential over columns
+__global__ void __launch_bounds__(256) k_matvec(int n, const double *__restrict__ A, const double *__restrict__ x,
+                                                double *__restrict__ y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0;
+    const double *a = A + i;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        double v0 = a[(size_t)n * j] * x[j];
+        double v1 = a[(size_t)n * (j + 1)] * x[j + 1];
+        double v2 = a[(size_t)n * (j + 2)] * x[j + 2];
+        double v3 = a[(size_t)n * (j + 3)] * x[j + 3];
+        acc = acc + v0;
+        acc = acc + v1;
+        acc = acc + v2;
+        acc = acc + v3;
+    }
+    for (; j < n; j++) acc = acc + a[(size_t)n * j] * x[j];
+    y[i] = acc;
+}
+
+// partial dot products: out[blockIdx.x] for pairs (x1,y1), (x2,y2)
+__global__ void k_dot2(int n, const double *__restrict__ x1, const double *__restrict__ y1, const double *__restrict__ x2,
+                       const double *__restrict__ y2, double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double a = 0, b = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        a += x1[i] * y1[i];
+        if (x2) b += x2[i] * y2[i];
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = a;
+        out[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// final reduction of k_dot2 partials into sc[slot], sc[slot+1]
+__global__ void k_reduce2(int nblk, const double *__restrict__ part, double *__restrict__ sc, int slot)
+{
+    __shared__ double sh[256];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        sc[slot] = a;
+        sc[slot + 1] = b;
+    }
+}
+
+// v0 = grad; vmin -= num/den conj; grad -= num/den aux    (sc[0]=num, sc[1]=den)
+__global__ void k_cg_step(int n, const double *__restrict__ sc, double *__restrict__ vmin, double *__restrict__ grad,
+                          double *__restrict__ v0, const double *__restrict__ conj, const double *__restrict__ aux)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r = sc[0] / sc[1];
+    v0[i] = grad[i];
+    vmin[i] = vmin[i] - r * conj[i];
+    grad[i] = grad[i] - r * aux[i];
+}
+
+// conj = -grad + (num/den) conj       (sc[4]=grad.grad, sc[5]=v0.v0)
+__global__ void k_cg_dir(int n, const double *__restrict__ sc, const double *__restrict__ grad, double *__restrict__ conj)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    conj[i] = -grad[i] + sc[4] / sc[5] * conj[i];
+}
+
+__global__ void k_axpy_init(int n, const double *__restrict__ ax, const double *__restrict__ b, double *__restrict__ grad,
+                            double *__restrict__ conj)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double g = ax[i] - b[i];
+    grad[i] = g;
+    conj[i] = -g;
+}
+
+// star-averaged velocities with inverse operations (veloc_avergeIBZ_invsyFBZ)
+__global__ void k_veloci(int nibz, int nb, const double *__restrict__ vel, const int *__restrict__ fstar_n,
+                         const int *__restrict__ fstar_op, const int *__restrict__ fstar_n0,
+                         const double *__restrict__ opk_inv, double *__restrict__ veloci)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nibz * nb) return;
+    int i = x / nb, j = x % nb;
+    int narms = fstar_n[i];
+    double va[3] = {0, 0, 0};
+    for (int n = 0; n < narms; n++) {
+        int n0 = fstar_n0[48 * i + n];
+        double vi[3];
+        opmul(opk_inv + 9 * fstar_op[48 * i + n], vel + ((size_t)n0 * nb + j) * 3, vi);
+        for (int r = 0; r < 3; r++) va[r] = va[r] + (vi[r] / narms);
+    }
+    for (int r = 0; r < 3; r++) veloci[(size_t)x * 3 + r] = va[r];
+}
+
+__global__ void k_kappa_bte(int nibz, int nb, const double *__restrict__ x, const double *__restrict__ veloci,
+                            const double *__restrict__ eival, double temp, int classical, double const1,
+                            double const2, double cnst, const int *__restrict__ star_n,
+                            const int *__restrict__ star_op, const int *__restrict__ star_n0,
+                            const double *__restrict__ opk, double *__restrict__ kpart, double *__restrict__ taueff,
+                            int *__restrict__ err)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nibz * nb) return;
+    int i = t / nb, ii = t % nb;
+    double f[3] = {x[3 * (size_t)t], x[3 * (size_t)t + 1], x[3 * (size_t)t + 2]};
+    const double *vi = veloci + (size_t)t * 3;
+    double kp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double te = taueff[i + (size_t)nibz * ii];
+    int narms = star_n[i];
+    for (int s = 0; s < narms; s++) {
+        const double *op = opk + 9 * star_op[48 * i + s];
+        double F12[3], vc[3];
+        opmul(op, f, F12);
+        opmul(op, vi, vc);
+        int n0 = star_n0[48 * i + s];
+        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
+        if (!(om > 0)) atomicExch(err, 1);
+        double nb0 = nbe(om, temp, classical);
+        for (int j = 0; j < 3; j++) {
+            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
+            double fv = F12[0] * vc[0] + F12[1] * vc[1] + F12[2] * vc[2];
+            double vv = vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2];
+            te = te - const2 * fv / om / vv / (narms * 3.0);
+        }
+    }
+    for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
+    taueff[i + (size_t)nibz * ii] = te;
+}
+
+}  // namespace
+
+namespace alt3 {
+
+static double temp_cm(double tempk) { return tempk * K_B / (100 * H_PLANK * C_LIGHT); }
+
+int upload_bte_tables(alt3_ctx *ctx)
+{
+    HostTables &T = ctx->T;
+    ALT3_TRY(ctx->d_star_n.alloc(T.nibz));
+    ALT3_TRY(ctx->d_star_op.alloc(48 * T.nibz));
+    ALT3_TRY(ctx->d_star_n0.alloc(48 * T.nibz));
+    ALT3_TRY(ctx->d_fstar_n.alloc(T.nibz));
+    ALT3_TRY(ctx->d_fstar_op.alloc(48 * T.nibz));
+    ALT3_TRY(ctx->d_fstar_n0.alloc(48 * T.nibz));
+    ALT3_TRY(ctx->d_opk.alloc(9 * T.nops));
+    ALT3_TRY(ctx->d_opk_inv.alloc(9 * T.nops));
+    ALT3_HIP(hipMemcpy(ctx->d_star_n.p, T.star_n.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_star_op.p, T.star_op.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_star_n0.p, T.star_n0.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_fstar_n.p, T.fstar_n.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_fstar_op.p, T.fstar_op.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_fstar_n0.p, T.fstar_n0.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_opk.p, T.opk.data(), sizeof(double) * 9 * T.nops, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_opk_inv.p, T.opk_inv.data(), sizeof(double) * 9 * T.nops, hipMemcpyHostToDevice));
+    return ALT3_OK;
+}
+
+int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT)
+{
+    if (!ctx->have_fgr) return fail(ALT3_E_STATE, "alt3_tau needs alt3_fgr first");
+    if (!(tempk > 0)) return fail(ALT3_E_ARG, "ERROR in N_BE: t=<0");
+    const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
+    DBuf<int> derr;
+    ALT3_TRY(derr.alloc(1));
+    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    ALT3_TRY(ctx->d_dist.alloc((size_t)nk * nb));
+    hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
+                       temp_cm(tempk), ctx->classical, ctx->d_dist.p, derr.p);
+    ALT3_HIP(hipGetLastError());
+    ALT3_TRY(ctx->d_q.alloc((size_t)ksub * nb));
+    ALT3_TRY(ctx->d_qn.alloc((size_t)ksub * nb));
+    ALT3_TRY(ctx->d_qu.alloc((size_t)ksub * nb));
+    ALT3_TRY(ctx->d_r1.alloc((size_t)ksub * nb * nb * nk));
+    ALT3_TRY(ctx->d_r2.alloc((size_t)ksub * nb * nb * nk));
+    hipLaunchKernelGGL(k_tau, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
+                       ctx->d_dist.p, ctx->d_p1.p, ctx->d_p2.p, ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p, ctx->d_r1.p,
+                       ctx->d_r2.p);
+    ALT3_HIP(hipGetLastError());
+    int herr = 0;
+    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
+    ctx->tempk = tempk;
+    ctx->have_tau = true;
+    ctx->have_cm = false;
+    if (Q || QN || QU || tN || tU || tT) {
+        DBuf<double> o;
+        size_t n = (size_t)ksub * nb;
+        ALT3_TRY(o.alloc(6 * n));
+        hipLaunchKernelGGL(k_qandtau, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub, nb, ctx->ibz_lo,
+                           ctx->d_mapinv.p, ctx->d_dist.p, ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p, o.p, o.p + n,
+                           o.p + 2 * n, o.p + 3 * n, o.p + 4 * n, o.p + 5 * n);
+        ALT3_HIP(hipGetLastError());
+        double *dst[6] = {Q, QN, QU, tN, tU, tT};
+        for (int k = 0; k < 6; k++)
+            if (dst[k]) ALT3_HIP(hipMemcpyAsync(dst[k], o.p + k * n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return ALT3_OK;
+}
+
+static int need_full_ibz(alt3_ctx *ctx, const char *who)
+{
+    if (!ctx->have_tau) return fail(ALT3_E_STATE, std::string(who) + " needs alt3_tau first");
+    if (ctx->ibz_lo != 0 || ctx->ibz_hi != ctx->nibz - 1)
+        return fail(ALT3_E_STATE, std::string(who) + " needs Q over the whole irreducible zone (ibz 0..nibz-1)");
+    return ALT3_OK;
+}
+
+int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff)
+{
+    ALT3_TRY(need_full_ibz(ctx, "alt3_kappa_rta"));
+    if (tempk != ctx->tempk) return fail(ALT3_E_STATE, "alt3_kappa_rta temperature differs from the last alt3_tau");
+    const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;
+    ALT3_TRY(ctx->d_frta.alloc((size_t)nibz * nb * 3));
+    ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
+    ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
+    ALT3_TRY(ctx->d_taueff.alloc((size_t)nibz * nb));
+    DBuf<int> derr;
+    ALT3_TRY(derr.alloc(1));
+    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
+    double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
+    hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, nk,
+                       ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_dist.p, ctx->d_eival.p, ctx->d_vel.p, ctx->d_q.p, tempk,
+                       temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
+                       ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr.p);
+    ALT3_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_kappa_sum, dim3((nb * 9 + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    ALT3_HIP(hipGetLastError());
+    int herr = 0;
+    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
+    if (FRTA) ALT3_HIP(hipMemcpyAsync(FRTA, ctx->d_frta.p, sizeof(double) * nibz * nb * 3, hipMemcpyDeviceToHost, ctx->stream));
+    if (taueff) ALT3_HIP(hipMemcpyAsync(taueff, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    return ALT3_OK;
+}
+
+int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
+{
+    if (!ctx->have_tau) return fail(ALT3_E_STATE, "alt3_collision needs alt3_tau first");
+    if (tempk != ctx->tempk) return fail(ALT3_E_STATE, "alt3_collision temperature differs from the last alt3_tau");
+    const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
+    const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
+    ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
+    ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
+    dim3 grid((ksub * nb + 63) / 64, nibz * nb);
+    hipLaunchKernelGGL(k_collision, grid, dim3(64), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ksub, nibz, ctx->d_mapinv.p,
+                       ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p, ctx->d_star_n0.p,
+                       ctx->d_opk.p, ctx->d_cm.p);
+    ALT3_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
+                       ctx->d_freq.p, ctx->d_dist.p, ctx->d_vel.p, ctx->d_rhs.p);
+    ALT3_HIP(hipGetLastError());
+    if (CM) ALT3_HIP(hipMemcpyAsync(CM, ctx->d_cm.p, sizeof(double) * nrow * ncol, hipMemcpyDeviceToHost, ctx->stream));
+    if (RHS) ALT3_HIP(hipMemcpyAsync(RHS, ctx->d_rhs.p, sizeof(double) * nibz * nb * 3, hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_cm = true;
+    return ALT3_OK;
+}
+
+// CG on a device matrix (dA, db); x is a device vector, overwritten
+int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
+              double *funcmin, int *iters)
+{
+    ALT3_TRY(ctx->d_cg.alloc((size_t)4 * n));
+    double *grad = ctx->d_cg.p, *conj = grad + n, *aux = conj + n, *v0 = aux + n;
+    const int nblk = 256;
+    ALT3_TRY(ctx->d_red.alloc(2 * nblk + 16));
+    double *part = ctx->d_red.p, *sc = part + 2 * nblk;
+    const int gb = (n + 255) / 256;
+    const double tol = 1.0 / std::pow(10.0, 2 * ndig);
+    hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, dx, aux);
+    hipLaunchKernelGGL(k_axpy_init, dim3(gb), dim3(256), 0, ctx->stream, n, aux, db, grad, conj);
+    ALT3_HIP(hipGetLastError());
+    int iter = 0;
+    double hs[8];
+    while (iter < max_iter) {
+        iter++;
+        hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, conj, aux);
+        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, grad, conj, aux, conj, part);
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);  // num, den
+        hipLaunchKernelGGL(k_cg_step, dim3(gb), dim3(256), 0, ctx->stream, n, sc, dx, grad, v0, conj, aux);
+        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, dx, dx, conj, conj, part);
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 2);  // v2, cc
+        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, grad, grad, v0, v0, part);
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 4);  // gg, v0v0
+        ALT3_HIP(hipGetLastError());
+        ALT3_HIP(hipMemcpyAsync(hs, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        double num = hs[0], den = hs[1], v2 = hs[2], cc = hs[3];
+        double err = (num * num / den / den) * cc / v2;
+        if (err < tol) break;
+        hipLaunchKernelGGL(k_cg_dir, dim3(gb), dim3(256), 0, ctx->stream, n, sc, grad, conj);
+        ALT3_HIP(hipGetLastError());
+    }
+    if (funcmin) {
+        hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, dx, v0);
+        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, dx, v0, dx, db, part);
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);
+        ALT3_HIP(hipMemcpyAsync(hs, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        *funcmin = 0.5 * hs[0] - hs[1];
+    }
+    if (iters) *iters = iter;
+    return ALT3_OK;
+}
+
+int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev)
+{
+    const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;
+    for (int j = 0; j < nibz; j++)
+        for (int s = 0; s < ctx->T.fstar_n[j]; s++)
+            if (ctx->T.fstar_n0[48 * j + s] < 0)
+                return fail(ALT3_E_MESH, "ERROR: a star vector of an IBZ point is not a kpoint (mesh breaks the lattice symmetry)");
+    ALT3_TRY(ctx->d_veloci.alloc((size_t)nibz * nb * 3));
+    ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
+    ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
+    DBuf<int> derr;
+    ALT3_TRY(derr.alloc(1));
+    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
+                       ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
+    double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
+    double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
+    hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
+                       ctx->d_eival.p, temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p,
+                       ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr.p);
+    hipLaunchKernelGGL(k_kappa_sum, dim3((nb * 9 + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    ALT3_HIP(hipGetLastError());
+    int herr = 0;
+    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    return ALT3_OK;
+}
+
+}  // namespace alt3

@@ -1,0 +1,378 @@
+// C ABI of the alt3 library (declared in include/alt3.h).
+#include <chrono>
+#include <cstring>
+
+#include "alt3_impl.h"
+
+namespace alt3 {
+
+thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int eig_prepare(alt3_ctx *ctx, const alt3_system *sys);
+int eig_run(alt3_ctx *ctx);
+int v3_prepare(alt3_ctx *ctx);
+int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec);
+int v3_copy_out(alt3_ctx *ctx, double *host);
+int copy_fortran5(alt3_ctx *ctx, const double *dev, double *host);
+int fgr_run(alt3_ctx *ctx);
+int upload_bte_tables(alt3_ctx *ctx);
+int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT);
+int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff);
+int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
+int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
+              double *funcmin, int *iters);
+int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
+
+namespace {
+struct Timer {
+    alt3_ctx *c;
+    explicit Timer(alt3_ctx *ctx) : c(ctx) { (void)hipEventRecord(c->ev0, c->stream); }
+    void stop()
+    {
+        float ms = 0;
+        (void)hipEventRecord(c->ev1, c->stream);
+        (void)hipEventSynchronize(c->ev1);
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        c->last_ms = ms;
+    }
+};
+}  // namespace
+
+}  // namespace alt3
+
+using namespace alt3;
+
+extern "C" {
+
+const char *alt3_last_error(void) { return g_last_error.c_str(); }
+
+int alt3_create(const alt3_system *sys, alt3_ctx **out)
+{
+    if (!sys || !out) return fail(ALT3_E_ARG, "null argument");
+    *out = nullptr;
+    if (sys->natoms0 < 1 || sys->natoms < sys->natoms0 || !sys->mass || !sys->r123 || !sys->atompos || !sys->iatomcell0)
+        return fail(ALT3_E_ARG, "incomplete crystal description");
+    if (3 * sys->natoms0 > 64) return fail(ALT3_E_ARG, "more than 64 branches per cell are not supported");
+    for (int i = 0; i < sys->natoms; i++)
+        if (sys->iatomcell0[i] < 1 || sys->iatomcell0[i] > sys->natoms0) return fail(ALT3_E_ARG, "iatomcell0 out of range");
+    for (int i = 0; i < sys->natoms0; i++)
+        if (sys->iatomcell0[i] != i + 1) return fail(ALT3_E_ARG, "the first natoms0 listed atoms must be the primitive cell");
+    int ndev = 0;
+    if (!sys->host_only && (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1))
+        return fail(ALT3_E_HIP, "no HIP device available");
+    alt3_ctx *ctx = new alt3_ctx();
+    ctx->host_only = sys->host_only != 0;
+    ctx->natoms0 = sys->natoms0;
+    ctx->nb = 3 * sys->natoms0;
+    ctx->classical = sys->classical;
+    for (int a = 0; a < 3; a++) ctx->nc[a] = sys->nc[a];
+    int rc = build_tables(sys, ctx->T);
+    if (rc == ALT3_OK) rc = build_v3_tables(sys, ctx->V);
+    if (rc != ALT3_OK) {
+        delete ctx;
+        return rc;
+    }
+    ctx->nk = ctx->T.nk;
+    ctx->nibz = ctx->T.nibz;
+    ctx->nops = ctx->T.nops;
+    if (ctx->host_only) {
+        *out = ctx;
+        return ALT3_OK;
+    }
+    auto init = [&]() -> int {
+        ALT3_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ALT3_HIP(hipEventCreate(&ctx->ev0));
+        ALT3_HIP(hipEventCreate(&ctx->ev1));
+        ALT3_TRY(ctx->d_kpc_eig.alloc(3 * ctx->nk));
+        ALT3_TRY(ctx->d_kpc.alloc(3 * ctx->nk));
+        ALT3_TRY(ctx->d_mapinv.alloc(ctx->nibz));
+        ALT3_HIP(hipMemcpy(ctx->d_kpc_eig.p, ctx->T.kpc_eig.data(), sizeof(double) * 3 * ctx->nk, hipMemcpyHostToDevice));
+        ALT3_HIP(hipMemcpy(ctx->d_kpc.p, ctx->T.kpc.data(), sizeof(double) * 3 * ctx->nk, hipMemcpyHostToDevice));
+        ALT3_HIP(hipMemcpy(ctx->d_mapinv.p, ctx->T.mapinv.data(), sizeof(int) * ctx->nibz, hipMemcpyHostToDevice));
+        ALT3_TRY(eig_prepare(ctx, sys));
+        ALT3_TRY(v3_prepare(ctx));
+        ALT3_TRY(upload_bte_tables(ctx));
+        return ALT3_OK;
+    };
+    rc = init();
+    if (rc != ALT3_OK) {
+        alt3_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return ALT3_OK;
+}
+
+int alt3_destroy(alt3_ctx *ctx)
+{
+    if (!ctx) return ALT3_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    // DBuf members release in their destructors? they are PODs; release explicitly
+    ctx->d_fc2_rc.release(); ctx->d_fc2_amp.release(); ctx->d_fc2_rr.release();
+    ctx->d_kpc_eig.release(); ctx->d_kpc.release(); ctx->d_eival.release(); ctx->d_freq.release();
+    ctx->d_vel.release(); ctx->d_evec.release(); ctx->d_mappos.release();
+    ctx->d_t_abc.release(); ctx->d_t_psi.release(); ctx->d_t_rj.release(); ctx->d_t_rk.release();
+    ctx->d_dr.release(); ctx->d_slot_dr.release(); ctx->d_slot_bc.release(); ctx->d_slot_ptr.release();
+    ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
+    ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
+    ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
+    ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release(); ctx->d_qall.release();
+    ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
+    ctx->d_fstar_n.release(); ctx->d_fstar_op.release(); ctx->d_fstar_n0.release();
+    ctx->d_opk.release(); ctx->d_opk_inv.release(); ctx->d_kibz.release(); ctx->d_frta.release();
+    ctx->d_kpart.release(); ctx->d_kappa.release(); ctx->d_taueff.release(); ctx->d_veloci.release();
+    ctx->d_cm.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ALT3_OK;
+}
+
+int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
+{
+    if (!ctx || !bytes) return fail(ALT3_E_ARG, "null argument");
+    long long b = 0;
+    b += ctx->d_evec.bytes() + ctx->d_eival.bytes() + ctx->d_freq.bytes() + ctx->d_vel.bytes();
+    b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
+    b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
+    *bytes = b;
+    return ALT3_OK;
+}
+
+int alt3_sizes(const alt3_ctx *ctx, int *nk, int *nibz, int *nb, int *nops)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (nk) *nk = ctx->nk;
+    if (nibz) *nibz = ctx->nibz;
+    if (nb) *nb = ctx->nb;
+    if (nops) *nops = ctx->nops;
+    return ALT3_OK;
+}
+
+int alt3_get_mesh(const alt3_ctx *ctx, double *kpc_eig, double *kpc, int *mapibz, int *mapinv, double *kibz,
+                  double *wibz, int *tet, double *opk, int *mappos, int *npos)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    const HostTables &T = ctx->T;
+    if (kpc_eig) std::memcpy(kpc_eig, T.kpc_eig.data(), sizeof(double) * T.kpc_eig.size());
+    if (kpc) std::memcpy(kpc, T.kpc.data(), sizeof(double) * T.kpc.size());
+    if (mapibz) std::memcpy(mapibz, T.mapibz.data(), sizeof(int) * T.mapibz.size());
+    if (mapinv) std::memcpy(mapinv, T.mapinv.data(), sizeof(int) * T.mapinv.size());
+    if (kibz) std::memcpy(kibz, T.kibz.data(), sizeof(double) * T.kibz.size());
+    if (wibz) std::memcpy(wibz, T.wibz.data(), sizeof(double) * T.wibz.size());
+    if (tet) std::memcpy(tet, T.tet.data(), sizeof(int) * T.tet.size());
+    if (opk) std::memcpy(opk, T.opk.data(), sizeof(double) * T.opk.size());
+    if (mappos) std::memcpy(mappos, T.mappos.data(), sizeof(int) * T.mappos.size());
+    if (npos) *npos = T.npos;
+    return ALT3_OK;
+}
+
+int alt3_kstars(const alt3_ctx *ctx, int ibz, int *narms, int *ops, int *n0)
+{
+    if (!ctx || ibz < 0 || ibz >= ctx->nibz) return fail(ALT3_E_ARG, "bad ibz index");
+    const HostTables &T = ctx->T;
+    int n = T.star_n[ibz];
+    if (narms) *narms = n;
+    for (int s = 0; s < n; s++) {
+        if (ops) ops[s] = T.star_op[48 * ibz + s];
+        if (n0) n0[s] = T.star_n0[48 * ibz + s];
+    }
+    return ALT3_OK;
+}
+
+int alt3_eig(alt3_ctx *ctx, double *eigenval, double *eigenvec, double *veloc)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(eig_run(ctx));
+    tm.stop();
+    const size_t nk = ctx->nk, nb = ctx->nb;
+    if (eigenval) ALT3_HIP(hipMemcpy(eigenval, ctx->d_eival.p, sizeof(double) * nk * nb, hipMemcpyDeviceToHost));
+    if (eigenvec) ALT3_HIP(hipMemcpy(eigenvec, ctx->d_evec.p, sizeof(double) * nk * nb * nb * 2, hipMemcpyDeviceToHost));
+    if (veloc) ALT3_HIP(hipMemcpy(veloc, ctx->d_vel.p, sizeof(double) * nk * nb * 3, hipMemcpyDeviceToHost));
+    ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
+    return ALT3_OK;
+}
+
+int alt3_set_eig(alt3_ctx *ctx, const double *eigenval, const double *eigenvec, const double *veloc)
+{
+    if (!ctx || !eigenval || !eigenvec || !veloc) return fail(ALT3_E_ARG, "null argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    const size_t nk = ctx->nk, nb = ctx->nb;
+    ALT3_TRY(ctx->d_eival.alloc(nk * nb));
+    ALT3_TRY(ctx->d_freq.alloc(nk * nb));
+    ALT3_TRY(ctx->d_vel.alloc(nk * nb * 3));
+    ALT3_TRY(ctx->d_evec.alloc(nk * nb * nb * 2));
+    std::vector<double> fr(nk * nb);
+    for (size_t i = 0; i < nk * nb; i++) fr[i] = std::sqrt(eigenval[i]) * cnst();  // cal_freq
+    ALT3_HIP(hipMemcpy(ctx->d_eival.p, eigenval, sizeof(double) * nk * nb, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_freq.p, fr.data(), sizeof(double) * nk * nb, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_evec.p, eigenvec, sizeof(double) * nk * nb * nb * 2, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_vel.p, veloc, sizeof(double) * nk * nb * 3, hipMemcpyHostToDevice));
+    ctx->have_eig = true;
+    ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
+    return ALT3_OK;
+}
+
+int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *v33s8)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(v3_run(ctx, ibz_lo, ibz_hi, precision));
+    tm.stop();
+    if (v33s8) ALT3_TRY(v3_copy_out(ctx, v33s8));
+    return ALT3_OK;
+}
+
+int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(fgr_run(ctx));
+    tm.stop();
+    if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
+    if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+    return ALT3_OK;
+}
+
+int alt3_tau(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tauN, double *tauU, double *tautot)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(tau_run(ctx, tempk, Q, QN, QU, tauN, tauU, tautot));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_kappa_rta(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(kappa_rta_run(ctx, tempk, kappa, FRTA, taueff));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_collision(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(collision_run(ctx, tempk, CM, RHS));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, int max_iter, int ndig, double *funcmin,
+            int *iters)
+{
+    if (!ctx || !x || n <= 0) return fail(ALT3_E_ARG, "bad argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    if ((A == nullptr) != (b == nullptr)) return fail(ALT3_E_ARG, "A and b must both be given or both be NULL");
+    DBuf<double> dA, db, dx;
+    const double *pA, *pb;
+    if (!A) {
+        if (!ctx->have_cm) return fail(ALT3_E_STATE, "alt3_cg(A=NULL) needs alt3_collision first");
+        if (ctx->ibz_lo != 0 || ctx->ibz_hi != ctx->nibz - 1) return fail(ALT3_E_STATE, "collision matrix does not cover the IBZ");
+        if (n != ctx->nibz * ctx->nb * 3) return fail(ALT3_E_ARG, "n must be nibz*nb*3");
+        pA = ctx->d_cm.p;
+        pb = ctx->d_rhs.p;
+    } else {
+        ALT3_TRY(dA.alloc((size_t)n * n));
+        ALT3_TRY(db.alloc(n));
+        ALT3_HIP(hipMemcpy(dA.p, A, sizeof(double) * n * n, hipMemcpyHostToDevice));
+        ALT3_HIP(hipMemcpy(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+        pA = dA.p;
+        pb = db.p;
+    }
+    ALT3_TRY(dx.alloc(n));
+    ALT3_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    Timer tm(ctx);
+    ALT3_TRY(cg_device(ctx, n, pA, pb, dx.p, max_iter, ndig, funcmin, iters));
+    tm.stop();
+    ALT3_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return ALT3_OK;
+}
+
+int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, double *taueff)
+{
+    if (!ctx || !x) return fail(ALT3_E_ARG, "null argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    const int n = ctx->nibz * ctx->nb * 3;
+    DBuf<double> dx, dte;
+    ALT3_TRY(dx.alloc(n));
+    ALT3_TRY(dte.alloc((size_t)ctx->nibz * ctx->nb));
+    ALT3_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (taueff)
+        ALT3_HIP(hipMemcpy(dte.p, taueff, sizeof(double) * ctx->nibz * ctx->nb, hipMemcpyHostToDevice));
+    else
+        ALT3_HIP(hipMemset(dte.p, 0, sizeof(double) * ctx->nibz * ctx->nb));
+    Timer tm(ctx);
+    ALT3_TRY(kappa_bte_run(ctx, tempk, dx.p, kappa, dte.p));
+    tm.stop();
+    if (taueff) ALT3_HIP(hipMemcpy(taueff, dte.p, sizeof(double) * ctx->nibz * ctx->nb, hipMemcpyDeviceToHost));
+    return ALT3_OK;
+}
+
+int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, double *kappa_rta, double *kappa_bte,
+               int *cg_iters, double *stage_ms)
+{
+    if (!ctx || ntemps < 1 || !temps) return fail(ALT3_E_ARG, "bad argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int nb = ctx->nb, nibz = ctx->nibz, n = nibz * nb * 3;
+    auto stage = [&](int slot, auto &&fn) -> int {
+        Timer tm(ctx);
+        int rc = fn();
+        tm.stop();
+        ms[slot] += ctx->last_ms;
+        return rc;
+    };
+    if (!ctx->have_eig) ALT3_TRY(stage(0, [&] { return eig_run(ctx); }));
+    ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
+    ALT3_TRY(stage(2, [&] { return fgr_run(ctx); }));
+    DBuf<double> dx, dte;
+    ALT3_TRY(dx.alloc(n));
+    ALT3_TRY(dte.alloc((size_t)nibz * nb));
+    for (int it = 0; it < ntemps; it++) {
+        double T = temps[it];
+        ALT3_TRY(stage(3, [&] { return tau_run(ctx, T, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr); }));
+        ALT3_TRY(stage(4, [&] { return kappa_rta_run(ctx, T, kappa_rta ? kappa_rta + (size_t)it * nb * 9 : nullptr, nullptr, nullptr); }));
+        ALT3_TRY(stage(5, [&] { return collision_run(ctx, T, nullptr, nullptr); }));
+        // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843)
+        std::vector<double> fr((size_t)nibz * nb * 3), x0(n);
+        ALT3_HIP(hipMemcpy(fr.data(), ctx->d_frta.p, sizeof(double) * fr.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < nibz; i++)
+            for (int l = 0; l < nb; l++)
+                for (int a = 0; a < 3; a++) x0[3 * ((size_t)i * nb + l) + a] = fr[i + (size_t)nibz * (l + (size_t)nb * a)];
+        ALT3_HIP(hipMemcpy(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        ALT3_HIP(hipMemcpy(dte.p, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToDevice));
+        int its = 0;
+        ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its); }));
+        if (cg_iters) cg_iters[it] = its;
+        ALT3_TRY(stage(7, [&] { return kappa_bte_run(ctx, T, dx.p, kappa_bte ? kappa_bte + (size_t)it * nb * 9 : nullptr, dte.p); }));
+    }
+    if (stage_ms) std::memcpy(stage_ms, ms, sizeof(ms));
+    return ALT3_OK;
+}
+
+int alt3_last_kernel_ms(const alt3_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return fail(ALT3_E_ARG, "null argument");
+    *ms = ctx->last_ms;
+    return ALT3_OK;
+}
+
+}  // extern "C"

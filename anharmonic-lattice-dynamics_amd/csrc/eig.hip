@@ -1,0 +1,371 @@
+// Harmonic phonons on the q-mesh, FP64 on the GPU.
+//
+// Replaces get_frequencies (THERMACOND/rest.f90:2230-2405): for every mesh point the
+// dynamical matrix (set_dynamical_matrix, rest.f90:445-556) is built and diagonalised at q
+// and at q +- dq e_a (finitedif_vel, rest.f90:2587-2631, dq = REAL*4 0.001), i.e. 7
+// Hermitian eigenproblems per q.  One wavefront owns one matrix in LDS and runs cyclic
+// complex Jacobi rotations (lane-parallel row/column updates); Jacobi converges to full
+// FP64 accuracy on the small (3*natoms)^2 matrices, including degenerate branches.
+// Eigenvalue ordering follows zheev (ascending), then the reference's post-processing:
+// |lambda|, eigenvectors multiplied by i when "pure imaginary", the second sort of
+// get_frequencies that permutes eigenvalues/eigenvectors but not velocities, and
+// subst_eivecs (rest.f90:3262-3293).
+#include "alt3_impl.h"
+
+namespace {
+
+struct zd {
+    double r, i;
+};
+__device__ __forceinline__ zd zmul(zd a, zd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+
+constexpr int kDq = 7;  // q, q+dx, q-dx, q+dy, q-dy, q+dz, q-dz
+
+// entries: e = row*nb + col; per entry a CSR range of FC2 terms (reference term order)
+__global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *__restrict__ kpc,
+                                                  const int *__restrict__ ent_ptr,
+                                                  const double *__restrict__ term_amp,
+                                                  const double *__restrict__ term_rr, double dq,
+                                                  double *__restrict__ evals7,   // [nk][7][nb]
+                                                  double *__restrict__ evec0)    // [nk][nb col][nb comp] complex
+{
+    extern __shared__ double lds[];
+    const int m = blockIdx.x;  // matrix id = q*7 + variant
+    const int q = m / kDq, var = m % kDq;
+    if (q >= nk) return;
+    const int lane = threadIdx.x;
+    zd *A = reinterpret_cast<zd *>(lds);  // [nb][nb] row-major
+    zd *V = A + nb * nb;                  // [nb][nb] row-major, V[k][col]
+    double *red = reinterpret_cast<double *>(V + nb * nb);  // [64]
+    __shared__ double s_rot[5];           // c, s, cos(phi), sin(phi), skip
+    __shared__ int s_ord[64];
+
+    double kq[3] = {kpc[3 * q], kpc[3 * q + 1], kpc[3 * q + 2]};
+    if (var > 0) {
+        int a = (var - 1) / 2;
+        kq[a] = (var & 1) ? kq[a] + dq : kq[a] - dq;
+    }
+    const int nn = nb * nb;
+    // 1. D(q): each entry sums its own terms in reference order
+    for (int e = lane; e < nn; e += 64) {
+        double re = 0, im = 0;
+        for (int t = ent_ptr[e]; t < ent_ptr[e + 1]; t++) {
+            double ph = kq[0] * term_rr[3 * t] + kq[1] * term_rr[3 * t + 1] + kq[2] * term_rr[3 * t + 2];
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            re = re + term_amp[t] * cs;
+            im = im + term_amp[t] * sn;
+        }
+        A[e] = {re, im};
+        V[e] = {(e / nb == e % nb) ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    // 2. Hermitisation exactly as rest.f90:514-543 (note: column nb is left untouched)
+    for (int e = lane; e < nn; e += 64) {
+        int t = e / nb, j = e % nb;
+        if (t == j) {
+            zd d = A[e];
+            if (fabs(d.i) > 9e-4 * fabs(d.r)) A[e] = {d.r, 0.0};
+        }
+    }
+    __syncthreads();
+    for (int e = lane; e < nn; e += 64) {
+        int t = e / nb, j = e % nb;
+        if (j > t && j < nb - 1) {
+            zd a = A[t * nb + j], b = A[j * nb + t];
+            double mi = (a.i - b.i) / 2, mj = (a.r + b.r) / 2;
+            A[t * nb + j] = {mj, mi};
+            A[j * nb + t] = {mj, -mi};
+        }
+    }
+    __syncthreads();
+    {
+        double si = 0, sr = 0;
+        for (int e = lane; e < nn; e += 64) {
+            si += fabs(A[e].i);
+            sr += fabs(A[e].r);
+        }
+        red[lane] = si;
+        __syncthreads();
+        if (lane == 0) {
+            double s = 0;
+            for (int l = 0; l < 64; l++) s += red[l];
+            s_rot[0] = s;
+        }
+        __syncthreads();
+        red[lane] = sr;
+        __syncthreads();
+        if (lane == 0) {
+            double s = 0;
+            for (int l = 0; l < 64; l++) s += red[l];
+            s_rot[1] = s;
+        }
+        __syncthreads();
+        bool realify = (s_rot[0] / nn) < (s_rot[1] / nn) * 1e-10;
+        __syncthreads();
+        if (realify)
+            for (int e = lane; e < nn; e += 64) A[e] = {A[e].r, 0.0};
+        __syncthreads();
+    }
+    // zheev('U') reads the upper triangle and a real diagonal: make A exactly Hermitian
+    for (int e = lane; e < nn; e += 64) {
+        int r = e / nb, c = e % nb;
+        if (r > c) A[e] = {A[c * nb + r].r, -A[c * nb + r].i};
+        if (r == c) A[e] = {A[e].r, 0.0};
+    }
+    __syncthreads();
+    // 3. cyclic complex Jacobi
+    for (int sweep = 0; sweep < 40; sweep++) {
+        double off = 0, dia = 0;
+        for (int e = lane; e < nn; e += 64) {
+            int r = e / nb, c = e % nb;
+            double v2 = A[e].r * A[e].r + A[e].i * A[e].i;
+            if (r != c) off += v2; else dia += v2;
+        }
+        red[lane] = off;
+        __syncthreads();
+        if (lane == 0) {
+            double s = 0;
+            for (int l = 0; l < 64; l++) s += red[l];
+            s_rot[0] = s;
+        }
+        __syncthreads();
+        red[lane] = dia;
+        __syncthreads();
+        if (lane == 0) {
+            double s = 0;
+            for (int l = 0; l < 64; l++) s += red[l];
+            s_rot[1] = s;
+        }
+        __syncthreads();
+        bool done = !(s_rot[0] > 1e-32 * s_rot[1]) || s_rot[0] == 0.0;
+        __syncthreads();
+        if (done) break;
+        for (int p = 0; p < nb - 1; p++)
+            for (int qq = p + 1; qq < nb; qq++) {
+                if (lane == 0) {
+                    zd apq = A[p * nb + qq];
+                    double mag = sqrt(apq.r * apq.r + apq.i * apq.i);
+                    double app = A[p * nb + p].r, aqq = A[qq * nb + qq].r;
+                    if (mag <= 1e-300 || mag * mag <= 1e-36 * (app * app + aqq * aqq)) {
+                        s_rot[4] = 1.0;
+                        A[p * nb + qq] = {0.0, 0.0};
+                        A[qq * nb + p] = {0.0, 0.0};
+                    } else {
+                        double tau = (aqq - app) / (2.0 * mag);
+                        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        double c = 1.0 / sqrt(1.0 + t * t);
+                        s_rot[0] = c;
+                        s_rot[1] = t * c;
+                        s_rot[2] = apq.r / mag;
+                        s_rot[3] = apq.i / mag;
+                        s_rot[4] = 0.0;
+                    }
+                }
+                __syncthreads();
+                if (s_rot[4] == 0.0) {
+                    const double c = s_rot[0], s = s_rot[1];
+                    const zd em = {s_rot[2], -s_rot[3]};  // e^{-i phi}
+                    const zd ep = {s_rot[2], s_rot[3]};   // e^{+i phi}
+                    if (lane < nb) {  // columns of A and V
+                        int k = lane;
+                        zd x = A[k * nb + p], y = zmul(em, A[k * nb + qq]);
+                        A[k * nb + p] = {c * x.r - s * y.r, c * x.i - s * y.i};
+                        A[k * nb + qq] = {s * x.r + c * y.r, s * x.i + c * y.i};
+                        zd vx = V[k * nb + p], vy = zmul(em, V[k * nb + qq]);
+                        V[k * nb + p] = {c * vx.r - s * vy.r, c * vx.i - s * vy.i};
+                        V[k * nb + qq] = {s * vx.r + c * vy.r, s * vx.i + c * vy.i};
+                    }
+                    __syncthreads();
+                    if (lane < nb) {  // rows of A
+                        int k = lane;
+                        zd x = A[p * nb + k], y = zmul(ep, A[qq * nb + k]);
+                        A[p * nb + k] = {c * x.r - s * y.r, c * x.i - s * y.i};
+                        A[qq * nb + k] = {s * x.r + c * y.r, s * x.i + c * y.i};
+                    }
+                    __syncthreads();
+                    if (lane == 0) {
+                        A[p * nb + qq] = {0.0, 0.0};
+                        A[qq * nb + p] = {0.0, 0.0};
+                        A[p * nb + p].i = 0.0;
+                        A[qq * nb + qq].i = 0.0;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    // 4. ascending order of the signed eigenvalues (zheev), ties keep index order
+    if (lane == 0) {
+        for (int k = 0; k < nb; k++) s_ord[k] = k;
+        for (int a = 1; a < nb; a++) {
+            int x = s_ord[a];
+            double vx = A[x * nb + x].r;
+            int b = a - 1;
+            while (b >= 0 && A[s_ord[b] * nb + s_ord[b]].r > vx) {
+                s_ord[b + 1] = s_ord[b];
+                b--;
+            }
+            s_ord[b + 1] = x;
+        }
+    }
+    __syncthreads();
+    if (lane < nb) {
+        int src = s_ord[lane];
+        evals7[((size_t)q * kDq + var) * nb + lane] = fabs(A[src * nb + src].r);
+    }
+    if (var == 0) {
+        // column j (sorted) of V; "pure imaginary" columns (sum |Re| < 1e-3) times i
+        for (int j = lane; j < nb; j += 64) {
+            int src = s_ord[j];
+            double sre = 0;
+            for (int k = 0; k < nb; k++) sre += fabs(V[k * nb + src].r);
+            bool rot = sre < 1e-3;
+            for (int k = 0; k < nb; k++) {
+                zd v = V[k * nb + src];
+                if (rot) v = {-v.i, v.r};
+                size_t o = 2 * (((size_t)q * nb + j) * nb + k);
+                evec0[o] = v.r;
+                evec0[o + 1] = v.i;
+            }
+        }
+    }
+}
+
+// finitedif_vel velocities + get_frequencies' second sort
+__global__ void k_eig_finish(int nk, int nb, double dq, double cnst, const double *__restrict__ evals7,
+                             const double *__restrict__ evec0, double *__restrict__ eival,
+                             double *__restrict__ evec, double *__restrict__ vel, double *__restrict__ freq)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nk) return;
+    const double *e = evals7 + (size_t)q * kDq * nb;
+    int mp[64];
+    for (int i = 0; i < nb; i++) mp[i] = i;
+    for (int i = 0; i < nb; i++)  // sort (extratools.f90:154-175)
+        for (int j = i + 1; j < nb; j++)
+            if (e[mp[i]] > e[mp[j]]) {
+                int t = mp[i];
+                mp[i] = mp[j];
+                mp[j] = t;
+            }
+    for (int l = 0; l < nb; l++) {
+        double e0 = e[l];
+        for (int a = 0; a < 3; a++) {
+            double ep = e[(1 + 2 * a) * nb + l], em = e[(2 + 2 * a) * nb + l];
+            vel[((size_t)q * nb + l) * 3 + a] = (ep - em) / 2 / dq / 2 / sqrt(e0) * cnst * 1e-8 * 2 * alt3::PI;
+        }
+    }
+    for (int j = 0; j < nb; j++) {
+        double ev = e[mp[j]];
+        eival[(size_t)q * nb + j] = ev;
+        freq[(size_t)q * nb + j] = sqrt(ev) * cnst;
+        const double *src = evec0 + 2 * ((size_t)q * nb + mp[j]) * nb;
+        double *dst = evec + 2 * ((size_t)q * nb + j) * nb;
+        for (int k = 0; k < 2 * nb; k++) dst[k] = src[k];
+    }
+}
+
+// subst_eivecs: for k in the negative half, e(-k) <- conj(e(k))
+__global__ void k_subst(int nneg, int nb, const int *__restrict__ negs, const int *__restrict__ negidx,
+                        double *__restrict__ evec)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int per = 2 * nb * nb;
+    if (t >= nneg * nb * nb) return;
+    int n = t / (nb * nb), e = t % (nb * nb);
+    const double *src = evec + (size_t)negs[n] * per;
+    double *dst = evec + (size_t)negidx[n] * per;
+    dst[2 * e] = src[2 * e];
+    dst[2 * e + 1] = -src[2 * e + 1];
+}
+
+}  // namespace
+
+namespace alt3 {
+
+int eig_prepare(alt3_ctx *ctx, const alt3_system *sys)
+{
+    const int nb = ctx->nb, nn = nb * nb;
+    std::vector<std::vector<int>> per(nn);
+    for (int t = 0; t < sys->nt2; t++) {
+        int i0 = sys->fc2_atoms[2 * t], j = sys->fc2_atoms[2 * t + 1];
+        if (i0 < 1 || i0 > sys->natoms0) continue;  // only home-cell rows (rest.f90:481)
+        if (j < 1 || j > sys->natoms) return fail(ALT3_E_ARG, "fc2 atom index out of range");
+        if (sys->fc2_xyz[2 * t] < 1 || sys->fc2_xyz[2 * t] > 3 || sys->fc2_xyz[2 * t + 1] < 1 || sys->fc2_xyz[2 * t + 1] > 3)
+            return fail(ALT3_E_ARG, "fc2 xyz index out of range");
+        int j0 = sys->iatomcell0[j - 1];
+        int row = sys->fc2_xyz[2 * t] - 1 + 3 * (i0 - 1), col = sys->fc2_xyz[2 * t + 1] - 1 + 3 * (j0 - 1);
+        per[row * nb + col].push_back(t);
+    }
+    std::vector<int> ptr(nn + 1, 0);
+    std::vector<double> amp, rr;
+    for (int e = 0; e < nn; e++) {
+        for (int t : per[e]) {
+            int i0 = sys->fc2_atoms[2 * t], j = sys->fc2_atoms[2 * t + 1], j0 = sys->iatomcell0[j - 1];
+            amp.push_back(sys->fc2[t] / std::sqrt(sys->mass[i0 - 1] * sys->mass[j0 - 1]));
+            for (int a = 0; a < 3; a++) rr.push_back(sys->atompos[3 * (j - 1) + a] - sys->atompos[3 * (j0 - 1) + a]);
+        }
+        ptr[e + 1] = (int)amp.size();
+    }
+    ctx->nt2h = (int)amp.size();
+    ALT3_TRY(ctx->d_fc2_rc.alloc(nn + 1));
+    ALT3_TRY(ctx->d_fc2_amp.alloc(std::max<size_t>(1, amp.size())));
+    ALT3_TRY(ctx->d_fc2_rr.alloc(std::max<size_t>(1, rr.size())));
+    ALT3_HIP(hipMemcpy(ctx->d_fc2_rc.p, ptr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
+    if (!amp.empty()) {
+        ALT3_HIP(hipMemcpy(ctx->d_fc2_amp.p, amp.data(), sizeof(double) * amp.size(), hipMemcpyHostToDevice));
+        ALT3_HIP(hipMemcpy(ctx->d_fc2_rr.p, rr.data(), sizeof(double) * rr.size(), hipMemcpyHostToDevice));
+    }
+    return ALT3_OK;
+}
+
+int eig_run(alt3_ctx *ctx)
+{
+    const int nk = ctx->nk, nb = ctx->nb;
+    if (nb > 64) return fail(ALT3_E_ARG, "more than 64 branches per cell");
+    DBuf<double> ev7, evec0;
+    ALT3_TRY(ev7.alloc((size_t)nk * kDq * nb));
+    ALT3_TRY(evec0.alloc((size_t)nk * nb * nb * 2));
+    ALT3_TRY(ctx->d_eival.alloc((size_t)nk * nb));
+    ALT3_TRY(ctx->d_freq.alloc((size_t)nk * nb));
+    ALT3_TRY(ctx->d_vel.alloc((size_t)nk * nb * 3));
+    ALT3_TRY(ctx->d_evec.alloc((size_t)nk * nb * nb * 2));
+    const double dq = (double)0.001f;
+    size_t lds = (size_t)2 * nb * nb * sizeof(zd) + 64 * sizeof(double);
+    hipLaunchKernelGGL(k_eig_solve, dim3(nk * kDq), dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,
+                       ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+    ALT3_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_eig_finish, dim3((nk + 63) / 64), dim3(64), 0, ctx->stream, nk, nb, dq, cnst(), ev7.p,
+                       evec0.p, ctx->d_eival.p, ctx->d_evec.p, ctx->d_vel.p, ctx->d_freq.p);
+    ALT3_HIP(hipGetLastError());
+    // subst_eivecs
+    const int npos = ctx->T.npos, nneg = nk - npos;
+    if (nneg > 0) {
+        std::vector<int> negs(nneg), negidx(nneg);
+        for (int n = 0; n < nneg; n++) {
+            int k = ctx->T.mappos[npos + n];
+            negs[n] = k;
+            int c2 = k % ctx->nc[2], c1 = (k / ctx->nc[2]) % ctx->nc[1], c0 = k / (ctx->nc[1] * ctx->nc[2]);
+            int m0 = (ctx->nc[0] - c0) % ctx->nc[0], m1 = (ctx->nc[1] - c1) % ctx->nc[1], m2 = (ctx->nc[2] - c2) % ctx->nc[2];
+            negidx[n] = m2 + ctx->nc[2] * (m1 + ctx->nc[1] * m0);
+        }
+        DBuf<int> dn, di;
+        ALT3_TRY(dn.alloc(nneg));
+        ALT3_TRY(di.alloc(nneg));
+        ALT3_HIP(hipMemcpyAsync(dn.p, negs.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
+        ALT3_HIP(hipMemcpyAsync(di.p, negidx.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
+        int tot = nneg * nb * nb;
+        hipLaunchKernelGGL(k_subst, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, nneg, nb, dn.p, di.p, ctx->d_evec.p);
+        ALT3_HIP(hipGetLastError());
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        dn.release();
+        di.release();
+    }
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ev7.release();
+    evec0.release();
+    ctx->have_eig = true;
+    return ALT3_OK;
+}
+
+}  // namespace alt3

@@ -1,0 +1,260 @@
+// Tetrahedron-integrated three-phonon scattering weights on the GPU.
+//
+// Replaces calculate_FGR_mpi2 (THERMACOND/sub.f90:7199-7499) and weight_tet
+// (tetrahedron.f90:188-384).  For every (q1 in the irreducible zone, l1, l2, l3) and process
+// (1: coalescence, corner energy -w(q2,l2)+w(q1+q2,l3); 2: decay, w(q2,l2)+w(q1-q2,l3)) the
+// linear-tetrahedron weights of delta(w(q1,l1) - e) over the 6*nk tetrahedra are
+// accumulated per mesh point in LDS and multiplied by F(q2) = |V3|^2 (2pi)^2/nk with the
+// acoustic-Gamma mask:
+//   P_sm(q1,q2) = F(q2) * sum_{tetra containing q2} c/6.
+// The corner energies and weights keep the reference's REAL*4 types (energies rounded to
+// float, differences in float, mixed float/double formulae) so ties and the degenerate-gap
+// clamp behave identically; FMA contraction is disabled in this file for the same reason.
+#include "alt3_impl.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ __forceinline__ void coords(const alt3::MeshDims &d, int k, int &c0, int &c1, int &c2)
+{
+    c2 = k % d.n2;
+    c1 = (k / d.n2) % d.n1;
+    c0 = k / (d.n1 * d.n2);
+}
+__device__ __forceinline__ int idx(const alt3::MeshDims &d, int c0, int c1, int c2) { return c2 + d.n2 * (c1 + d.n1 * c0); }
+__device__ __forceinline__ int wrap(int x, int n)
+{
+    x %= n;
+    return x < 0 ? x + n : x;
+}
+
+// weight_tet for one tetrahedron: corner energies w[4] (already rounded to REAL*4) at e
+__device__ void weight_tet1(const float *w, double e, double *c)
+{
+    float en[4] = {w[0], w[1], w[2], w[3]};
+    // SSORT: descending, first maximum wins
+    for (int i = 0; i < 3; i++) {
+        int m = i;
+        for (int k = i + 1; k < 4; k++)
+            if (en[k] > en[m]) m = k;
+        if (m != i) {
+            float t = en[i];
+            en[i] = en[m];
+            en[m] = t;
+        }
+    }
+    int uns[4];
+    if (en[0] != en[1] && en[1] != en[2] && en[2] != en[3]) {
+        for (int k = 0; k < 4; k++) uns[k] = (en[k] == w[0]) ? 1 : (en[k] == w[1]) ? 2 : (en[k] == w[2]) ? 3 : 4;
+    } else {
+        for (int k = 0; k < 4; k++) {
+            uns[k] = (en[k] == w[0]) ? 1 : (en[k] == w[1]) ? 2 : (en[k] == w[2]) ? 3 : 4;
+            for (int l = 0; l < k; l++) {
+                if (uns[l] == uns[k] && uns[k] == 1) {
+                    if (en[k] == w[1]) uns[k] = 2;
+                    else if (en[k] == w[2]) uns[k] = 3;
+                    else if (en[k] == w[3]) uns[k] = 4;
+                } else if (uns[l] == uns[k] && uns[k] == 2) {
+                    if (en[k] == w[2]) uns[k] = 3;
+                    else if (en[k] == w[3]) uns[k] = 4;
+                } else if (uns[l] == uns[k] && uns[k] == 3) {
+                    uns[k] = 4;
+                }
+            }
+        }
+    }
+    float e1 = en[3], e2 = en[2], e3 = en[1], e4 = en[0];
+    float e21 = e2 - e1, e31 = e3 - e1, e41 = e4 - e1, e32 = e3 - e2, e42 = e4 - e2, e43 = e4 - e3;
+    const double small1 = 1e-10;
+    const float small2 = (float)1e-10;
+#define CLAMP(x)                                              \
+    {                                                         \
+        if (fabs((double)(x)) < small1 && (x) > 0.0f) (x) = small2;  \
+        if (fabs((double)(x)) < small1 && (x) < 0.0f) (x) = -small2; \
+    }
+    CLAMP(e21) CLAMP(e31) CLAMP(e41) CLAMP(e42) CLAMP(e43)
+#undef CLAMP
+    const double E1 = e1, E2 = e2, E3 = e3, E4 = e4;
+    double cc[4];
+    if (e < E1 || e > E4) {
+        c[0] = c[1] = c[2] = c[3] = 0;
+        return;
+    } else if (E1 <= e && e <= E2) {
+        double d1 = e - E1;
+        cc[3] = ((E2 - e) / (double)e21 + (E3 - e) / (double)e31 + (E4 - e) / (double)e41) * (d1 * d1) / (double)(e41 * e31 * e21);
+        cc[2] = (d1 * d1 * d1) / (double)(e21 * e21 * e31 * e41);
+        cc[1] = (d1 * d1 * d1) / (double)(e21 * (e31 * e31) * e41);
+        cc[0] = (d1 * d1 * d1) / (double)(e21 * e31 * (e41 * e41));
+    } else if (E2 <= e && e <= E3) {
+        float c11 = (float)((E3 - e) / (double)(e31 * e31));
+        float c12 = (float)((E4 - e) / (double)(e41 * e41));
+        float c21 = (float)((E3 - e) / (double)(e32 * e32));
+        float c22 = (float)((E4 - e) / (double)(e42 * e42));
+        float c31 = (float)((e - E2) / (double)(e32 * e32));
+        float c32 = (float)((e - E1) / (double)(e31 * e31));
+        float c41 = (float)((e - E2) / (double)(e42 * e42));
+        float c42 = (float)((e - E1) / (double)(e41 * e41));
+        const double a32 = (E3 - e) * (e - E2), a41 = (E4 - e) * (e - E1), a31 = (E3 - e) * (e - E1), a42 = (E4 - e) * (e - E2);
+        float b11 = (float)(a32 / (double)(e42 * e32) + a41 / (double)(e41 * e42) + a31 / (double)(e32 * e41));
+        float b12 = (float)(a41 / (double)(e42 * e31) + a42 / (double)(e42 * e32) + a31 / (double)(e31 * e32));
+        float b21 = (float)(a32 / (double)(e42 * e31) + a42 / (double)(e42 * e41) + a31 / (double)(e31 * e41));
+        float b22 = (float)(a32 / (double)(e32 * e31) + a41 / (double)(e41 * e31) + a42 / (double)(e32 * e41));
+        float b31 = b21;
+        float b32 = b11;
+        float b41 = b22;
+        float b42 = b12;
+        cc[3] = 0.5f * (c11 * b11 + c12 * b12);
+        cc[2] = 0.5f * (c21 * b21 + c22 * b22);
+        cc[1] = 0.5f * (c31 * b31 + c32 * b32);
+        cc[0] = 0.5f * (c41 * b41 + c42 * b42);
+    } else {
+        double d4 = E4 - e;
+        cc[3] = (d4 * d4 * d4) / (double)(e41 * e41 * e42 * e43);
+        cc[2] = (d4 * d4 * d4) / (double)(e41 * (e42 * e42) * e43);
+        cc[1] = (d4 * d4 * d4) / (double)(e41 * e42 * (e43 * e43));
+        cc[0] = ((e - E3) / (double)e43 + (e - E2) / (double)e42 + (e - E1) / (double)e41) * (d4 * d4) / (double)(e41 * e42 * e43);
+    }
+    double out[4] = {0, 0, 0, 0};
+    for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
+    c[0] = out[0];
+    c[1] = out[1];
+    c[2] = out[2];
+    c[3] = out[3];
+}
+
+// one workgroup per (s, l1, l2, l3, process); LDS holds the float corner energy and the
+// double weight sum of every mesh point
+template <typename VT>
+__global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, int ibz_lo,
+                                             const int *__restrict__ mapinv, const double *__restrict__ freq,
+                                             const VT *__restrict__ v33, double *__restrict__ P1,
+                                             double *__restrict__ P2)
+{
+    extern __shared__ double lds[];
+    double *W = lds;                                   // [nk]
+    float *en = reinterpret_cast<float *>(W + nk);     // [nk]
+    __shared__ float s_lo[256], s_hi[256];
+    const int proc = blockIdx.x & 1;
+    int r = blockIdx.x >> 1;
+    const int l3 = r % nb; r /= nb;
+    const int l2 = r % nb; r /= nb;
+    const int l1 = r % nb; r /= nb;
+    const int s = r;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    coords(md, k1, a0, a1, a2);
+    const double omega1 = freq[(size_t)k1 * nb + l1];
+    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    float lo = 3.4e38f, hi = -3.4e38f;
+    for (int w = threadIdx.x; w < nk; w += blockDim.x) {
+        int c0, c1, c2;
+        coords(md, w, c0, c1, c2);
+        double e;
+        if (proc == 0) {  // -w(q2,l2) + w(q1+q2,l3); q1+q2 -> c1+c2-n/2
+            int p = idx(md, wrap(a0 + c0 - h0, md.n0), wrap(a1 + c1 - h1, md.n1), wrap(a2 + c2 - h2, md.n2));
+            e = -freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+        } else {          // w(q2,l2) + w(q1-q2,l3); q1-q2 -> c1-c2+n/2
+            int p = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));
+            e = freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+        }
+        float ef = (float)e;
+        en[w] = ef;
+        W[w] = 0.0;
+        lo = fminf(lo, ef);
+        hi = fmaxf(hi, ef);
+    }
+    s_lo[threadIdx.x] = lo;
+    s_hi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s_lo[threadIdx.x] = fminf(s_lo[threadIdx.x], s_lo[threadIdx.x + st]);
+            s_hi[threadIdx.x] = fmaxf(s_hi[threadIdx.x], s_hi[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    const bool any = !(omega1 < (double)s_lo[0] || omega1 > (double)s_hi[0]);
+    if (any) {
+        // tetrahedra of cube k: corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2} (make_kp_reg_tet)
+        const int ntet = 6 * nk;
+        for (int t = threadIdx.x; t < ntet; t += blockDim.x) {
+            int k = t / 6, l = t % 6;
+            int c0, c1, c2;
+            coords(md, k, c0, c1, c2);
+            int d0 = (c0 + 1) % md.n0, d1 = (c1 + 1) % md.n1, d2 = (c2 + 1) % md.n2;
+            int pt[8] = {idx(md, c0, c1, c2), idx(md, d0, c1, c2), idx(md, c0, d1, c2), idx(md, d0, d1, c2),
+                         idx(md, c0, c1, d2), idx(md, d0, c1, d2), idx(md, c0, d1, d2), idx(md, d0, d1, d2)};
+            int v[4];
+            switch (l) {
+                case 0: v[0] = pt[0]; v[1] = pt[1]; v[2] = pt[2]; v[3] = pt[5]; break;
+                case 1: v[0] = pt[1]; v[1] = pt[2]; v[2] = pt[3]; v[3] = pt[5]; break;
+                case 2: v[0] = pt[0]; v[1] = pt[2]; v[2] = pt[4]; v[3] = pt[5]; break;
+                case 3: v[0] = pt[2]; v[1] = pt[3]; v[2] = pt[5]; v[3] = pt[7]; break;
+                case 4: v[0] = pt[2]; v[1] = pt[4]; v[2] = pt[5]; v[3] = pt[6]; break;
+                default: v[0] = pt[2]; v[1] = pt[5]; v[2] = pt[6]; v[3] = pt[7]; break;
+            }
+            float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
+            float mn = fminf(fminf(w4[0], w4[1]), fminf(w4[2], w4[3]));
+            float mx = fmaxf(fmaxf(w4[0], w4[1]), fmaxf(w4[2], w4[3]));
+            if (omega1 < (double)mn || omega1 > (double)mx) continue;
+            double c[4];
+            weight_tet1(w4, omega1, c);
+            for (int x = 0; x < 4; x++)
+                if (c[x] != 0.0) atomicAdd(&W[v[x]], c[x] / 6.0);
+        }
+    }
+    __syncthreads();
+    const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
+    const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+    double *P = proc == 0 ? P1 : P2;
+    for (int w = threadIdx.x; w < nk; w += blockDim.x) {
+        int c0, c1, c2;
+        coords(md, w, c0, c1, c2);
+        double F;
+        if (proc == 0) {
+            F = (double)v33[base + w] * tpi2 / nk;
+            bool g2 = (c0 == h0 && c1 == h1 && c2 == h2);
+            bool g3 = (wrap(a0 + c0 - h0, md.n0) == h0 && wrap(a1 + c1 - h1, md.n1) == h1 && wrap(a2 + c2 - h2, md.n2) == h2);
+            if ((l2 < 3 && g2) || (l3 < 3 && g3)) F = 0.0;
+        } else {
+            // F(w) = |V3(q1, -q_w, ...)|^2 ; masks at -q_j = q_w and q1+q_j = q1-q_w
+            int jn = idx(md, (md.n0 - c0) % md.n0, (md.n1 - c1) % md.n1, (md.n2 - c2) % md.n2);
+            F = (double)v33[base + jn] * tpi2 / nk;
+            bool g2 = (c0 == h0 && c1 == h1 && c2 == h2);
+            bool g3 = (wrap(a0 - c0 + h0, md.n0) == h0 && wrap(a1 - c1 + h1, md.n1) == h1 && wrap(a2 - c2 + h2, md.n2) == h2);
+            if ((l2 < 3 && g2) || (l3 < 3 && g3)) F = 0.0;
+        }
+        P[base + w] = W[w] * F;
+    }
+}
+
+}  // namespace
+
+namespace alt3 {
+
+int fgr_run(alt3_ctx *ctx)
+{
+    if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr needs alt3_v3sq_ibz first");
+    const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    const size_t nel = (size_t)ksub * nb * nb * nb * nk;
+    size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
+    if (lds > 65536) return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 5461");
+    ALT3_TRY(ctx->d_p1.alloc(nel));
+    ALT3_TRY(ctx->d_p2.alloc(nel));
+    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
+    dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb * nb));
+    if (ctx->v3_prec == ALT3_V3_MIXED)
+        hipLaunchKernelGGL(k_fgr<float>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
+                           ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
+    else
+        hipLaunchKernelGGL(k_fgr<double>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
+                           ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_fgr = true;
+    ctx->have_tau = ctx->have_cm = false;
+    return ALT3_OK;
+}
+
+}  // namespace alt3

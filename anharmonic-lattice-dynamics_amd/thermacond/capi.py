@@ -1,0 +1,239 @@
+"""ctypes binding of libalt3.so (include/alt3.h) and a thin Context wrapper.
+
+Array conventions follow the C ABI: every array argument/result is in the reference's
+Fortran column-major layout (numpy order='F' with the Fortran shape) with 0-based index
+values.  The library must be built (thermacond.build); there is no CPU fallback: loading
+fails loudly when libalt3.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .refio import InputSet
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libalt3.so")
+
+V3_FP64_REF, V3_FP64, V3_MIXED = 0, 1, 2
+
+ERRORS = {-1: "ALT3_E_ARG", -2: "ALT3_E_HIP", -3: "ALT3_E_MESH", -4: "ALT3_E_STATE", -5: "ALT3_E_DEN0",
+          -6: "ALT3_E_NOMEM", -7: "ALT3_E_NBE"}
+
+
+class Alt3Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class System(C.Structure):
+    _fields_ = [
+        ("natoms0", C.c_int), ("mass", C.c_void_p), ("r123", C.c_void_p), ("latparams", C.c_void_p),
+        ("primlatt", C.c_void_p), ("natoms", C.c_int), ("atompos", C.c_void_p), ("iatomcell0", C.c_void_p),
+        ("nt2", C.c_int), ("fc2_atoms", C.c_void_p), ("fc2_xyz", C.c_void_p), ("fc2", C.c_void_p),
+        ("nt3", C.c_int), ("fc3_atoms", C.c_void_p), ("fc3_xyz", C.c_void_p), ("fc3", C.c_void_p),
+        ("nc", C.c_int * 3), ("classical", C.c_int), ("host_only", C.c_int),
+    ]
+
+
+EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
+           "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
+           "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with thermacond.build (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        L.alt3_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            if name != "alt3_last_error":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise Alt3Error(rc, lib().alt3_last_error().decode())
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """A device-resident crystal + force constants + mesh (alt3_ctx)."""
+
+    def __init__(self, inp: InputSet, nc: Optional[Sequence[int]] = None, classical: Optional[int] = None,
+                 host_only: bool = False):
+        self.inp = inp
+        nc = tuple(inp.params.nc if nc is None else nc)
+        self._keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        s = System()
+        s.natoms0 = inp.natoms0
+        s.mass = arr(inp.masses, np.float64)
+        s.r123 = arr(inp.r123, np.float64)
+        s.latparams = arr(inp.latparams, np.float64)
+        s.primlatt = arr(np.asfortranarray(inp.primlatt).reshape(-1, order="F"), np.float64)
+        s.natoms = len(inp.iatomcell0)
+        s.atompos = arr(inp.atompos, np.float64)
+        s.iatomcell0 = arr(inp.iatomcell0, np.int32)
+        s.nt2 = len(inp.fc2)
+        s.fc2_atoms = arr(inp.fc2_atoms, np.int32)
+        s.fc2_xyz = arr(inp.fc2_xyz, np.int32)
+        s.fc2 = arr(inp.fc2, np.float64)
+        s.nt3 = len(inp.fc3)
+        s.fc3_atoms = arr(inp.fc3_atoms, np.int32)
+        s.fc3_xyz = arr(inp.fc3_xyz, np.int32)
+        s.fc3 = arr(inp.fc3, np.float64)
+        s.nc = (C.c_int * 3)(*nc)
+        s.classical = int(inp.params.classical if classical is None else classical)
+        s.host_only = 1 if host_only else 0
+        self._sys = s
+        h = C.c_void_p()
+        _check(lib().alt3_create(C.byref(s), C.byref(h)))
+        self.h = h
+        nk, nibz, nb, nops = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().alt3_sizes(h, C.byref(nk), C.byref(nibz), C.byref(nb), C.byref(nops)))
+        self.nk, self.nibz, self.nb, self.nops = nk.value, nibz.value, nb.value, nops.value
+        self.nc = nc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().alt3_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- tables
+    def mesh(self) -> dict:
+        nk, ni, no = self.nk, self.nibz, self.nops
+        d = dict(kpc_eig=np.zeros((nk, 3)), kpc=np.zeros((nk, 3)), mapibz=np.zeros(nk, np.int32),
+                 mapinv=np.zeros(ni, np.int32), kibz=np.zeros((ni, 3)), wibz=np.zeros(ni),
+                 tet=np.zeros((6 * nk, 4), np.int32), opk=np.zeros((no, 3, 3)), mappos=np.zeros(nk, np.int32))
+        npos = C.c_int()
+        _check(lib().alt3_get_mesh(self.h, *[_p(d[k]) for k in ("kpc_eig", "kpc", "mapibz", "mapinv", "kibz", "wibz",
+                                                                 "tet", "opk", "mappos")], C.byref(npos)))
+        d["npos"] = npos.value
+        return d
+
+    def kstars(self, i: int):
+        n = C.c_int(); ops = np.zeros(48, np.int32); n0 = np.zeros(48, np.int32)
+        _check(lib().alt3_kstars(self.h, i, C.byref(n), _p(ops), _p(n0)))
+        return ops[:n.value].copy(), n0[:n.value].copy()
+
+    # ---------------------------------------------------------------- path
+    def eig(self, want: bool = True):
+        nk, nb = self.nk, self.nb
+        if not want:
+            _check(lib().alt3_eig(self.h, None, None, None))
+            return None
+        ev = np.zeros((nb, nk), order="F"); vec = np.zeros((nb, nb, nk), dtype=np.complex128, order="F")
+        vel = np.zeros((3, nb, nk), order="F")
+        _check(lib().alt3_eig(self.h, _p(ev), _p(vec), _p(vel)))
+        return ev, vec, vel
+
+    def set_eig(self, eigenval, eigenvec, veloc):
+        ev = np.asfortranarray(eigenval, dtype=np.float64)
+        vec = np.asfortranarray(eigenvec, dtype=np.complex128)
+        vel = np.asfortranarray(veloc, dtype=np.float64)
+        _check(lib().alt3_set_eig(self.h, _p(ev), _p(vec), _p(vel)))
+
+    def v3sq(self, lo: int, hi: int, precision: int = V3_FP64, want: bool = True):
+        out = np.zeros((hi - lo + 1, self.nk, self.nb, self.nb, self.nb), order="F") if want else None
+        _check(lib().alt3_v3sq_ibz(self.h, lo, hi, precision, _p(out)))
+        self._last_ksub = hi - lo + 1
+        return out
+
+    def fgr(self, want: bool = True):
+        if not want:
+            _check(lib().alt3_fgr(self.h, None, None))
+            return None
+        ksub = self._ksub()
+        P1 = np.zeros((ksub, self.nk, self.nb, self.nb, self.nb), order="F"); P2 = np.zeros_like(P1)
+        _check(lib().alt3_fgr(self.h, _p(P1), _p(P2)))
+        return P1, P2
+
+    def _ksub(self):
+        return getattr(self, "_last_ksub", self.nibz)
+
+    def tau(self, tempk: float):
+        ksub = self._ksub()
+        outs = [np.zeros((ksub, self.nb), order="F") for _ in range(6)]
+        _check(lib().alt3_tau(self.h, C.c_double(tempk), *[_p(o) for o in outs]))
+        return dict(zip(("Q", "QN", "QU", "tauN", "tauU", "tautot"), outs))
+
+    def kappa_rta(self, tempk: float):
+        k = np.zeros((self.nb, 3, 3), order="F"); F = np.zeros((self.nibz, self.nb, 3), order="F")
+        te = np.zeros((self.nibz, self.nb), order="F")
+        _check(lib().alt3_kappa_rta(self.h, C.c_double(tempk), _p(k), _p(F), _p(te)))
+        return k, F, te
+
+    def collision(self, tempk: float, want: bool = True):
+        n = self.nibz * self.nb * 3
+        CM = np.zeros((n, n), order="F") if want else None
+        R = np.zeros(n)
+        _check(lib().alt3_collision(self.h, C.c_double(tempk), _p(CM), _p(R)))
+        return CM, R
+
+    def cg(self, x0, A=None, b=None, max_iter: int = 100, ndig: int = 2):
+        x = np.array(x0, dtype=np.float64, copy=True)
+        n = len(x)
+        fm = C.c_double(); it = C.c_int()
+        Af = None if A is None else np.asfortranarray(A, dtype=np.float64)
+        bf = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().alt3_cg(self.h, n, _p(Af), _p(bf), _p(x), max_iter, ndig, C.byref(fm), C.byref(it)))
+        return x, it.value, fm.value
+
+    def kappa_bte(self, tempk: float, x, taueff=None):
+        k = np.zeros((self.nb, 3, 3), order="F")
+        te = None if taueff is None else np.asfortranarray(np.array(taueff, dtype=np.float64, copy=True))
+        _check(lib().alt3_kappa_bte(self.h, C.c_double(tempk), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(k), _p(te)))
+        return k, te
+
+    def solve(self, temps: Sequence[float], precision: int = V3_FP64):
+        nt = len(temps)
+        kr = np.zeros((nt, self.nb, 3, 3)); kb = np.zeros_like(kr)
+        its = np.zeros(nt, np.int32); ms = np.zeros(8)
+        t = np.ascontiguousarray(temps, dtype=np.float64)
+        kr_f = np.zeros(nt * self.nb * 9); kb_f = np.zeros_like(kr_f)
+        _check(lib().alt3_solve(self.h, nt, _p(t), precision, _p(kr_f), _p(kb_f), _p(its), _p(ms)))
+        for i in range(nt):
+            kr[i] = kr_f[i * self.nb * 9:(i + 1) * self.nb * 9].reshape((self.nb, 3, 3), order="F")
+            kb[i] = kb_f[i * self.nb * 9:(i + 1) * self.nb * 9].reshape((self.nb, 3, 3), order="F")
+        return dict(kappa_rta=kr, kappa_bte=kb, cg_iters=its, stage_ms=ms)
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().alt3_last_kernel_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def device_bytes(self) -> int:
+        b = C.c_longlong()
+        _check(lib().alt3_device_bytes(self.h, C.byref(b)))
+        return b.value

@@ -1,0 +1,313 @@
+"""Synthetic crystals and their force constants, written in THERMACOND's input format.
+
+The reference takes harmonic and cubic force constants from FOCEX fits
+(FOCEX/ios.f90:1180-1260 writes fc2.dat/fc3.dat, :2256-2350 writes lat_fc.dat).  Here
+they come from model potentials instead, by central finite differences of the forces:
+
+    Phi2(ia,kb)    = -[F_kb(+h_ia) - F_kb(-h_ia)] / (2h)
+    Phi3(ia,jb,kg) = -[F_kg(+h_ia,+h_jb) - F_kg(+h_ia,-h_jb)
+                       - F_kg(-h_ia,+h_jb) + F_kg(-h_ia,-h_jb)] / (4h^2)
+
+with forces F = -dE/du taken exactly (torch autograd, float64) on a finite cluster that
+contains every atom interacting with the displaced ones.  Because the cluster forces sum
+to zero for any displacement, both tensors obey the acoustic sum rule to rounding.
+
+Models
+  * Stillinger-Weber silicon (Stillinger & Weber, PRB 31, 5262 (1985)):
+    eps=2.1683 eV, sigma=2.0951 A, A=7.049556277, B=0.6022245584, p=4, q=0, a=1.80,
+    lambda=21.0, gamma=1.20; diamond lattice a=5.43 A, m=28.0855 amu.
+  * Born-Mayer pair crystals (rock-salt and zincblende type): V(r) = A exp(-r/rho) for
+    unlike neighbours with a harmonic bond-bending-free stabiliser; see born_mayer_*.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# crystal description
+# ---------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class Crystal:
+    """A crystal in the reference's conventions.
+
+    latparams   a,b,c,alpha,beta,gamma of the conventional cell (a,b,c in units of
+                `lattice_parameter`), as structure.params line 1 (rest.f90:96)
+    primlatt    3x3, column j = j-th primitive vector in conventional units (rest.f90:97)
+    lattice_parameter  scale in Angstrom (rest.f90:98)
+    types       1-based type of each primitive atom
+    masses      mass (amu) per type
+    names       element label per type
+    frac        (natoms0,3) positions in conventional reduced coordinates (rest.f90:120)
+    """
+
+    latparams: Tuple[float, ...]
+    primlatt: np.ndarray
+    lattice_parameter: float
+    types: List[int]
+    masses: List[float]
+    names: List[str]
+    frac: np.ndarray
+
+    @property
+    def natoms0(self) -> int:
+        return len(self.types)
+
+    @property
+    def conv_to_cart(self) -> np.ndarray:
+        """force_constants.f90:53-63 (angles in degrees)."""
+        a, b, c, al, be, ga = self.latparams
+        a, b, c = a * self.lattice_parameter, b * self.lattice_parameter, c * self.lattice_parameter
+        d2r = math.atan(1.0) * 4 / 180.0
+        al, be, ga = al * d2r, be * d2r, ga * d2r
+        m = np.zeros((3, 3))
+        m[0, 0] = a
+        m[0, 1] = b * math.cos(ga)
+        m[1, 1] = b * math.sin(ga)
+        m[0, 2] = c * math.cos(be)
+        m[1, 2] = c * (math.cos(al) - math.cos(ga) * math.cos(be)) / math.sin(ga)
+        m[2, 2] = math.sqrt(c ** 2 - m[0, 2] ** 2 - m[1, 2] ** 2)
+        return m
+
+    @property
+    def prim_cart(self) -> np.ndarray:
+        """Rows are the Cartesian primitive vectors r1,r2,r3."""
+        return (self.conv_to_cart @ self.primlatt).T
+
+    def home_positions(self) -> np.ndarray:
+        """Cartesian home-cell positions, folded into the primitive cell as
+        force_constants.f90:148-156 (unitcell, force_constants.f90:1357-1377) does."""
+        P = self.prim_cart.T  # columns = primitive vectors
+        Pinv = np.linalg.inv(P)
+        out = []
+        for f in self.frac:
+            x = self.conv_to_cart @ f
+            b = Pinv @ x
+            for i in range(3):
+                while b[i] > 1.0 or abs(b[i] - 1) <= 1e-6:
+                    b[i] -= 1
+                while b[i] < 0.0 and abs(b[i]) > 1e-6:
+                    b[i] += 1
+            out.append(P @ b)
+        return np.array(out)
+
+    def atom_masses(self) -> np.ndarray:
+        return np.array([self.masses[t - 1] for t in self.types], dtype=np.float64)
+
+
+def diamond(a: float = 5.43, mass: float = 28.0855, name: str = "Si") -> Crystal:
+    return Crystal((1, 1, 1, 90, 90, 90),
+                   np.array([[0, .5, .5], [.5, 0, .5], [.5, .5, 0]], dtype=np.float64).T,
+                   a, [1, 1], [mass], [name], np.array([[0, 0, 0], [.25, .25, .25]]))
+
+
+def zincblende(a: float = 5.65, m1: float = 69.72, m2: float = 74.92,
+               n1: str = "Ga", n2: str = "As") -> Crystal:
+    return Crystal((1, 1, 1, 90, 90, 90),
+                   np.array([[0, .5, .5], [.5, 0, .5], [.5, .5, 0]], dtype=np.float64).T,
+                   a, [1, 2], [m1, m2], [n1, n2], np.array([[0, 0, 0], [.25, .25, .25]]))
+
+
+def rocksalt_conventional(a: float = 5.6, m1: float = 72.63, m2: float = 78.97,
+                          n1: str = "Ge", n2: str = "Se") -> Crystal:
+    """8-atom simple-cubic cell of a rock-salt-type binary (4 + 4 atoms)."""
+    fcc = [[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]
+    frac = [f for f in fcc] + [[(f[0] + .5) % 1, f[1], f[2]] for f in fcc]
+    return Crystal((1, 1, 1, 90, 90, 90), np.eye(3), a, [1] * 4 + [2] * 4,
+                   [m1, m2], [n1, n2], np.array(frac, dtype=np.float64))
+
+
+def supercell_2x1x1(c: Crystal) -> Crystal:
+    """2x1x1 supercell of a simple-cubic-cell crystal (primlatt = identity)."""
+    assert np.allclose(c.primlatt, np.eye(3))
+    frac = np.concatenate([c.frac, c.frac + np.array([1.0, 0, 0])]) * np.array([.5, 1, 1])
+    lp = (2.0,) + tuple(c.latparams[1:])
+    return Crystal(lp, np.eye(3), c.lattice_parameter, c.types + c.types,
+                   c.masses, c.names, frac)
+
+
+# ---------------------------------------------------------------------------
+# potentials (torch, float64) and finite-difference force constants
+# ---------------------------------------------------------------------------
+
+SW_SI = dict(eps=2.1683, sigma=2.0951, A=7.049556277, B=0.6022245584, p=4.0, q=0.0,
+             a=1.80, lam=21.0, gamma=1.20)
+
+
+def _sw_energy(x, pairs_i, pairs_j, trip_c, trip_j, trip_k, P=SW_SI):
+    import torch
+    eps, sig, A, B, p, q, a, lam, gam = (P[k] for k in
+                                         ("eps", "sigma", "A", "B", "p", "q", "a", "lam", "gamma"))
+    rc = a * sig
+    d = x[pairs_j] - x[pairs_i]
+    r = torch.sqrt((d * d).sum(-1))
+    e2 = eps * A * (B * (sig / r) ** p - (sig / r) ** q) * torch.exp(sig / (r - rc))
+    dj = x[trip_j] - x[trip_c]
+    dk = x[trip_k] - x[trip_c]
+    rj = torch.sqrt((dj * dj).sum(-1))
+    rk = torch.sqrt((dk * dk).sum(-1))
+    cth = (dj * dk).sum(-1) / (rj * rk)
+    e3 = eps * lam * torch.exp(gam * sig / (rj - rc) + gam * sig / (rk - rc)) * (cth + 1.0 / 3.0) ** 2
+    return e2.sum() + e3.sum()
+
+
+def _bm_energy(x, pairs_i, pairs_j, kind, P):
+    """Born-Mayer repulsion A exp(-r/rho) between unlike first neighbours plus a
+    Born-Mayer-shaped attraction -C exp(-r/rho2) that holds the lattice at the given
+    spacing, and a weaker like-pair term for second neighbours (kind = 1/2)."""
+    import torch
+    d = x[pairs_j] - x[pairs_i]
+    r = torch.sqrt((d * d).sum(-1))
+    A1, rho1, C1, rho1b = P["A1"], P["rho1"], P["C1"], P["rho1b"]
+    A2, rho2, C2, rho2b = P["A2"], P["rho2"], P["C2"], P["rho2b"]
+    e = torch.where(kind == 1,
+                    A1 * torch.exp(-r / rho1) - C1 * torch.exp(-r / rho1b),
+                    A2 * torch.exp(-r / rho2) - C2 * torch.exp(-r / rho2b))
+    # smooth cutoff so higher shells do not interact
+    return e.sum()
+
+
+def _lattice_cluster(crys: Crystal, radius: float):
+    """All atoms with |R + tau_b - centre| < radius, home atoms first."""
+    P = crys.prim_cart
+    tau = crys.home_positions()
+    centre = tau.mean(0)
+    n = int(math.ceil(radius / min(np.linalg.norm(P, axis=1)))) + 2
+    home, others = [], []
+    for n1 in range(-n, n + 1):
+        for n2 in range(-n, n + 1):
+            for n3 in range(-n, n + 1):
+                R = n1 * P[0] + n2 * P[1] + n3 * P[2]
+                for b in range(crys.natoms0):
+                    pos = R + tau[b]
+                    if np.linalg.norm(pos - centre) < radius:
+                        rec = (b + 1, (n1, n2, n3), pos)
+                        (home if (n1, n2, n3) == (0, 0, 0) else others).append(rec)
+    home.sort(key=lambda t: t[0])
+    others.sort(key=lambda t: (round(float(np.linalg.norm(t[2] - centre)), 6), t[1], t[0]))
+    return home + others
+
+
+@dataclasses.dataclass
+class ForceConstants:
+    """Atom list (lat_fc.dat neighbour block) plus FC2/FC3 term lists.
+
+    atompos (natoms,3) Cartesian; iatomcell0 (natoms,) 1-based basis atom;
+    iatomcell (natoms,3) cell indices.  fc2: list of (i,al,j,be,value) with 1-based atom
+    indices into the atom list, fc3: (i,al,j,be,k,ga,value)."""
+    crystal: Crystal
+    atompos: np.ndarray
+    iatomcell0: np.ndarray
+    iatomcell: np.ndarray
+    fc2: np.ndarray     # (nt2, 5) float: i, al, j, be, value
+    fc3: np.ndarray     # (nt3, 7) float: i, al, j, be, k, ga, value
+
+    @property
+    def natoms(self) -> int:
+        return len(self.iatomcell0)
+
+
+def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3: float,
+                        h: float = 0.01, radius: float = 10.0, rel_drop: float = 1e-9,
+                        fc3_perturb=None) -> ForceConstants:
+    import torch
+    cl = _lattice_cluster(crys, radius)
+    x0 = torch.tensor(np.array([c[2] for c in cl]), dtype=torch.float64)
+    nat = x0.shape[0]
+
+    def forces(x):
+        x = x.clone().requires_grad_(True)
+        e = energy_fn(x)
+        (g,) = torch.autograd.grad(e, x)
+        return (-g).detach()
+
+    home = list(range(crys.natoms0))
+    # FC2
+    fc2 = {}
+    for i in home:
+        for al in range(3):
+            xp = x0.clone(); xp[i, al] += h
+            xm = x0.clone(); xm[i, al] -= h
+            d = -(forces(xp) - forces(xm)) / (2 * h)
+            for k in range(nat):
+                for be in range(3):
+                    v = float(d[k, be])
+                    if v != 0.0:
+                        fc2[(i, al, k, be)] = v
+    # FC3: j within cutoff_fc3 of i
+    fc3 = {}
+    for i in home:
+        near = [j for j in range(nat) if float(torch.linalg.norm(x0[j] - x0[i])) < cutoff_fc3]
+        for al in range(3):
+            for j in near:
+                for be in range(3):
+                    f = []
+                    for si, sj in ((1, 1), (1, -1), (-1, 1), (-1, -1)):
+                        x = x0.clone(); x[i, al] += si * h; x[j, be] += sj * h
+                        f.append(forces(x))
+                    d = -(f[0] - f[1] - f[2] + f[3]) / (4 * h * h)
+                    for k in range(nat):
+                        for ga in range(3):
+                            v = float(d[k, ga])
+                            if v != 0.0:
+                                fc3[(i, al, j, be, k, ga)] = v
+    mx2 = max(abs(v) for v in fc2.values())
+    mx3 = max(abs(v) for v in fc3.values())
+    fc2 = {k: v for k, v in fc2.items() if abs(v) > rel_drop * mx2}
+    # symmetrise FC3 in (j,be) <-> (k,ga) (finite-difference error is O(h^2)) and keep union
+    sym = {}
+    for (i, al, j, be, k, ga), v in fc3.items():
+        key2 = (i, al, k, ga, j, be)
+        sym[(i, al, j, be, k, ga)] = 0.5 * (v + fc3.get(key2, 0.0))
+    fc3 = {k: v for k, v in sym.items()
+           if abs(v) > rel_drop * mx3 or abs(sym.get((k[0], k[1], k[4], k[5], k[2], k[3]), 0.0)) > rel_drop * mx3}
+    if fc3_perturb is not None:
+        fc3 = fc3_perturb(fc3)
+    used = set()
+    for (i, al, k, be) in fc2:
+        used.add(i); used.add(k)
+    for (i, al, j, be, k, ga) in fc3:
+        used.add(i); used.add(j); used.add(k)
+    order = [a for a in range(nat) if a in used or a < crys.natoms0]
+    remap = {a: n + 1 for n, a in enumerate(order)}
+    atompos = np.array([cl[a][2] for a in order])
+    iat0 = np.array([cl[a][0] for a in order], dtype=np.int64)
+    iatc = np.array([cl[a][1] for a in order], dtype=np.int64)
+    fc2a = np.array(sorted([(remap[i], al + 1, remap[k], be + 1, v) for (i, al, k, be), v in fc2.items()]),
+                    dtype=np.float64)
+    fc3a = np.array(sorted([(remap[i], al + 1, remap[j], be + 1, remap[k], ga + 1, v)
+                            for (i, al, j, be, k, ga), v in fc3.items()]), dtype=np.float64)
+    return ForceConstants(crys, atompos, iat0, iatc, fc2a, fc3a)
+
+
+def sw_silicon(a: float = 5.43, h: float = 0.01) -> ForceConstants:
+    """Stillinger-Weber Si force constants: FC2 to the 2nd shell, FC3 to the 2nd shell
+    (the SW range 1.8*sigma = 3.77 A reaches only first neighbours, so three-body terms
+    couple first and second neighbours)."""
+    import torch
+    crys = diamond(a)
+    cl = _lattice_cluster(crys, 10.0)
+    x0 = np.array([c[2] for c in cl])
+    rc = SW_SI["a"] * SW_SI["sigma"]
+    pi, pj, tc, tj, tk = [], [], [], [], []
+    nbr = [[] for _ in range(len(x0))]
+    for i in range(len(x0)):
+        d = np.linalg.norm(x0 - x0[i], axis=1)
+        for j in np.nonzero((d < rc) & (d > 0))[0]:
+            nbr[i].append(int(j))
+            if j > i:
+                pi.append(i); pj.append(int(j))
+    for c in range(len(x0)):
+        for u in range(len(nbr[c])):
+            for v in range(u + 1, len(nbr[c])):
+                tc.append(c); tj.append(nbr[c][u]); tk.append(nbr[c][v])
+    T = lambda l: torch.tensor(l, dtype=torch.long)
+    pi, pj, tc, tj, tk = T(pi), T(pj), T(tc), T(tj), T(tk)
+    fn = lambda x: _sw_energy(x, pi, pj, tc, tj, tk)
+    nn = a * math.sqrt(3) / 4
+    return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h)

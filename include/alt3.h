@@ -1,0 +1,145 @@
+/*
+ * alt3 -- MI355X-native three-phonon path of THERMACOND (C ABI).
+ *
+ * One context (alt3_ctx) holds a crystal, its force constants and a q-mesh in device
+ * memory; the entry points below replace the reference subroutines named in each comment
+ * (paths relative to the reference repository).  All buffers are host pointers owned by
+ * the caller; arrays use the reference's Fortran column-major layouts with 0-based index
+ * values unless stated.  A NULL output pointer means "keep on the device, do not copy".
+ * Every function returns 0 on success or a negative ALT3_E* code; alt3_last_error()
+ * returns the message of the last failure on the calling thread.  One host thread per
+ * context; distinct contexts may be used from distinct threads.  No MPI inside.
+ */
+#ifndef ALT3_H
+#define ALT3_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALT3_OK 0
+#define ALT3_E_ARG (-1)      /* bad argument / size mismatch */
+#define ALT3_E_HIP (-2)      /* HIP runtime error */
+#define ALT3_E_MESH (-3)     /* q is not a mesh point (get_k_info inside=2), odd mesh */
+#define ALT3_E_STATE (-4)    /* prerequisite stage not run */
+#define ALT3_E_DEN0 (-5)     /* zero denominator in V3 (matrix_elt_simplified den=0) */
+#define ALT3_E_NOMEM (-6)    /* device allocation failed */
+#define ALT3_E_NBE (-7)      /* occupation of a non-positive frequency (nbe w<=0) */
+
+/* V3 arithmetic */
+#define ALT3_V3_FP64_REF 0   /* per-element sum over all FC3 terms (matrix_elt_simplified) */
+#define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
+#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA with hi/lo split, FP32 contraction */
+
+typedef struct alt3_ctx alt3_ctx;
+
+/* Crystal, force constants and mesh: the module state THERMACOND holds after
+ * read_lattice/read_fc234 (THERMACOND/rest.f90:152-443). */
+typedef struct {
+    int natoms0;               /* atoms in the primitive cell */
+    const double *mass;        /* [natoms0] amu */
+    const double *r123;        /* [3][3] rows r1,r2,r3 (Angstrom), lat_fc.dat */
+    const double *latparams;   /* [6] structure.params line 1 (a,b,c scaled, angles deg) */
+    const double *primlatt;    /* [9] column-major, column j = primitive vector j, conv units */
+    int natoms;                /* atoms in the neighbour list */
+    const double *atompos;     /* [natoms][3] Cartesian */
+    const int *iatomcell0;     /* [natoms] 1-based basis atom of each listed atom */
+    int nt2;                   /* FC2 terms (fc2.dat) */
+    const int *fc2_atoms;      /* [nt2][2] 1-based atom indices */
+    const int *fc2_xyz;        /* [nt2][2] 1..3 */
+    const double *fc2;         /* [nt2] eV/A^2 */
+    int nt3;                   /* FC3 terms (fc3.dat) */
+    const int *fc3_atoms;      /* [nt3][3] */
+    const int *fc3_xyz;        /* [nt3][3] */
+    const double *fc3;         /* [nt3] eV/A^3 */
+    int nc[3];                 /* q-mesh (even), latdyn.params record 1 */
+    int classical;             /* 0 quantum, 1 classical occupations */
+    int host_only;             /* 1: build the mesh/symmetry tables only (no device needed);
+                                  such a context answers alt3_sizes/alt3_get_mesh/alt3_kstars */
+} alt3_system;
+
+int alt3_create(const alt3_system *sys, alt3_ctx **out);
+int alt3_destroy(alt3_ctx *ctx);
+const char *alt3_last_error(void);
+int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes);
+
+/* Sizes: nk = nc1*nc2*nc3, nibz, nb = 3*natoms0, nops (lattice point group). */
+int alt3_sizes(const alt3_ctx *ctx, int *nk, int *nibz, int *nb, int *nops);
+
+/* Mesh and symmetry tables built at creation, copied out:
+ *   kpc_eig [nk][3]  make_kp_reg mesh (REAL*4 fractions), modules.f90:1721-1784
+ *   kpc     [nk][3]  make_kp_reg_tet mesh (FP64), tetrahedron.f90:2-142
+ *   mapibz  [nk], mapinv [nibz], kibz [nibz][3], wibz [nibz]   get_weights, modules.f90:2006-2162
+ *   tet     [6nk][4] tetrahedron corners (make_kp_reg_tet)
+ *   opk     [nops][9] op_kmatrix column-major (force_constants.f90:154-186)
+ *   mappos  [nk]     get_negatives map, rest.f90:3208-3260; *npos its positive count
+ * Any pointer may be NULL. */
+int alt3_get_mesh(const alt3_ctx *ctx, double *kpc_eig, double *kpc, int *mapibz, int *mapinv,
+                  double *kibz, double *wibz, int *tet, double *opk, int *mappos, int *npos);
+
+/* Stars on the mesh of IBZ point i (kstarsinfbz1, sub.f90:5551-5590): returns narms,
+ * fills ops[48] (0-based op index) and n0[48] (mesh index). */
+int alt3_kstars(const alt3_ctx *ctx, int ibz, int *narms, int *ops, int *n0);
+
+/* Harmonic phonons on the mesh: get_frequencies (rest.f90:2230-2405) with finite-difference
+ * velocities (finitedif_vel, :2587-2631), then subst_eivecs (:3262-3293).  FP64 on device.
+ *   eigenval [nk][nb]  (= Fortran eigenval(nb,nk)), |omega^2| in eV/A^2/amu
+ *   eigenvec [nk][nb][nb] complex interleaved (= Fortran eigenvec(nb,nb,nk))
+ *   veloc    [nk][nb][3]  (= Fortran veloc(3,nb,nk)), units of c */
+int alt3_eig(alt3_ctx *ctx, double *eigenval, double *eigenvec, double *veloc);
+
+/* Load externally computed eigen data instead of alt3_eig (same layouts). */
+int alt3_set_eig(alt3_ctx *ctx, const double *eigenval, const double *eigenvec, const double *veloc);
+
+/* |V3|^2 for q1 = IBZ points ibz_lo..ibz_hi (0-based, inclusive), q2 over the mesh,
+ * q3 = -q1-q2: calculate_w3_ibz_split_sq (sub.f90:1050-1233).  Kept on the device for
+ * alt3_fgr; v33s8 (optional) receives Fortran v33s8(ksub,nk,nb,nb,nb), FP64. */
+int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *v33s8);
+
+/* Tetrahedron-integrated Fermi golden rule: calculate_FGR_mpi2 (sub.f90:7199-7499,
+ * weight_tet tetrahedron.f90:188-384).  Uses the |V3|^2 of the last alt3_v3sq_ibz call
+ * (same ibz range).  P1sm/P2sm optional, Fortran (ksub,nk,nb,nb,nb). */
+int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm);
+
+/* Occupations and scattering rates at tempk: distribution (sub.f90:1955-1980),
+ * calculate_tauRTA_mpi2 (:7953-8194), Qandtau (:8198-8305).  Outputs Fortran (ksub,nb):
+ * Q, QN, QU, tauinv_N, tauinv_U, tauinv_tot (1/s).  Any may be NULL. */
+int alt3_tau(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU,
+             double *tauN, double *tauU, double *tautot);
+
+/* RTA conductivity: calculate_RTA_distributionfunction (sub.f90:1238-1289) and mode_kappa
+ * (:1292-1401).  Needs alt3_tau over the whole IBZ.  kappa Fortran (nb,3,3) W/m/K;
+ * FRTA (nibz,nb,3); taueff (nibz,nb). */
+int alt3_kappa_rta(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff);
+
+/* Collision matrix and right-hand side: CollisionMatrix_mpi (sub.f90:3034-3210) for the
+ * current ibz range, cal_RHS_IBZ (:2618-2691).  CM Fortran (ksub*nb*3, nibz*nb*3); RHS
+ * (nibz*nb*3), index 3*(i*nb+l)+alpha.  Both stay on the device for alt3_cg. */
+int alt3_collision(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
+
+/* Conjugate gradients, cg_quadratic (cg.f90:19-69): minimises 1/2 x.A.x - b.x from the
+ * initial x (overwritten).  A == NULL and b == NULL use the device collision matrix and
+ * RHS of the last alt3_collision (n must then be nibz*nb*3).  Stops when
+ * (dx/x)^2 < 10^(-2*ndig) or after max_iter iterations. */
+int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, int max_iter,
+            int ndig, double *funcmin, int *iters);
+
+/* Full-BTE conductivity from the CG solution x: mode_kappa_d (sub.f90:1407-1530) with
+ * veloc_avergeIBZ_invsyFBZ (:5479-5537).  taueff accumulates onto its input, like the
+ * reference module array (pass the alt3_kappa_rta output). */
+int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, double *taueff);
+
+/* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
+ * IBZ, FGR, then per T tau, kappa_RTA, collision, CG (x0 = F_RTA, 100 iterations, 2
+ * digits), kappa_BTE.  kappa_rta/kappa_bte: [ntemps][nb*9] Fortran (nb,3,3) per T.
+ * stage_ms (optional, 8 doubles): eig, v3, fgr, tau, kappa_rta, collision, cg, kappa_bte. */
+int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision,
+               double *kappa_rta, double *kappa_bte, int *cg_iters, double *stage_ms);
+
+/* Wall time (ms, HIP events on the context stream) of the kernels of the last call. */
+int alt3_last_kernel_ms(const alt3_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

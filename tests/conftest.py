@@ -1,0 +1,46 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (os.path.join(REPO, "anharmonic-lattice-dynamics_amd"), os.path.join(REPO, "oracle"),
+          os.path.join(REPO, "tests", "golden")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libalt3.so")
+
+
+def load_fixture(name: str) -> dict:
+    with np.load(os.path.join(GOLDEN, f"ref_{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def ref4():
+    return load_fixture("si4")
+
+
+@pytest.fixture(scope="session")
+def ref8():
+    return load_fixture("si8")
+
+
+@pytest.fixture(scope="session")
+def si_inputs(tmp_path_factory):
+    """SW-Si reference-format inputs for 4^3, 8^3 and 16^3 meshes (same force constants)."""
+    from thermacond import models, refio
+    fc = models.sw_silicon()
+    out = {}
+    for n in (4, 8, 16):
+        d = str(tmp_path_factory.mktemp(f"si{n}"))
+        refio.write_inputs(d, fc, refio.RunParams(nc=(n, n, n)))
+        out[n] = refio.read_inputs(d)
+    out["fc"] = fc
+    return out
