@@ -8,6 +8,8 @@
 
 #include "../../include/alt3.h"
 
+struct alt3_ctx;
+
 namespace alt3 {
 
 // ---------------------------------------------------------------- errors
@@ -106,6 +108,7 @@ struct V3Tables {
     std::vector<double> sterm_rk;  // [nterm][3]
 };
 int build_v3_tables(const alt3_system *sys, V3Tables &v);
+int v3_prep_launch(alt3_ctx *ctx, int ksub);
 
 }  // namespace alt3
 
@@ -137,6 +140,7 @@ struct alt3_ctx {
     alt3::DBuf<double> d_dr;
     alt3::DBuf<int> d_slot_dr, d_slot_bc, d_slot_ptr, d_sterm_a;
     alt3::DBuf<double> d_sterm_psi, d_sterm_rk;
+    alt3::DBuf<double> d_eph;                      // [nk][ndr] complex e^{i q2.dR}
     alt3::DBuf<double> d_gp;                       // per q1: [ndr][nb][nb*nb] complex (l1,b,c)
     alt3::DBuf<float> d_gpf;                       // FP32 hi/lo copy for the mixed path
     alt3::DBuf<int> d_mapinv;

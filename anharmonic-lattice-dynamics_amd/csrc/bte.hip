@@ -64,20 +64,20 @@ __device__ double block_sum(double v, double *sh)
     return r;
 }
 
-// one workgroup per (s, l1): Q, QN, QU and the l3-reduced R1/R2
+// one workgroup per (s, l1, l2): the l3-reduced R1/R2 row and partial Q, QN, QU
 __global__ void __launch_bounds__(256) k_tau(alt3::MeshDims md, int nk, int nb, int ibz_lo,
                                              const int *__restrict__ mapinv, const double *__restrict__ dist,
                                              const double *__restrict__ P1sm, const double *__restrict__ P2sm,
-                                             double *__restrict__ Q, double *__restrict__ QN, double *__restrict__ QU,
-                                             double *__restrict__ R1, double *__restrict__ R2)
+                                             double *__restrict__ qpart, double *__restrict__ R1, double *__restrict__ R2)
 {
     __shared__ double sh[256];
-    const int s = blockIdx.x / nb, l1 = blockIdx.x % nb;
+    const int l2 = blockIdx.x % nb, sl = blockIdx.x / nb, s = sl / nb, l1 = sl % nb;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
     coords(md, k1, a0, a1, a2);
     const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
     const double d1 = dist[(size_t)k1 * nb + l1];
+    const size_t row = ((size_t)sl * nb + l2) * (size_t)nb * nk;   // (s,l1,l2,l3=0,j=0)
     double q = 0, qn = 0, qu = 0;
     for (int j = threadIdx.x; j < nk; j += blockDim.x) {
         int c0, c1, c2;
@@ -88,32 +88,48 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDims md, int nk, int nb, 
         int t0 = a0 - c0, t1 = a1 - c1, t2 = a2 - c2;  // q1-q2 unfolded: u = (c1-c2)/n
         bool in2 = (t0 >= -h0 && t0 < h0) && (t1 >= -h1 && t1 < h1) && (t2 >= -h2 && t2 < h2);
         int nm = idx(md, wrap(t0 + h0, md.n0), wrap(t1 + h1, md.n1), wrap(t2 + h2, md.n2));
-        for (int l2 = 0; l2 < nb; l2++) {
-            const double d2 = dist[(size_t)j * nb + l2];
-            double r1 = 0, r2 = 0;
-            for (int l3 = 0; l3 < nb; l3++) {
-                size_t x = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk + j;
-                double p1 = P1sm[x] * (d1 * d2 * (dist[(size_t)np * nb + l3] + 1));
-                double p2 = P2sm[x] * (d1 * (d2 + 1) * (dist[(size_t)nm * nb + l3] + 1));
-                q = q + (p1 + 0.5 * p2);
-                if (in1) qn += p1; else qu += p1;
-                if (in2) qn += 0.5 * p2; else qu += 0.5 * p2;
-                r1 += p1;
-                r2 += p2;
-            }
-            size_t o = (((size_t)s * nb + l1) * nb + l2) * (size_t)nk + j;
-            R1[o] = r1;
-            R2[o] = r2;
+        const double d2 = dist[(size_t)j * nb + l2];
+        const double f1 = d1 * d2, f2 = d1 * (d2 + 1);
+        double r1 = 0, r2 = 0;
+        for (int l3 = 0; l3 < nb; l3++) {
+            size_t x = row + (size_t)l3 * nk + j;
+            double p1 = P1sm[x] * (f1 * (dist[(size_t)np * nb + l3] + 1));
+            double p2 = P2sm[x] * (f2 * (dist[(size_t)nm * nb + l3] + 1));
+            q = q + (p1 + 0.5 * p2);
+            if (in1) qn += p1; else qu += p1;
+            if (in2) qn += 0.5 * p2; else qu += 0.5 * p2;
+            r1 += p1;
+            r2 += p2;
         }
+        size_t o = ((size_t)sl * nb + l2) * (size_t)nk + j;
+        R1[o] = r1;
+        R2[o] = r2;
     }
     q = block_sum(q, sh);
     qn = block_sum(qn, sh);
     qu = block_sum(qu, sh);
     if (threadIdx.x == 0) {
-        Q[(size_t)s * nb + l1] = q;
-        QN[(size_t)s * nb + l1] = qn;
-        QU[(size_t)s * nb + l1] = qu;
+        qpart[3 * (size_t)blockIdx.x] = q;
+        qpart[3 * (size_t)blockIdx.x + 1] = qn;
+        qpart[3 * (size_t)blockIdx.x + 2] = qu;
     }
+}
+
+__global__ void k_tau_sum(int n, int nb, const double *__restrict__ qpart, double *__restrict__ Q,
+                          double *__restrict__ QN, double *__restrict__ QU)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;   // (s, l1)
+    if (x >= n) return;
+    double q = 0, qn = 0, qu = 0;
+    for (int l2 = 0; l2 < nb; l2++) {
+        const double *p = qpart + 3 * ((size_t)x * nb + l2);
+        q += p[0];
+        qn += p[1];
+        qu += p[2];
+    }
+    Q[x] = q;
+    QN[x] = qn;
+    QU[x] = qu;
 }
 
 // tauinv = Q/(n(n+1)) c 100 ; outputs Fortran (ksub,nb)
@@ -203,43 +219,71 @@ __global__ void k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart, 
     kappa[ii + (size_t)nb * e] = acc;
 }
 
-// collision matrix, one thread per (row block (s,l1), column block (n2,l2)); CM column-major
-__global__ void k_collision(alt3::MeshDims md, int nk, int nb, int ibz_lo, int ksub, int nibz,
-                            const int *__restrict__ mapinv, const double *__restrict__ Q,
-                            const double *__restrict__ R1, const double *__restrict__ R2,
-                            const int *__restrict__ star_n, const int *__restrict__ star_op,
-                            const int *__restrict__ star_n0, const double *__restrict__ opk, double *__restrict__ CM)
+// collision matrix.  One workgroup per row block (s,l1); threads over column blocks
+// (n2,l2), whose star gathers stay inside the (s,l1) slab of R1/R2.  The device matrix is
+// row-major (CMr[row*ncol + col]) so the three rows of a block are written contiguously.
+__global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, int nb, int ibz_lo, int nibz,
+                                                   const int *__restrict__ mapinv, const double *__restrict__ Q,
+                                                   const double *__restrict__ R1, const double *__restrict__ R2,
+                                                   const int *__restrict__ star_n, const int *__restrict__ star_op,
+                                                   const int *__restrict__ star_n0, const double *__restrict__ opk,
+                                                   double *__restrict__ CMr)
 {
-    int rb = blockIdx.x * blockDim.x + threadIdx.x;  // (s, l1)
-    int cb = blockIdx.y;                              // (n2, l2)
-    if (rb >= ksub * nb) return;
-    int s = rb / nb, l1 = rb % nb, n2 = cb / nb, l2 = cb % nb;
-    int k1 = mapinv[ibz_lo + s];
+    const int rb = blockIdx.x;  // (s, l1)
+    const int s = rb / nb, l1 = rb % nb;
+    const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
     coords(md, k1, a0, a1, a2);
     const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
-    const size_t nrow = (size_t)ksub * nb * 3;
-    const size_t rbase = (((size_t)s * nb + l1) * nb + l2) * (size_t)nk;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int ns = 0; ns < star_n[n2]; ns++) {
-        int n0 = star_n0[48 * n2 + ns];
-        double C;
-        if (n0 == k1 && l1 == l2) {
-            C = -Q[(size_t)s * nb + l1];
-        } else {
-            int c0, c1, c2;
-            coords(md, n0, c0, c1, c2);
-            int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
-            int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
-            C = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+    const size_t ncol = (size_t)nibz * nb * 3;
+    const double qd = Q[(size_t)s * nb + l1];
+    for (int cb = threadIdx.x; cb < nibz * nb; cb += blockDim.x) {
+        const int n2 = cb / nb, l2 = cb % nb;
+        const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ns = 0; ns < star_n[n2]; ns++) {
+            int n0 = star_n0[48 * n2 + ns];
+            double C;
+            if (n0 == k1 && l1 == l2) {
+                C = -qd;
+            } else {
+                int c0, c1, c2;
+                coords(md, n0, c0, c1, c2);
+                int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
+                int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
+                C = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+            }
+            const double *op = opk + 9 * star_op[48 * n2 + ns];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (C * op[3 * j + i]);
         }
-        const double *op = opk + 9 * star_op[48 * n2 + ns];
+        const size_t d1 = 3 * (size_t)rb, d2 = 3 * (size_t)cb;
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (C * op[3 * j + i]);
+            for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
     }
-    const size_t d1 = 3 * (size_t)rb, d2 = 3 * (size_t)cb;
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) CM[(d1 + i) + nrow * (d2 + j)] = acc[i + 3 * j];
+}
+
+// row-major -> Fortran column-major copy (nrow x ncol)
+__global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restrict__ in, double *__restrict__ out)
+{
+    size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nrow * ncol) return;
+    size_t r = o % nrow, c = o / nrow;
+    out[o] = in[r * ncol + c];
+}
+
+// y = A x for a row-major n x n matrix: one wavefront per row, columns in order per lane
+__global__ void __launch_bounds__(256) k_matvec_rows(int n, const double *__restrict__ A, const double *__restrict__ x,
+                                                     double *__restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double *a = A + (size_t)row * n;
+    double acc = 0;
+    for (int j = lane; j < n; j += 64) acc += a[j] * x[j];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) y[row] = acc;
 }
 
 __global__ void k_rhs(int nibz, int nb, double tempk, const int *__restrict__ mapinv, const double *__restrict__ freq,
@@ -443,9 +487,13 @@ int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, doub
     ALT3_TRY(ctx->d_qu.alloc((size_t)ksub * nb));
     ALT3_TRY(ctx->d_r1.alloc((size_t)ksub * nb * nb * nk));
     ALT3_TRY(ctx->d_r2.alloc((size_t)ksub * nb * nb * nk));
-    hipLaunchKernelGGL(k_tau, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
-                       ctx->d_dist.p, ctx->d_p1.p, ctx->d_p2.p, ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p, ctx->d_r1.p,
-                       ctx->d_r2.p);
+    DBuf<double> qpart;
+    ALT3_TRY(qpart.alloc((size_t)3 * ksub * nb * nb));
+    hipLaunchKernelGGL(k_tau, dim3(ksub * nb * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
+                       ctx->d_dist.p, ctx->d_p1.p, ctx->d_p2.p, qpart.p, ctx->d_r1.p, ctx->d_r2.p);
+    ALT3_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, nb, qpart.p,
+                       ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -518,25 +566,39 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
     const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
-    dim3 grid((ksub * nb + 63) / 64, nibz * nb);
-    hipLaunchKernelGGL(k_collision, grid, dim3(64), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ksub, nibz, ctx->d_mapinv.p,
-                       ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p, ctx->d_star_n0.p,
-                       ctx->d_opk.p, ctx->d_cm.p);
+    hipLaunchKernelGGL(k_collision, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
+                       ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
+                       ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
                        ctx->d_freq.p, ctx->d_dist.p, ctx->d_vel.p, ctx->d_rhs.p);
     ALT3_HIP(hipGetLastError());
-    if (CM) ALT3_HIP(hipMemcpyAsync(CM, ctx->d_cm.p, sizeof(double) * nrow * ncol, hipMemcpyDeviceToHost, ctx->stream));
+    if (CM) {  // Fortran Collision_Matrixmpi(nrow, ncol) is column-major
+        DBuf<double> t;
+        ALT3_TRY(t.alloc(nrow * ncol));
+        hipLaunchKernelGGL(k_rows_to_cols, dim3((unsigned)((nrow * ncol + 255) / 256)), dim3(256), 0, ctx->stream, nrow,
+                           ncol, ctx->d_cm.p, t.p);
+        ALT3_HIP(hipGetLastError());
+        ALT3_HIP(hipMemcpyAsync(CM, t.p, sizeof(double) * nrow * ncol, hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    }
     if (RHS) ALT3_HIP(hipMemcpyAsync(RHS, ctx->d_rhs.p, sizeof(double) * nibz * nb * 3, hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_cm = true;
     return ALT3_OK;
 }
 
-// CG on a device matrix (dA, db); x is a device vector, overwritten
+// CG on a device matrix (dA, db); x is a device vector, overwritten.  rowmajor selects the
+// layout of dA (the collision matrix is kept row-major on the device).
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
-              double *funcmin, int *iters)
+              double *funcmin, int *iters, bool rowmajor)
 {
+    auto matvec = [&](const double *xin, double *yout) {
+        if (rowmajor)
+            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, dA, xin, yout);
+        else
+            hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout);
+    };
     ALT3_TRY(ctx->d_cg.alloc((size_t)4 * n));
     double *grad = ctx->d_cg.p, *conj = grad + n, *aux = conj + n, *v0 = aux + n;
     const int nblk = 256;
@@ -544,14 +606,14 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
     double *part = ctx->d_red.p, *sc = part + 2 * nblk;
     const int gb = (n + 255) / 256;
     const double tol = 1.0 / std::pow(10.0, 2 * ndig);
-    hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, dx, aux);
+    matvec(dx, aux);
     hipLaunchKernelGGL(k_axpy_init, dim3(gb), dim3(256), 0, ctx->stream, n, aux, db, grad, conj);
     ALT3_HIP(hipGetLastError());
     int iter = 0;
     double hs[8];
     while (iter < max_iter) {
         iter++;
-        hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, conj, aux);
+        matvec(conj, aux);
         hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, grad, conj, aux, conj, part);
         hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);  // num, den
         hipLaunchKernelGGL(k_cg_step, dim3(gb), dim3(256), 0, ctx->stream, n, sc, dx, grad, v0, conj, aux);
@@ -569,7 +631,7 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
         ALT3_HIP(hipGetLastError());
     }
     if (funcmin) {
-        hipLaunchKernelGGL(k_matvec, dim3(gb), dim3(256), 0, ctx->stream, n, dA, dx, v0);
+        matvec(dx, v0);
         hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, dx, v0, dx, db, part);
         hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);
         ALT3_HIP(hipMemcpyAsync(hs, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

