@@ -263,6 +263,59 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, in
     }
 }
 
+// Same matrix with the (s,l1,l2) rows of R1/R2 staged in LDS (2*nk doubles): one
+// workgroup per row block, branches l2 in turn, threads over the irreducible q2.  Every
+// star gather then hits LDS instead of re-fetching scattered HBM lines.
+__global__ void __launch_bounds__(256) k_collision_lds(alt3::MeshDims md, int nk, int nb, int ibz_lo, int nibz,
+                                                       const int *__restrict__ mapinv, const double *__restrict__ Q,
+                                                       const double *__restrict__ R1, const double *__restrict__ R2,
+                                                       const int *__restrict__ star_n, const int *__restrict__ star_op,
+                                                       const int *__restrict__ star_n0, const double *__restrict__ opk,
+                                                       double *__restrict__ CMr)
+{
+    extern __shared__ double lds[];
+    double *r1 = lds, *r2 = lds + nk;
+    const int rb = blockIdx.x;  // (s, l1)
+    const int s = rb / nb, l1 = rb % nb;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    coords(md, k1, a0, a1, a2);
+    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    const size_t ncol = (size_t)nibz * nb * 3;
+    const double qd = Q[(size_t)s * nb + l1];
+    for (int l2 = 0; l2 < nb; l2++) {
+        const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
+        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
+            r1[j] = R1[rbase + j];
+            r2[j] = R2[rbase + j];
+        }
+        __syncthreads();
+        for (int n2 = threadIdx.x; n2 < nibz; n2 += blockDim.x) {
+            double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int ns = 0; ns < star_n[n2]; ns++) {
+                int n0 = star_n0[48 * n2 + ns];
+                double C;
+                if (n0 == k1 && l1 == l2) {
+                    C = -qd;
+                } else {
+                    int c0, c1, c2;
+                    coords(md, n0, c0, c1, c2);
+                    int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
+                    int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
+                    C = -r1[n0] + r1[p2] + 0.5 * r2[n0] + 0.5 * r2[p1];
+                }
+                const double *op = opk + 9 * star_op[48 * n2 + ns];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (C * op[3 * j + i]);
+            }
+            const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+        }
+        __syncthreads();
+    }
+}
+
 // row-major -> Fortran column-major copy (nrow x ncol)
 __global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restrict__ in, double *__restrict__ out)
 {
@@ -566,9 +619,15 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
     const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
-    hipLaunchKernelGGL(k_collision, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
-                       ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
-                       ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+    const size_t lds = (size_t)2 * nk * sizeof(double);
+    if (lds <= 65536)
+        hipLaunchKernelGGL(k_collision_lds, dim3(ksub * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
+                           ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
+                           ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+    else
+        hipLaunchKernelGGL(k_collision, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
+                           ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
+                           ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
                        ctx->d_freq.p, ctx->d_dist.p, ctx->d_vel.p, ctx->d_rhs.p);

@@ -128,18 +128,20 @@ __constant__ int kTc[6][4] = {{0, 1, 2, 5}, {1, 2, 3, 5}, {0, 2, 4, 5}, {2, 3, 5
 
 // one workgroup per (s, l1, l2, l3, process); LDS holds the float corner energy and the
 // double weight sum of every mesh point
+constexpr int kFgrThreads = 1024;  // 16 waves: LDS (12 B per mesh point) limits residency to 2 groups/CU
+
 template <typename VT>
-__global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, int ibz_lo,
-                                             const int *__restrict__ mapinv, const double *__restrict__ freq,
-                                             const VT *__restrict__ v33, double *__restrict__ P1,
-                                             double *__restrict__ P2)
+__global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDims md, int nk, int nb, int ibz_lo,
+                                                     const int *__restrict__ mapinv, const double *__restrict__ freq,
+                                                     const VT *__restrict__ v33, double *__restrict__ P1,
+                                                     double *__restrict__ P2)
 {
     extern __shared__ double lds[];
     double *W = lds;                                   // [nk]
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
-    __shared__ float s_lo[256], s_hi[256];
+    __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
     __shared__ int s_cnt;
-    __shared__ int s_list[6 * 256];
+    __shared__ unsigned short s_list[6 * kFgrThreads];  // (cube << 3 | tetrahedron), nk <= 8192
     const int proc = blockIdx.x & 1;
     int r = blockIdx.x >> 1;
     const int l3 = r % nb; r /= nb;
@@ -169,17 +171,20 @@ __global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, 
         lo = fminf(lo, ef);
         hi = fmaxf(hi, ef);
     }
-    s_lo[threadIdx.x] = lo;
-    s_hi[threadIdx.x] = hi;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            s_lo[threadIdx.x] = fminf(s_lo[threadIdx.x], s_lo[threadIdx.x + st]);
-            s_hi[threadIdx.x] = fmaxf(s_hi[threadIdx.x], s_hi[threadIdx.x + st]);
-        }
-        __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
     }
-    const bool any = !(omega1 < (double)s_lo[0] || omega1 > (double)s_hi[0]);
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    for (int w = 0; w < kFgrThreads / 64; w++) {
+        lo = fminf(lo, s_lo[w]);
+        hi = fmaxf(hi, s_hi[w]);
+    }
+    const bool any = !(omega1 < (double)lo || omega1 > (double)hi);
     if (any) {
         // cube k has corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2} (pt1..pt8 of
         // make_kp_reg_tet) and tetrahedra (1,2,3,6) (2,3,4,6) (1,3,5,6) (3,4,6,8) (3,5,6,7)
@@ -187,7 +192,7 @@ __global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, 
         // they are few; they are compacted into an LDS list per chunk of cubes so that the
         // weight formula runs with every lane busy instead of once per divergent lane.
         const int tid = threadIdx.x;
-        for (int base = 0; base < nk; base += 256) {
+        for (int base = 0; base < nk; base += kFgrThreads) {
             if (tid == 0) s_cnt = 0;
             __syncthreads();
             const int k = base + tid;
@@ -209,14 +214,15 @@ __global__ void __launch_bounds__(256) k_fgr(alt3::MeshDims md, int nk, int nb, 
                     for (int l = 0; l < 6; l++) {
                         float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
                         float tmn = fminf(fminf(a, b), fminf(c, d)), tmx = fmaxf(fmaxf(a, b), fmaxf(c, d));
-                        if (!(omega1 < (double)tmn || omega1 > (double)tmx)) s_list[atomicAdd(&s_cnt, 1)] = (k << 3) | l;
+                        if (!(omega1 < (double)tmn || omega1 > (double)tmx))
+                            s_list[atomicAdd(&s_cnt, 1)] = (unsigned short)((k << 3) | l);
                     }
                 }
             }
             __syncthreads();
             const int cnt = s_cnt;
             for (int x = tid; x < cnt; x += blockDim.x) {
-                const int code = s_list[x], kk = code >> 3, l = code & 7;
+                const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
                 int c0, c1, c2;
                 coords(md, kk, c0, c1, c2);
                 int v[4];
@@ -273,17 +279,18 @@ int fgr_run(alt3_ctx *ctx)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    if (lds > 65536) return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 5461");
+    if (lds + 6 * kFgrThreads * sizeof(unsigned short) + 256 > 65536 || nk > 8192)
+        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4400");
     ALT3_TRY(ctx->d_p1.alloc(nel));
     ALT3_TRY(ctx->d_p2.alloc(nel));
     MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb * nb));
     if (ctx->v3_prec == ALT3_V3_MIXED)
-        hipLaunchKernelGGL(k_fgr<float>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
-                           ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
+        hipLaunchKernelGGL(k_fgr<float>, grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
     else
-        hipLaunchKernelGGL(k_fgr<double>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
-                           ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
+        hipLaunchKernelGGL(k_fgr<double>, grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;

@@ -8,10 +8,12 @@ conjugate gradients at T = 100, 300, 800 K.  One step = the whole path on the de
 eigen solve on the mesh, V3 over the irreducible zone, tetrahedron FGR, then per
 temperature tau, kappa_RTA, collision matrix, CG and kappa_BTE.
 
-value = |V3|^2 elements per second of the whole step (higher is better).  The JSON line
-also carries the V3-kernel-only rate, stage times, p50/p99 kappa-solve latency, the
-roofline of the dominant kernel and the CPU baseline (the reference Fortran V3 routine,
-built from source, on one host core, over one irreducible q1).
+value = |V3|^2 elements per second of the whole step with the mixed-precision V3 (FP32
+MFMA phase GEMM); the same step with FP64 V3 is timed beside it ("fp64"), together with
+the kappa difference between the two ("mixed_vs_fp64").  The line also carries the
+V3-kernel-only rates, stage times, p50/p99 kappa-solve latency, the roofline of the
+dominant kernel (HIP-event time of k_fgr over the timed steps) and the CPU baseline (the
+reference Fortran V3 routine, built from source, on one host core, over one irreducible q1).
 """
 from __future__ import annotations
 
@@ -28,8 +30,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "anharmonic-lattice-dynamics_amd"))
 
 PEAK_HBM_GBS = 8000.0
-PEAK_FP64_TFLOPS = 78.6
-PEAK_FP32_TFLOPS = 157.3
+STAGES = ["eig", "v3", "fgr", "tau", "kappa_rta", "collision", "cg", "kappa_bte"]
 
 
 def parse():
@@ -38,11 +39,25 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mesh", type=int, default=16)
-    ap.add_argument("--precision", choices=["mixed", "fp64", "fp64ref"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64", "fp64ref"], default="both")
     ap.add_argument("--temps", default="100,300,800")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (reference for the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    import torch
+    n = mib * 1024 * 1024 // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
 
 
 def log(msg, quiet=False):
@@ -57,7 +72,7 @@ def cpu_baseline_reference(inputs_dir: str, quiet: bool):
     if not os.path.exists(probe):
         return None
     env = dict(os.environ, REF_PROBE_V3_SAMPLE="1 1", OMP_NUM_THREADS="1")
-    log("cpu baseline: reference V3 on one IBZ q1 (about 45 s)", quiet)
+    log("cpu baseline: reference V3 on one IBZ q1 (20-45 s)", quiet)
     r = subprocess.run([probe], cwd=inputs_dir, env=env, capture_output=True, text=True, timeout=600)
     for line in r.stdout.splitlines():
         m = re.match(r"\s*REF_PROBE_V3 elements=\s*(\d+)\s+seconds=\s*([0-9.EeDd+-]+)", line)
@@ -69,69 +84,92 @@ def cpu_baseline_reference(inputs_dir: str, quiet: bool):
     return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding over q1 is "
-                                   "described in DESIGN.md but not launched", "n_gpus": args.gpus}))
-        sys.exit(2)
-    from thermacond import capi, models, refio
-
-    prec = {"mixed": capi.V3_MIXED, "fp64": capi.V3_FP64, "fp64ref": capi.V3_FP64_REF}[args.precision]
-    temps = [float(t) for t in args.temps.split(",")]
-    n = args.mesh
-    tmp = tempfile.mkdtemp(prefix="alt3_bench_")
-    fc = models.sw_silicon()
-    refio.write_inputs(tmp, fc, refio.RunParams(nc=(n, n, n)))
-    inp = refio.read_inputs(tmp)
-    log(f"bench: SW-Si {n}^3, precision={args.precision}, temps={temps}", args.quiet)
-    ctx = capi.Context(inp)
-    nel = ctx.nibz * ctx.nk * ctx.nb ** 3
-    lat = []
-    stage = None
-    for it in range(args.warmup + args.steps):
+def timed(ctx, prec, temps, warmup, steps, label, quiet):
+    lat, stage, out = [], None, None
+    for it in range(warmup + steps):
         t0 = time.perf_counter()
         ctx.eig(want=False)
         out = ctx.solve(temps, prec)
         dt = time.perf_counter() - t0
-        if it >= args.warmup:
+        if it >= warmup:
             lat.append(dt)
-            stage = out["stage_ms"] if stage is None else stage + out["stage_ms"]
-        log(f"step {it}: {dt * 1e3:.1f} ms  stages(ms)={[round(x, 2) for x in out['stage_ms']]} "
-            f"kappa_RTA(300K)={out['kappa_rta'][min(1, len(temps) - 1)].sum(0).trace() / 3:.3f}", args.quiet)
-    K = args.steps
-    stage = stage / K
-    ms_step = 1e3 * sum(lat) / K
-    lat_sorted = sorted(lat)
-    p50 = lat_sorted[len(lat) // 2] * 1e3
-    p99 = lat_sorted[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))] * 1e3
-    # dominant kernel: the tetrahedron FGR kernel (one launch over the whole IBZ)
-    fgr_ms = stage[2]
-    fgr_bytes = nel * (8 + 16)   # read |V3|^2 (FP64), write P1sm + P2sm (FP64)
+            stage = out["stage_ms"].copy() if stage is None else stage + out["stage_ms"]
+        log(f"{label} step {it}: {dt * 1e3:.2f} ms  stages(ms)={[round(float(x), 3) for x in out['stage_ms']]}", quiet)
+    stage = stage / steps
+    lat_s = sorted(lat)
+    return dict(ms=1e3 * sum(lat) / steps, p50=1e3 * lat_s[len(lat) // 2],
+                p99=1e3 * lat_s[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], stage=stage, out=out)
+
+
+def kappa_iso(k):
+    return float(k.sum(0).trace() / 3)
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; sharding over q1 across GPUs is "
+                                   "described in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from thermacond import capi, models, refio
+
+    temps = [float(t) for t in args.temps.split(",")]
+    n = args.mesh
+    tmp = tempfile.mkdtemp(prefix="alt3_bench_")
+    refio.write_inputs(tmp, models.sw_silicon(), refio.RunParams(nc=(n, n, n)))
+    inp = refio.read_inputs(tmp)
+    log(f"bench: SW-Si {n}^3, precision={args.precision}, temps={temps}", args.quiet)
+    ctx = capi.Context(inp)
+    nel = ctx.nibz * ctx.nk * ctx.nb ** 3
+    sizes = {"nibz": ctx.nibz, "nk": ctx.nk, "nb": ctx.nb}
+    precs = {"both": [("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)], "mixed": [("mixed", capi.V3_MIXED)],
+             "fp64": [("fp64", capi.V3_FP64)], "fp64ref": [("fp64ref", capi.V3_FP64_REF)]}[args.precision]
+    runs = {name: timed(ctx, p, temps, args.warmup, args.steps, name, args.quiet) for name, p in precs}
+    ctx.close()
+    head_name = precs[0][0]
+    head = runs[head_name]
+    ti = min(1, len(temps) - 1)
+
+    def summary(r):
+        return {"value": nel / (r["ms"] * 1e-3), "ms_per_step": r["ms"],
+                "v3_kernel_elements_per_s": nel / (r["stage"][1] * 1e-3),
+                "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, r["stage"])},
+                "latency_ms": {"p50": r["p50"], "p99": r["p99"]},
+                "kappa_300K": {"rta": kappa_iso(r["out"]["kappa_rta"][ti]), "bte": kappa_iso(r["out"]["kappa_bte"][ti])}}
+
+    # dominant kernel of the step: the tetrahedron FGR kernel (one launch per step)
+    v3_bytes = 4 if head_name == "mixed" else 8
+    fgr_ms = float(head["stage"][2])
+    fgr_bytes = nel * (v3_bytes + 16)  # read |V3|^2, write P1sm + P2sm (FP64)
     fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
-    v3_ms = stage[1]
     res = {
         "metric": "3-ph V3 elements/s (mixed vs FP64) at kappa err <=0.5%; p50/p99 kappa-solve latency",
-        "value": nel / (ms_step * 1e-3),
+        "value": nel / (head["ms"] * 1e-3),
         "unit": "V3 elements/s",
-        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": ms_step,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"mixed": "fp32-mfma V3 / f64", "fp64": "f64", "fp64ref": "f64"}[args.precision],
+        "dtype": {"mixed": "fp32 (V3 phase GEMM on MFMA) + f64", "fp64": "f64", "fp64ref": "f64"}[head_name],
         "data": "synthetic (Stillinger-Weber Si force constants by finite differences)",
-        "config": {"workload": f"SW-Si 2-atom diamond, {n}^3 mesh, tetrahedron + CG BTE at {temps} K",
-                   "nibz": ctx.nibz, "nk": ctx.nk, "nb": ctx.nb, "v3_elements": nel,
-                   "precision": args.precision},
-        "v3_kernel_elements_per_s": nel / (v3_ms * 1e-3),
-        "stage_ms": {k: round(v, 3) for k, v in zip(
-            ["eig", "v3", "fgr", "tau", "kappa_rta", "collision", "cg", "kappa_bte"], stage)},
-        "latency_ms": {"p50": p50, "p99": p99},
+        "config": {"workload": f"SW-Si 2-atom diamond, {n}^3 mesh, tetrahedron FGR + CG BTE at {temps} K",
+                   **sizes, "v3_elements": nel, "precision": head_name},
+        "latency_ms": {"p50": head["p50"], "p99": head["p99"]},
         "roofline": {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": fgr_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_fgr",
                      "algorithmic_bytes_per_launch": fgr_bytes, "kernel_ms": fgr_ms},
-        "kappa_300K": {"rta": float(out["kappa_rta"][min(1, len(temps) - 1)].sum(0).trace() / 3),
-                       "bte": float(out["kappa_bte"][min(1, len(temps) - 1)].sum(0).trace() / 3)},
     }
-    ctx.close()
+    for name, r in runs.items():
+        res[name] = summary(r)
+    if "mixed" in runs and "fp64" in runs:
+        m, f = runs["mixed"], runs["fp64"]
+        errs = {}
+        for k in ("kappa_rta", "kappa_bte"):
+            errs[k] = max(abs(kappa_iso(m["out"][k][i]) / kappa_iso(f["out"][k][i]) - 1) for i in range(len(temps)))
+        res["mixed_vs_fp64"] = {"whole_step_speedup": f["ms"] / m["ms"],
+                                "v3_kernel_speedup": float(f["stage"][1] / m["stage"][1]),
+                                "kappa_rel_err_max": errs}
     if not args.no_cpu_baseline:
         cb = cpu_baseline_reference(tmp, args.quiet)
         if cb is not None:

@@ -176,6 +176,25 @@ def test_solve_own_eigenvectors_vs_reference(si_inputs, ref4, ref8):
                 assert abs(kb / rb - 1) < 2e-2, (n, T, kb, rb)
 
 
+def test_mixed_v3_vs_fp64(si_inputs):
+    """ALT3_V3_MIXED (FP32 MFMA phase GEMM) against the FP64 path on the same eigenvectors:
+    |V3|^2 within 1e-5 of the largest element, kappa within 0.5 %."""
+    print("case: mixed vs fp64 8^3", flush=True)
+    with capi.Context(si_inputs[8]) as c:
+        c.eig(want=False)
+        v64 = c.v3sq(0, c.nibz - 1, capi.V3_FP64)
+        v32 = c.v3sq(0, c.nibz - 1, capi.V3_MIXED)
+        check(v32, v64, 1e-5, "mixed |V3|^2")
+        temps = [100.0, 300.0, 800.0]
+        o64 = c.solve(temps, capi.V3_FP64)
+        o32 = c.solve(temps, capi.V3_MIXED)
+        for i in range(3):
+            for k in ("kappa_rta", "kappa_bte"):
+                a = np.trace(o32[k][i].sum(0)) / 3
+                b = np.trace(o64[k][i].sum(0)) / 3
+                assert abs(a / b - 1) < 5e-3, (k, temps[i], a, b)
+
+
 def test_solve_matches_stagewise(si_inputs, ref8):
     """alt3_solve equals the stage-by-stage entry points on the same device state."""
     with capi.Context(si_inputs[8]) as c:
