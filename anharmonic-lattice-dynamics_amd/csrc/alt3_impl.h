@@ -153,14 +153,24 @@ struct alt3_ctx {
     alt3::DBuf<double> d_p1, d_p2;                 // [ksub][nb][nb][nb][nk]
     bool have_fgr = false;
 
-    // per temperature
+    // per temperature: one tau pass fills slot t of each array for every temperature it
+    // was given (tau_temps); the consumers read the slot of the temperature they are asked for
     double tempk = -1;
-    alt3::DBuf<double> d_dist;                     // [nk][nb]
-    alt3::DBuf<double> d_q, d_qn, d_qu;            // [ksub][nb]
-    alt3::DBuf<double> d_r1, d_r2;                 // [ksub][nb][nb][nk]  sum_l3 P1, P2
-    alt3::DBuf<double> d_qall;                     // [nibz][nb] (whole IBZ, for F_RTA)
+    std::vector<double> tau_temps;
+    int tau_cur = 0;
+    alt3::DBuf<double> d_dist;                     // [t][nk][nb]
+    alt3::DBuf<double> d_q, d_qn, d_qu;            // [t][ksub][nb]
+    alt3::DBuf<double> d_r1, d_r2;                 // [t][ksub][nb][nb][nk]  sum_l3 P1, P2
+    size_t ksub_nb() const { return (size_t)(ibz_hi - ibz_lo + 1) * nb; }
+    double *dist() const { return d_dist.p + (size_t)tau_cur * nk * nb; }
+    double *q() const { return d_q.p + (size_t)tau_cur * ksub_nb(); }
+    double *qn() const { return d_qn.p + (size_t)tau_cur * ksub_nb(); }
+    double *qu() const { return d_qu.p + (size_t)tau_cur * ksub_nb(); }
+    double *r1() const { return d_r1.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
+    double *r2() const { return d_r2.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
     bool have_tau = false;
     alt3::DBuf<int> d_star_n, d_star_op, d_star_n0, d_fstar_n, d_fstar_op, d_fstar_n0;
+    alt3::DBuf<int> d_cm_slot_n2, d_cm_slot_cnt, d_cm_tab;   // collision star table, see k_collision_c
     alt3::DBuf<double> d_opk, d_opk_inv, d_kibz;
     alt3::DBuf<double> d_frta, d_kpart, d_kappa, d_taueff, d_veloci;
     alt3::DBuf<double> d_cm, d_rhs;

@@ -9,22 +9,12 @@
 // Layouts on the device: P1sm/P2sm [s][l1][l2][l3][q2]; the tau kernel also writes
 // R1/R2[s][l1][l2][q2] = sum_l3 P1, P2 (occupations applied), which is all the collision
 // matrix reads: C = -R1(n0) + R1(q2-q1) + R2(n0)/2 + R2(q1-q2)/2 off the diagonal and -Q on it.
+#include <algorithm>
+
 #include "alt3_impl.h"
+#include "mesh_dev.h"
 
 namespace {
-
-__device__ __forceinline__ void coords(const alt3::MeshDims &d, int k, int &c0, int &c1, int &c2)
-{
-    c2 = k % d.n2;
-    c1 = (k / d.n2) % d.n1;
-    c0 = k / (d.n1 * d.n2);
-}
-__device__ __forceinline__ int idx(const alt3::MeshDims &d, int c0, int c1, int c2) { return c2 + d.n2 * (c1 + d.n1 * c0); }
-__device__ __forceinline__ int wrap(int x, int n)
-{
-    x %= n;
-    return x < 0 ? x + n : x;
-}
 
 __device__ __forceinline__ double nbe(double w, double t, int m)
 {
@@ -64,54 +54,72 @@ __device__ double block_sum(double v, double *sh)
     return r;
 }
 
-// one workgroup per (s, l1, l2): the l3-reduced R1/R2 row and partial Q, QN, QU
-__global__ void __launch_bounds__(256) k_tau(alt3::MeshDims md, int nk, int nb, int ibz_lo,
+// one workgroup per (s, l1, l2): the l3-reduced R1/R2 rows and partial Q, QN, QU of NT
+// temperatures at once, so that P1sm/P2sm are read once for all of them
+template <int NT>
+__global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                              const int *__restrict__ mapinv, const double *__restrict__ dist,
-                                             const double *__restrict__ P1sm, const double *__restrict__ P2sm,
-                                             double *__restrict__ qpart, double *__restrict__ R1, double *__restrict__ R2)
+                                             size_t dstr, const double *__restrict__ P1sm,
+                                             const double *__restrict__ P2sm, double *__restrict__ qpart, size_t qstr,
+                                             double *__restrict__ R1, double *__restrict__ R2, size_t rstr)
 {
     __shared__ double sh[256];
     const int l2 = blockIdx.x % nb, sl = blockIdx.x / nb, s = sl / nb, l1 = sl % nb;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
-    coords(md, k1, a0, a1, a2);
-    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
-    const double d1 = dist[(size_t)k1 * nb + l1];
+    m.coords(k1, a0, a1, a2);
+    double d1[NT], q[NT], qn[NT], qu[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        d1[t] = dist[t * dstr + (size_t)k1 * nb + l1];
+        q[t] = qn[t] = qu[t] = 0;
+    }
     const size_t row = ((size_t)sl * nb + l2) * (size_t)nb * nk;   // (s,l1,l2,l3=0,j=0)
-    double q = 0, qn = 0, qu = 0;
     for (int j = threadIdx.x; j < nk; j += blockDim.x) {
         int c0, c1, c2;
-        coords(md, j, c0, c1, c2);
-        int s0 = a0 + c0, s1 = a1 + c1, s2 = a2 + c2;  // q1+q2 unfolded: u = (c1+c2)/n - 1
-        bool in1 = (s0 >= h0 && s0 < 3 * h0) && (s1 >= h1 && s1 < 3 * h1) && (s2 >= h2 && s2 < 3 * h2);
-        int np = idx(md, wrap(s0 - h0, md.n0), wrap(s1 - h1, md.n1), wrap(s2 - h2, md.n2));
-        int t0 = a0 - c0, t1 = a1 - c1, t2 = a2 - c2;  // q1-q2 unfolded: u = (c1-c2)/n
-        bool in2 = (t0 >= -h0 && t0 < h0) && (t1 >= -h1 && t1 < h1) && (t2 >= -h2 && t2 < h2);
-        int nm = idx(md, wrap(t0 + h0, md.n0), wrap(t1 + h1, md.n1), wrap(t2 + h2, md.n2));
-        const double d2 = dist[(size_t)j * nb + l2];
-        const double f1 = d1 * d2, f2 = d1 * (d2 + 1);
-        double r1 = 0, r2 = 0;
-        for (int l3 = 0; l3 < nb; l3++) {
-            size_t x = row + (size_t)l3 * nk + j;
-            double p1 = P1sm[x] * (f1 * (dist[(size_t)np * nb + l3] + 1));
-            double p2 = P2sm[x] * (f2 * (dist[(size_t)nm * nb + l3] + 1));
-            q = q + (p1 + 0.5 * p2);
-            if (in1) qn += p1; else qu += p1;
-            if (in2) qn += 0.5 * p2; else qu += 0.5 * p2;
-            r1 += p1;
-            r2 += p2;
+        m.coords(j, c0, c1, c2);
+        const bool in1 = m.sum_inside(a0, a1, a2, c0, c1, c2);    // q1+q2 inside the first zone
+        const int np = m.sum(a0, a1, a2, c0, c1, c2);
+        const bool in2 = m.diff_inside(a0, a1, a2, c0, c1, c2);   // q1-q2
+        const int nm = m.diff(a0, a1, a2, c0, c1, c2);
+        double f1[NT], f2[NT], r1[NT], r2[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const double d2 = dist[t * dstr + (size_t)j * nb + l2];
+            f1[t] = d1[t] * d2;
+            f2[t] = d1[t] * (d2 + 1);
+            r1[t] = r2[t] = 0;
         }
-        size_t o = ((size_t)sl * nb + l2) * (size_t)nk + j;
-        R1[o] = r1;
-        R2[o] = r2;
+#pragma unroll 2
+        for (int l3 = 0; l3 < nb; l3++) {
+            const size_t x = row + (size_t)l3 * nk + j;
+            const double P1 = P1sm[x], P2 = P2sm[x];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const double p1 = P1 * (f1[t] * (dist[t * dstr + (size_t)np * nb + l3] + 1));
+                const double p2 = P2 * (f2[t] * (dist[t * dstr + (size_t)nm * nb + l3] + 1));
+                q[t] = q[t] + (p1 + 0.5 * p2);
+                if (in1) qn[t] += p1; else qu[t] += p1;
+                if (in2) qn[t] += 0.5 * p2; else qu[t] += 0.5 * p2;
+                r1[t] += p1;
+                r2[t] += p2;
+            }
+        }
+        const size_t o = ((size_t)sl * nb + l2) * (size_t)nk + j;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            R1[t * rstr + o] = r1[t];
+            R2[t * rstr + o] = r2[t];
+        }
     }
-    q = block_sum(q, sh);
-    qn = block_sum(qn, sh);
-    qu = block_sum(qu, sh);
-    if (threadIdx.x == 0) {
-        qpart[3 * (size_t)blockIdx.x] = q;
-        qpart[3 * (size_t)blockIdx.x + 1] = qn;
-        qpart[3 * (size_t)blockIdx.x + 2] = qu;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const double a = block_sum(q[t], sh), b = block_sum(qn[t], sh), c = block_sum(qu[t], sh);
+        if (threadIdx.x == 0) {
+            qpart[t * qstr + 3 * (size_t)blockIdx.x] = a;
+            qpart[t * qstr + 3 * (size_t)blockIdx.x + 1] = b;
+            qpart[t * qstr + 3 * (size_t)blockIdx.x + 2] = c;
+        }
     }
 }
 
@@ -219,10 +227,11 @@ __global__ void k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart, 
     kappa[ii + (size_t)nb * e] = acc;
 }
 
-// collision matrix.  One workgroup per row block (s,l1); threads over column blocks
-// (n2,l2), whose star gathers stay inside the (s,l1) slab of R1/R2.  The device matrix is
-// row-major (CMr[row*ncol + col]) so the three rows of a block are written contiguously.
-__global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, int nb, int ibz_lo, int nibz,
+// collision matrix, general fallback.  One workgroup per row block (s,l1); threads over
+// column blocks (n2,l2), whose star gathers stay inside the (s,l1) slab of R1/R2.  The
+// device matrix is row-major (CMr[row*ncol + col]) so the three rows of a block are written
+// contiguously.
+__global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz,
                                                    const int *__restrict__ mapinv, const double *__restrict__ Q,
                                                    const double *__restrict__ R1, const double *__restrict__ R2,
                                                    const int *__restrict__ star_n, const int *__restrict__ star_op,
@@ -233,8 +242,7 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, in
     const int s = rb / nb, l1 = rb % nb;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
-    coords(md, k1, a0, a1, a2);
-    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    m.coords(k1, a0, a1, a2);
     const size_t ncol = (size_t)nibz * nb * 3;
     const double qd = Q[(size_t)s * nb + l1];
     for (int cb = threadIdx.x; cb < nibz * nb; cb += blockDim.x) {
@@ -248,9 +256,9 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, in
                 C = -qd;
             } else {
                 int c0, c1, c2;
-                coords(md, n0, c0, c1, c2);
-                int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
-                int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
+                m.coords(n0, c0, c1, c2);
+                const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q2
+                const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q2-q1
                 C = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
             }
             const double *op = opk + 9 * star_op[48 * n2 + ns];
@@ -263,56 +271,57 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDims md, int nk, in
     }
 }
 
-// Same matrix with the (s,l1,l2) rows of R1/R2 staged in LDS (2*nk doubles): one
-// workgroup per row block, branches l2 in turn, threads over the irreducible q2.  Every
-// star gather then hits LDS instead of re-fetching scattered HBM lines.
-__global__ void __launch_bounds__(256) k_collision_lds(alt3::MeshDims md, int nk, int nb, int ibz_lo, int nibz,
-                                                       const int *__restrict__ mapinv, const double *__restrict__ Q,
-                                                       const double *__restrict__ R1, const double *__restrict__ R2,
-                                                       const int *__restrict__ star_n, const int *__restrict__ star_op,
-                                                       const int *__restrict__ star_n0, const double *__restrict__ opk,
-                                                       double *__restrict__ CMr)
+// Collision matrix with the per-point collision term in LDS.  One workgroup per (row block
+// (s,l1), l2).  Phase 1 builds C(q0) = -R1(q0) + R1(q0-q1) + R2(q0)/2 + R2(q1-q0)/2 (or -Q
+// on the diagonal) for every mesh point q0 from row-contiguous reads of R1/R2; phase 2 gives
+// each thread one irreducible q2 (slots ordered by decreasing star size, so that the lanes of
+// a wave run the same number of star members) and sums C(q0) op over its star from LDS.  The
+// star table is stored member-major (tab[ns][slot] = q0 << 6 | op) for coalesced reads.
+__global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
+                                                     const int *__restrict__ mapinv, const double *__restrict__ Q,
+                                                     const double *__restrict__ R1, const double *__restrict__ R2,
+                                                     const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
+                                                     const int *__restrict__ tab, const double *__restrict__ opk,
+                                                     double *__restrict__ CMr)
 {
     extern __shared__ double lds[];
-    double *r1 = lds, *r2 = lds + nk;
-    const int rb = blockIdx.x;  // (s, l1)
+    double *C = lds;            // [nk]
+    double *ops = lds + nk;     // [nops][9]
+    const int l2 = blockIdx.x % nb, rb = blockIdx.x / nb;  // rb = (s, l1)
     const int s = rb / nb, l1 = rb % nb;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
-    coords(md, k1, a0, a1, a2);
-    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    m.coords(k1, a0, a1, a2);
+    const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
+    for (int x = threadIdx.x; x < 9 * nops; x += blockDim.x) ops[x] = opk[x];
+    for (int n0 = threadIdx.x; n0 < nk; n0 += blockDim.x) {
+        int c0, c1, c2;
+        m.coords(n0, c0, c1, c2);
+        const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q0
+        const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q0-q1
+        C[n0] = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && l1 == l2) C[k1] = -Q[(size_t)s * nb + l1];
+    __syncthreads();
     const size_t ncol = (size_t)nibz * nb * 3;
-    const double qd = Q[(size_t)s * nb + l1];
-    for (int l2 = 0; l2 < nb; l2++) {
-        const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
-        for (int j = threadIdx.x; j < nk; j += blockDim.x) {
-            r1[j] = R1[rbase + j];
-            r2[j] = R2[rbase + j];
-        }
-        __syncthreads();
-        for (int n2 = threadIdx.x; n2 < nibz; n2 += blockDim.x) {
-            double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int ns = 0; ns < star_n[n2]; ns++) {
-                int n0 = star_n0[48 * n2 + ns];
-                double C;
-                if (n0 == k1 && l1 == l2) {
-                    C = -qd;
-                } else {
-                    int c0, c1, c2;
-                    coords(md, n0, c0, c1, c2);
-                    int p1 = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));  // q1-q2
-                    int p2 = idx(md, wrap(c0 - a0 + h0, md.n0), wrap(c1 - a1 + h1, md.n1), wrap(c2 - a2 + h2, md.n2));  // q2-q1
-                    C = -r1[n0] + r1[p2] + 0.5 * r2[n0] + 0.5 * r2[p1];
-                }
-                const double *op = opk + 9 * star_op[48 * n2 + ns];
-                for (int i = 0; i < 3; i++)
-                    for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (C * op[3 * j + i]);
-            }
-            const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
+    for (int x = threadIdx.x; x < nibz; x += blockDim.x) {
+        const int n2 = slot_n2[x], cnt = slot_cnt[x];
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ns = 0; ns < cnt; ns++) {
+            const int e = tab[(size_t)ns * nibz + x];
+            const double c = C[e >> 6];
+            const double *op = ops + 9 * (e & 63);
+#pragma unroll
             for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (c * op[3 * j + i]);
         }
-        __syncthreads();
+        const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
     }
 }
 
@@ -519,48 +528,109 @@ int upload_bte_tables(alt3_ctx *ctx)
     ALT3_HIP(hipMemcpy(ctx->d_fstar_n0.p, T.fstar_n0.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
     ALT3_HIP(hipMemcpy(ctx->d_opk.p, T.opk.data(), sizeof(double) * 9 * T.nops, hipMemcpyHostToDevice));
     ALT3_HIP(hipMemcpy(ctx->d_opk_inv.p, T.opk_inv.data(), sizeof(double) * 9 * T.nops, hipMemcpyHostToDevice));
+    // collision star table: irreducible points ordered by decreasing star size (stable),
+    // members stored member-major as q0 << 6 | op
+    std::vector<int> order(T.nibz);
+    for (int i = 0; i < T.nibz; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return T.star_n[a] > T.star_n[b]; });
+    std::vector<int> cnt(T.nibz), tab((size_t)48 * T.nibz, 0);
+    for (int x = 0; x < T.nibz; x++) {
+        const int n2 = order[x];
+        cnt[x] = T.star_n[n2];
+        if (cnt[x] < 0 || cnt[x] > 48) return fail(ALT3_E_MESH, "star larger than 48 members");
+        for (int ns = 0; ns < cnt[x]; ns++) {
+            const int n0 = T.star_n0[48 * n2 + ns], op = T.star_op[48 * n2 + ns];
+            if (n0 < 0 || n0 >= T.nk || op < 0 || op >= 48 || op >= T.nops)
+                return fail(ALT3_E_MESH, "star member outside the mesh or operation table");
+            tab[(size_t)ns * T.nibz + x] = (n0 << 6) | op;
+        }
+    }
+    ALT3_TRY(ctx->d_cm_slot_n2.alloc(T.nibz));
+    ALT3_TRY(ctx->d_cm_slot_cnt.alloc(T.nibz));
+    ALT3_TRY(ctx->d_cm_tab.alloc((size_t)48 * T.nibz));
+    ALT3_HIP(hipMemcpy(ctx->d_cm_slot_n2.p, order.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_cm_slot_cnt.p, cnt.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_cm_tab.p, tab.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
     return ALT3_OK;
 }
 
-int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT)
+// Q, QN, QU and the l3-reduced R1/R2 for each of nt temperatures, P1sm/P2sm read once per
+// group of up to four temperatures.  Temperature t lands in slot t of the per-temperature
+// arrays; slot 0 is selected afterwards.
+int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
 {
     if (!ctx->have_fgr) return fail(ALT3_E_STATE, "alt3_tau needs alt3_fgr first");
-    if (!(tempk > 0)) return fail(ALT3_E_ARG, "ERROR in N_BE: t=<0");
+    if (nt < 1) return fail(ALT3_E_ARG, "no temperature");
+    for (int t = 0; t < nt; t++)
+        if (!(temps[t] > 0)) return fail(ALT3_E_ARG, "ERROR in N_BE: t=<0");
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
-    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
+    const MeshDev md = MeshDev::make(ctx->nc);
+    ctx->have_tau = ctx->have_cm = false;
+    ctx->tau_temps.clear();
     DBuf<int> derr;
     ALT3_TRY(derr.alloc(1));
     ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
-    ALT3_TRY(ctx->d_dist.alloc((size_t)nk * nb));
-    hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
-                       temp_cm(tempk), ctx->classical, ctx->d_dist.p, derr.p);
+    const size_t dstr = (size_t)nk * nb, qstr = (size_t)ksub * nb, rstr = (size_t)ksub * nb * nb * nk;
+    ALT3_TRY(ctx->d_dist.alloc(nt * dstr));
+    ALT3_TRY(ctx->d_q.alloc(nt * qstr));
+    ALT3_TRY(ctx->d_qn.alloc(nt * qstr));
+    ALT3_TRY(ctx->d_qu.alloc(nt * qstr));
+    ALT3_TRY(ctx->d_r1.alloc(nt * rstr));
+    ALT3_TRY(ctx->d_r2.alloc(nt * rstr));
+    for (int t = 0; t < nt; t++)
+        hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
+                           temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr.p);
     ALT3_HIP(hipGetLastError());
-    ALT3_TRY(ctx->d_q.alloc((size_t)ksub * nb));
-    ALT3_TRY(ctx->d_qn.alloc((size_t)ksub * nb));
-    ALT3_TRY(ctx->d_qu.alloc((size_t)ksub * nb));
-    ALT3_TRY(ctx->d_r1.alloc((size_t)ksub * nb * nb * nk));
-    ALT3_TRY(ctx->d_r2.alloc((size_t)ksub * nb * nb * nk));
+    const size_t nblk = (size_t)ksub * nb * nb, pstr = 3 * nblk;
     DBuf<double> qpart;
-    ALT3_TRY(qpart.alloc((size_t)3 * ksub * nb * nb));
-    hipLaunchKernelGGL(k_tau, dim3(ksub * nb * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
-                       ctx->d_dist.p, ctx->d_p1.p, ctx->d_p2.p, qpart.p, ctx->d_r1.p, ctx->d_r2.p);
-    ALT3_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, nb, qpart.p,
-                       ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p);
+    ALT3_TRY(qpart.alloc((size_t)nt * pstr));
+    for (int t0 = 0; t0 < nt; t0 += 4) {
+        const int g = std::min(4, nt - t0);
+        auto *kern = g == 1 ? k_tau<1> : g == 2 ? k_tau<2> : g == 3 ? k_tau<3> : k_tau<4>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                           ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1.p, ctx->d_p2.p,
+                           qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+        ALT3_HIP(hipGetLastError());
+    }
+    for (int t = 0; t < nt; t++)
+        hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, nb,
+                           qpart.p + t * pstr, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr, ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
-    ctx->tempk = tempk;
+    ctx->tau_temps.assign(temps, temps + nt);
+    ctx->tau_cur = 0;
+    ctx->tempk = temps[0];
     ctx->have_tau = true;
-    ctx->have_cm = false;
+    return ALT3_OK;
+}
+
+// make the slot of temperature tempk (from the last tau pass) current
+int tau_select(alt3_ctx *ctx, double tempk, const char *who)
+{
+    if (!ctx->have_tau) return fail(ALT3_E_STATE, std::string(who) + " needs alt3_tau first");
+    for (size_t t = 0; t < ctx->tau_temps.size(); t++)
+        if (ctx->tau_temps[t] == tempk) {
+            if ((int)t != ctx->tau_cur || ctx->tempk != tempk) ctx->have_cm = false;
+            ctx->tau_cur = (int)t;
+            ctx->tempk = tempk;
+            return ALT3_OK;
+        }
+    return fail(ALT3_E_STATE, std::string(who) + " temperature differs from the last alt3_tau");
+}
+
+int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT)
+{
+    ALT3_TRY(tau_multi_run(ctx, &tempk, 1));
+    const int nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     if (Q || QN || QU || tN || tU || tT) {
         DBuf<double> o;
         size_t n = (size_t)ksub * nb;
         ALT3_TRY(o.alloc(6 * n));
         hipLaunchKernelGGL(k_qandtau, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub, nb, ctx->ibz_lo,
-                           ctx->d_mapinv.p, ctx->d_dist.p, ctx->d_q.p, ctx->d_qn.p, ctx->d_qu.p, o.p, o.p + n,
+                           ctx->d_mapinv.p, ctx->dist(), ctx->q(), ctx->qn(), ctx->qu(), o.p, o.p + n,
                            o.p + 2 * n, o.p + 3 * n, o.p + 4 * n, o.p + 5 * n);
         ALT3_HIP(hipGetLastError());
         double *dst[6] = {Q, QN, QU, tN, tU, tT};
@@ -571,9 +641,9 @@ int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, doub
     return ALT3_OK;
 }
 
-static int need_full_ibz(alt3_ctx *ctx, const char *who)
+static int need_full_ibz(alt3_ctx *ctx, double tempk, const char *who)
 {
-    if (!ctx->have_tau) return fail(ALT3_E_STATE, std::string(who) + " needs alt3_tau first");
+    ALT3_TRY(tau_select(ctx, tempk, who));
     if (ctx->ibz_lo != 0 || ctx->ibz_hi != ctx->nibz - 1)
         return fail(ALT3_E_STATE, std::string(who) + " needs Q over the whole irreducible zone (ibz 0..nibz-1)");
     return ALT3_OK;
@@ -581,8 +651,7 @@ static int need_full_ibz(alt3_ctx *ctx, const char *who)
 
 int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff)
 {
-    ALT3_TRY(need_full_ibz(ctx, "alt3_kappa_rta"));
-    if (tempk != ctx->tempk) return fail(ALT3_E_STATE, "alt3_kappa_rta temperature differs from the last alt3_tau");
+    ALT3_TRY(need_full_ibz(ctx, tempk, "alt3_kappa_rta"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;
     ALT3_TRY(ctx->d_frta.alloc((size_t)nibz * nb * 3));
     ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
@@ -594,7 +663,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, nk,
-                       ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_dist.p, ctx->d_eival.p, ctx->d_vel.p, ctx->d_q.p, tempk,
+                       ctx->d_mapinv.p, ctx->d_freq.p, ctx->dist(), ctx->d_eival.p, ctx->d_vel.p, ctx->q(), tempk,
                        temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
                        ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr.p);
     ALT3_HIP(hipGetLastError());
@@ -612,25 +681,24 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
 
 int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
 {
-    if (!ctx->have_tau) return fail(ALT3_E_STATE, "alt3_collision needs alt3_tau first");
-    if (tempk != ctx->tempk) return fail(ALT3_E_STATE, "alt3_collision temperature differs from the last alt3_tau");
+    ALT3_TRY(tau_select(ctx, tempk, "alt3_collision"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
-    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
+    const MeshDev md = MeshDev::make(ctx->nc);
     const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
-    const size_t lds = (size_t)2 * nk * sizeof(double);
+    const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
     if (lds <= 65536)
-        hipLaunchKernelGGL(k_collision_lds, dim3(ksub * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
-                           ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
-                           ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+        hipLaunchKernelGGL(k_collision_c, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                           nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), ctx->r1(), ctx->r2(), ctx->d_cm_slot_n2.p,
+                           ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, ctx->d_cm.p);
     else
         hipLaunchKernelGGL(k_collision, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
-                           ctx->d_mapinv.p, ctx->d_q.p, ctx->d_r1.p, ctx->d_r2.p, ctx->d_star_n.p, ctx->d_star_op.p,
+                           ctx->d_mapinv.p, ctx->q(), ctx->r1(), ctx->r2(), ctx->d_star_n.p, ctx->d_star_op.p,
                            ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
-                       ctx->d_freq.p, ctx->d_dist.p, ctx->d_vel.p, ctx->d_rhs.p);
+                       ctx->d_freq.p, ctx->dist(), ctx->d_vel.p, ctx->d_rhs.p);
     ALT3_HIP(hipGetLastError());
     if (CM) {  // Fortran Collision_Matrixmpi(nrow, ncol) is column-major
         DBuf<double> t;

@@ -1,4 +1,5 @@
 // C ABI of the alt3 library (declared in include/alt3.h).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -23,6 +24,7 @@ int copy_fortran5(alt3_ctx *ctx, const double *dev, double *host);
 int fgr_run(alt3_ctx *ctx);
 int upload_bte_tables(alt3_ctx *ctx);
 int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT);
+int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt);
 int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff);
 int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
@@ -122,8 +124,9 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
     ctx->d_eph.release(); ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
     ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
-    ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release(); ctx->d_qall.release();
+    ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release();
     ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
+    ctx->d_cm_slot_n2.release(); ctx->d_cm_slot_cnt.release(); ctx->d_cm_tab.release();
     ctx->d_fstar_n.release(); ctx->d_fstar_op.release(); ctx->d_fstar_n0.release();
     ctx->d_opk.release(); ctx->d_opk_inv.release(); ctx->d_kibz.release(); ctx->d_frta.release();
     ctx->d_kpart.release(); ctx->d_kappa.release(); ctx->d_taueff.release(); ctx->d_veloci.release();
@@ -348,7 +351,11 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     ALT3_TRY(dte.alloc((size_t)nibz * nb));
     for (int it = 0; it < ntemps; it++) {
         double T = temps[it];
-        ALT3_TRY(stage(3, [&] { return tau_run(ctx, T, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr); }));
+        // one tau pass (one read of P1sm/P2sm) per group of four temperatures
+        if (it % 4 == 0) {
+            const int g = std::min(4, ntemps - it);
+            ALT3_TRY(stage(3, [&] { return tau_multi_run(ctx, temps + it, g); }));
+        }
         ALT3_TRY(stage(4, [&] { return kappa_rta_run(ctx, T, kappa_rta ? kappa_rta + (size_t)it * nb * 9 : nullptr, nullptr, nullptr); }));
         ALT3_TRY(stage(5, [&] { return collision_run(ctx, T, nullptr, nullptr); }));
         // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843)

@@ -11,23 +11,11 @@
 // float, differences in float, mixed float/double formulae) so ties and the degenerate-gap
 // clamp behave identically; FMA contraction is disabled in this file for the same reason.
 #include "alt3_impl.h"
+#include "mesh_dev.h"
 
 #pragma clang fp contract(off)
 
 namespace {
-
-__device__ __forceinline__ void coords(const alt3::MeshDims &d, int k, int &c0, int &c1, int &c2)
-{
-    c2 = k % d.n2;
-    c1 = (k / d.n2) % d.n1;
-    c0 = k / (d.n1 * d.n2);
-}
-__device__ __forceinline__ int idx(const alt3::MeshDims &d, int c0, int c1, int c2) { return c2 + d.n2 * (c1 + d.n1 * c0); }
-__device__ __forceinline__ int wrap(int x, int n)
-{
-    x %= n;
-    return x < 0 ? x + n : x;
-}
 
 // weight_tet for one tetrahedron: corner energies w[4] (already rounded to REAL*4) at e
 __device__ void weight_tet1(const float *w, double e, double *c)
@@ -126,12 +114,25 @@ __device__ void weight_tet1(const float *w, double e, double *c)
 // corners (0..7 = bits g1,g2,g3 of the cube) of the 6 tetrahedra of make_kp_reg_tet
 __constant__ int kTc[6][4] = {{0, 1, 2, 5}, {1, 2, 3, 5}, {0, 2, 4, 5}, {2, 3, 5, 7}, {2, 4, 5, 6}, {2, 5, 6, 7}};
 
-// one workgroup per (s, l1, l2, l3, process); LDS holds the float corner energy and the
-// double weight sum of every mesh point
+// a[j] for a register array and a runtime index, as a chain of selects
+template <int N>
+__device__ __forceinline__ float pick(const float (&a)[N], int j)
+{
+    float r = a[0];
+#pragma unroll
+    for (int i = 1; i < N; i++) r = (j == i) ? a[i] : r;
+    return r;
+}
+
+// One workgroup per (s, l2, l3, process).  The corner energies e(q2) depend on l2, l3 and
+// the process but not on l1, so they are built once in LDS (with the energy range of every
+// cube kept in registers) and reused for the nb values omega(q1, l1); per l1 the LDS weight
+// sums W are accumulated and then written out as P = W * F.
 constexpr int kFgrThreads = 1024;  // 16 waves: LDS (12 B per mesh point) limits residency to 2 groups/CU
+constexpr int kFgrPer = 5;         // mesh points (and cubes) per thread: nk <= 5120
 
 template <typename VT>
-__global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDims md, int nk, int nb, int ibz_lo,
+__global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
                                                      const VT *__restrict__ v33, double *__restrict__ P1,
                                                      double *__restrict__ P2)
@@ -144,128 +145,149 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDims md, int nk, 
     __shared__ unsigned short s_list[6 * kFgrThreads];  // (cube << 3 | tetrahedron), nk <= 8192
     const int proc = blockIdx.x & 1;
     int r = blockIdx.x >> 1;
-    const int l3 = r % nb; r /= nb;
-    const int l2 = r % nb; r /= nb;
-    const int l1 = r % nb; r /= nb;
-    const int s = r;
+    const int l3 = r % nb;
+    r /= nb;
+    const int l2 = r % nb;
+    const int s = r / nb;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
-    coords(md, k1, a0, a1, a2);
-    const double omega1 = freq[(size_t)k1 * nb + l1];
-    const int h0 = md.n0 / 2, h1 = md.n1 / 2, h2 = md.n2 / 2;
+    m.coords(k1, a0, a1, a2);
+    const int tid = threadIdx.x, gam = m.gamma();
     float lo = 3.4e38f, hi = -3.4e38f;
-    for (int w = threadIdx.x; w < nk; w += blockDim.x) {
-        int c0, c1, c2;
-        coords(md, w, c0, c1, c2);
-        double e;
-        if (proc == 0) {  // -w(q2,l2) + w(q1+q2,l3); q1+q2 -> c1+c2-n/2
-            int p = idx(md, wrap(a0 + c0 - h0, md.n0), wrap(a1 + c1 - h1, md.n1), wrap(a2 + c2 - h2, md.n2));
-            e = -freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
-        } else {          // w(q2,l2) + w(q1-q2,l3); q1-q2 -> c1-c2+n/2
-            int p = idx(md, wrap(a0 - c0 + h0, md.n0), wrap(a1 - c1 + h1, md.n1), wrap(a2 - c2 + h2, md.n2));
-            e = freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+    unsigned vzero = 0;  // bit j: acoustic branch at Gamma in q2 or q3
+#pragma unroll
+    for (int j = 0; j < kFgrPer; j++) {
+        const int w = tid + j * kFgrThreads;
+        if (w < nk) {
+            int c0, c1, c2;
+            m.coords(w, c0, c1, c2);
+            int p;
+            double e;
+            if (proc == 0) {  // -w(q2,l2) + w(q1+q2,l3)
+                p = m.sum(a0, a1, a2, c0, c1, c2);
+                e = -freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+            } else {          // w(q2,l2) + w(q1-q2,l3)
+                p = m.diff(a0, a1, a2, c0, c1, c2);
+                e = freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+            }
+            if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) vzero |= 1u << j;
+            const float ef = (float)e;
+            en[w] = ef;
+            W[w] = 0.0;
+            lo = fminf(lo, ef);
+            hi = fmaxf(hi, ef);
         }
-        float ef = (float)e;
-        en[w] = ef;
-        W[w] = 0.0;
-        lo = fminf(lo, ef);
-        hi = fmaxf(hi, ef);
     }
     for (int off = 32; off > 0; off >>= 1) {
         lo = fminf(lo, __shfl_xor(lo, off, 64));
         hi = fmaxf(hi, __shfl_xor(hi, off, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
-        s_lo[threadIdx.x >> 6] = lo;
-        s_hi[threadIdx.x >> 6] = hi;
+    if ((tid & 63) == 0) {
+        s_lo[tid >> 6] = lo;
+        s_hi[tid >> 6] = hi;
     }
     __syncthreads();
     for (int w = 0; w < kFgrThreads / 64; w++) {
         lo = fminf(lo, s_lo[w]);
         hi = fmaxf(hi, s_hi[w]);
     }
-    const bool any = !(omega1 < (double)lo || omega1 > (double)hi);
-    if (any) {
-        // cube k has corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2} (pt1..pt8 of
-        // make_kp_reg_tet) and tetrahedra (1,2,3,6) (2,3,4,6) (1,3,5,6) (3,4,6,8) (3,5,6,7)
-        // (3,6,7,8).  Only tetrahedra whose energy range contains omega1 have weights, and
-        // they are few; they are compacted into an LDS list per chunk of cubes so that the
-        // weight formula runs with every lane busy instead of once per divergent lane.
-        const int tid = threadIdx.x;
-        for (int base = 0; base < nk; base += kFgrThreads) {
-            if (tid == 0) s_cnt = 0;
-            __syncthreads();
-            const int k = base + tid;
-            if (k < nk) {
-                int c0, c1, c2;
-                coords(md, k, c0, c1, c2);
-                const int d0 = (c0 + 1 == md.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == md.n1) ? 0 : c1 + 1,
-                          d2 = (c2 + 1 == md.n2) ? 0 : c2 + 1;
-                float ce[8] = {en[idx(md, c0, c1, c2)], en[idx(md, d0, c1, c2)], en[idx(md, c0, d1, c2)],
-                               en[idx(md, d0, d1, c2)], en[idx(md, c0, c1, d2)], en[idx(md, d0, c1, d2)],
-                               en[idx(md, c0, d1, d2)], en[idx(md, d0, d1, d2)]};
-                float mn = ce[0], mx = ce[0];
-                for (int p = 1; p < 8; p++) {
-                    mn = fminf(mn, ce[p]);
-                    mx = fmaxf(mx, ce[p]);
-                }
-                if (!(omega1 < (double)mn || omega1 > (double)mx)) {
+    // energy range of cube k = tid + j*T; corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2}
+    // (pt1..pt8 of make_kp_reg_tet), tetrahedra (1,2,3,6) (2,3,4,6) (1,3,5,6) (3,4,6,8)
+    // (3,5,6,7) (3,6,7,8)
+    float cmn[kFgrPer], cmx[kFgrPer];
+#pragma unroll
+    for (int j = 0; j < kFgrPer; j++) {
+        const int k = tid + j * kFgrThreads;
+        cmn[j] = 3.4e38f;
+        cmx[j] = -3.4e38f;
+        if (k < nk) {
+            int c0, c1, c2;
+            m.coords(k, c0, c1, c2);
+            const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                      d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+            const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
+                                 en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
+                                 en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
+            for (int p = 0; p < 8; p++) {
+                cmn[j] = fminf(cmn[j], ce[p]);
+                cmx[j] = fmaxf(cmx[j], ce[p]);
+            }
+        }
+    }
+    const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
+    double *P = proc == 0 ? P1 : P2;
+    for (int l1 = 0; l1 < nb; l1++) {
+        const double omega1 = freq[(size_t)k1 * nb + l1];
+        if (!(omega1 < (double)lo || omega1 > (double)hi)) {
+            // Only tetrahedra whose energy range contains omega1 have weights, and they are
+            // few; per chunk of cubes they are compacted into an LDS list so that the weight
+            // formula runs with every lane busy instead of once per divergent lane.
+#pragma unroll 1
+            for (int j = 0; j * kFgrThreads < nk; j++) {
+                if (tid == 0) s_cnt = 0;
+                __syncthreads();
+                if (!(omega1 < (double)pick(cmn, j) || omega1 > (double)pick(cmx, j))) {
+                    const int k = tid + j * kFgrThreads;
+                    int c0, c1, c2;
+                    m.coords(k, c0, c1, c2);
+                    const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                              d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+                    const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
+                                         en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
+                                         en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
 #pragma unroll
                     for (int l = 0; l < 6; l++) {
-                        float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
-                        float tmn = fminf(fminf(a, b), fminf(c, d)), tmx = fmaxf(fmaxf(a, b), fmaxf(c, d));
+                        const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
+                        const float tmn = fminf(fminf(a, b), fminf(c, d)), tmx = fmaxf(fmaxf(a, b), fmaxf(c, d));
                         if (!(omega1 < (double)tmn || omega1 > (double)tmx))
                             s_list[atomicAdd(&s_cnt, 1)] = (unsigned short)((k << 3) | l);
                     }
                 }
-            }
-            __syncthreads();
-            const int cnt = s_cnt;
-            for (int x = tid; x < cnt; x += blockDim.x) {
-                const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
-                int c0, c1, c2;
-                coords(md, kk, c0, c1, c2);
-                int v[4];
+                __syncthreads();
+                const int cnt = s_cnt;
+                for (int x = tid; x < cnt; x += kFgrThreads) {
+                    const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
+                    int c0, c1, c2;
+                    m.coords(kk, c0, c1, c2);
+                    int v[4];
 #pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    const int p = kTc[l][y];
-                    int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
-                    if (e0 == md.n0) e0 = 0;
-                    if (e1 == md.n1) e1 = 0;
-                    if (e2 == md.n2) e2 = 0;
-                    v[y] = idx(md, e0, e1, e2);
+                    for (int y = 0; y < 4; y++) {
+                        const int p = kTc[l][y];
+                        int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
+                        if (e0 == m.n0) e0 = 0;
+                        if (e1 == m.n1) e1 = 0;
+                        if (e2 == m.n2) e2 = 0;
+                        v[y] = m.index(e0, e1, e2);
+                    }
+                    const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
+                    double c[4];
+                    weight_tet1(w4, omega1, c);
+                    for (int y = 0; y < 4; y++)
+                        if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
                 }
-                float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
-                double c[4];
-                weight_tet1(w4, omega1, c);
-                for (int y = 0; y < 4; y++)
-                    if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+                __syncthreads();
             }
-            __syncthreads();
         }
-    }
-    __syncthreads();
-    const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
-    const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
-    double *P = proc == 0 ? P1 : P2;
-    for (int w = threadIdx.x; w < nk; w += blockDim.x) {
-        int c0, c1, c2;
-        coords(md, w, c0, c1, c2);
-        double F;
-        if (proc == 0) {
-            F = (double)v33[base + w] * tpi2 / nk;
-            bool g2 = (c0 == h0 && c1 == h1 && c2 == h2);
-            bool g3 = (wrap(a0 + c0 - h0, md.n0) == h0 && wrap(a1 + c1 - h1, md.n1) == h1 && wrap(a2 + c2 - h2, md.n2) == h2);
-            if ((l2 < 3 && g2) || (l3 < 3 && g3)) F = 0.0;
-        } else {
-            // F(w) = |V3(q1, -q_w, ...)|^2 ; masks at -q_j = q_w and q1+q_j = q1-q_w
-            int jn = idx(md, (md.n0 - c0) % md.n0, (md.n1 - c1) % md.n1, (md.n2 - c2) % md.n2);
-            F = (double)v33[base + jn] * tpi2 / nk;
-            bool g2 = (c0 == h0 && c1 == h1 && c2 == h2);
-            bool g3 = (wrap(a0 - c0 + h0, md.n0) == h0 && wrap(a1 - c1 + h1, md.n1) == h1 && wrap(a2 - c2 + h2, md.n2) == h2);
-            if ((l2 < 3 && g2) || (l3 < 3 && g3)) F = 0.0;
+        // P(q_w) = W(q_w) F(q_w); each thread then clears the weight sums it read
+        const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+#pragma unroll
+        for (int j = 0; j < kFgrPer; j++) {
+            const int w = tid + j * kFgrThreads;
+            if (w < nk) {
+                // F(q_w) reads |V3|^2 at q2 = q_w (coalescence) or q2 = -q_w (decay)
+                int vo = w;
+                if (proc) {
+                    int c0, c1, c2;
+                    m.coords(w, c0, c1, c2);
+                    vo = m.neg(c0, c1, c2);
+                }
+                double F = (double)v33[base + vo] * tpi2 / nk;
+                if (vzero >> j & 1u) F = 0.0;
+                P[base + w] = W[w] * F;
+                W[w] = 0.0;
+            }
         }
-        P[base + w] = W[w] * F;
+        __syncthreads();
     }
 }
 
@@ -279,12 +301,12 @@ int fgr_run(alt3_ctx *ctx)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    if (lds + 6 * kFgrThreads * sizeof(unsigned short) + 256 > 65536 || nk > 8192)
+    if (lds + 6 * kFgrThreads * sizeof(unsigned short) + 256 > 65536 || nk > kFgrPer * kFgrThreads)
         return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4400");
     ALT3_TRY(ctx->d_p1.alloc(nel));
     ALT3_TRY(ctx->d_p2.alloc(nel));
-    MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
-    dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb * nb));
+    const MeshDev md = MeshDev::make(ctx->nc);
+    dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     if (ctx->v3_prec == ALT3_V3_MIXED)
         hipLaunchKernelGGL(k_fgr<float>, grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
                            ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);

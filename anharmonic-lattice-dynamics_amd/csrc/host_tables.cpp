@@ -167,6 +167,9 @@ int build_tables(const alt3_system *sys, HostTables &t)
     for (int a = 0; a < 3; a++)
         if (nc[a] <= 0 || nc[a] % 2 != 0)
             return fail(ALT3_E_MESH, "mesh must be even in every direction (make_kp_reg_tet corners are mesh points only then)");
+    // the device index arithmetic divides by float reciprocals, exact for n <= 256 and nk < 4e6
+    if (nc[0] > 256 || nc[1] > 256 || nc[2] > 256 || (long long)nc[0] * nc[1] * nc[2] >= 4000000)
+        return fail(ALT3_E_MESH, "mesh too large: at most 256 points per direction and 4e6 in total");
     const int nk = nc[0] * nc[1] * nc[2];
     t.nk = nk;
     // reciprocal lattice, make_reciprocal_lattice_2pi (extratools.f90:209-228)
