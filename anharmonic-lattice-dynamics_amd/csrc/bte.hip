@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
             f2[t] = d1[t] * (d2 + 1);
             r1[t] = r2[t] = 0;
         }
-#pragma unroll 2
+#pragma unroll 3
         for (int l3 = 0; l3 < nb; l3++) {
             const size_t x = row + (size_t)l3 * nk + j;
             const double P1 = P1sm[x], P2 = P2sm[x];

@@ -128,10 +128,13 @@ __device__ __forceinline__ float pick(const float (&a)[N], int j)
 // the process but not on l1, so they are built once in LDS (with the energy range of every
 // cube kept in registers) and reused for the nb values omega(q1, l1); per l1 the LDS weight
 // sums W are accumulated and then written out as P = W * F.
-constexpr int kFgrThreads = 1024;  // 16 waves: LDS (12 B per mesh point) limits residency to 2 groups/CU
-constexpr int kFgrPer = 5;         // mesh points (and cubes) per thread: nk <= 5120
+constexpr int kFgrThreads = 512;   // 8 waves; LDS (12 B per mesh point) allows 2 groups per CU
+constexpr int kFgrList = 6144;     // LDS list of active tetrahedra; one chunk of cubes always fits
 
-template <typename VT>
+// e inside [lo, hi] as the reference tests it (no weight outside, tetrahedron.f90:343)
+__device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return !(e < (double)lo || e > (double)hi); }
+
+template <typename VT, int kFgrPer>   // kFgrPer: mesh points (and cubes) per thread
 __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
                                                      const VT *__restrict__ v33, double *__restrict__ P1,
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
     __shared__ int s_cnt;
-    __shared__ unsigned short s_list[6 * kFgrThreads];  // (cube << 3 | tetrahedron), nk <= 8192
+    __shared__ unsigned short s_list[kFgrList];  // (cube << 3 | tetrahedron), nk <= 8192
     const int proc = blockIdx.x & 1;
     int r = blockIdx.x >> 1;
     const int l3 = r % nb;
@@ -216,72 +219,102 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     }
     const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
     double *P = proc == 0 ? P1 : P2;
-    for (int l1 = 0; l1 < nb; l1++) {
-        const double omega1 = freq[(size_t)k1 * nb + l1];
-        if (!(omega1 < (double)lo || omega1 > (double)hi)) {
-            // Only tetrahedra whose energy range contains omega1 have weights, and they are
-            // few; per chunk of cubes they are compacted into an LDS list so that the weight
-            // formula runs with every lane busy instead of once per divergent lane.
-#pragma unroll 1
-            for (int j = 0; j * kFgrThreads < nk; j++) {
-                if (tid == 0) s_cnt = 0;
-                __syncthreads();
-                if (!(omega1 < (double)pick(cmn, j) || omega1 > (double)pick(cmx, j))) {
-                    const int k = tid + j * kFgrThreads;
-                    int c0, c1, c2;
-                    m.coords(k, c0, c1, c2);
-                    const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
-                              d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
-                    const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
-                                         en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
-                                         en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
+    // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list
+    auto push = [&](int j, double e) {
+        if (omega_in(e, pick(cmn, j), pick(cmx, j))) {
+            const int k = tid + j * kFgrThreads;
+            int c0, c1, c2;
+            m.coords(k, c0, c1, c2);
+            const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                      d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+            const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
+                                 en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
+                                 en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
 #pragma unroll
-                    for (int l = 0; l < 6; l++) {
-                        const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
-                        const float tmn = fminf(fminf(a, b), fminf(c, d)), tmx = fmaxf(fmaxf(a, b), fmaxf(c, d));
-                        if (!(omega1 < (double)tmn || omega1 > (double)tmx))
-                            s_list[atomicAdd(&s_cnt, 1)] = (unsigned short)((k << 3) | l);
-                    }
+            for (int l = 0; l < 6; l++) {
+                const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
+                if (omega_in(e, fminf(fminf(a, b), fminf(c, d)), fmaxf(fmaxf(a, b), fmaxf(c, d)))) {
+                    const int x = atomicAdd(&s_cnt, 1);
+                    if (x < kFgrList) s_list[x] = (unsigned short)((k << 3) | l);
                 }
-                __syncthreads();
-                const int cnt = s_cnt;
-                for (int x = tid; x < cnt; x += kFgrThreads) {
-                    const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
-                    int c0, c1, c2;
-                    m.coords(kk, c0, c1, c2);
-                    int v[4];
-#pragma unroll
-                    for (int y = 0; y < 4; y++) {
-                        const int p = kTc[l][y];
-                        int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
-                        if (e0 == m.n0) e0 = 0;
-                        if (e1 == m.n1) e1 = 0;
-                        if (e2 == m.n2) e2 = 0;
-                        v[y] = m.index(e0, e1, e2);
-                    }
-                    const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
-                    double c[4];
-                    weight_tet1(w4, omega1, c);
-                    for (int y = 0; y < 4; y++)
-                        if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
-                }
-                __syncthreads();
             }
         }
-        // P(q_w) = W(q_w) F(q_w); each thread then clears the weight sums it read
+    };
+    // weights of the listed tetrahedra, added to W at their corners
+    auto process = [&](int cnt, double e) {
+        for (int x = tid; x < cnt; x += kFgrThreads) {
+            const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
+            int c0, c1, c2;
+            m.coords(kk, c0, c1, c2);
+            int v[4];
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                const int p = kTc[l][y];
+                int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
+                if (e0 == m.n0) e0 = 0;
+                if (e1 == m.n1) e1 = 0;
+                if (e2 == m.n2) e2 = 0;
+                v[y] = m.index(e0, e1, e2);
+            }
+            const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
+            double c[4];
+            weight_tet1(w4, e, c);
+            for (int y = 0; y < 4; y++)
+                if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+        }
+    };
+    for (int l1 = 0; l1 < nb; l1++) {
+        const double omega1 = freq[(size_t)k1 * nb + l1];
         const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+        // |V3|^2 of this l1 (at q2 = q_w for coalescence, q2 = -q_w for decay), requested
+        // before the weight phase so that the load latency overlaps it
+        VT fv[kFgrPer];
 #pragma unroll
         for (int j = 0; j < kFgrPer; j++) {
             const int w = tid + j * kFgrThreads;
+            fv[j] = 0;
             if (w < nk) {
-                // F(q_w) reads |V3|^2 at q2 = q_w (coalescence) or q2 = -q_w (decay)
                 int vo = w;
                 if (proc) {
                     int c0, c1, c2;
                     m.coords(w, c0, c1, c2);
                     vo = m.neg(c0, c1, c2);
                 }
-                double F = (double)v33[base + vo] * tpi2 / nk;
+                fv[j] = v33[base + vo];
+            }
+        }
+        if (omega_in(omega1, lo, hi)) {
+            // Only tetrahedra whose energy range contains omega1 have weights, and they are
+            // few: they are compacted into an LDS list so that the weight formula runs with
+            // every lane busy.  One list normally holds all of them; if it overflows, the
+            // cubes are taken again chunk by chunk (at most 6 entries per cube and thread).
+            if (tid == 0) s_cnt = 0;
+            __syncthreads();
+#pragma unroll 1
+            for (int j = 0; j * kFgrThreads < nk; j++) push(j, omega1);
+            __syncthreads();
+            const int cnt = s_cnt;
+            if (cnt <= kFgrList) {
+                process(cnt, omega1);
+            } else {
+#pragma unroll 1
+                for (int j = 0; j * kFgrThreads < nk; j++) {
+                    __syncthreads();
+                    if (tid == 0) s_cnt = 0;
+                    __syncthreads();
+                    push(j, omega1);
+                    __syncthreads();
+                    process(s_cnt, omega1);
+                }
+            }
+            __syncthreads();
+        }
+        // P(q_w) = W(q_w) F(q_w); each thread then clears the weight sums it read
+#pragma unroll
+        for (int j = 0; j < kFgrPer; j++) {
+            const int w = tid + j * kFgrThreads;
+            if (w < nk) {
+                double F = (double)fv[j] * tpi2 / nk;
                 if (vzero >> j & 1u) F = 0.0;
                 P[base + w] = W[w] * F;
                 W[w] = 0.0;
@@ -301,18 +334,34 @@ int fgr_run(alt3_ctx *ctx)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    if (lds + 6 * kFgrThreads * sizeof(unsigned short) + 256 > 65536 || nk > kFgrPer * kFgrThreads)
+    if (lds + kFgrList * sizeof(unsigned short) + 256 > 65536 || nk > 9 * kFgrThreads)
         return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4400");
     ALT3_TRY(ctx->d_p1.alloc(nel));
     ALT3_TRY(ctx->d_p2.alloc(nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
-    if (ctx->v3_prec == ALT3_V3_MIXED)
-        hipLaunchKernelGGL(k_fgr<float>, grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
-                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
-    else
-        hipLaunchKernelGGL(k_fgr<double>, grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
-                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
+    const int per = (nk + kFgrThreads - 1) / kFgrThreads;
+    const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
+    const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
+#define ALT3_FGR_LAUNCH(PER)                                                                                     \
+    if (mixed)                                                                                                   \
+        hipLaunchKernelGGL((k_fgr<float, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
+                           ctx->d_mapinv.p, ctx->d_freq.p, (const float *)v, ctx->d_p1.p, ctx->d_p2.p);          \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_fgr<double, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,            \
+                           ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, (const double *)v, ctx->d_p1.p, ctx->d_p2.p);
+    if (per <= 2) {
+        ALT3_FGR_LAUNCH(2)
+    } else if (per <= 4) {
+        ALT3_FGR_LAUNCH(4)
+    } else if (per <= 6) {
+        ALT3_FGR_LAUNCH(6)
+    } else if (per <= 8) {
+        ALT3_FGR_LAUNCH(8)
+    } else {
+        ALT3_FGR_LAUNCH(9)
+    }
+#undef ALT3_FGR_LAUNCH
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;
