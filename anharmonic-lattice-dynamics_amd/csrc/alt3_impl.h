@@ -142,7 +142,7 @@ struct alt3_ctx {
     alt3::DBuf<double> d_sterm_psi, d_sterm_rk;
     alt3::DBuf<double> d_eph;                      // [nk][ndr] complex e^{i q2.dR}
     alt3::DBuf<double> d_gp;                       // per q1: [ndr][nb][nb*nb] complex (l1,b,c)
-    alt3::DBuf<float> d_gpf;                       // FP32 hi/lo copy for the mixed path
+    alt3::DBuf<float> d_gpf;                       // packed FP32 B operand of the mixed path
     alt3::DBuf<int> d_mapinv;
     int ibz_lo = -1, ibz_hi = -1, v3_prec = -1;
     alt3::DBuf<double> d_v33;                      // [ksub][nb][nb][nb][nk] FP64

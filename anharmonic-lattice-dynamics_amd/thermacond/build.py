@@ -1,4 +1,5 @@
-"""Builds libalt3.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree.
+"""Builds libalt3.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree, and the Fortran
+job-1 driver fortran/alt3_kappa over it with amdflang.
 
     python -m thermacond.build            (with anharmonic-lattice-dynamics_amd/ on sys.path)
 """
@@ -31,6 +32,34 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN_DIR = os.path.join(PKG_DIR, "fortran")
+FORTRAN_EXE = os.path.join(FORTRAN_DIR, "alt3_kappa")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """The Fortran job-1 driver (fortran/alt3_kappa.f90 over the iso_c_binding module
+    fortran/alt3_iface.f90), linked against libalt3.so with an rpath relative to itself."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = []
+    for src in ("alt3_iface.f90", "alt3_kappa.f90"):
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".f.o")
+        cmd = [FLANG, "-O2", "-J", OBJ_DIR, "-c", os.path.join(FORTRAN_DIR, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    tmp = FORTRAN_EXE + ".tmp"
+    cmd = [FLANG, "-o", tmp] + objs + ["-L" + PKG_DIR, "-lalt3", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of alt3_kappa failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FORTRAN_EXE)
+    if verbose:
+        print(f"built {FORTRAN_EXE}")
+    return FORTRAN_EXE
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -43,6 +72,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB_PATH)
     if verbose:
         print(f"built {LIB_PATH}")
+    build_fortran(verbose)
     return LIB_PATH
 
 

@@ -88,7 +88,10 @@ class Context:
         s.natoms0 = inp.natoms0
         s.mass = arr(inp.masses, np.float64)
         s.r123 = arr(inp.r123, np.float64)
-        s.latparams = arr(inp.latparams, np.float64)
+        # structure.params line 1 with a,b,c times the scale, as read_input_fit holds it (rest.f90:89-92)
+        lp = np.array(inp.latparams, dtype=np.float64)
+        lp[:3] *= inp.lattice_parameter
+        s.latparams = arr(lp, np.float64)
         s.primlatt = arr(np.asfortranarray(inp.primlatt).reshape(-1, order="F"), np.float64)
         s.natoms = len(inp.iatomcell0)
         s.atompos = arr(inp.atompos, np.float64)

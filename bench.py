@@ -101,6 +101,21 @@ def timed(ctx, prec, temps, warmup, steps, label, quiet):
                 p99=1e3 * lat_s[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], stage=stage, out=out)
 
 
+def profiled_traffic(kernel_prefix: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a plain copy), or None when it has no entry."""
+    path = os.path.join(REPO, "profiles", "sw_si_16cubed_mixed_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch "):
+            table = json.loads(line.split(" ", 1)[1])
+            for name, b in table.items():
+                if name.split("<")[0] == kernel_prefix:
+                    return float(b)
+    return None
+
+
 def kappa_iso(k):
     return float(k.sum(0).trace() / 3)
 
@@ -157,7 +172,9 @@ def main():
                    **sizes, "v3_elements": nel, "precision": head_name},
         "latency_ms": {"p50": head["p50"], "p99": head["p99"]},
         "roofline": {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": fgr_gbs / PEAK_HBM_GBS, "traffic": None, "kernel": "k_fgr",
+                     "frac": fgr_gbs / PEAK_HBM_GBS,
+                     "traffic": profiled_traffic("k_fgr") if (n == 16 and head_name == "mixed") else None,
+                     "kernel": "k_fgr",
                      "algorithmic_bytes_per_launch": fgr_bytes, "kernel_ms": fgr_ms},
     }
     for name, r in runs.items():

@@ -29,7 +29,7 @@ extern "C" {
 /* V3 arithmetic */
 #define ALT3_V3_FP64_REF 0   /* per-element sum over all FC3 terms (matrix_elt_simplified) */
 #define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
-#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA with hi/lo split, FP32 contraction */
+#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA, FP32 contractions, FP64 scale */
 
 typedef struct alt3_ctx alt3_ctx;
 
@@ -130,8 +130,8 @@ int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, i
 int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, double *taueff);
 
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
- * IBZ, FGR, then per T tau, kappa_RTA, collision, CG (x0 = F_RTA, 100 iterations, 2
- * digits), kappa_BTE.  kappa_rta/kappa_bte: [ntemps][nb*9] Fortran (nb,3,3) per T.
+ * IBZ, FGR, then per T tau (one pass over P1sm/P2sm per group of four temperatures),
+ * kappa_RTA, collision, CG (x0 = F_RTA, 100 iterations, 2 digits), kappa_BTE.  kappa_rta/kappa_bte: [ntemps][nb*9] Fortran (nb,3,3) per T.
  * stage_ms (optional, 8 doubles): eig, v3, fgr, tau, kappa_rta, collision, cg, kappa_bte. */
 int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision,
                double *kappa_rta, double *kappa_bte, int *cg_iters, double *stage_ms);

@@ -42,5 +42,6 @@ def si_inputs(tmp_path_factory):
         d = str(tmp_path_factory.mktemp(f"si{n}"))
         refio.write_inputs(d, fc, refio.RunParams(nc=(n, n, n)))
         out[n] = refio.read_inputs(d)
+        out[f"dir{n}"] = d
     out["fc"] = fc
     return out
