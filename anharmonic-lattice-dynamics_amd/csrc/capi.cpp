@@ -22,6 +22,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec);
 int v3_copy_out(alt3_ctx *ctx, double *host);
 int copy_fortran5(alt3_ctx *ctx, const double *dev, double *host);
 int fgr_run(alt3_ctx *ctx);
+int fgr_gauss_run(alt3_ctx *ctx, double sigma);
 int upload_bte_tables(alt3_ctx *ctx);
 int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT);
 int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt);
@@ -248,6 +249,28 @@ int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm)
     return ALT3_OK;
 }
 
+int alt3_fgr_gauss(alt3_ctx *ctx, double sigma, double *P1sm, double *P2sm)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(fgr_gauss_run(ctx, sigma));
+    tm.stop();
+    if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
+    if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+    return ALT3_OK;
+}
+
+int alt3_set_delta(alt3_ctx *ctx, int kind, double sigma)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (kind != ALT3_DELTA_TETRA && kind != ALT3_DELTA_GAUSS) return fail(ALT3_E_ARG, "unknown delta kind");
+    if (kind == ALT3_DELTA_GAUSS && !(sigma > 0)) return fail(ALT3_E_ARG, "Gaussian width must be positive");
+    ctx->delta_kind = kind;
+    ctx->delta_sigma = sigma;
+    return ALT3_OK;
+}
+
 int alt3_tau(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tauN, double *tauU, double *tautot)
 {
     if (!ctx) return fail(ALT3_E_ARG, "null context");
@@ -345,7 +368,9 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     };
     if (!ctx->have_eig) ALT3_TRY(stage(0, [&] { return eig_run(ctx); }));
     ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
-    ALT3_TRY(stage(2, [&] { return fgr_run(ctx); }));
+    ALT3_TRY(stage(2, [&] {
+        return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
+    }));
     DBuf<double> dx, dte;
     ALT3_TRY(dx.alloc(n));
     ALT3_TRY(dte.alloc((size_t)nibz * nb));

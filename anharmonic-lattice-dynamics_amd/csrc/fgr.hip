@@ -324,9 +324,75 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     }
 }
 
+// Gaussian-smeared variant (delta_g, extratools.f90:239-246): the same corner energies and
+// masks as calculate_FGR_mpi2, but delta(w1 - e) -> exp(-x^2/2s^2)/(sqrt(2pi) s) at the mesh
+// point itself:  P_sm(q1,q2) = F(q2) delta_g(w(q1,l1) - e(q2), sigma).  FP64 throughout.
+// One thread per (s, l1, l2, l3, q2), both processes.
+template <typename VT>
+__global__ void __launch_bounds__(256) k_fgr_gauss(alt3::MeshDev m, int nk, int nb, int ibz_lo, double sigma,
+                                                   const int *__restrict__ mapinv, const double *__restrict__ freq,
+                                                   const VT *__restrict__ v33, double *__restrict__ P1,
+                                                   double *__restrict__ P2)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nk) return;
+    int r = blockIdx.y;                  // (l1, l2, l3)
+    const int l3 = r % nb;
+    r /= nb;
+    const int l2 = r % nb, l1 = r / nb;
+    const int s = blockIdx.z;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2, c0, c1, c2;
+    m.coords(k1, a0, a1, a2);
+    m.coords(w, c0, c1, c2);
+    const int gam = m.gamma();
+    const double omega1 = freq[(size_t)k1 * nb + l1];
+    const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
+    const double norm = 1.0 / (sqrt(2 * alt3::PI) * sigma);
+    const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+    const double f2 = freq[(size_t)w * nb + l2];
+    {   // coalescence: -w(q2,l2) + w(q1+q2,l3), F at q2
+        const int p = m.sum(a0, a1, a2, c0, c1, c2);
+        const double x = omega1 - (-f2 + freq[(size_t)p * nb + l3]);
+        double F = (double)v33[base + w] * tpi2 / nk;
+        if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) F = 0.0;
+        P1[base + w] = F * (exp(-x * x / 2 / sigma / sigma) * norm);
+    }
+    {   // decay: w(q2,l2) + w(q1-q2,l3), F at -q2
+        const int p = m.diff(a0, a1, a2, c0, c1, c2);
+        const double x = omega1 - (f2 + freq[(size_t)p * nb + l3]);
+        double F = (double)v33[base + m.neg(c0, c1, c2)] * tpi2 / nk;
+        if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) F = 0.0;
+        P2[base + w] = F * (exp(-x * x / 2 / sigma / sigma) * norm);
+    }
+}
+
 }  // namespace
 
 namespace alt3 {
+
+int fgr_gauss_run(alt3_ctx *ctx, double sigma)
+{
+    if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr_gauss needs alt3_v3sq_ibz first");
+    if (!(sigma > 0)) return fail(ALT3_E_ARG, "Gaussian width must be positive");
+    const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    const size_t nel = (size_t)ksub * nb * nb * nb * nk;
+    ALT3_TRY(ctx->d_p1.alloc(nel));
+    ALT3_TRY(ctx->d_p2.alloc(nel));
+    const MeshDev md = MeshDev::make(ctx->nc);
+    dim3 grid((unsigned)((nk + 255) / 256), (unsigned)(nb * nb * nb), (unsigned)ksub);
+    if (ctx->v3_prec == ALT3_V3_MIXED)
+        hipLaunchKernelGGL(k_fgr_gauss<float>, grid, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, sigma,
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
+    else
+        hipLaunchKernelGGL(k_fgr_gauss<double>, grid, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, sigma,
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_fgr = true;
+    ctx->have_tau = ctx->have_cm = false;
+    return ALT3_OK;
+}
 
 int fgr_run(alt3_ctx *ctx)
 {

@@ -19,6 +19,8 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libalt3.so")
 
 V3_FP64_REF, V3_FP64, V3_MIXED = 0, 1, 2
+DELTA_TETRA, DELTA_GAUSS = 0, 1
+THZ_TO_CM = 33.35640951981521  # 1 THz in cm^-1
 
 ERRORS = {-1: "ALT3_E_ARG", -2: "ALT3_E_HIP", -3: "ALT3_E_MESH", -4: "ALT3_E_STATE", -5: "ALT3_E_DEN0",
           -6: "ALT3_E_NOMEM", -7: "ALT3_E_NBE"}
@@ -42,7 +44,8 @@ class System(C.Structure):
 
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
-           "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms"]
+           "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
+           "alt3_set_delta"]
 
 _lib = None
 
@@ -184,6 +187,18 @@ class Context:
 
     def _ksub(self):
         return getattr(self, "_last_ksub", self.nibz)
+
+    def fgr_gauss(self, sigma: float, want: bool = True):
+        if not want:
+            _check(lib().alt3_fgr_gauss(self.h, C.c_double(sigma), None, None))
+            return None
+        ksub = self._ksub()
+        P1 = np.zeros((ksub, self.nk, self.nb, self.nb, self.nb), order="F"); P2 = np.zeros_like(P1)
+        _check(lib().alt3_fgr_gauss(self.h, C.c_double(sigma), _p(P1), _p(P2)))
+        return P1, P2
+
+    def set_delta(self, kind: int, sigma: float = 0.0):
+        _check(lib().alt3_set_delta(self.h, kind, C.c_double(sigma)))
 
     def tau(self, tempk: float):
         ksub = self._ksub()

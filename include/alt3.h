@@ -31,6 +31,10 @@ extern "C" {
 #define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
 #define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA, FP32 contractions, FP64 scale */
 
+/* delta-function integration of the scattering rates */
+#define ALT3_DELTA_TETRA 0   /* linear tetrahedra, weight_tet (tetrahedron.f90:188-384) */
+#define ALT3_DELTA_GAUSS 1   /* Gaussian delta_g (extratools.f90:239-246) at the mesh points */
+
 typedef struct alt3_ctx alt3_ctx;
 
 /* Crystal, force constants and mesh: the module state THERMACOND holds after
@@ -101,6 +105,16 @@ int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *
  * (same ibz range).  P1sm/P2sm optional, Fortran (ksub,nk,nb,nb,nb). */
 int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm);
 
+/* Gaussian-smeared variant of alt3_fgr: the same processes, corner energies and acoustic
+ * masks, with delta(w1 - e) replaced by delta_g(w1 - e, sigma) = exp(-x^2/2 sigma^2) /
+ * (sqrt(2 pi) sigma) (extratools.f90:239-246) evaluated at each mesh q2; sigma in cm^-1
+ * (0.1 THz = 3.33564 cm^-1).  The reference's job 1 has no caller of this form (its
+ * Gaussian rates, cubic_rates rest.f90:1698-1775, are unreachable from kappa.f90).
+ * P1sm/P2sm optional, Fortran (ksub,nk,nb,nb,nb). */
+int alt3_fgr_gauss(alt3_ctx *ctx, double sigma, double *P1sm, double *P2sm);
+/* Delta integration used by alt3_solve: ALT3_DELTA_TETRA (default) or ALT3_DELTA_GAUSS with
+ * width sigma (cm^-1). */
+int alt3_set_delta(alt3_ctx *ctx, int kind, double sigma);
 /* Occupations and scattering rates at tempk: distribution (sub.f90:1955-1980),
  * calculate_tauRTA_mpi2 (:7953-8194), Qandtau (:8198-8305).  Outputs Fortran (ksub,nb):
  * Q, QN, QU, tauinv_N, tauinv_U, tauinv_tot (1/s).  Any may be NULL. */

@@ -787,6 +787,71 @@ int or_fgr(int ibz_lo, int ibz_hi, const int *mapinv, int nk, const double *kp, 
     return 0;
 }
 
+/* Gaussian-smeared form of or_fgr: delta_g(x, sigma) = exp(-x^2/2/sigma^2)/sqrt(2 pi)/sigma
+ * (extratools.f90:239-246) at each mesh point instead of the tetrahedron weights; same
+ * processes, energies and acoustic masks as calculate_FGR_mpi2 (sub.f90:7199-7499). */
+double or_delta_g(double x, double w) { return exp(-x * x / 2.0 / w / w) / sqrt(2 * PI) / w; }
+
+int or_fgr_gauss(int ibz_lo, int ibz_hi, const int *mapinv, int nk, const double *kp, const int *nc,
+                 const double *g, int nb, const double *freq, double sigma, const double *v33,
+                 double *P1, double *P2)
+{
+    int ksub = ibz_hi - ibz_lo + 1;
+    int *ijk2 = malloc(sizeof(int) * 3 * nk), *nneg = malloc(sizeof(int) * nk), *ijkn = malloc(sizeof(int) * 3 * nk);
+    for (int j = 0; j < nk; j++) {
+        int nn;
+        or_get_k_info(kp + 3 * j, nc, g, &nn, ijk2 + 3 * j);
+        double m[3] = {-kp[3 * j], -kp[3 * j + 1], -kp[3 * j + 2]};
+        or_get_k_info(m, nc, g, &nneg[j], ijkn + 3 * j);
+    }
+    const double tpi2 = (2 * PI) * (2 * PI);
+    int h1 = nc[0] / 2 + 1, h2 = nc[1] / 2 + 1, h3 = nc[2] / 2 + 1;
+#define ISG(p) ((p)[0] == h1 && (p)[1] == h2 && (p)[2] == h3)
+#pragma omp parallel
+    {
+        int *np = malloc(sizeof(int) * nk), *ijkp = malloc(sizeof(int) * 3 * nk), *nm = malloc(sizeof(int) * nk);
+        double *F = malloc(sizeof(double) * nk);
+#pragma omp for collapse(2) schedule(dynamic)
+        for (int s = 0; s < ksub; s++)
+            for (int ii = 0; ii < nb; ii++) {
+                int i = mapinv[ibz_lo + s];
+                const double *q1 = kp + 3 * i;
+                for (int j = 0; j < nk; j++) {
+                    double a[3] = {q1[0] + kp[3 * j], q1[1] + kp[3 * j + 1], q1[2] + kp[3 * j + 2]};
+                    or_get_k_info(a, nc, g, &np[j], ijkp + 3 * j);
+                    double b[3] = {q1[0] - kp[3 * j], q1[1] - kp[3 * j + 1], q1[2] - kp[3 * j + 2]};
+                    or_get_k_info(b, nc, g, &nm[j], NULL);
+                }
+                double omega1 = freq[i + (size_t)nk * ii];
+                for (int jj = 0; jj < nb; jj++)
+                    for (int kk = 0; kk < nb; kk++) {
+                        size_t base = (size_t)s + (size_t)ksub * ((size_t)nk * (ii + (size_t)nb * (jj + (size_t)nb * kk)));
+                        for (int j = 0; j < nk; j++) {   /* coalescence */
+                            double v = v33[base + (size_t)ksub * j] * tpi2 / nk;
+                            if (jj < 3 && ISG(ijk2 + 3 * j)) v = 0.0;
+                            if (kk < 3 && ISG(ijkp + 3 * j)) v = 0.0;
+                            double e = -freq[j + (size_t)nk * jj] + freq[np[j] + (size_t)nk * kk];
+                            P1[base + (size_t)ksub * j] = v * or_delta_g(omega1 - e, sigma);
+                        }
+                        for (int j = 0; j < nk; j++) {   /* decay, F at -q */
+                            double v = v33[base + (size_t)ksub * j] * tpi2 / nk;
+                            if (jj < 3 && ISG(ijkn + 3 * j)) v = 0.0;
+                            if (kk < 3 && ISG(ijkp + 3 * j)) v = 0.0;
+                            F[nneg[j]] = v;
+                        }
+                        for (int j = 0; j < nk; j++) {
+                            double e = freq[j + (size_t)nk * jj] + freq[nm[j] + (size_t)nk * kk];
+                            P2[base + (size_t)ksub * j] = F[j] * or_delta_g(omega1 - e, sigma);
+                        }
+                    }
+            }
+        free(np); free(ijkp); free(nm); free(F);
+    }
+#undef ISG
+    free(ijk2); free(nneg); free(ijkn);
+    return 0;
+}
+
 /* ---------------------------------------------------------------------------- */
 /* occupations, Q and tau (sub.f90:1955-1980, 7953-8194, 8198-8305; extratools:257-288) */
 /* ---------------------------------------------------------------------------- */

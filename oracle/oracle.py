@@ -46,6 +46,8 @@ def lib():
         _lib.or_nbe.argtypes = [C.c_double, C.c_double, C.c_int]
         _lib.or_temp_cm.restype = C.c_double
         _lib.or_temp_cm.argtypes = [C.c_double]
+        _lib.or_delta_g.restype = C.c_double
+        _lib.or_delta_g.argtypes = [C.c_double, C.c_double]
     return _lib
 
 
@@ -242,6 +244,18 @@ def fgr(ibz_lo, ibz_hi, mapinv, kp, nc, g, freq, tet, v33):
                  _p(np.ascontiguousarray(g)), nb, _p(_f64(freq)), _p(np.ascontiguousarray(tet, dtype=np.int32)),
                  _p(_f64(v33)), _p(P1), _p(P2))
     return P1, P2
+
+
+def fgr_gauss(ibz_lo, ibz_hi, mapinv, kp, nc, g, freq, sigma, v33):
+    nk = len(kp); nb = freq.shape[1]; ksub = ibz_hi - ibz_lo + 1
+    P1 = np.zeros((ksub, nk, nb, nb, nb), order="F"); P2 = np.zeros_like(P1)
+    lib().or_fgr_gauss(ibz_lo, ibz_hi, _p(_i32(mapinv)), nk, _p(np.ascontiguousarray(kp)), _p(_i32(np.asarray(nc))),
+                       _p(np.ascontiguousarray(g)), nb, _p(_f64(freq)), C.c_double(sigma), _p(_f64(v33)), _p(P1), _p(P2))
+    return P1, P2
+
+
+def delta_g(x, w):
+    return lib().or_delta_g(x, w)
 
 
 def weight_tet(w4, e):

@@ -226,3 +226,51 @@ def test_solve_temperature_groups(si_inputs, ref4):
         check(kap, many["kappa_rta"][2], 1e-13, "stagewise kappa_rta")
         with pytest.raises(capi.Alt3Error):
             c.kappa_rta(200.0)   # not held by the last tau pass
+
+
+def test_fgr_gauss_vs_oracle_4(ctx4, ref4, si_inputs):
+    """alt3_fgr_gauss against the oracle's Gaussian form on the same |V3|^2 (FP64 both, 1e-10)."""
+    print("case: Gaussian FGR 4^3", flush=True)
+    _load_ref_eig(ctx4, ref4)
+    v = ctx4.v3sq(0, ctx4.nibz - 1, capi.V3_FP64)
+    sigma = 0.1 * capi.THZ_TO_CM
+    P1, P2 = ctx4.fgr_gauss(sigma)
+    inp = si_inputs[4]
+    nc = np.array(inp.params.nc)
+    g, _ = O.reciprocal(inp.r123)
+    kp, _ = O.make_kp_reg_tet(nc, g)
+    mapinv = (ref4["mapinv"] - 1).astype(np.int32)
+    freq = O.freq_from_eigenval(ref4["eigenval"])
+    O1, O2 = O.fgr_gauss(0, ctx4.nibz - 1, mapinv, kp, nc, g, freq, sigma, v)
+    check(P1, O1, 1e-10, "gauss P1sm")
+    check(P2, O2, 1e-10, "gauss P2sm")
+
+
+def test_gauss_rta_kappa_8(si_inputs, ref8):
+    """Gaussian-smeared RTA kappa on SW-Si 8^3 (sigma = 0.1 THz, T = 300 K) with the reference
+    eigen data: alt3_solve with ALT3_DELTA_GAUSS equals the oracle chain (tau, F_RTA,
+    mode_kappa) on the device's P to 1e-8, and lies within 25 % of the tetrahedron result."""
+    print("case: Gaussian RTA kappa 8^3", flush=True)
+    inp = si_inputs[8]
+    sigma, T = 0.1 * capi.THZ_TO_CM, 300.0
+    with capi.Context(inp) as c:
+        c.set_eig(ref8["eigenval"], ref8["eigenvec"], ref8["veloc"])
+        c.set_delta(capi.DELTA_GAUSS, sigma)
+        out = c.solve([T], capi.V3_FP64)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        P1, P2 = c.fgr_gauss(sigma)
+    nc = np.array(inp.params.nc)
+    g, vol = O.reciprocal(inp.r123)
+    opk = O.lattice_ops(inp.latparams, inp.primlatt)
+    kp, _ = O.make_kp_reg_tet(nc, g)
+    mapinv = (ref8["mapinv"] - 1).astype(np.int32)
+    kibz = ref8["kibz"].T.copy()
+    E, V = ref8["eigenval"], ref8["veloc"]
+    freq, dist = O.distribution(E, T)
+    _, _, Q, QN, QU = O.tau(0, len(mapinv) - 1, mapinv, kp, nc, g, dist, P1, P2, want_p=False)
+    F = O.frta(mapinv, freq, dist, V, Q, T)
+    kap, _ = O.mode_kappa(kibz, mapinv, kp, nc, g, inp.primlatt, opk, E, V, F, T, vol)
+    check(out["kappa_rta"][0], kap, 1e-8, "Gaussian kappa_RTA vs oracle chain")
+    kg = np.trace(out["kappa_rta"][0].sum(0)) / 3
+    kt = np.trace(ref8["T300_kappaRTA"].sum(0)) / 3
+    assert 0.75 < kg / kt < 1.25, (kg, kt)

@@ -151,6 +151,41 @@ def test_fgr_vs_reference_8_subset(si_inputs, ref8):
         assert np.array_equal(P1[0], ref8["P1sm"][j]) and np.array_equal(P2[0], ref8["P2sm"][j])
 
 
+def test_delta_g_closed_form():
+    """delta_g (extratools.f90:239-246) is a normalised Gaussian of variance w^2."""
+    for w in (0.5, 3.3356, 10.0):
+        x = np.linspace(-12 * w, 12 * w, 200001)
+        y = np.array([O.delta_g(float(v), w) for v in x[::50]])
+        xs = x[::50]
+        assert abs(np.trapezoid(y, xs) - 1) < 1e-8
+        assert abs(np.trapezoid(y * xs * xs, xs) / w ** 2 - 1) < 1e-6
+        assert abs(O.delta_g(0.0, w) - 1 / (np.sqrt(2 * np.pi) * w)) < 1e-15 / w
+
+
+def test_fgr_gauss_vs_tetrahedra(si_inputs, ref8):
+    """With |V3|^2 = 1 the summed P1sm/P2sm are two-phonon densities of states at omega(q1,l1)
+    (times (2 pi)^2); the Gaussian form (sigma = 5 cm^-1) and the tetrahedron form must agree
+    to within the mesh error (20 % at 8^3 for general q1; a wrong normalisation -- a missing
+    1/nk, 2pi or 1/sqrt(2pi) -- is off by factors of 2 or more)."""
+    inp = si_inputs[8]
+    nc = np.array(inp.params.nc)
+    g, _ = O.reciprocal(inp.r123)
+    kp, tet = O.make_kp_reg_tet(nc, g)
+    mapinv = (ref8["mapinv"] - 1).astype(np.int32)
+    freq = O.freq_from_eigenval(ref8["eigenval"])
+    nk, nb = freq.shape
+    v33 = np.ones((1, nk, nb, nb, nb), order="F")
+    for s in (1, 3):
+        T1, T2 = O.fgr(s, s, mapinv, kp, nc, g, freq, tet, v33)
+        G1, G2 = O.fgr_gauss(s, s, mapinv, kp, nc, g, freq, 5.0, v33)
+        for t, gg in ((T1, G1), (T2, G2)):
+            r = gg.sum() / t.sum()
+            assert 0.8 < r < 1.2, (s, r)
+        # acoustic branches l2 < 3 at q2 = Gamma carry no weight (sub.f90:7349-7356)
+        gam = int(np.argmin(np.abs(kp).sum(axis=1)))
+        assert np.all(G1[0, gam, :, :3, :] == 0) and np.all(G2[0, gam, :, :3, :] == 0)
+
+
 # ------------------------------------------------------------------ Q, tau, kappa, CM, CG
 def test_tau_kappa_cm_cg_vs_reference(geo4, ref4):
     G = geo4
