@@ -152,6 +152,8 @@ struct alt3_ctx {
     // FGR
     int delta_kind = ALT3_DELTA_TETRA;             // used by alt3_solve
     double delta_sigma = 0;                        // Gaussian width, cm^-1
+    int fgr_kind = ALT3_DELTA_TETRA;               // what d_p1/d_p2 hold
+    double fgr_sigma = 0;
     alt3::DBuf<double> d_p1, d_p2;                 // [ksub][nb][nb][nb][nk]
     bool have_fgr = false;
 

@@ -150,6 +150,31 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     return ALT3_OK;
 }
 
+int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *bytes)
+{
+    if (!ctx || !bytes || ntemps < 1) return fail(ALT3_E_ARG, "bad argument");
+    if (precision < ALT3_V3_FP64_REF || precision > ALT3_V3_MIXED) return fail(ALT3_E_ARG, "unknown precision");
+    const long long nk = ctx->nk, nb = ctx->nb, ni = ctx->nibz, ndr = ctx->V.ndr, nt = std::min(ntemps, 4);
+    const long long nel = ni * nb * nb * nb * nk, n = ni * nb * 3;
+    long long b = 0;
+    b += nk * nb * nb * 16 + nk * nb * 8 * 3 + nk * nb * 3 * 8 + nk * 3 * 8 * 2 + nk * 4 * 2;   // eigen data, meshes
+    b += (long long)ctx->nt2h * 40 + (long long)ctx->V.nterm * 80 + nk * ndr * 16;             // FC tables, phases
+    b += ni * ndr * nb * nb * nb * 16;                                                          // G'
+    if (precision == ALT3_V3_MIXED) {
+        const long long Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb * nb + 15) / 16) * 16;
+        b += nel * 4 + ni * nb * Kp * Np * 4 + nk * nb * nb * 8 + nk * ndr * 8;
+    } else {
+        b += nel * 8;
+    }
+    b += 2 * nel * 8;                                                                           // P1sm, P2sm
+    b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * 8 + 3 * ni * nb * nb * 8);  // tau pass
+    b += n * n * 8 + n * 8 * 8 + ni * nb * 9 * 8 * 2;                                           // CM, RHS, CG, kappa
+    b += (long long)ni * 48 * 4 * 7 + 9 * 48 * 8 * 2;                                           // star tables
+    b += 16ll << 20;                                                                            // small buffers
+    *bytes = b;
+    return ALT3_OK;
+}
+
 int alt3_sizes(const alt3_ctx *ctx, int *nk, int *nibz, int *nb, int *nops)
 {
     if (!ctx) return fail(ALT3_E_ARG, "null context");
@@ -367,10 +392,16 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         return rc;
     };
     if (!ctx->have_eig) ALT3_TRY(stage(0, [&] { return eig_run(ctx); }));
-    ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
-    ALT3_TRY(stage(2, [&] {
-        return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
-    }));
+    // |V3|^2 and the scattering weights do not depend on temperature: a context that still
+    // holds them for the whole IBZ at this precision and delta integration reuses them
+    const bool v3_held = ctx->have_v3 && ctx->v3_prec == precision && ctx->ibz_lo == 0 && ctx->ibz_hi == nibz - 1;
+    if (!v3_held) ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
+    const bool fgr_held = v3_held && ctx->have_fgr && ctx->fgr_kind == ctx->delta_kind &&
+                          (ctx->delta_kind == ALT3_DELTA_TETRA || ctx->fgr_sigma == ctx->delta_sigma);
+    if (!fgr_held)
+        ALT3_TRY(stage(2, [&] {
+            return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
+        }));
     DBuf<double> dx, dte;
     ALT3_TRY(dx.alloc(n));
     ALT3_TRY(dte.alloc((size_t)nibz * nb));

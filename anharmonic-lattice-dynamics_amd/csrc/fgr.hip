@@ -390,6 +390,8 @@ int fgr_gauss_run(alt3_ctx *ctx, double sigma)
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;
+    ctx->fgr_kind = ALT3_DELTA_GAUSS;
+    ctx->fgr_sigma = sigma;
     ctx->have_tau = ctx->have_cm = false;
     return ALT3_OK;
 }
@@ -431,6 +433,7 @@ int fgr_run(alt3_ctx *ctx)
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;
+    ctx->fgr_kind = ALT3_DELTA_TETRA;
     ctx->have_tau = ctx->have_cm = false;
     return ALT3_OK;
 }

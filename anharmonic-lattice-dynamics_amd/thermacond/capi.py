@@ -45,7 +45,7 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta"]
+           "alt3_set_delta", "alt3_footprint"]
 
 _lib = None
 
@@ -250,6 +250,11 @@ class Context:
         ms = C.c_double()
         _check(lib().alt3_last_kernel_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def footprint(self, ntemps: int, precision: int = V3_FP64) -> int:
+        b = C.c_longlong()
+        _check(lib().alt3_footprint(self.h, ntemps, precision, C.byref(b)))
+        return b.value
 
     def device_bytes(self) -> int:
         b = C.c_longlong()

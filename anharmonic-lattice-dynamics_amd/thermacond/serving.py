@@ -1,0 +1,273 @@
+"""A device-resident kappa-solve server with continuous batching and HBM-footprint admission.
+
+Requests (a crystal with its force constants, a q-mesh, temperatures, V3 precision, delta
+integration) arrive at any time.  The scheduler admits them in arrival order as soon as a
+worker slot is free and their device footprint (alt3_footprint, computed on a host-only
+context, so before any device memory is touched) fits in what the budget has left; a
+request that does not fit waits while later, smaller ones may pass it (first fit), but no
+request is passed more than `max_bypass` times.  Admitted requests run concurrently, one
+host thread per slot, each on its own context and HIP stream, so their kernels overlap on
+the device.
+
+Contexts are kept device-resident across requests: a finished request leaves its context
+(force constants, eigenvectors, |V3|^2 and scattering weights) in a cache keyed by crystal
+and mesh, and a later request with the same key skips the eigen solve, V3 and FGR
+(alt3_solve reuses what the context holds) and pays only the temperature-dependent part.
+Cached contexts count against the budget and are evicted, least recently used first, when
+an admission needs their memory.  Requests with the same key are serialised on that
+context.
+
+Every result is a pure function of its request: a context computes the same kappa whether
+it runs alone or beside others (tests/test_serving.py checks this on the device).
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import itertools
+import threading
+import time
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import capi
+from .refio import InputSet
+
+
+@dataclasses.dataclass
+class Request:
+    key: str                          # identifies crystal + force constants (+ mesh below)
+    inputs: InputSet
+    temps: Sequence[float]
+    nc: Optional[Sequence[int]] = None
+    precision: int = capi.V3_FP64
+    delta: int = capi.DELTA_TETRA
+    sigma: float = 0.0
+    # filled by the server
+    id: int = -1
+    t_arrive: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    footprint: int = 0
+    reused: bool = False
+    result: Optional[dict] = None
+    error: Optional[str] = None
+
+    @property
+    def mesh_key(self) -> str:
+        nc = tuple(self.inputs.params.nc if self.nc is None else self.nc)
+        return f"{self.key}|{nc[0]}x{nc[1]}x{nc[2]}"
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_arrive
+
+
+class _Cached:
+    def __init__(self, ctx: capi.Context, reserve: int):
+        self.ctx = ctx
+        self.reserve = reserve          # bytes counted against the budget while cached
+        self.lock = threading.Lock()
+        self.busy = 0
+        self.last_used = time.monotonic()
+
+
+def device_free_bytes() -> int:
+    import torch
+    free, _ = torch.cuda.mem_get_info()
+    return int(free)
+
+
+class KappaServer:
+    """Continuous-batching scheduler over alt3 contexts (see the module docstring)."""
+
+    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, max_bypass: int = 8,
+                 cache: bool = True):
+        capi.lib()                      # load the library once, before any worker thread
+        self.slots = slots
+        self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes())
+        self.max_bypass = max_bypass
+        self.use_cache = cache
+        self._pending: collections.deque = collections.deque()
+        self._bypassed: Dict[int, int] = {}
+        self._cache: "collections.OrderedDict[str, _Cached]" = collections.OrderedDict()
+        self._in_use = 0                # bytes reserved by running requests and cached contexts
+        self._running = 0
+        self._cv = threading.Condition()
+        self._ids = itertools.count()
+        self._done: List[Request] = []
+        self._stop = False
+        self._threads: List[threading.Thread] = []
+        self._sched = threading.Thread(target=self._schedule, daemon=True)
+        self._sched.start()
+        self.max_concurrent = 0
+
+    # ------------------------------------------------------------------ API
+    def submit(self, req: Request) -> Request:
+        req.id = next(self._ids)
+        req.t_arrive = time.perf_counter()
+        req.footprint = self._estimate(req)
+        with self._cv:
+            if req.footprint > self.budget:
+                req.error = f"footprint {req.footprint} B exceeds the HBM budget {self.budget} B"
+                req.t_done = time.perf_counter()
+                self._done.append(req)
+                self._cv.notify_all()
+                return req
+            self._pending.append(req)
+            self._cv.notify_all()
+        return req
+
+    def wait(self, reqs: Sequence[Request], timeout: Optional[float] = None) -> None:
+        t_end = None if timeout is None else time.monotonic() + timeout
+        with self._cv:
+            while any(r.t_done == 0.0 for r in reqs):
+                left = None if t_end is None else t_end - time.monotonic()
+                if left is not None and left <= 0:
+                    raise TimeoutError("requests still running")
+                self._cv.wait(timeout=left if left is not None else 1.0)
+
+    def close(self) -> None:
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        self._sched.join()
+        for t in self._threads:
+            t.join()
+        for c in self._cache.values():
+            c.ctx.close()
+        self._cache.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ internals
+    def _estimate(self, req: Request) -> int:
+        with capi.Context(req.inputs, nc=req.nc, host_only=True) as h:
+            return h.footprint(len(req.temps), req.precision)
+
+    def _evict_for(self, need: int) -> bool:
+        """Drop idle cached contexts (least recently used first) until `need` bytes fit."""
+        while self.budget - self._in_use < need:
+            victim = next((k for k, c in self._cache.items() if c.busy == 0), None)
+            if victim is None:
+                return False
+            c = self._cache.pop(victim)
+            c.ctx.close()
+            self._in_use -= c.reserve
+        return True
+
+    def _try_admit(self, req: Request) -> bool:
+        cached = self._cache.get(req.mesh_key) if self.use_cache else None
+        if cached is not None:
+            extra = max(0, req.footprint - cached.reserve)
+            if self.budget - self._in_use < extra and not self._evict_for(extra):
+                return False
+            cached.reserve += extra
+            self._in_use += extra
+            cached.busy += 1
+            self._cache.move_to_end(req.mesh_key)
+            return True
+        if self.budget - self._in_use < req.footprint and not self._evict_for(req.footprint):
+            return False
+        self._in_use += req.footprint
+        return True
+
+    def _schedule(self) -> None:
+        with self._cv:
+            while True:
+                if self._stop and not self._pending:
+                    return
+                admitted = None
+                if self._running < self.slots:
+                    for i, req in enumerate(self._pending):
+                        # a request passed over too often blocks everything behind it
+                        if self._try_admit(req):
+                            admitted = req
+                            del self._pending[i]
+                            for earlier in itertools.islice(self._pending, 0, i):
+                                self._bypassed[earlier.id] = self._bypassed.get(earlier.id, 0) + 1
+                            break
+                        if self._bypassed.get(req.id, 0) >= self.max_bypass:
+                            break
+                if admitted is None:
+                    self._cv.wait(timeout=0.5)
+                    continue
+                admitted.t_admit = time.perf_counter()
+                self._running += 1
+                self.max_concurrent = max(self.max_concurrent, self._running)
+                t = threading.Thread(target=self._work, args=(admitted,), daemon=True)
+                self._threads.append(t)
+                t.start()
+
+    def _work(self, req: Request) -> None:
+        ctx_entry: Optional[_Cached] = None
+        own_ctx: Optional[capi.Context] = None
+        try:
+            with self._cv:
+                ctx_entry = self._cache.get(req.mesh_key) if self.use_cache else None
+            if ctx_entry is None:
+                own_ctx = capi.Context(req.inputs, nc=req.nc)
+                ctx, lock = own_ctx, threading.Lock()
+            else:
+                ctx, lock = ctx_entry.ctx, ctx_entry.lock
+                req.reused = True
+            with lock:
+                ctx.set_delta(req.delta, req.sigma)
+                out = ctx.solve([float(t) for t in req.temps], req.precision)
+            req.result = dict(kappa_rta=out["kappa_rta"], kappa_bte=out["kappa_bte"], cg_iters=out["cg_iters"],
+                              stage_ms=out["stage_ms"])
+        except Exception as e:  # the request fails, the server keeps serving
+            req.error = f"{type(e).__name__}: {e}"
+        finally:
+            with self._cv:
+                if ctx_entry is not None:
+                    ctx_entry.busy -= 1
+                    ctx_entry.last_used = time.monotonic()
+                elif own_ctx is not None:
+                    if self.use_cache and req.error is None and req.mesh_key not in self._cache:
+                        self._cache[req.mesh_key] = _Cached(own_ctx, req.footprint)
+                    else:
+                        own_ctx.close()
+                        self._in_use -= req.footprint
+                else:
+                    self._in_use -= req.footprint
+                self._running -= 1
+                req.t_done = time.perf_counter()
+                self._done.append(req)
+                self._cv.notify_all()
+
+
+def latency_summary(reqs: Sequence[Request]) -> dict:
+    ok = [r for r in reqs if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    span = (max(r.t_done for r in reqs) - min(r.t_arrive for r in reqs)) if reqs else 0.0
+    return {"requests": len(reqs), "completed": len(ok), "failed": len(reqs) - len(ok),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "span_s": span,
+            "solves_per_hour": len(ok) / span * 3600 if span > 0 else 0.0,
+            "reused_contexts": sum(1 for r in ok if r.reused)}
+
+
+def poisson_arrivals(n: int, rate_per_s: float, seed: int) -> np.ndarray:
+    """Arrival times (s from the start) of a Poisson process."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_per_s, size=n))
+
+
+def replay(server: KappaServer, reqs: Sequence[Request], arrivals: Sequence[float],
+           on_submit: Optional[Callable[[Request], None]] = None) -> None:
+    """Submit each request at its arrival time (relative to now) and wait for all."""
+    t0 = time.perf_counter()
+    for req, ta in zip(reqs, arrivals):
+        dt = t0 + ta - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        server.submit(req)
+        if on_submit:
+            on_submit(req)
+    server.wait(reqs)

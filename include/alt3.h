@@ -67,6 +67,11 @@ int alt3_destroy(alt3_ctx *ctx);
 const char *alt3_last_error(void);
 int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes);
 
+/* Device bytes alt3_solve needs on this context for ntemps temperatures at the given V3
+ * precision (an upper bound of the library's allocations; answers on host-only contexts,
+ * so that a scheduler can admit work by HBM footprint before creating a device context).
+ * No counterpart in the reference, which sizes its arrays per MPI rank. */
+int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *bytes);
 /* Sizes: nk = nc1*nc2*nc3, nibz, nb = 3*natoms0, nops (lattice point group). */
 int alt3_sizes(const alt3_ctx *ctx, int *nk, int *nibz, int *nb, int *nops);
 
@@ -144,7 +149,8 @@ int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, i
 int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, double *taueff);
 
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
- * IBZ, FGR, then per T tau (one pass over P1sm/P2sm per group of four temperatures),
+ * IBZ and FGR (unless the context still holds them from an earlier call with the same
+ * precision and delta integration; alt3_eig/alt3_set_eig clear them), then per T tau (one pass over P1sm/P2sm per group of four temperatures),
  * kappa_RTA, collision, CG (x0 = F_RTA, 100 iterations, 2 digits), kappa_BTE.  kappa_rta/kappa_bte: [ntemps][nb*9] Fortran (nb,3,3) per T.
  * stage_ms (optional, 8 doubles): eig, v3, fgr, tau, kappa_rta, collision, cg, kappa_bte. */
 int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision,

@@ -1,0 +1,87 @@
+"""The kappa-solve server (thermacond/serving.py): footprint admission and continuous batching.
+
+CPU: footprints come from host-only contexts and grow with the mesh, the branches and the
+number of temperatures; a request larger than the budget is refused without touching the device.
+GPU: a mixed stream of uneven requests (meshes 4^3-8^3, one to three temperatures, FP64 and
+mixed V3, tetrahedron and Gaussian delta) run concurrently through the server gives, for
+every request, the kappa the same request gives alone on a fresh context (1e-9 relative;
+the FGR's LDS atomics reorder sums at the 1e-16 level), the device held more than one
+request at a time, and requests with a cached context reused it.
+"""
+import numpy as np
+import pytest
+
+from thermacond import capi, serving
+
+
+def test_footprint_grows_with_work(si_inputs):
+    sizes = {}
+    for n in (4, 8, 16):
+        with capi.Context(si_inputs[n], host_only=True) as h:
+            sizes[n] = (h.footprint(1, capi.V3_FP64), h.footprint(3, capi.V3_FP64), h.footprint(1, capi.V3_MIXED))
+    assert sizes[4][0] < sizes[8][0] < sizes[16][0]
+    for n in sizes:
+        assert sizes[n][1] > sizes[n][0]
+    assert sizes[16][2] < sizes[16][0]          # FP32 |V3|^2 halves the largest V3 array
+    # 16^3 SW-Si: P1sm+P2sm alone are 3.9 GB
+    assert sizes[16][0] > 2 * 244187136 * 8
+
+
+def test_oversized_request_is_refused_on_the_host(si_inputs, monkeypatch):
+    monkeypatch.setattr(serving, "device_free_bytes", lambda: 1 << 20)
+    srv = serving.KappaServer.__new__(serving.KappaServer)
+    # only the admission path: no scheduler thread, no device
+    srv.budget = 1 << 20
+    srv._cv = serving.threading.Condition()
+    srv._done, srv._pending = [], serving.collections.deque()
+    srv._ids = serving.itertools.count()
+    req = srv.submit(serving.Request("si", si_inputs[8], [300.0]))
+    assert req.error is not None and "exceeds the HBM budget" in req.error and req.t_done > 0
+
+
+def _mix(si_inputs):
+    S = si_inputs
+    g = 0.1 * capi.THZ_TO_CM
+    return [serving.Request("si", S[4], [300.0]),
+            serving.Request("si", S[8], [100.0, 800.0]),
+            serving.Request("si", S[4], [500.0], precision=capi.V3_MIXED),
+            serving.Request("si", S[4], [200.0, 300.0, 400.0]),
+            serving.Request("si", S[8], [300.0], delta=capi.DELTA_GAUSS, sigma=g),
+            serving.Request("si", S[4], [300.0], nc=(6, 6, 6)),
+            serving.Request("si", S[8], [250.0]),
+            serving.Request("si", S[4], [700.0], nc=(6, 6, 6), precision=capi.V3_MIXED)]
+
+
+@pytest.mark.gpu
+def test_alone_equals_in_the_mix(si_inputs):
+    print("case: serving mix of 8 requests", flush=True)
+    alone = []
+    with serving.KappaServer(slots=1, cache=False) as srv:
+        for r in _mix(si_inputs):
+            srv.submit(r)
+            srv.wait([r])
+            assert r.error is None, r.error
+            alone.append(r)
+    mix = _mix(si_inputs)
+    with serving.KappaServer(slots=4, cache=True) as srv:
+        serving.replay(srv, mix, serving.poisson_arrivals(len(mix), rate_per_s=200.0, seed=5))
+        conc = srv.max_concurrent
+    for a, m in zip(alone, mix):
+        assert m.error is None, m.error
+        for k in ("kappa_rta", "kappa_bte"):
+            d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
+            assert d < 1e-9, (m.mesh_key, m.temps, k, d)
+        assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
+    assert conc > 1
+    assert any(m.reused for m in mix)
+    s = serving.latency_summary(mix)
+    print("serving mix:", s, flush=True)
+    assert s["completed"] == len(mix)
+
+
+@pytest.mark.gpu
+def test_footprint_bounds_device_allocations(si_inputs):
+    with capi.Context(si_inputs[8]) as c:
+        est = c.footprint(3, capi.V3_FP64)
+        c.solve([100.0, 300.0, 800.0], capi.V3_FP64)
+        assert c.device_bytes() <= est
