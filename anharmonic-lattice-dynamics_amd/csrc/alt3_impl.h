@@ -123,6 +123,7 @@ struct alt3_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     long long peak_bytes = 0;
+    size_t lds_max = 65536;                        // LDS per workgroup on this device
 
     // harmonic
     int nt2h = 0;                                 // FC2 terms with a home-cell first atom
@@ -144,7 +145,8 @@ struct alt3_ctx {
     alt3::DBuf<double> d_gp;                       // per q1: [ndr][nb][nb*nb] complex (l1,b,c)
     alt3::DBuf<float> d_gpf;                       // packed FP32 B operand of the mixed path
     alt3::DBuf<int> d_mapinv;
-    int ibz_lo = -1, ibz_hi = -1, v3_prec = -1;
+    int ibz_lo = -1, ibz_hi = -1, v3_prec = -1;    // v3_prec: what d_v33/d_v33f hold
+    int v3_req = -1;                               // precision the caller asked for
     alt3::DBuf<double> d_v33;                      // [ksub][nb][nb][nb][nk] FP64
     alt3::DBuf<float> d_v33f;                      // same, FP32 (mixed path)
     bool have_v3 = false;

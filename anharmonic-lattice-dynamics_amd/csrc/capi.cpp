@@ -90,6 +90,10 @@ int alt3_create(const alt3_system *sys, alt3_ctx **out)
     }
     auto init = [&]() -> int {
         ALT3_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        int dev = 0, lds = 0;
+        ALT3_HIP(hipGetDevice(&dev));
+        ALT3_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (lds > 0) ctx->lds_max = (size_t)lds;
         ALT3_HIP(hipEventCreate(&ctx->ev0));
         ALT3_HIP(hipEventCreate(&ctx->ev1));
         ALT3_TRY(ctx->d_kpc_eig.alloc(3 * ctx->nk));
@@ -394,7 +398,7 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     if (!ctx->have_eig) ALT3_TRY(stage(0, [&] { return eig_run(ctx); }));
     // |V3|^2 and the scattering weights do not depend on temperature: a context that still
     // holds them for the whole IBZ at this precision and delta integration reuses them
-    const bool v3_held = ctx->have_v3 && ctx->v3_prec == precision && ctx->ibz_lo == 0 && ctx->ibz_hi == nibz - 1;
+    const bool v3_held = ctx->have_v3 && ctx->v3_req == precision && ctx->ibz_lo == 0 && ctx->ibz_hi == nibz - 1;
     if (!v3_held) ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
     const bool fgr_held = v3_held && ctx->have_fgr && ctx->fgr_kind == ctx->delta_kind &&
                           (ctx->delta_kind == ALT3_DELTA_TETRA || ctx->fgr_sigma == ctx->delta_sigma);

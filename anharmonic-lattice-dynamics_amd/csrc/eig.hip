@@ -332,6 +332,9 @@ int eig_run(alt3_ctx *ctx)
     ALT3_TRY(ctx->d_evec.alloc((size_t)nk * nb * nb * 2));
     const double dq = (double)0.001f;
     size_t lds = (size_t)2 * nb * nb * sizeof(zd) + 64 * sizeof(double);
+    if (lds > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
+    if (lds > 65536)   // cells with more than 44 branches
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_eig_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_eig_solve, dim3(nk * kDq), dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,
                        ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
     ALT3_HIP(hipGetLastError());

@@ -262,6 +262,7 @@ __global__ void k_to_fortran5(int ksub, int nk, int nb, const T *__restrict__ in
 namespace alt3 {
 
 int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err);  // v3_mixed.hip
+bool v3_mixed_fits(const alt3_ctx *ctx);                 // v3_mixed.hip
 
 int v3_prepare(alt3_ctx *ctx)
 {
@@ -335,7 +336,14 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
     ctx->ibz_lo = ibz_lo;
     ctx->ibz_hi = ibz_hi;
     ctx->v3_prec = prec;
+    ctx->v3_req = prec;
     const double c33 = const33();
+    if (prec == ALT3_V3_MIXED && !v3_mixed_fits(ctx)) {
+        // the 16-point MFMA tile of the mixed path does not fit in LDS for this many
+        // branches: the FP64 path computes |V3|^2 instead (and consumers read FP64)
+        prec = ALT3_V3_FP64;
+        ctx->v3_prec = prec;
+    }
     if (prec == ALT3_V3_MIXED) {
         ctx->d_v33.release();
         ALT3_TRY(ctx->d_v33f.alloc(nel));
@@ -352,13 +360,17 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
         } else {
             const int ndr = ctx->V.ndr, nb2 = nb * nb;
             ALT3_TRY(v3_prep_launch(ctx, ksub));
-            // tile of q2 sized so the three LDS arrays stay within 64 KB
+            // tile of q2 sized so the three LDS arrays stay within 64 KB (two groups per CU);
+            // large cells take one q2 per group and up to the device's LDS per group
+            auto need = [&](int t) { return (size_t)t * (ndr + 2 * nb2) * sizeof(zd); };
             int tq = 64;
-            while (tq > 1 && (size_t)tq * (ndr + 2 * nb2) * sizeof(zd) > 65536) tq /= 2;
-            size_t lds = (size_t)tq * (ndr + 2 * nb2) * sizeof(zd);
-            if (lds > 65536) return fail(ALT3_E_ARG, "V3 tile does not fit in LDS");
+            while (tq > 1 && need(tq) > 65536) tq /= 2;
+            size_t lds = need(tq);
+            if (lds > ctx->lds_max) return fail(ALT3_E_ARG, "V3 tile does not fit in LDS");
             dim3 grid((nk + tq - 1) / tq, nb, ksub);
 #define LAUNCH(T)                                                                                              \
+    if (lds > 65536)                                                                                           \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_main<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     hipLaunchKernelGGL(k_v3_main<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ibz_lo, ndr, ctx->d_mapinv.p, \
                        ctx->d_eph.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr.p)
             switch (tq) {

@@ -156,14 +156,23 @@ __global__ void __launch_bounds__(256) k_v3_mfma(alt3::MeshDims md, int nk, int 
 
 namespace alt3 {
 
+static size_t mixed_lds(const alt3_ctx *ctx, int t)
+{
+    const int nb2 = ctx->nb * ctx->nb, ndr = ctx->V.ndr;
+    const int Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb2 + 15) / 16) * 16;
+    return (size_t)t * (Kp + Np + 2 * nb2) * sizeof(float);
+}
+
+bool v3_mixed_fits(const alt3_ctx *ctx) { return mixed_lds(ctx, 16) <= ctx->lds_max; }
+
 int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err)
 {
     const int nk = ctx->nk, nb = ctx->nb, nb2 = nb * nb, ndr = ctx->V.ndr;
     const int Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb2 + 15) / 16) * 16;
     int tq = 64;
-    auto lds_of = [&](int t) { return (size_t)t * (Kp + Np + 2 * nb2) * sizeof(float); };
+    auto lds_of = [&](int t) { return mixed_lds(ctx, t); };
     while (tq > 16 && lds_of(tq) > 65536) tq /= 2;
-    if (lds_of(tq) > 65536) return fail(ALT3_E_ARG, "ALT3_V3_MIXED: a 16-point q2 tile does not fit in LDS (nb too large)");
+    if (lds_of(tq) > ctx->lds_max) return fail(ALT3_E_ARG, "ALT3_V3_MIXED: a 16-point q2 tile does not fit in LDS (nb too large)");
     MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
     // G' (FP64) then packed FP32 B
     ALT3_TRY(v3_prep_launch(ctx, ksub));
@@ -184,6 +193,8 @@ int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err)
     size_t lds = lds_of(tq);
     const double c33 = const33();
 #define LAUNCH(T)                                                                                                     \
+    if (lds > 65536)                                                                                                  \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_mfma<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     hipLaunchKernelGGL(k_v3_mfma<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ndr, Kp, Np,         \
                        ctx->d_mapinv.p, ephf.p, evf.p, ctx->d_eival.p, ctx->d_gpf.p, c33, ctx->d_v33f.p, d_err)
     switch (tq) {

@@ -23,7 +23,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -122,6 +122,15 @@ def rocksalt_conventional(a: float = 5.6, m1: float = 72.63, m2: float = 78.97,
                    [m1, m2], [n1, n2], np.array(frac, dtype=np.float64))
 
 
+def zincblende_tetragonal4(a: float = 5.65, m1: float = 69.72, m2: float = 74.92,
+                           n1: str = "Ga", n2: str = "As") -> Crystal:
+    """4-atom tetragonal cell of a zincblende-type binary: a1 = (1/2,-1/2,0) a,
+    a2 = (1/2,1/2,0) a, a3 = (0,0,1) a (two primitive cells), 2 + 2 atoms."""
+    prim = np.array([[.5, -.5, 0], [.5, .5, 0], [0, 0, 1]], dtype=np.float64).T
+    frac = np.array([[0, 0, 0], [.5, 0, .5], [.25, .25, .25], [.75, .25, .75]], dtype=np.float64)
+    return Crystal((1, 1, 1, 90, 90, 90), prim, a, [1, 1, 2, 2], [m1, m2], [n1, n2], frac)
+
+
 def supercell_2x1x1(c: Crystal) -> Crystal:
     """2x1x1 supercell of a simple-cubic-cell crystal (primlatt = identity)."""
     assert np.allclose(c.primlatt, np.eye(3))
@@ -156,20 +165,16 @@ def _sw_energy(x, pairs_i, pairs_j, trip_c, trip_j, trip_k, P=SW_SI):
     return e2.sum() + e3.sum()
 
 
-def _bm_energy(x, pairs_i, pairs_j, kind, P):
-    """Born-Mayer repulsion A exp(-r/rho) between unlike first neighbours plus a
-    Born-Mayer-shaped attraction -C exp(-r/rho2) that holds the lattice at the given
-    spacing, and a weaker like-pair term for second neighbours (kind = 1/2)."""
+def _shell_energy(x, pi, pj, D, r0, delta, rho):
+    """Born-Mayer-type double exponential per bonded pair (a Morse form with its minimum at
+    the pair's equilibrium distance r0, so the lattice is in equilibrium and the harmonic
+    force constants are radial springs k = 2D/rho^2), plus a cubic term that scales the
+    third derivative -6D/rho^3 by (1 + delta)."""
+    d = x[pj] - x[pi]
+    r = (d * d).sum(-1).sqrt()
+    u = (r - r0) / rho
     import torch
-    d = x[pairs_j] - x[pairs_i]
-    r = torch.sqrt((d * d).sum(-1))
-    A1, rho1, C1, rho1b = P["A1"], P["rho1"], P["C1"], P["rho1b"]
-    A2, rho2, C2, rho2b = P["A2"], P["rho2"], P["C2"], P["rho2b"]
-    e = torch.where(kind == 1,
-                    A1 * torch.exp(-r / rho1) - C1 * torch.exp(-r / rho1b),
-                    A2 * torch.exp(-r / rho2) - C2 * torch.exp(-r / rho2b))
-    # smooth cutoff so higher shells do not interact
-    return e.sum()
+    return (D * (torch.exp(-2 * u) - 2 * torch.exp(-u)) - delta * D * u ** 3).sum()
 
 
 def _lattice_cluster(crys: Crystal, radius: float):
@@ -214,7 +219,7 @@ class ForceConstants:
 
 def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3: float,
                         h: float = 0.01, radius: float = 10.0, rel_drop: float = 1e-9,
-                        fc3_perturb=None) -> ForceConstants:
+                        fc3_perturb=None, close_swap: bool = False) -> ForceConstants:
     import torch
     cl = _lattice_cluster(crys, radius)
     x0 = torch.tensor(np.array([c[2] for c in cl]), dtype=torch.float64)
@@ -259,11 +264,21 @@ def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3
     mx2 = max(abs(v) for v in fc2.values())
     mx3 = max(abs(v) for v in fc3.values())
     fc2 = {k: v for k, v in fc2.items() if abs(v) > rel_drop * mx2}
-    # symmetrise FC3 in (j,be) <-> (k,ga) (finite-difference error is O(h^2)) and keep union
+    # symmetrise FC3 in (j,be) <-> (k,ga) (finite-difference error is O(h^2)).  With
+    # close_swap a term whose partner was not differentiated (k beyond cutoff_fc3, e.g.
+    # Phi(i,i,k) of a pair potential) gives the partner its own value, so the list is closed
+    # under the swap: the reference sizes its phase table by the largest second atom index
+    # (cal_eiqr2, sub.f90:6035) and reads it with the third, which is only in bounds when the
+    # third never exceeds the second's range.
     sym = {}
     for (i, al, j, be, k, ga), v in fc3.items():
         key2 = (i, al, k, ga, j, be)
-        sym[(i, al, j, be, k, ga)] = 0.5 * (v + fc3.get(key2, 0.0))
+        if close_swap:
+            w = 0.5 * (v + fc3[key2]) if key2 in fc3 else v
+            sym[key2] = w
+        else:   # the SW-Si lists of the golden fixtures: partners within cutoff_fc3 only
+            w = 0.5 * (v + fc3.get(key2, 0.0))
+        sym[(i, al, j, be, k, ga)] = w
     fc3 = {k: v for k, v in sym.items()
            if abs(v) > rel_drop * mx3 or abs(sym.get((k[0], k[1], k[4], k[5], k[2], k[3]), 0.0)) > rel_drop * mx3}
     if fc3_perturb is not None:
@@ -311,3 +326,87 @@ def sw_silicon(a: float = 5.43, h: float = 0.01) -> ForceConstants:
     fn = lambda x: _sw_energy(x, pi, pj, tc, tj, tk)
     nn = a * math.sqrt(3) / 4
     return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h)
+
+
+def _sw_force_constants(crys: Crystal, P: dict, a: float, h: float) -> ForceConstants:
+    import torch
+    cl = _lattice_cluster(crys, 10.0 + 0.5 * a * (crys.natoms0 // 2 - 1))
+    x0 = np.array([c[2] for c in cl])
+    rc = P["a"] * P["sigma"]
+    pi, pj, tc, tj, tk = [], [], [], [], []
+    nbr = [[] for _ in range(len(x0))]
+    for i in range(len(x0)):
+        d = np.linalg.norm(x0 - x0[i], axis=1)
+        for j in np.nonzero((d < rc) & (d > 0))[0]:
+            nbr[i].append(int(j))
+            if j > i:
+                pi.append(i); pj.append(int(j))
+    for c in range(len(x0)):
+        for u in range(len(nbr[c])):
+            for v in range(u + 1, len(nbr[c])):
+                tc.append(c); tj.append(nbr[c][u]); tk.append(nbr[c][v])
+    T = lambda l: torch.tensor(l, dtype=torch.long)
+    pi, pj, tc, tj, tk = T(pi), T(pj), T(tc), T(tj), T(tk)
+    fn = lambda x: _sw_energy(x, pi, pj, tc, tj, tk, P)
+    return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h,
+                               radius=10.0 + 0.5 * a * (crys.natoms0 // 2 - 1), close_swap=True)
+
+
+def sw_zincblende4(a: float = 5.65, h: float = 0.01) -> ForceConstants:
+    """4-atom zincblende-type cell (masses 69.72/74.92) with the Stillinger-Weber form, its
+    length scale sigma stretched by a/5.43 so the lattice sits at the same reduced geometry
+    as SW silicon (a tetrahedral bond-angle term keeps the zincblende structure stable)."""
+    P = dict(SW_SI)
+    P["sigma"] = SW_SI["sigma"] * a / 5.43
+    return _sw_force_constants(zincblende_tetragonal4(a), P, a, h)
+
+
+# shells of the rock-salt structure: (distance / a, species relation, spring constant eV/A^2)
+ROCKSALT_SHELLS = ((0.5, "unlike", 2.5), (math.sqrt(0.5), "like", 0.5), (math.sqrt(0.75), "unlike", 0.12))
+
+
+def born_mayer_rocksalt(crys: Optional[Crystal] = None, a: float = 5.6, rho: float = 0.35, seed: int = 0,
+                        perturb: float = 0.05, h: float = 0.01) -> ForceConstants:
+    """Rock-salt-type binary with Born-Mayer-type pair springs (no Coulomb term).
+
+    Harmonic force constants reach the 3rd shell (unlike pairs at a/2, like pairs at a/sqrt2,
+    unlike pairs at a sqrt3/2; ROCKSALT_SHELLS).  The cubic force constants of a pair
+    potential couple a home atom to its bonded partners only, and they are kept for every
+    bonded shell: truncating them at the 2nd shell would leave Phi(i,i,k) of the 3rd-shell
+    bonds without their swap partners Phi(i,k,i), Phi(i,k,k), breaking the (j,k) symmetry and
+    the acoustic sum rule.  The cubic part carries a seeded Gaussian perturbation of relative width
+    `perturb` (5 %): one draw per symmetry class of bonds (shell x species pair), applied to
+    the pair potential's third derivative, so the perturbed constants keep the space-group
+    symmetry and the acoustic sum rule exactly.  `crys` defaults to the 8-atom cubic cell;
+    a supercell of it (supercell_2x1x1) gives the 16-atom cell."""
+    import torch
+    crys = rocksalt_conventional(a) if crys is None else crys
+    radius = 10.5 if crys.natoms0 <= 8 else 12.5
+    cl = _lattice_cluster(crys, radius)
+    x0 = np.array([c[2] for c in cl])
+    ty = np.array([crys.types[c[0] - 1] for c in cl])
+    rng = np.random.default_rng(seed)
+    classes = {}
+    pi, pj, D, r0, dl = [], [], [], [], []
+    for i in range(len(x0)):
+        d = np.linalg.norm(x0 - x0[i], axis=1)
+        for j in range(i + 1, len(x0)):
+            for s, (f, rel, k) in enumerate(ROCKSALT_SHELLS):
+                if abs(d[j] - f * a) < 1e-3 and ((ty[i] != ty[j]) == (rel == "unlike")):
+                    key = (s, min(ty[i], ty[j]), max(ty[i], ty[j]))
+                    if key not in classes:
+                        classes[key] = float(rng.standard_normal()) * perturb
+                    pi.append(i); pj.append(j); D.append(k * rho * rho / 2); r0.append(f * a)
+                    dl.append(classes[key])
+    T = lambda l, dt: torch.tensor(l, dtype=dt)
+    pi, pj = T(pi, torch.long), T(pj, torch.long)
+    D, r0, dl = T(D, torch.float64), T(r0, torch.float64), T(dl, torch.float64)
+    fn = lambda x: _shell_energy(x, pi, pj, D, r0, dl, rho)
+    return _fd_force_constants(crys, fn, cutoff_pair=ROCKSALT_SHELLS[-1][0] * a * 1.01,
+                               cutoff_fc3=ROCKSALT_SHELLS[-1][0] * a * 1.05, h=h, radius=radius, close_swap=True)
+
+
+def born_mayer_supercell16(a: float = 5.6, seed: int = 0, h: float = 0.01) -> ForceConstants:
+    """16-atom 2x1x1 supercell of the 8-atom rock-salt-type cell (same potential)."""
+    return born_mayer_rocksalt(supercell_2x1x1(rocksalt_conventional(a)), a=a, seed=seed, h=h)
+

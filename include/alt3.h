@@ -29,7 +29,9 @@ extern "C" {
 /* V3 arithmetic */
 #define ALT3_V3_FP64_REF 0   /* per-element sum over all FC3 terms (matrix_elt_simplified) */
 #define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
-#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA, FP32 contractions, FP64 scale */
+#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA, FP32 contractions, FP64 scale;
+                                cells whose 16-point MFMA tile exceeds the LDS (nb > ~28) run
+                                the FP64 path instead */
 
 /* delta-function integration of the scattering rates */
 #define ALT3_DELTA_TETRA 0   /* linear tetrahedra, weight_tet (tetrahedron.f90:188-384) */

@@ -542,9 +542,14 @@ int or_v3sq(int ibz_lo, int ibz_hi, const int *mapinv, int nk, const double *kp,
 {
     const zc *ev = (const zc *)eigenvec_ri;
     int ksub = ibz_hi - ibz_lo + 1;
+    /* cal_eiqr2 (sub.f90:6035) sizes the table by the largest second atom index and reads it
+     * with the third as well; the two agree for lists closed under the (j,k) swap (FOCEX's and
+     * the generators'), and sizing by both keeps every read in bounds */
     int maxat = 0;
-    for (int t = 0; t < nt3; t++)
+    for (int t = 0; t < nt3; t++) {
         if (at3[3 * t + 1] > maxat) maxat = at3[3 * t + 1];
+        if (at3[3 * t + 2] > maxat) maxat = at3[3 * t + 2];
+    }
     zc *eiqr2 = malloc(sizeof(zc) * (size_t)nk * maxat);
     for (int i = 0; i < nk; i++)
         for (int j = 1; j <= maxat; j++) {

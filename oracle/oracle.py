@@ -74,7 +74,8 @@ def const33() -> float:
 # ---------------------------------------------------------------------------- indexing
 def reciprocal(r: np.ndarray):
     g = np.zeros((3, 3)); vol = C.c_double()
-    lib().or_reciprocal_2pi(_p(_f64(np.ascontiguousarray(r))), _p(g), C.byref(vol))
+    # r: rows r1, r2, r3 (C order, as lat_fc.dat lists them)
+    lib().or_reciprocal_2pi(_p(np.ascontiguousarray(r, dtype=np.float64)), _p(g), C.byref(vol))
     return g, vol.value
 
 

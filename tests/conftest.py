@@ -45,3 +45,20 @@ def si_inputs(tmp_path_factory):
         out[f"dir{n}"] = d
     out["fc"] = fc
     return out
+
+
+@pytest.fixture(scope="session")
+def cells(tmp_path_factory):
+    """Force constants of the other cells: 4-atom zincblende-type (SW form), 8-atom
+    rock-salt-type and its 16-atom 2x1x1 supercell (Born-Mayer-type springs, seed 0)."""
+    from thermacond import models
+    return {"zb4": models.sw_zincblende4(), "rs8": models.born_mayer_rocksalt(seed=0),
+            "rs16": models.born_mayer_supercell16(seed=0)}
+
+
+def inputs_for(fc, n, tmp_path_factory, name):
+    from thermacond import refio
+    d = str(tmp_path_factory.mktemp(f"{name}_{n}"))
+    refio.write_inputs(d, fc, refio.RunParams(nc=(n, n, n)))
+    return refio.read_inputs(d)
+
