@@ -42,10 +42,16 @@ inline double cnst() { return (double)521.1098918f; }
 double const33();
 
 // ---------------------------------------------------------------- device buffer
+// owns its allocation: freed by release() or when the buffer goes out of scope (hipFree
+// waits for the device, so a scoped temporary outlives the kernels that use it)
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
     int alloc(size_t count) {
         if (count == n && p) return ALT3_OK;
         release();
@@ -109,6 +115,8 @@ struct V3Tables {
 };
 int build_v3_tables(const alt3_system *sys, V3Tables &v);
 int v3_prep_launch(alt3_ctx *ctx, int ksub);
+bool v3_rows_supported(const alt3_ctx *ctx);
+int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 
 }  // namespace alt3
 

@@ -344,7 +344,19 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
         prec = ALT3_V3_FP64;
         ctx->v3_prec = prec;
     }
-    if (prec == ALT3_V3_MIXED) {
+    if (prec != ALT3_V3_FP64_REF && v3_rows_supported(ctx)) {
+        // 6, 12 or 24 branches: one workgroup per q2 tile handles every l1 (v3_rows.hip)
+        const bool fp32 = prec == ALT3_V3_MIXED;
+        if (fp32) {
+            ctx->d_v33.release();
+            ALT3_TRY(ctx->d_v33f.alloc(nel));
+        } else {
+            ctx->d_v33f.release();
+            ALT3_TRY(ctx->d_v33.alloc(nel));
+        }
+        ALT3_TRY(v3_prep_launch(ctx, ksub));
+        ALT3_TRY(v3_rows_run(ctx, ksub, fp32, derr.p));
+    } else if (prec == ALT3_V3_MIXED) {
         ctx->d_v33.release();
         ALT3_TRY(ctx->d_v33f.alloc(nel));
         ALT3_TRY(v3_mixed_run(ctx, ksub, derr.p));

@@ -1,0 +1,206 @@
+// Batched |V3|^2 with every l1 of a q2 tile in one workgroup (cells of 6, 12 or 24 branches).
+//
+// Same algebra as k_v3_main (THERMACOND/sub.f90:5888-6011 in batched form):
+//   Phi'(q2)[b,c] = sum_r e^{i q2.dR_r} G'[r][l1][b,c]
+//   U[b,l3]       = sum_c Phi'[b,c] e3[c,l3]            e3 = e(q3 = -q1-q2)
+//   V[l2,l3]      = sum_b e2[b,l2] U[b,l3]
+//   |V3|^2        = |V|^2 const33^2 / (8 sqrt|l1 l2 l3|)
+// but organised for the hardware: one workgroup holds a tile of TQ q2 points and loops over
+// l1, so the phase rows, the e3 blocks and the per-mode scales are staged once per tile
+// instead of once per (tile, l1).  Thread (q, j) owns row b = j of Phi'/U (the phase sum
+// over r streams into the U row one column c at a time, so Phi' never materialises) and,
+// after a barrier, column l2 = j of V.  LDS arrays are stored q-fastest so that the lanes of
+// a wave (consecutive q) hit consecutive banks, and the output rows [s][l1][l2][l3][q2] are
+// written with consecutive q2 per lane.  T = double (ALT3_V3_FP64) or float (ALT3_V3_MIXED:
+// FP32 phase sum and contractions, FP64 scale); the scale uses the per-mode table
+// |lambda|^(-1/4), so 1/den = w1 w2 w3 / sqrt 8.
+#include "alt3_impl.h"
+#include "mesh_dev.h"
+
+namespace {
+
+// |lambda(k,l)|^(-1/4); +inf for a zero eigenvalue (the reference's den = 0 stop)
+__global__ void k_lam_quarter(int n, const double *__restrict__ eival, double *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 1.0 / sqrt(sqrt(fabs(eival[i])));
+}
+
+template <typename T>
+__global__ void k_copy_real(size_t n, const double *__restrict__ in, T *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[i];
+}
+
+template <typename T, int NB, int TQ>
+__global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, int ibz_lo, int ndr,
+                                                     const int *__restrict__ mapinv, const T *__restrict__ eph,
+                                                     const T *__restrict__ evec, const double *__restrict__ wq,
+                                                     const T *__restrict__ gp, double c33sq8, T *__restrict__ out,
+                                                     int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *E = reinterpret_cast<T *>(smem);     // [r][q] re, then im        (2 ndr TQ)
+    T *E3 = E + 2 * ndr * TQ;                // [c][l3][q] re, then im    (2 NB NB TQ)
+    T *U = E3 + 2 * NB * NB * TQ;            // [b][l3][q] re, then im    (2 NB NB TQ)
+    __shared__ double w3s[NB][TQ];
+    __shared__ int k3s[TQ];
+    constexpr int NT = TQ * NB, NN = NB * NB;
+    const int tid = threadIdx.x, q = tid % TQ, j = tid / TQ;
+    const int s = blockIdx.y, q0 = blockIdx.x * TQ;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    m.coords(k1, a0, a1, a2);
+    const int n2 = q0 + q;
+    const bool valid = n2 < nk;
+    const int n2c = valid ? n2 : nk - 1;
+    if (j == 0) {
+        int b0, b1, b2;
+        m.coords(n2c, b0, b1, b2);
+        k3s[q] = m.neg_sum(a0, a1, a2, b0, b1, b2);
+    }
+    for (int x = tid; x < ndr * TQ; x += NT) {
+        const int r = x / TQ, qq = x - r * TQ;
+        const int nn = min(q0 + qq, nk - 1);
+        E[x] = eph[2 * ((size_t)nn * ndr + r)];
+        E[ndr * TQ + x] = eph[2 * ((size_t)nn * ndr + r) + 1];
+    }
+    __syncthreads();
+    for (int x = tid; x < NN * TQ; x += NT) {      // x = (c*NB + l3)*TQ + qq
+        const int cl = x / TQ, qq = x - cl * TQ, c = cl / NB, l3 = cl - c * NB;
+        const T *e = evec + 2 * (((size_t)k3s[qq] * NB + l3) * NB + c);
+        E3[x] = e[0];
+        E3[NN * TQ + x] = e[1];
+    }
+    for (int x = tid; x < NB * TQ; x += NT) {
+        const int l3 = x / TQ, qq = x - l3 * TQ;
+        w3s[l3][qq] = wq[(size_t)k3s[qq] * NB + l3];
+    }
+    const double w2 = wq[(size_t)n2c * NB + j];
+    const T *e2 = evec + 2 * ((size_t)n2c * NB + j) * NB;   // e(q2) column l2 = j
+    __syncthreads();
+    for (int l1 = 0; l1 < NB; l1++) {
+        // row b = j of U = Phi' e3, Phi' streamed column by column
+        T ur[NB], ui[NB];
+#pragma unroll
+        for (int l3 = 0; l3 < NB; l3++) ur[l3] = ui[l3] = 0;
+        const T *g = gp + 2 * (((size_t)s * ndr * NB + l1) * NN + j * NB);   // + r*NB*NN + c
+#pragma unroll 1
+        for (int c = 0; c < NB; c++) {
+            T pr = 0, pi = 0;
+#pragma unroll 4
+            for (int r = 0; r < ndr; r++) {
+                const T gr = g[2 * ((size_t)r * NB * NN + c)], gi = g[2 * ((size_t)r * NB * NN + c) + 1];
+                const T er = E[r * TQ + q], ei = E[ndr * TQ + r * TQ + q];
+                pr += er * gr - ei * gi;
+                pi += er * gi + ei * gr;
+            }
+#pragma unroll
+            for (int l3 = 0; l3 < NB; l3++) {
+                const T xr = E3[(c * NB + l3) * TQ + q], xi = E3[NN * TQ + (c * NB + l3) * TQ + q];
+                ur[l3] += pr * xr - pi * xi;
+                ui[l3] += pr * xi + pi * xr;
+            }
+        }
+#pragma unroll
+        for (int l3 = 0; l3 < NB; l3++) {
+            U[(j * NB + l3) * TQ + q] = ur[l3];
+            U[NN * TQ + (j * NB + l3) * TQ + q] = ui[l3];
+        }
+        __syncthreads();
+        // column l2 = j of V = e2^T U, scaled and squared
+        if (valid) {
+            T vr[NB], vi[NB];
+#pragma unroll
+            for (int l3 = 0; l3 < NB; l3++) vr[l3] = vi[l3] = 0;
+#pragma unroll 2
+            for (int b = 0; b < NB; b++) {
+                const T er = e2[2 * b], ei = e2[2 * b + 1];
+#pragma unroll
+                for (int l3 = 0; l3 < NB; l3++) {
+                    const T xr = U[(b * NB + l3) * TQ + q], xi = U[NN * TQ + (b * NB + l3) * TQ + q];
+                    vr[l3] += er * xr - ei * xi;
+                    vi[l3] += er * xi + ei * xr;
+                }
+            }
+            const double w12 = wq[(size_t)k1 * NB + l1] * w2;
+            T *o = out + ((((size_t)s * NB + l1) * NB + j) * NB) * (size_t)nk + n2;
+#pragma unroll
+            for (int l3 = 0; l3 < NB; l3++) {
+                const double w = w12 * w3s[l3][q];
+                if (!(w <= 1.7e308)) atomicExch(err, 1);   // a zero eigenvalue in the triple
+                o[(size_t)l3 * nk] = (vr[l3] * vr[l3] + vi[l3] * vi[l3]) * (T)(c33sq8 * w * w);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int NB, int TQ>
+int launch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const double *wq, const T *gp, T *out,
+                int *d_err)
+{
+    const int ndr = ctx->V.ndr;
+    const size_t lds = (size_t)2 * (ndr + 2 * NB * NB) * TQ * sizeof(T);
+    if (lds > ctx->lds_max) return alt3::fail(ALT3_E_ARG, "V3 tile does not fit in LDS");
+    if (lds > 65536)
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_rows<T, NB, TQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    const double c33 = alt3::const33();
+    dim3 grid((unsigned)((ctx->nk + TQ - 1) / TQ), (unsigned)ksub);
+    hipLaunchKernelGGL((k_v3_rows<T, NB, TQ>), grid, dim3(TQ * NB), lds, ctx->stream, alt3::MeshDev::make(ctx->nc),
+                       ctx->nk, ctx->ibz_lo, ndr, ctx->d_mapinv.p, eph, evec, wq, gp, c33 * c33 / 8, out, d_err);
+    ALT3_HIP(hipGetLastError());
+    return ALT3_OK;
+}
+
+template <typename T>
+int dispatch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const double *wq, const T *gp, T *out,
+                  int *d_err)
+{
+    switch (ctx->nb) {
+        case 6: return launch_rows<T, 6, 32>(ctx, ksub, eph, evec, wq, gp, out, d_err);
+        case 12: return launch_rows<T, 12, 16>(ctx, ksub, eph, evec, wq, gp, out, d_err);
+        default: return launch_rows<T, 24, 4>(ctx, ksub, eph, evec, wq, gp, out, d_err);
+    }
+}
+
+}  // namespace
+
+namespace alt3 {
+
+bool v3_rows_supported(const alt3_ctx *ctx) { return ctx->V.ndr > 0 && (ctx->nb == 6 || ctx->nb == 12 || ctx->nb == 24); }
+
+// G' must be prepared (v3_prep_launch); out holds ksub*nb^3*nk values of the precision's type
+int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
+{
+    const int nk = ctx->nk, nb = ctx->nb, ndr = ctx->V.ndr;
+    DBuf<double> wq;
+    ALT3_TRY(wq.alloc((size_t)nk * nb));
+    hipLaunchKernelGGL(k_lam_quarter, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_eival.p,
+                       wq.p);
+    ALT3_HIP(hipGetLastError());
+    if (!fp32) {
+        ALT3_TRY(dispatch_rows<double>(ctx, ksub, ctx->d_eph.p, ctx->d_evec.p, wq.p, ctx->d_gp.p, ctx->d_v33.p, d_err));
+    } else {
+        DBuf<float> ephf, evf, gpf;
+        const size_t ne = (size_t)nk * nb * nb * 2, np = (size_t)nk * ndr * 2, ng = (size_t)ksub * ndr * nb * nb * nb * 2;
+        ALT3_TRY(ephf.alloc(np));
+        ALT3_TRY(evf.alloc(ne));
+        ALT3_TRY(gpf.alloc(ng));
+        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, np,
+                           ctx->d_eph.p, ephf.p);
+        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, ne,
+                           ctx->d_evec.p, evf.p);
+        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ng,
+                           ctx->d_gp.p, gpf.p);
+        ALT3_HIP(hipGetLastError());
+        ALT3_TRY(dispatch_rows<float>(ctx, ksub, ephf.p, evf.p, wq.p, gpf.p, ctx->d_v33f.p, d_err));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));   // before the FP32 copies are freed
+    }
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    return ALT3_OK;
+}
+
+}  // namespace alt3
