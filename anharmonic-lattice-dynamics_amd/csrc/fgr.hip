@@ -75,19 +75,25 @@ __device__ void weight_tet1(const float *w, double e, double *c)
         cc[1] = (d1 * d1 * d1) / (double)(e21 * (e31 * e31) * e41);
         cc[0] = (d1 * d1 * d1) / (double)(e21 * e31 * (e41 * e41));
     } else if (E2 <= e && e <= E3) {
-        float c11 = (float)((E3 - e) / (double)(e31 * e31));
-        float c12 = (float)((E4 - e) / (double)(e41 * e41));
-        float c21 = (float)((E3 - e) / (double)(e32 * e32));
-        float c22 = (float)((E4 - e) / (double)(e42 * e42));
-        float c31 = (float)((e - E2) / (double)(e32 * e32));
-        float c32 = (float)((e - E1) / (double)(e31 * e31));
-        float c41 = (float)((e - E2) / (double)(e42 * e42));
-        float c42 = (float)((e - E1) / (double)(e41 * e41));
+        // the reference's quotients, each denominator (a REAL*4 product) inverted once: ten
+        // FP64 reciprocals instead of twenty divisions; the quotients are rounded to REAL*4
+        const double r31 = 1.0 / (double)(e31 * e31), r41 = 1.0 / (double)(e41 * e41);
+        const double r32 = 1.0 / (double)(e32 * e32), r42 = 1.0 / (double)(e42 * e42);
+        float c11 = (float)((E3 - e) * r31);
+        float c12 = (float)((E4 - e) * r41);
+        float c21 = (float)((E3 - e) * r32);
+        float c22 = (float)((E4 - e) * r42);
+        float c31 = (float)((e - E2) * r32);
+        float c32 = (float)((e - E1) * r31);
+        float c41 = (float)((e - E2) * r42);
+        float c42 = (float)((e - E1) * r41);
         const double a32 = (E3 - e) * (e - E2), a41 = (E4 - e) * (e - E1), a31 = (E3 - e) * (e - E1), a42 = (E4 - e) * (e - E2);
-        float b11 = (float)(a32 / (double)(e42 * e32) + a41 / (double)(e41 * e42) + a31 / (double)(e32 * e41));
-        float b12 = (float)(a41 / (double)(e42 * e31) + a42 / (double)(e42 * e32) + a31 / (double)(e31 * e32));
-        float b21 = (float)(a32 / (double)(e42 * e31) + a42 / (double)(e42 * e41) + a31 / (double)(e31 * e41));
-        float b22 = (float)(a32 / (double)(e32 * e31) + a41 / (double)(e41 * e31) + a42 / (double)(e32 * e41));
+        const double i1 = 1.0 / (double)(e42 * e32), i2 = 1.0 / (double)(e41 * e42), i3 = 1.0 / (double)(e32 * e41);
+        const double i4 = 1.0 / (double)(e42 * e31), i5 = 1.0 / (double)(e31 * e32), i6 = 1.0 / (double)(e31 * e41);
+        float b11 = (float)(a32 * i1 + a41 * i2 + a31 * i3);
+        float b12 = (float)(a41 * i4 + a42 * i1 + a31 * i5);
+        float b21 = (float)(a32 * i4 + a42 * i2 + a31 * i6);
+        float b22 = (float)(a32 * i5 + a41 * i6 + a42 * i3);
         float b31 = b21;
         float b32 = b11;
         float b41 = b22;
@@ -129,7 +135,8 @@ __device__ __forceinline__ float pick(const float (&a)[N], int j)
 // cube kept in registers) and reused for the nb values omega(q1, l1); per l1 the LDS weight
 // sums W are accumulated and then written out as P = W * F.
 constexpr int kFgrThreads = 512;   // 8 waves; LDS (12 B per mesh point) allows 2 groups per CU
-constexpr int kFgrList = 6144;     // LDS list of active tetrahedra; one chunk of cubes always fits
+constexpr int kFgrCase = 4096;     // LDS list per weight_tet case; one chunk of cubes (<= 6*512) always fits
+constexpr int kFgrList = 3 * kFgrCase;
 
 // e inside [lo, hi] as the reference tests it (no weight outside, tetrahedron.f90:343)
 __device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return !(e < (double)lo || e > (double)hi); }
@@ -144,8 +151,8 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     double *W = lds;                                   // [nk]
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
-    __shared__ int s_cnt;
-    __shared__ unsigned short s_list[kFgrList];  // (cube << 3 | tetrahedron), nk <= 8192
+    __shared__ int s_cnt[3];
+    __shared__ unsigned short s_list[kFgrList];  // [case][kFgrCase] of (cube << 3 | tetrahedron), nk <= 8192
     const int proc = blockIdx.x & 1;
     int r = blockIdx.x >> 1;
     const int l3 = r % nb;
@@ -219,7 +226,9 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     }
     const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
     double *P = proc == 0 ? P1 : P2;
-    // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list
+    // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list of
+    // their weight_tet case (e below the second-lowest corner, below the third, or above), so
+    // that the lanes of a wave take the same branch of the weight formula
     auto push = [&](int j, double e) {
         if (omega_in(e, pick(cmn, j), pick(cmx, j))) {
             const int k = tid + j * kFgrThreads;
@@ -233,17 +242,21 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
 #pragma unroll
             for (int l = 0; l < 6; l++) {
                 const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
-                if (omega_in(e, fminf(fminf(a, b), fminf(c, d)), fmaxf(fmaxf(a, b), fmaxf(c, d)))) {
-                    const int x = atomicAdd(&s_cnt, 1);
-                    if (x < kFgrList) s_list[x] = (unsigned short)((k << 3) | l);
+                const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
+                if (omega_in(e, fminf(s1, s3), fmaxf(s2, s4))) {
+                    const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
+                    const int cs = (e <= (double)fminf(m1, m2)) ? 0 : (e <= (double)fmaxf(m1, m2)) ? 1 : 2;
+                    const int x = atomicAdd(&s_cnt[cs], 1);
+                    if (x < kFgrCase) s_list[cs * kFgrCase + x] = (unsigned short)((k << 3) | l);
                 }
             }
         }
     };
     // weights of the listed tetrahedra, added to W at their corners
-    auto process = [&](int cnt, double e) {
-        for (int x = tid; x < cnt; x += kFgrThreads) {
-            const int code = (int)s_list[x], kk = code >> 3, l = code & 7;
+    auto process = [&](int n0, int n1, int n2, double e) {
+        for (int x = tid; x < n0 + n1 + n2; x += kFgrThreads) {
+            const int at = x < n0 ? x : x < n0 + n1 ? kFgrCase + x - n0 : 2 * kFgrCase + x - n0 - n1;
+            const int code = (int)s_list[at], kk = code >> 3, l = code & 7;
             int c0, c1, c2;
             m.coords(kk, c0, c1, c2);
             int v[4];
@@ -285,26 +298,26 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
         }
         if (omega_in(omega1, lo, hi)) {
             // Only tetrahedra whose energy range contains omega1 have weights, and they are
-            // few: they are compacted into an LDS list so that the weight formula runs with
-            // every lane busy.  One list normally holds all of them; if it overflows, the
+            // few: they are compacted into LDS lists so that the weight formula runs with
+            // every lane busy.  The lists normally hold all of them; if one overflows, the
             // cubes are taken again chunk by chunk (at most 6 entries per cube and thread).
-            if (tid == 0) s_cnt = 0;
+            if (tid < 3) s_cnt[tid] = 0;
             __syncthreads();
 #pragma unroll 1
             for (int j = 0; j * kFgrThreads < nk; j++) push(j, omega1);
             __syncthreads();
-            const int cnt = s_cnt;
-            if (cnt <= kFgrList) {
-                process(cnt, omega1);
+            const int n0 = s_cnt[0], n1 = s_cnt[1], n2 = s_cnt[2];
+            if (n0 <= kFgrCase && n1 <= kFgrCase && n2 <= kFgrCase) {
+                process(n0, n1, n2, omega1);
             } else {
 #pragma unroll 1
                 for (int j = 0; j * kFgrThreads < nk; j++) {
                     __syncthreads();
-                    if (tid == 0) s_cnt = 0;
+                    if (tid < 3) s_cnt[tid] = 0;
                     __syncthreads();
                     push(j, omega1);
                     __syncthreads();
-                    process(s_cnt, omega1);
+                    process(s_cnt[0], s_cnt[1], s_cnt[2], omega1);
                 }
             }
             __syncthreads();
@@ -402,8 +415,8 @@ int fgr_run(alt3_ctx *ctx)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    if (lds + kFgrList * sizeof(unsigned short) + 256 > 65536 || nk > 9 * kFgrThreads)
-        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4400");
+    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 9 * kFgrThreads)
+        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4608");
     ALT3_TRY(ctx->d_p1.alloc(nel));
     ALT3_TRY(ctx->d_p2.alloc(nel));
     const MeshDev md = MeshDev::make(ctx->nc);
@@ -411,7 +424,12 @@ int fgr_run(alt3_ctx *ctx)
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
     const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
+    const bool big = lds + kFgrList * sizeof(unsigned short) + 256 > 65536;   // static + dynamic LDS
 #define ALT3_FGR_LAUNCH(PER)                                                                                     \
+    if (big) {                                                                                                   \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<float, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<double, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    }                                                                                                            \
     if (mixed)                                                                                                   \
         hipLaunchKernelGGL((k_fgr<float, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
                            ctx->d_mapinv.p, ctx->d_freq.p, (const float *)v, ctx->d_p1.p, ctx->d_p2.p);          \
