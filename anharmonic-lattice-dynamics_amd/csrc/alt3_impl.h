@@ -142,6 +142,7 @@ struct alt3_ctx {
     alt3::DBuf<double> d_evec;                     // [nk][nb][nb] complex (interleaved)
     alt3::DBuf<int> d_mappos;
     bool have_eig = false;
+    bool have_veloci = false;                      // d_veloci holds the star averages of d_vel
 
     // V3
     alt3::DBuf<int> d_t_abc;
@@ -165,6 +166,8 @@ struct alt3_ctx {
     int fgr_kind = ALT3_DELTA_TETRA;               // what d_p1/d_p2 hold
     double fgr_sigma = 0;
     alt3::DBuf<double> d_p1, d_p2;                 // [ksub][nb][nb][nb][nk]
+    alt3::DBuf<float> d_p1f, d_p2f;                // same in FP32 (mixed path: P = W |V3|^2 with FP32 |V3|^2)
+    bool p_fp32 = false;                           // which pair holds the current P
     bool have_fgr = false;
 
     // per temperature: one tau pass fills slot t of each array for every temperature it

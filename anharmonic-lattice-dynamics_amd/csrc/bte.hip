@@ -56,11 +56,11 @@ __device__ double block_sum(double v, double *sh)
 
 // one workgroup per (s, l1, l2): the l3-reduced R1/R2 rows and partial Q, QN, QU of NT
 // temperatures at once, so that P1sm/P2sm are read once for all of them
-template <int NT>
+template <int NT, typename PT>
 __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                              const int *__restrict__ mapinv, const double *__restrict__ dist,
-                                             size_t dstr, const double *__restrict__ P1sm,
-                                             const double *__restrict__ P2sm, double *__restrict__ qpart, size_t qstr,
+                                             size_t dstr, const PT *__restrict__ P1sm,
+                                             const PT *__restrict__ P2sm, double *__restrict__ qpart, size_t qstr,
                                              double *__restrict__ R1, double *__restrict__ R2, size_t rstr)
 {
     __shared__ double sh[256];
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
 #pragma unroll 3
         for (int l3 = 0; l3 < nb; l3++) {
             const size_t x = row + (size_t)l3 * nk + j;
-            const double P1 = P1sm[x], P2 = P2sm[x];
+            const double P1 = (double)P1sm[x], P2 = (double)P2sm[x];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const double p1 = P1 * (f1[t] * (dist[t * dstr + (size_t)np * nb + l3] + 1));
@@ -216,15 +216,16 @@ __global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ ma
     taueff[i + (size_t)nibz * ii] = (i == nibz - 1 && ii == nb - 1) ? te : 0.0;
 }
 
-// kappa(ii, j, jj) Fortran (nb,3,3) = sum over i of the partials, in IBZ order
-__global__ void k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart, double *__restrict__ kappa)
+// kappa(ii, j, jj) Fortran (nb,3,3) = sum over i of the partials: one wavefront per output,
+// lane partial sums over i = lane, lane+64, ... then a shuffle tree
+__global__ void __launch_bounds__(64) k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart,
+                                                  double *__restrict__ kappa)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nb * 9) return;
-    int ii = x / 9, e = x % 9;
+    const int x = blockIdx.x, ii = x / 9, e = x % 9;
     double acc = 0;
-    for (int i = 0; i < nibz; i++) acc = acc + kpart[((size_t)i * nb + ii) * 9 + e];
-    kappa[ii + (size_t)nb * e] = acc;
+    for (int i = threadIdx.x; i < nibz; i += 64) acc += kpart[((size_t)i * nb + ii) * 9 + e];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) kappa[ii + (size_t)nb * e] = acc;
 }
 
 // collision matrix, general fallback.  One workgroup per row block (s,l1); threads over
@@ -586,10 +587,17 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     ALT3_TRY(qpart.alloc((size_t)nt * pstr));
     for (int t0 = 0; t0 < nt; t0 += 4) {
         const int g = std::min(4, nt - t0);
-        auto *kern = g == 1 ? k_tau<1> : g == 2 ? k_tau<2> : g == 3 ? k_tau<3> : k_tau<4>;
-        hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
-                           ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1.p, ctx->d_p2.p,
-                           qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+        if (ctx->p_fp32) {
+            auto *kern = g == 1 ? k_tau<1, float> : g == 2 ? k_tau<2, float> : g == 3 ? k_tau<3, float> : k_tau<4, float>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                               ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1f.p, ctx->d_p2f.p,
+                               qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+        } else {
+            auto *kern = g == 1 ? k_tau<1, double> : g == 2 ? k_tau<2, double> : g == 3 ? k_tau<3, double> : k_tau<4, double>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                               ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1.p, ctx->d_p2.p,
+                               qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+        }
         ALT3_HIP(hipGetLastError());
     }
     for (int t = 0; t < nt; t++)
@@ -667,7 +675,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
                        temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
                        ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr.p);
     ALT3_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_kappa_sum, dim3((nb * 9 + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -782,14 +790,17 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     DBuf<int> derr;
     ALT3_TRY(derr.alloc(1));
     ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
-    hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
-                       ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
+    if (!ctx->have_veloci) {   // star-averaged IBZ velocities depend on the eigen data only
+        hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
+                           ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
+        ctx->have_veloci = true;
+    }
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
                        ctx->d_eival.p, temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p,
                        ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr.p);
-    hipLaunchKernelGGL(k_kappa_sum, dim3((nb * 9 + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -20,7 +20,8 @@ int eig_run(alt3_ctx *ctx);
 int v3_prepare(alt3_ctx *ctx);
 int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec);
 int v3_copy_out(alt3_ctx *ctx, double *host);
-int copy_fortran5(alt3_ctx *ctx, const double *dev, double *host);
+template <typename T>
+int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host);
 int fgr_run(alt3_ctx *ctx);
 int fgr_gauss_run(alt3_ctx *ctx, double sigma);
 int upload_bte_tables(alt3_ctx *ctx);
@@ -33,6 +34,18 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
 
 namespace {
+int copy_p(alt3_ctx *ctx, double *P1sm, double *P2sm)
+{
+    if (ctx->p_fp32) {
+        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1f.p, P1sm));
+        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2f.p, P2sm));
+    } else {
+        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
+        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+    }
+    return ALT3_OK;
+}
+
 struct Timer {
     alt3_ctx *c;
     explicit Timer(alt3_ctx *ctx) : c(ctx) { (void)hipEventRecord(c->ev0, c->stream); }
@@ -128,7 +141,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_dr.release(); ctx->d_slot_dr.release(); ctx->d_slot_bc.release(); ctx->d_slot_ptr.release();
     ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
     ctx->d_eph.release(); ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
-    ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
+    ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_p1f.release(); ctx->d_p2f.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
     ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release();
     ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
     ctx->d_cm_slot_n2.release(); ctx->d_cm_slot_cnt.release(); ctx->d_cm_tab.release();
@@ -149,7 +162,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     long long b = 0;
     b += ctx->d_evec.bytes() + ctx->d_eival.bytes() + ctx->d_freq.bytes() + ctx->d_vel.bytes();
     b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
-    b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
+    b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
     *bytes = b;
     return ALT3_OK;
 }
@@ -170,7 +183,7 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
     } else {
         b += nel * 8;
     }
-    b += 2 * nel * 8;                                                                           // P1sm, P2sm
+    b += 2 * nel * (precision == ALT3_V3_MIXED ? 4 : 8);                                       // P1sm, P2sm
     b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * 8 + 3 * ni * nb * nb * 8);  // tau pass
     b += n * n * 8 + n * 8 * 8 + ni * nb * 9 * 8 * 2;                                           // CM, RHS, CG, kappa
     b += (long long)ni * 48 * 4 * 7 + 9 * 48 * 8 * 2;                                           // star tables
@@ -251,6 +264,7 @@ int alt3_set_eig(alt3_ctx *ctx, const double *eigenval, const double *eigenvec, 
     ALT3_HIP(hipMemcpy(ctx->d_evec.p, eigenvec, sizeof(double) * nk * nb * nb * 2, hipMemcpyHostToDevice));
     ALT3_HIP(hipMemcpy(ctx->d_vel.p, veloc, sizeof(double) * nk * nb * 3, hipMemcpyHostToDevice));
     ctx->have_eig = true;
+    ctx->have_veloci = false;
     ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
     return ALT3_OK;
 }
@@ -273,8 +287,7 @@ int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm)
     Timer tm(ctx);
     ALT3_TRY(fgr_run(ctx));
     tm.stop();
-    if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
-    if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+    ALT3_TRY(copy_p(ctx, P1sm, P2sm));
     return ALT3_OK;
 }
 
@@ -285,8 +298,7 @@ int alt3_fgr_gauss(alt3_ctx *ctx, double sigma, double *P1sm, double *P2sm)
     Timer tm(ctx);
     ALT3_TRY(fgr_gauss_run(ctx, sigma));
     tm.stop();
-    if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
-    if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+    ALT3_TRY(copy_p(ctx, P1sm, P2sm));
     return ALT3_OK;
 }
 

@@ -368,6 +368,7 @@ int eig_run(alt3_ctx *ctx)
     ev7.release();
     evec0.release();
     ctx->have_eig = true;
+    ctx->have_veloci = false;
     return ALT3_OK;
 }
 

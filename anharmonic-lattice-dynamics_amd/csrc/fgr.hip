@@ -144,8 +144,8 @@ __device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return 
 template <typename VT, int kFgrPer>   // kFgrPer: mesh points (and cubes) per thread
 __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
-                                                     const VT *__restrict__ v33, double *__restrict__ P1,
-                                                     double *__restrict__ P2)
+                                                     const VT *__restrict__ v33, VT *__restrict__ P1,
+                                                     VT *__restrict__ P2)
 {
     extern __shared__ double lds[];
     double *W = lds;                                   // [nk]
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
         }
     }
     const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
-    double *P = proc == 0 ? P1 : P2;
+    VT *P = proc == 0 ? P1 : P2;
     // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list of
     // their weight_tet case (e below the second-lowest corner, below the third, or above), so
     // that the lanes of a wave take the same branch of the weight formula
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             if (w < nk) {
                 double F = (double)fv[j] * tpi2 / nk;
                 if (vzero >> j & 1u) F = 0.0;
-                P[base + w] = W[w] * F;
+                P[base + w] = (VT)(W[w] * F);
                 W[w] = 0.0;
             }
         }
@@ -344,8 +344,8 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
 template <typename VT>
 __global__ void __launch_bounds__(256) k_fgr_gauss(alt3::MeshDev m, int nk, int nb, int ibz_lo, double sigma,
                                                    const int *__restrict__ mapinv, const double *__restrict__ freq,
-                                                   const VT *__restrict__ v33, double *__restrict__ P1,
-                                                   double *__restrict__ P2)
+                                                   const VT *__restrict__ v33, VT *__restrict__ P1,
+                                                   VT *__restrict__ P2)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nk) return;
@@ -369,14 +369,14 @@ __global__ void __launch_bounds__(256) k_fgr_gauss(alt3::MeshDev m, int nk, int 
         const double x = omega1 - (-f2 + freq[(size_t)p * nb + l3]);
         double F = (double)v33[base + w] * tpi2 / nk;
         if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) F = 0.0;
-        P1[base + w] = F * (exp(-x * x / 2 / sigma / sigma) * norm);
+        P1[base + w] = (VT)(F * (exp(-x * x / 2 / sigma / sigma) * norm));
     }
     {   // decay: w(q2,l2) + w(q1-q2,l3), F at -q2
         const int p = m.diff(a0, a1, a2, c0, c1, c2);
         const double x = omega1 - (f2 + freq[(size_t)p * nb + l3]);
         double F = (double)v33[base + m.neg(c0, c1, c2)] * tpi2 / nk;
         if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) F = 0.0;
-        P2[base + w] = F * (exp(-x * x / 2 / sigma / sigma) * norm);
+        P2[base + w] = (VT)(F * (exp(-x * x / 2 / sigma / sigma) * norm));
     }
 }
 
@@ -384,19 +384,36 @@ __global__ void __launch_bounds__(256) k_fgr_gauss(alt3::MeshDev m, int nk, int 
 
 namespace alt3 {
 
+// P in the precision of |V3|^2: FP32 for the mixed path, FP64 otherwise
+static int alloc_p(alt3_ctx *ctx, size_t nel)
+{
+    ctx->p_fp32 = ctx->v3_prec == ALT3_V3_MIXED;
+    if (ctx->p_fp32) {
+        ctx->d_p1.release();
+        ctx->d_p2.release();
+        ALT3_TRY(ctx->d_p1f.alloc(nel));
+        ALT3_TRY(ctx->d_p2f.alloc(nel));
+    } else {
+        ctx->d_p1f.release();
+        ctx->d_p2f.release();
+        ALT3_TRY(ctx->d_p1.alloc(nel));
+        ALT3_TRY(ctx->d_p2.alloc(nel));
+    }
+    return ALT3_OK;
+}
+
 int fgr_gauss_run(alt3_ctx *ctx, double sigma)
 {
     if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr_gauss needs alt3_v3sq_ibz first");
     if (!(sigma > 0)) return fail(ALT3_E_ARG, "Gaussian width must be positive");
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
-    ALT3_TRY(ctx->d_p1.alloc(nel));
-    ALT3_TRY(ctx->d_p2.alloc(nel));
+    ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     dim3 grid((unsigned)((nk + 255) / 256), (unsigned)(nb * nb * nb), (unsigned)ksub);
-    if (ctx->v3_prec == ALT3_V3_MIXED)
+    if (ctx->p_fp32)
         hipLaunchKernelGGL(k_fgr_gauss<float>, grid, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, sigma,
-                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1.p, ctx->d_p2.p);
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33f.p, ctx->d_p1f.p, ctx->d_p2f.p);
     else
         hipLaunchKernelGGL(k_fgr_gauss<double>, grid, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, sigma,
                            ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p);
@@ -417,8 +434,7 @@ int fgr_run(alt3_ctx *ctx)
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
     if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 9 * kFgrThreads)
         return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4608");
-    ALT3_TRY(ctx->d_p1.alloc(nel));
-    ALT3_TRY(ctx->d_p2.alloc(nel));
+    ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
@@ -432,7 +448,7 @@ int fgr_run(alt3_ctx *ctx)
     }                                                                                                            \
     if (mixed)                                                                                                   \
         hipLaunchKernelGGL((k_fgr<float, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
-                           ctx->d_mapinv.p, ctx->d_freq.p, (const float *)v, ctx->d_p1.p, ctx->d_p2.p);          \
+                           ctx->d_mapinv.p, ctx->d_freq.p, (const float *)v, ctx->d_p1f.p, ctx->d_p2f.p);        \
     else                                                                                                         \
         hipLaunchKernelGGL((k_fgr<double, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,            \
                            ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, (const double *)v, ctx->d_p1.p, ctx->d_p2.p);

@@ -424,17 +424,21 @@ int v3_copy_out(alt3_ctx *ctx, double *host)
     return ALT3_OK;
 }
 
-int copy_fortran5(alt3_ctx *ctx, const double *dev, double *host)
+template <typename T>
+int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host)
 {
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     size_t tot = (size_t)ksub * nk * nb * nb * nb;
     DBuf<double> tmp;
     ALT3_TRY(tmp.alloc(tot));
-    hipLaunchKernelGGL(k_to_fortran5<double>, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, ksub, nk, nb, dev, tmp.p);
+    hipLaunchKernelGGL(k_to_fortran5<T>, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, ksub, nk, nb, dev, tmp.p);
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipMemcpyAsync(host, tmp.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
 }
+
+template int copy_fortran5<double>(alt3_ctx *, const double *, double *);
+template int copy_fortran5<float>(alt3_ctx *, const float *, double *);
 
 }  // namespace alt3

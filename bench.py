@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--temps", default="100,300,800")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving"], default="step",
+                    help="step: the SW-Si 16^3 headline step (default); rocksalt: the 8-atom rock-salt-type "
+                         "cell at 12^3; sweep: 64 SW-Si instances through the server; serving: a Poisson "
+                         "stream of 512 kappa requests over four cells")
+    ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (reference for the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -120,6 +125,36 @@ def kappa_iso(k):
     return float(k.sum(0).trace() / 3)
 
 
+def run_workload(args):
+    """One JSON line for a workload other than the headline step (not timed by the driver)."""
+    from thermacond import capi, workloads
+    say = lambda m: log(m, args.quiet)
+    if args.workload == "rocksalt":
+        out = workloads.run_rocksalt(12, [300.0], log=say)
+        res = {"metric": "3-ph V3 elements/s (whole path, mixed vs FP64)", "unit": "V3 elements/s",
+               "value": out["mixed"]["elements_per_s"], "higher_is_better": True, "n_gpus": 1, "dtype": "f64 / fp32 V3",
+               "data": "synthetic (Born-Mayer-type rock-salt force constants, seed 0)", **out}
+    elif args.workload == "sweep":
+        out = workloads.run_sweep(64, 0, log=say)
+        res = {"metric": "batched sweep: V3 elements/s over 64 SW-Si instances", "unit": "V3 elements/s",
+               "value": out["elements_per_s"], "higher_is_better": True, "n_gpus": 1, "dtype": "fp32 V3 + f64",
+               "data": "synthetic (SW-Si, a ~ U[5.4,5.9] A, seed 0)",
+               "config": {"workload": "64 SW-Si instances, meshes {8,10,12,14}^3, T = 100,200,300,500,800 K, "
+                                      "RTA + CG, 4 server slots"}, **out}
+    else:
+        runs = {}
+        for name, p in (("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)):
+            runs[name] = workloads.run_serving(args.requests, 0, precision=p, log=say)
+            say(f"serving {name}: {runs[name]}")
+        res = {"metric": "kappa solves/hour at p99 latency (Poisson stream, 80 % load)", "unit": "solves/hour",
+               "value": runs["mixed"]["solves_per_hour"], "higher_is_better": True, "n_gpus": 1,
+               "dtype": "fp32 V3 + f64", "data": "synthetic (SW-Si, zincblende-type 4-atom, rock-salt-type 8- and "
+               "16-atom cells)", "config": {"workload": f"{args.requests} requests, cells uniform, mesh even "
+               f"6^3..cap {workloads.MESH_CAP}, T ~ U[100,800] K, 4 slots"},
+               "latency_ms": {"p50": runs["mixed"]["p50_ms"], "p99": runs["mixed"]["p99_ms"]}, **runs}
+    print(json.dumps(res))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -131,6 +166,9 @@ def main():
         sys.exit(2)
     from thermacond import capi, models, refio
 
+    if args.workload != "step":
+        run_workload(args)
+        return
     temps = [float(t) for t in args.temps.split(",")]
     n = args.mesh
     tmp = tempfile.mkdtemp(prefix="alt3_bench_")
@@ -156,9 +194,9 @@ def main():
                 "kappa_300K": {"rta": kappa_iso(r["out"]["kappa_rta"][ti]), "bte": kappa_iso(r["out"]["kappa_bte"][ti])}}
 
     # dominant kernel of the step: the tetrahedron FGR kernel (one launch per step)
-    v3_bytes = 4 if head_name == "mixed" else 8
+    v3_bytes = 4 if head_name == "mixed" else 8   # |V3|^2 and P1sm/P2sm share the precision
     fgr_ms = float(head["stage"][2])
-    fgr_bytes = nel * (v3_bytes + 16)  # read |V3|^2, write P1sm + P2sm (FP64)
+    fgr_bytes = nel * (v3_bytes + 2 * v3_bytes)  # read |V3|^2, write P1sm + P2sm
     fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
     res = {
         "metric": "3-ph V3 elements/s (mixed vs FP64) at kappa err <=0.5%; p50/p99 kappa-solve latency",
