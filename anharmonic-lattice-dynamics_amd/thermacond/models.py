@@ -153,11 +153,11 @@ def _sw_energy(x, pairs_i, pairs_j, trip_c, trip_j, trip_k, P=SW_SI):
     eps, sig, A, B, p, q, a, lam, gam = (P[k] for k in
                                          ("eps", "sigma", "A", "B", "p", "q", "a", "lam", "gamma"))
     rc = a * sig
-    d = x[pairs_j] - x[pairs_i]
+    d = x[..., pairs_j, :] - x[..., pairs_i, :]
     r = torch.sqrt((d * d).sum(-1))
     e2 = eps * A * (B * (sig / r) ** p - (sig / r) ** q) * torch.exp(sig / (r - rc))
-    dj = x[trip_j] - x[trip_c]
-    dk = x[trip_k] - x[trip_c]
+    dj = x[..., trip_j, :] - x[..., trip_c, :]
+    dk = x[..., trip_k, :] - x[..., trip_c, :]
     rj = torch.sqrt((dj * dj).sum(-1))
     rk = torch.sqrt((dk * dk).sum(-1))
     cth = (dj * dk).sum(-1) / (rj * rk)
@@ -170,7 +170,7 @@ def _shell_energy(x, pi, pj, D, r0, delta, rho):
     the pair's equilibrium distance r0, so the lattice is in equilibrium and the harmonic
     force constants are radial springs k = 2D/rho^2), plus a cubic term that scales the
     third derivative -6D/rho^3 by (1 + delta)."""
-    d = x[pj] - x[pi]
+    d = x[..., pj, :] - x[..., pi, :]
     r = (d * d).sum(-1).sqrt()
     u = (r - r0) / rho
     import torch
@@ -219,7 +219,10 @@ class ForceConstants:
 
 def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3: float,
                         h: float = 0.01, radius: float = 10.0, rel_drop: float = 1e-9,
-                        fc3_perturb=None, close_swap: bool = False) -> ForceConstants:
+                        fc3_perturb=None, close_swap: bool = False, batched: bool = False) -> ForceConstants:
+    """batched: evaluate all displaced clusters of one (i, alpha) in one autograd call (the
+    energy functions take a leading batch axis); the values are the same as one call per
+    configuration (False), only ten times faster."""
     import torch
     cl = _lattice_cluster(crys, radius)
     x0 = torch.tensor(np.array([c[2] for c in cl]), dtype=torch.float64)
@@ -246,21 +249,34 @@ def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3
                         fc2[(i, al, k, be)] = v
     # FC3: j within cutoff_fc3 of i
     fc3 = {}
+    signs = ((1, 1), (1, -1), (-1, 1), (-1, -1))
     for i in home:
         near = [j for j in range(nat) if float(torch.linalg.norm(x0[j] - x0[i])) < cutoff_fc3]
         for al in range(3):
+            if batched:
+                X = x0.repeat(len(near) * 3 * 4, 1, 1)
+                n = 0
+                for j in near:
+                    for be in range(3):
+                        for si, sj in signs:
+                            X[n, i, al] += si * h
+                            X[n, j, be] += sj * h
+                            n += 1
+                FB = forces(X)
+            n = 0
             for j in near:
                 for be in range(3):
-                    f = []
-                    for si, sj in ((1, 1), (1, -1), (-1, 1), (-1, -1)):
-                        x = x0.clone(); x[i, al] += si * h; x[j, be] += sj * h
-                        f.append(forces(x))
-                    d = -(f[0] - f[1] - f[2] + f[3]) / (4 * h * h)
-                    for k in range(nat):
-                        for ga in range(3):
-                            v = float(d[k, ga])
-                            if v != 0.0:
-                                fc3[(i, al, j, be, k, ga)] = v
+                    if batched:
+                        f = [FB[n + t] for t in range(4)]
+                        n += 4
+                    else:
+                        f = []
+                        for si, sj in signs:
+                            x = x0.clone(); x[i, al] += si * h; x[j, be] += sj * h
+                            f.append(forces(x))
+                    d = (-(f[0] - f[1] - f[2] + f[3]) / (4 * h * h)).numpy()
+                    for k, ga in zip(*np.nonzero(d)):
+                        fc3[(i, al, j, be, int(k), int(ga))] = float(d[k, ga])
     mx2 = max(abs(v) for v in fc2.values())
     mx3 = max(abs(v) for v in fc3.values())
     fc2 = {k: v for k, v in fc2.items() if abs(v) > rel_drop * mx2}
@@ -300,7 +316,7 @@ def _fd_force_constants(crys: Crystal, energy_fn, cutoff_pair: float, cutoff_fc3
     return ForceConstants(crys, atompos, iat0, iatc, fc2a, fc3a)
 
 
-def sw_silicon(a: float = 5.43, h: float = 0.01) -> ForceConstants:
+def sw_silicon(a: float = 5.43, h: float = 0.01, batched: bool = True) -> ForceConstants:
     """Stillinger-Weber Si force constants: FC2 to the 2nd shell, FC3 to the 2nd shell
     (the SW range 1.8*sigma = 3.77 A reaches only first neighbours, so three-body terms
     couple first and second neighbours)."""
@@ -325,7 +341,7 @@ def sw_silicon(a: float = 5.43, h: float = 0.01) -> ForceConstants:
     pi, pj, tc, tj, tk = T(pi), T(pj), T(tc), T(tj), T(tk)
     fn = lambda x: _sw_energy(x, pi, pj, tc, tj, tk)
     nn = a * math.sqrt(3) / 4
-    return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h)
+    return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h, batched=batched)
 
 
 def _sw_force_constants(crys: Crystal, P: dict, a: float, h: float) -> ForceConstants:
@@ -349,7 +365,7 @@ def _sw_force_constants(crys: Crystal, P: dict, a: float, h: float) -> ForceCons
     pi, pj, tc, tj, tk = T(pi), T(pj), T(tc), T(tj), T(tk)
     fn = lambda x: _sw_energy(x, pi, pj, tc, tj, tk, P)
     return _fd_force_constants(crys, fn, cutoff_pair=rc, cutoff_fc3=a / math.sqrt(2) * 1.05, h=h,
-                               radius=10.0 + 0.5 * a * (crys.natoms0 // 2 - 1), close_swap=True)
+                               radius=10.0 + 0.5 * a * (crys.natoms0 // 2 - 1), close_swap=True, batched=True)
 
 
 def sw_zincblende4(a: float = 5.65, h: float = 0.01) -> ForceConstants:
@@ -403,7 +419,8 @@ def born_mayer_rocksalt(crys: Optional[Crystal] = None, a: float = 5.6, rho: flo
     D, r0, dl = T(D, torch.float64), T(r0, torch.float64), T(dl, torch.float64)
     fn = lambda x: _shell_energy(x, pi, pj, D, r0, dl, rho)
     return _fd_force_constants(crys, fn, cutoff_pair=ROCKSALT_SHELLS[-1][0] * a * 1.01,
-                               cutoff_fc3=ROCKSALT_SHELLS[-1][0] * a * 1.05, h=h, radius=radius, close_swap=True)
+                               cutoff_fc3=ROCKSALT_SHELLS[-1][0] * a * 1.05, h=h, radius=radius, close_swap=True,
+                               batched=True)
 
 
 def born_mayer_supercell16(a: float = 5.6, seed: int = 0, h: float = 0.01) -> ForceConstants:

@@ -88,27 +88,14 @@ def run_rocksalt(n: int = 12, temps: Sequence[float] = (300.0,), log=print) -> d
 
 
 # ---------------------------------------------------------------------------- batched sweep
-def _sw_fc(a: float):
-    import torch
-    torch.set_num_threads(1)
-    return models.sw_silicon(a)
-
-
-def sweep_requests(count: int = 64, seed: int = 0, workers: int = 16) -> List[serving.Request]:
+def sweep_requests(count: int = 64, seed: int = 0) -> List[serving.Request]:
     """SW-Si instances: lattice constant ~ U[5.4, 5.9] A per instance, mesh from
-    {8,10,12,14}^3, temperatures from {100,200,300,500,800} K (all five per instance).
-    The finite-difference force constants are generated in a pool of host processes
-    (started before this process touches the GPU)."""
-    import concurrent.futures as cf
-    import multiprocessing as mp
+    {8,10,12,14}^3, temperatures from {100,200,300,500,800} K (all five per instance)."""
     rng = np.random.default_rng(seed)
     draws = [(round(float(rng.uniform(5.4, 5.9)), 4), int(rng.choice([8, 10, 12, 14]))) for _ in range(count)]
-    todo = sorted({a for a, _ in draws if ("sw-si", a, 0) not in _fc_done})
-    if todo:
-        with cf.ProcessPoolExecutor(max_workers=min(workers, os.cpu_count() or 1, len(todo)),
-                                    mp_context=mp.get_context("fork")) as ex:
-            for a, fc in zip(todo, ex.map(_sw_fc, todo)):
-                _fc_done[("sw-si", a, 0)] = fc
+    for a, _ in draws:
+        if ("sw-si", a, 0) not in _fc_done:
+            _fc_done[("sw-si", a, 0)] = models.sw_silicon(a, batched=True)
     reqs = []
     for a, n in draws:
         reqs.append(serving.Request(f"sw-si-a{a}", cell_inputs("sw-si", n, a), [100.0, 200.0, 300.0, 500.0, 800.0],
