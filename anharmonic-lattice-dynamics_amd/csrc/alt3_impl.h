@@ -115,7 +115,7 @@ struct V3Tables {
 };
 int build_v3_tables(const alt3_system *sys, V3Tables &v);
 int v3_prep_launch(alt3_ctx *ctx, int ksub);
-bool v3_rows_supported(const alt3_ctx *ctx);
+bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 
 }  // namespace alt3

@@ -344,7 +344,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
         prec = ALT3_V3_FP64;
         ctx->v3_prec = prec;
     }
-    if (prec != ALT3_V3_FP64_REF && v3_rows_supported(ctx)) {
+    if (prec != ALT3_V3_FP64_REF && v3_rows_supported(ctx, prec == ALT3_V3_MIXED)) {
         // 6, 12 or 24 branches: one workgroup per q2 tile handles every l1 (v3_rows.hip)
         const bool fp32 = prec == ALT3_V3_MIXED;
         if (fp32) {

@@ -170,7 +170,12 @@ int dispatch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const do
 
 namespace alt3 {
 
-bool v3_rows_supported(const alt3_ctx *ctx) { return ctx->V.ndr > 0 && (ctx->nb == 6 || ctx->nb == 12 || ctx->nb == 24); }
+// 24 branches only in FP32: the FP64 tile holds 4 q2 points, too few to amortise the G' rows
+// (k_v3_main is faster there)
+bool v3_rows_supported(const alt3_ctx *ctx, bool fp32)
+{
+    return ctx->V.ndr > 0 && (ctx->nb == 6 || ctx->nb == 12 || (ctx->nb == 24 && fp32));
+}
 
 // G' must be prepared (v3_prep_launch); out holds ksub*nb^3*nk values of the precision's type
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
