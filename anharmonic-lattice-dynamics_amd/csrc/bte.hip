@@ -123,6 +123,97 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
     }
 }
 
+// one workgroup per (s, chunk of 256 mesh points q2), every (l1, l2, l3) inside, for cells of
+// NB branches: the occupations at q1+q2 and q1-q2 (the gathers of k_tau) are read once per
+// thread into registers, so the inner loops only stream the P rows (12 independent loads per
+// (l1, l2)) and write the R rows.  Partial Q sums per (s, l1, chunk).
+template <int NT, typename PT, int NB>
+__global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz_lo, const int *__restrict__ mapinv,
+                                                const double *__restrict__ dist, size_t dstr,
+                                                const PT *__restrict__ P1sm, const PT *__restrict__ P2sm,
+                                                double *__restrict__ qpart, size_t qstr, double *__restrict__ R1,
+                                                double *__restrict__ R2, size_t rstr)
+{
+    __shared__ double sh[256];
+    const int s = blockIdx.y, nch = gridDim.x, ch = blockIdx.x;
+    const int j = ch * 256 + threadIdx.x;
+    const bool valid = j < nk;
+    const int jc = valid ? j : nk - 1;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2, c0, c1, c2;
+    m.coords(k1, a0, a1, a2);
+    m.coords(jc, c0, c1, c2);
+    const bool in1 = m.sum_inside(a0, a1, a2, c0, c1, c2), in2 = m.diff_inside(a0, a1, a2, c0, c1, c2);
+    const int np = m.sum(a0, a1, a2, c0, c1, c2), nm = m.diff(a0, a1, a2, c0, c1, c2);
+    double n3p[NT][NB], n3m[NT][NB], d2[NT][NB];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int l = 0; l < NB; l++) {
+            n3p[t][l] = dist[t * dstr + (size_t)np * NB + l] + 1;
+            n3m[t][l] = dist[t * dstr + (size_t)nm * NB + l] + 1;
+            d2[t][l] = dist[t * dstr + (size_t)jc * NB + l];
+        }
+    for (int l1 = 0; l1 < NB; l1++) {
+        double d1[NT], q[NT], qn[NT], qu[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            d1[t] = dist[t * dstr + (size_t)k1 * NB + l1];
+            q[t] = qn[t] = qu[t] = 0;
+        }
+#pragma unroll 1
+        for (int l2 = 0; l2 < NB; l2++) {
+            const size_t row = (((size_t)s * NB + l1) * NB + l2) * NB * (size_t)nk + jc;   // (s,l1,l2,l3=0)
+            PT P1[NB], P2[NB];
+#pragma unroll
+            for (int l3 = 0; l3 < NB; l3++) {
+                P1[l3] = P1sm[row + (size_t)l3 * nk];
+                P2[l3] = P2sm[row + (size_t)l3 * nk];
+            }
+            // select the occupations of branch l2 without dynamic register indexing
+            double dd[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                dd[t] = d2[t][0];
+#pragma unroll
+                for (int l = 1; l < NB; l++) dd[t] = (l2 == l) ? d2[t][l] : dd[t];
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const double f1 = d1[t] * dd[t], f2 = d1[t] * (dd[t] + 1);
+                double r1 = 0, r2 = 0;
+#pragma unroll
+                for (int l3 = 0; l3 < NB; l3++) {
+                    const double p1 = (double)P1[l3] * (f1 * n3p[t][l3]);
+                    const double p2 = (double)P2[l3] * (f2 * n3m[t][l3]);
+                    q[t] = q[t] + (p1 + 0.5 * p2);
+                    if (in1) qn[t] += p1; else qu[t] += p1;
+                    if (in2) qn[t] += 0.5 * p2; else qu[t] += 0.5 * p2;
+                    r1 += p1;
+                    r2 += p2;
+                }
+                if (valid) {
+                    const size_t o = (((size_t)s * NB + l1) * NB + l2) * (size_t)nk + j;
+                    R1[t * rstr + o] = r1;
+                    R2[t * rstr + o] = r2;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            if (!valid) q[t] = qn[t] = qu[t] = 0;
+            const double a = block_sum(q[t], sh), b = block_sum(qn[t], sh), c = block_sum(qu[t], sh);
+            if (threadIdx.x == 0) {
+                double *o = qpart + t * qstr + 3 * (((size_t)s * NB + l1) * nch + ch);
+                o[0] = a;
+                o[1] = b;
+                o[2] = c;
+            }
+        }
+    }
+}
+
+// Q, QN, QU of (s, l1) = sum of the m partial sums of that row
 __global__ void k_tau_sum(int n, int nb, const double *__restrict__ qpart, double *__restrict__ Q,
                           double *__restrict__ QN, double *__restrict__ QU)
 {
@@ -582,12 +673,33 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
         hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
                            temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr.p);
     ALT3_HIP(hipGetLastError());
-    const size_t nblk = (size_t)ksub * nb * nb, pstr = 3 * nblk;
+    const int nch = (nk + 255) / 256;
+    const bool rows = nb == 6;   // k_tau_q2: occupations in registers (6 branches)
+    const int parts = rows ? nch : nb;   // partial sums per (s, l1)
+    const size_t nblk = (size_t)ksub * nb * nb, pstr = (size_t)3 * ksub * nb * parts;
     DBuf<double> qpart;
     ALT3_TRY(qpart.alloc((size_t)nt * pstr));
     for (int t0 = 0; t0 < nt; t0 += 4) {
         const int g = std::min(4, nt - t0);
-        if (ctx->p_fp32) {
+        if (rows) {
+            const dim3 grid((unsigned)nch, (unsigned)ksub);
+#define ALT3_TAU_Q2(G, PT, P1, P2)                                                                                 \
+    hipLaunchKernelGGL((k_tau_q2<G, PT, 6>), grid, dim3(256), 0, ctx->stream, md, nk, ctx->ibz_lo, ctx->d_mapinv.p, \
+                       ctx->d_dist.p + t0 * dstr, dstr, P1, P2, qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr,  \
+                       ctx->d_r2.p + t0 * rstr, rstr)
+            if (ctx->p_fp32) {
+                if (g == 1) ALT3_TAU_Q2(1, float, ctx->d_p1f.p, ctx->d_p2f.p);
+                else if (g == 2) ALT3_TAU_Q2(2, float, ctx->d_p1f.p, ctx->d_p2f.p);
+                else if (g == 3) ALT3_TAU_Q2(3, float, ctx->d_p1f.p, ctx->d_p2f.p);
+                else ALT3_TAU_Q2(4, float, ctx->d_p1f.p, ctx->d_p2f.p);
+            } else {
+                if (g == 1) ALT3_TAU_Q2(1, double, ctx->d_p1.p, ctx->d_p2.p);
+                else if (g == 2) ALT3_TAU_Q2(2, double, ctx->d_p1.p, ctx->d_p2.p);
+                else if (g == 3) ALT3_TAU_Q2(3, double, ctx->d_p1.p, ctx->d_p2.p);
+                else ALT3_TAU_Q2(4, double, ctx->d_p1.p, ctx->d_p2.p);
+            }
+#undef ALT3_TAU_Q2
+        } else if (ctx->p_fp32) {
             auto *kern = g == 1 ? k_tau<1, float> : g == 2 ? k_tau<2, float> : g == 3 ? k_tau<3, float> : k_tau<4, float>;
             hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1f.p, ctx->d_p2f.p,
@@ -601,7 +713,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
         ALT3_HIP(hipGetLastError());
     }
     for (int t = 0; t < nt; t++)
-        hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, nb,
+        hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, parts,
                            qpart.p + t * pstr, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr, ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
