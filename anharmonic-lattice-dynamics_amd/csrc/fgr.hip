@@ -17,10 +17,14 @@
 
 namespace {
 
-// weight_tet for one tetrahedron: corner energies w[4] (already rounded to REAL*4) at e
-__device__ void weight_tet1(const float *w, double e, double *c)
+// SSORT of the corner energies (descending, first maximum wins) and the reference's map
+// uns from sorted position to corner (tetrahedron.f90:248-300)
+__device__ __forceinline__ void tet_sort(const float *w, float *en, int *uns)
 {
-    float en[4] = {w[0], w[1], w[2], w[3]};
+    en[0] = w[0];
+    en[1] = w[1];
+    en[2] = w[2];
+    en[3] = w[3];
     // SSORT: descending, first maximum wins
     for (int i = 0; i < 3; i++) {
         int m = i;
@@ -32,7 +36,6 @@ __device__ void weight_tet1(const float *w, double e, double *c)
             en[m] = t;
         }
     }
-    int uns[4];
     if (en[0] != en[1] && en[1] != en[2] && en[2] != en[3]) {
         for (int k = 0; k < 4; k++) uns[k] = (en[k] == w[0]) ? 1 : (en[k] == w[1]) ? 2 : (en[k] == w[2]) ? 3 : 4;
     } else {
@@ -52,6 +55,14 @@ __device__ void weight_tet1(const float *w, double e, double *c)
             }
         }
     }
+}
+
+// weight_tet for one tetrahedron: corner energies w[4] (already rounded to REAL*4) at e
+__device__ void weight_tet1(const float *w, double e, double *c)
+{
+    float en[4];
+    int uns[4];
+    tet_sort(w, en, uns);
     float e1 = en[3], e2 = en[2], e3 = en[1], e4 = en[0];
     float e21 = e2 - e1, e31 = e3 - e1, e41 = e4 - e1, e32 = e3 - e2, e42 = e4 - e2, e43 = e4 - e3;
     const double small1 = 1e-10;
@@ -110,6 +121,59 @@ __device__ void weight_tet1(const float *w, double e, double *c)
         cc[0] = ((e - E3) / (double)e43 + (e - E2) / (double)e42 + (e - E1) / (double)e41) * (d4 * d4) / (double)(e41 * e42 * e43);
     }
     double out[4] = {0, 0, 0, 0};
+    for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
+    c[0] = out[0];
+    c[1] = out[1];
+    c[2] = out[2];
+    c[3] = out[3];
+}
+
+// The same weights entirely in FP32, for the mixed path (whose |V3|^2 and P are FP32): the
+// same sort, case split and 1e-10 clamp, with single-precision quotients.
+__device__ void weight_tet1f(const float *w, float e, float *c)
+{
+    float en[4];
+    int uns[4];
+    tet_sort(w, en, uns);
+    const float e1 = en[3], e2 = en[2], e3 = en[1], e4 = en[0];
+    float e21 = e2 - e1, e31 = e3 - e1, e41 = e4 - e1, e32 = e3 - e2, e42 = e4 - e2, e43 = e4 - e3;
+    const float small = 1e-10f;
+#define CLAMPF(x)                                \
+    {                                            \
+        if (fabsf(x) < small && (x) > 0.0f) (x) = small;  \
+        if (fabsf(x) < small && (x) < 0.0f) (x) = -small; \
+    }
+    CLAMPF(e21) CLAMPF(e31) CLAMPF(e41) CLAMPF(e42) CLAMPF(e43)
+#undef CLAMPF
+    float cc[4];
+    if (e < e1 || e > e4) {
+        c[0] = c[1] = c[2] = c[3] = 0;
+        return;
+    } else if (e1 <= e && e <= e2) {
+        const float d1 = e - e1, d13 = d1 * d1 * d1;
+        cc[3] = ((e2 - e) / e21 + (e3 - e) / e31 + (e4 - e) / e41) * (d1 * d1) / (e41 * e31 * e21);
+        cc[2] = d13 / (e21 * e21 * e31 * e41);
+        cc[1] = d13 / (e21 * (e31 * e31) * e41);
+        cc[0] = d13 / (e21 * e31 * (e41 * e41));
+    } else if (e2 <= e && e <= e3) {
+        const float r31 = 1.0f / (e31 * e31), r41 = 1.0f / (e41 * e41), r32 = 1.0f / (e32 * e32), r42 = 1.0f / (e42 * e42);
+        const float a32 = (e3 - e) * (e - e2), a41 = (e4 - e) * (e - e1), a31 = (e3 - e) * (e - e1), a42 = (e4 - e) * (e - e2);
+        const float i1 = 1.0f / (e42 * e32), i2 = 1.0f / (e41 * e42), i3 = 1.0f / (e32 * e41);
+        const float i4 = 1.0f / (e42 * e31), i5 = 1.0f / (e31 * e32), i6 = 1.0f / (e31 * e41);
+        const float b11 = a32 * i1 + a41 * i2 + a31 * i3, b12 = a41 * i4 + a42 * i1 + a31 * i5;
+        const float b21 = a32 * i4 + a42 * i2 + a31 * i6, b22 = a32 * i5 + a41 * i6 + a42 * i3;
+        cc[3] = 0.5f * ((e3 - e) * r31 * b11 + (e4 - e) * r41 * b12);
+        cc[2] = 0.5f * ((e3 - e) * r32 * b21 + (e4 - e) * r42 * b22);
+        cc[1] = 0.5f * ((e - e2) * r32 * b21 + (e - e1) * r31 * b11);
+        cc[0] = 0.5f * ((e - e2) * r42 * b22 + (e - e1) * r41 * b12);
+    } else {
+        const float d4 = e4 - e, d43 = d4 * d4 * d4;
+        cc[3] = d43 / (e41 * e41 * e42 * e43);
+        cc[2] = d43 / (e41 * (e42 * e42) * e43);
+        cc[1] = d43 / (e41 * e42 * (e43 * e43));
+        cc[0] = ((e - e3) / e43 + (e - e2) / e42 + (e - e1) / e41) * (d4 * d4) / (e41 * e42 * e43);
+    }
+    float out[4] = {0, 0, 0, 0};
     for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
     c[0] = out[0];
     c[1] = out[1];
@@ -270,10 +334,17 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
                 v[y] = m.index(e0, e1, e2);
             }
             const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
-            double c[4];
-            weight_tet1(w4, e, c);
-            for (int y = 0; y < 4; y++)
-                if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+            if constexpr (sizeof(VT) == sizeof(float)) {
+                float c[4];
+                weight_tet1f(w4, (float)e, c);
+                for (int y = 0; y < 4; y++)
+                    if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)(c[y] / 6.0f));
+            } else {
+                double c[4];
+                weight_tet1(w4, e, c);
+                for (int y = 0; y < 4; y++)
+                    if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+            }
         }
     };
     for (int l1 = 0; l1 < nb; l1++) {
