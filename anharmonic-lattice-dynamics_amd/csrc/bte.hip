@@ -428,8 +428,9 @@ __global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restric
 
 // y = A x for a row-major n x n matrix: one wavefront per row, columns in order per lane
 __global__ void __launch_bounds__(256) k_matvec_rows(int n, const double *__restrict__ A, const double *__restrict__ x,
-                                                     double *__restrict__ y)
+                                                     double *__restrict__ y, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;   // CG already converged
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
@@ -455,8 +456,9 @@ __global__ void k_rhs(int nibz, int nb, double tempk, const int *__restrict__ ma
 // ------------------------------------------------------------------ CG (HBM-bound)
 // y = A x, A column-major n x n: one thread per row, sequential over columns
 __global__ void __launch_bounds__(256) k_matvec(int n, const double *__restrict__ A, const double *__restrict__ x,
-                                                double *__restrict__ y)
+                                                double *__restrict__ y, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;   // CG already converged
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double acc = 0;
@@ -512,23 +514,51 @@ __global__ void k_reduce2(int nblk, const double *__restrict__ part, double *__r
 }
 
 // v0 = grad; vmin -= num/den conj; grad -= num/den aux    (sc[0]=num, sc[1]=den)
-__global__ void k_cg_step(int n, const double *__restrict__ sc, double *__restrict__ vmin, double *__restrict__ grad,
-                          double *__restrict__ v0, const double *__restrict__ conj, const double *__restrict__ aux)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double r = sc[0] / sc[1];
-    v0[i] = grad[i];
-    vmin[i] = vmin[i] - r * conj[i];
-    grad[i] = grad[i] - r * aux[i];
-}
-
 // conj = -grad + (num/den) conj       (sc[4]=grad.grad, sc[5]=v0.v0)
-__global__ void k_cg_dir(int n, const double *__restrict__ sc, const double *__restrict__ grad, double *__restrict__ conj)
+// One CG iteration of cg_quadratic (cg.f90:33-50) after aux = A conj, in a single workgroup
+// (the vectors have nibz*nb*3 entries): num, den, the steps of vmin and grad, the error
+// (num/den)^2 |conj|^2 / |vmin|^2 against the tolerance, and the new direction.  state[0]
+// turns 1 at the converged iteration (later launches return at once); state[1] = iteration.
+__global__ void __launch_bounds__(1024) k_cg_update(int n, int it, double tol, double *__restrict__ vmin,
+                                                    double *__restrict__ grad, double *__restrict__ conj,
+                                                    const double *__restrict__ aux, double *__restrict__ v0,
+                                                    int *__restrict__ state)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    conj[i] = -grad[i] + sc[4] / sc[5] * conj[i];
+    __shared__ double sh[1024];
+    if (state[0]) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double a = 0, b = 0;
+    for (int i = tid; i < n; i += nt) {
+        a += grad[i] * conj[i];
+        b += aux[i] * conj[i];
+    }
+    const double num = block_sum(a, sh), den = block_sum(b, sh);
+    const double r = num / den;
+    a = b = 0;
+    for (int i = tid; i < n; i += nt) {
+        v0[i] = grad[i];
+        vmin[i] = vmin[i] - r * conj[i];
+        grad[i] = grad[i] - r * aux[i];
+        a += vmin[i] * vmin[i];
+        b += conj[i] * conj[i];
+    }
+    const double v2 = block_sum(a, sh), cc = block_sum(b, sh);
+    const double err = (num * num / den / den) * cc / v2;
+    if (err < tol) {
+        if (tid == 0) {
+            state[0] = 1;
+            state[1] = it;
+        }
+        return;
+    }
+    a = b = 0;
+    for (int i = tid; i < n; i += nt) {
+        a += grad[i] * grad[i];
+        b += v0[i] * v0[i];
+    }
+    const double gg = block_sum(a, sh), vv = block_sum(b, sh);
+    for (int i = tid; i < n; i += nt) conj[i] = -grad[i] + gg / vv * conj[i];
+    if (tid == 0) state[1] = it;
 }
 
 __global__ void k_axpy_init(int n, const double *__restrict__ ax, const double *__restrict__ b, double *__restrict__ grad,
@@ -840,45 +870,45 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
               double *funcmin, int *iters, bool rowmajor)
 {
-    auto matvec = [&](const double *xin, double *yout) {
+    auto matvec = [&](const double *xin, double *yout, const int *skip) {
         if (rowmajor)
-            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, dA, xin, yout);
+            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
         else
-            hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout);
+            hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
     };
     ALT3_TRY(ctx->d_cg.alloc((size_t)4 * n));
     double *grad = ctx->d_cg.p, *conj = grad + n, *aux = conj + n, *v0 = aux + n;
     const int nblk = 256;
     ALT3_TRY(ctx->d_red.alloc(2 * nblk + 16));
     double *part = ctx->d_red.p, *sc = part + 2 * nblk;
+    DBuf<int> state;   // converged flag, iteration
+    ALT3_TRY(state.alloc(2));
+    ALT3_HIP(hipMemsetAsync(state.p, 0, 2 * sizeof(int), ctx->stream));
     const int gb = (n + 255) / 256;
     const double tol = 1.0 / std::pow(10.0, 2 * ndig);
-    matvec(dx, aux);
+    matvec(dx, aux, nullptr);
     hipLaunchKernelGGL(k_axpy_init, dim3(gb), dim3(256), 0, ctx->stream, n, aux, db, grad, conj);
     ALT3_HIP(hipGetLastError());
-    int iter = 0;
-    double hs[8];
-    while (iter < max_iter) {
-        iter++;
-        matvec(conj, aux);
-        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, grad, conj, aux, conj, part);
-        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);  // num, den
-        hipLaunchKernelGGL(k_cg_step, dim3(gb), dim3(256), 0, ctx->stream, n, sc, dx, grad, v0, conj, aux);
-        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, dx, dx, conj, conj, part);
-        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 2);  // v2, cc
-        hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, grad, grad, v0, v0, part);
-        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 4);  // gg, v0v0
+    // iterations run on the device; the host looks at the converged flag every 8 of them
+    int iter = max_iter;
+    for (int it = 1; it <= max_iter; it++) {
+        matvec(conj, aux, state.p);
+        hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, ctx->stream, n, it, tol, dx, grad, conj, aux, v0,
+                           state.p);
         ALT3_HIP(hipGetLastError());
-        ALT3_HIP(hipMemcpyAsync(hs, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        ALT3_HIP(hipStreamSynchronize(ctx->stream));
-        double num = hs[0], den = hs[1], v2 = hs[2], cc = hs[3];
-        double err = (num * num / den / den) * cc / v2;
-        if (err < tol) break;
-        hipLaunchKernelGGL(k_cg_dir, dim3(gb), dim3(256), 0, ctx->stream, n, sc, grad, conj);
-        ALT3_HIP(hipGetLastError());
+        if (it % 8 == 0 || it == max_iter) {
+            int st[2];
+            ALT3_HIP(hipMemcpyAsync(st, state.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+            ALT3_HIP(hipStreamSynchronize(ctx->stream));
+            if (st[0]) {
+                iter = st[1];
+                break;
+            }
+        }
     }
+    double hs[2];
     if (funcmin) {
-        matvec(dx, v0);
+        matvec(dx, v0, nullptr);
         hipLaunchKernelGGL(k_dot2, dim3(nblk), dim3(256), 0, ctx->stream, n, dx, v0, dx, db, part);
         hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(256), 0, ctx->stream, nblk, part, sc, 0);
         ALT3_HIP(hipMemcpyAsync(hs, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
