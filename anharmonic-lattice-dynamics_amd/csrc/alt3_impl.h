@@ -68,6 +68,8 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
+    // grow-only: keeps a larger allocation (scratch reused across calls without hipMalloc)
+    int reserve(size_t count) { return (p && count <= n) ? ALT3_OK : alloc(count); }
     size_t bytes() const { return n * sizeof(T); }
 };
 
@@ -193,5 +195,10 @@ struct alt3_ctx {
     alt3::DBuf<double> d_cm, d_rhs;
     bool have_cm = false;
     alt3::DBuf<double> d_cg;                       // CG work vectors
+    // per-call scratch kept across calls (no hipMalloc/hipFree on the solve path)
+    alt3::DBuf<int> d_flags;                       // [0] V3 den=0, [1] tau, [2] kappa_rta, [3] kappa_bte, [4,5] CG
+    alt3::DBuf<int> d_negs;                        // subst_eivecs pairs
+    alt3::DBuf<double> d_qpart, d_wq, d_ev7, d_evec0;
+    alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
     alt3::DBuf<double> d_red;                      // reduction scratch
 };

@@ -689,9 +689,8 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     const MeshDev md = MeshDev::make(ctx->nc);
     ctx->have_tau = ctx->have_cm = false;
     ctx->tau_temps.clear();
-    DBuf<int> derr;
-    ALT3_TRY(derr.alloc(1));
-    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    int *derr = ctx->d_flags.p + 1;
+    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     const size_t dstr = (size_t)nk * nb, qstr = (size_t)ksub * nb, rstr = (size_t)ksub * nb * nb * nk;
     ALT3_TRY(ctx->d_dist.alloc(nt * dstr));
     ALT3_TRY(ctx->d_q.alloc(nt * qstr));
@@ -701,14 +700,14 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     ALT3_TRY(ctx->d_r2.alloc(nt * rstr));
     for (int t = 0; t < nt; t++)
         hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
-                           temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr.p);
+                           temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr);
     ALT3_HIP(hipGetLastError());
     const int nch = (nk + 255) / 256;
     const bool rows = nb == 6;   // k_tau_q2: occupations in registers (6 branches)
     const int parts = rows ? nch : nb;   // partial sums per (s, l1)
     const size_t nblk = (size_t)ksub * nb * nb, pstr = (size_t)3 * ksub * nb * parts;
-    DBuf<double> qpart;
-    ALT3_TRY(qpart.alloc((size_t)nt * pstr));
+    DBuf<double> &qpart = ctx->d_qpart;
+    ALT3_TRY(qpart.reserve((size_t)nt * pstr));
     for (int t0 = 0; t0 < nt; t0 += 4) {
         const int g = std::min(4, nt - t0);
         if (rows) {
@@ -747,7 +746,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
                            qpart.p + t * pstr, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr, ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
     ctx->tau_temps.assign(temps, temps + nt);
@@ -807,20 +806,19 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
     ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
     ALT3_TRY(ctx->d_taueff.alloc((size_t)nibz * nb));
-    DBuf<int> derr;
-    ALT3_TRY(derr.alloc(1));
-    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    int *derr = ctx->d_flags.p + 2;
+    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, nk,
                        ctx->d_mapinv.p, ctx->d_freq.p, ctx->dist(), ctx->d_eival.p, ctx->d_vel.p, ctx->q(), tempk,
                        temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
-                       ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr.p);
+                       ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr);
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
     if (FRTA) ALT3_HIP(hipMemcpyAsync(FRTA, ctx->d_frta.p, sizeof(double) * nibz * nb * 3, hipMemcpyDeviceToHost, ctx->stream));
     if (taueff) ALT3_HIP(hipMemcpyAsync(taueff, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToHost, ctx->stream));
@@ -881,9 +879,8 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
     const int nblk = 256;
     ALT3_TRY(ctx->d_red.alloc(2 * nblk + 16));
     double *part = ctx->d_red.p, *sc = part + 2 * nblk;
-    DBuf<int> state;   // converged flag, iteration
-    ALT3_TRY(state.alloc(2));
-    ALT3_HIP(hipMemsetAsync(state.p, 0, 2 * sizeof(int), ctx->stream));
+    int *state = ctx->d_flags.p + 4;   // converged flag, iteration
+    ALT3_HIP(hipMemsetAsync(state, 0, 2 * sizeof(int), ctx->stream));
     const int gb = (n + 255) / 256;
     const double tol = 1.0 / std::pow(10.0, 2 * ndig);
     matvec(dx, aux, nullptr);
@@ -892,13 +889,13 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
     // iterations run on the device; the host looks at the converged flag every 8 of them
     int iter = max_iter;
     for (int it = 1; it <= max_iter; it++) {
-        matvec(conj, aux, state.p);
+        matvec(conj, aux, state);
         hipLaunchKernelGGL(k_cg_update, dim3(1), dim3(1024), 0, ctx->stream, n, it, tol, dx, grad, conj, aux, v0,
-                           state.p);
+                           state);
         ALT3_HIP(hipGetLastError());
         if (it % 8 == 0 || it == max_iter) {
             int st[2];
-            ALT3_HIP(hipMemcpyAsync(st, state.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+            ALT3_HIP(hipMemcpyAsync(st, state, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
             ALT3_HIP(hipStreamSynchronize(ctx->stream));
             if (st[0]) {
                 iter = st[1];
@@ -929,9 +926,8 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     ALT3_TRY(ctx->d_veloci.alloc((size_t)nibz * nb * 3));
     ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
     ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
-    DBuf<int> derr;
-    ALT3_TRY(derr.alloc(1));
-    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    int *derr = ctx->d_flags.p + 3;
+    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     if (!ctx->have_veloci) {   // star-averaged IBZ velocities depend on the eigen data only
         hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
                            ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
@@ -941,11 +937,11 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
                        ctx->d_eival.p, temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p,
-                       ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr.p);
+                       ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr);
     hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");

@@ -103,6 +103,7 @@ int alt3_create(const alt3_system *sys, alt3_ctx **out)
     }
     auto init = [&]() -> int {
         ALT3_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ALT3_TRY(ctx->d_flags.alloc(16));
         int dev = 0, lds = 0;
         ALT3_HIP(hipGetDevice(&dev));
         ALT3_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -149,6 +150,8 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_opk.release(); ctx->d_opk_inv.release(); ctx->d_kibz.release(); ctx->d_frta.release();
     ctx->d_kpart.release(); ctx->d_kappa.release(); ctx->d_taueff.release(); ctx->d_veloci.release();
     ctx->d_cm.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
+    ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
+    ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -163,6 +166,8 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     b += ctx->d_evec.bytes() + ctx->d_eival.bytes() + ctx->d_freq.bytes() + ctx->d_vel.bytes();
     b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
     b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
+    b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
+    b += ctx->d_ephf.bytes() + ctx->d_evf.bytes() + ctx->d_gpf32.bytes();
     *bytes = b;
     return ALT3_OK;
 }

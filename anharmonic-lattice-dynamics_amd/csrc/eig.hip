@@ -323,9 +323,9 @@ int eig_run(alt3_ctx *ctx)
 {
     const int nk = ctx->nk, nb = ctx->nb;
     if (nb > 64) return fail(ALT3_E_ARG, "more than 64 branches per cell");
-    DBuf<double> ev7, evec0;
-    ALT3_TRY(ev7.alloc((size_t)nk * kDq * nb));
-    ALT3_TRY(evec0.alloc((size_t)nk * nb * nb * 2));
+    DBuf<double> &ev7 = ctx->d_ev7, &evec0 = ctx->d_evec0;
+    ALT3_TRY(ev7.reserve((size_t)nk * kDq * nb));
+    ALT3_TRY(evec0.reserve((size_t)nk * nb * nb * 2));
     ALT3_TRY(ctx->d_eival.alloc((size_t)nk * nb));
     ALT3_TRY(ctx->d_freq.alloc((size_t)nk * nb));
     ALT3_TRY(ctx->d_vel.alloc((size_t)nk * nb * 3));
@@ -352,21 +352,16 @@ int eig_run(alt3_ctx *ctx)
             int m0 = (ctx->nc[0] - c0) % ctx->nc[0], m1 = (ctx->nc[1] - c1) % ctx->nc[1], m2 = (ctx->nc[2] - c2) % ctx->nc[2];
             negidx[n] = m2 + ctx->nc[2] * (m1 + ctx->nc[1] * m0);
         }
-        DBuf<int> dn, di;
-        ALT3_TRY(dn.alloc(nneg));
-        ALT3_TRY(di.alloc(nneg));
-        ALT3_HIP(hipMemcpyAsync(dn.p, negs.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
-        ALT3_HIP(hipMemcpyAsync(di.p, negidx.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
+        ALT3_TRY(ctx->d_negs.reserve(2 * (size_t)nneg));
+        int *dn = ctx->d_negs.p, *di = dn + nneg;
+        ALT3_HIP(hipMemcpyAsync(dn, negs.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
+        ALT3_HIP(hipMemcpyAsync(di, negidx.data(), sizeof(int) * nneg, hipMemcpyHostToDevice, ctx->stream));
         int tot = nneg * nb * nb;
-        hipLaunchKernelGGL(k_subst, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, nneg, nb, dn.p, di.p, ctx->d_evec.p);
+        hipLaunchKernelGGL(k_subst, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, nneg, nb, dn, di, ctx->d_evec.p);
         ALT3_HIP(hipGetLastError());
         ALT3_HIP(hipStreamSynchronize(ctx->stream));
-        dn.release();
-        di.release();
     }
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
-    ev7.release();
-    evec0.release();
     ctx->have_eig = true;
     ctx->have_veloci = false;
     return ALT3_OK;

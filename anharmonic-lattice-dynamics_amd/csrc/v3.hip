@@ -329,9 +329,8 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ibz_hi - ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     MeshDims md{ctx->nc[0], ctx->nc[1], ctx->nc[2]};
-    DBuf<int> derr;
-    ALT3_TRY(derr.alloc(1));
-    ALT3_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), ctx->stream));
+    int *derr = ctx->d_flags.p + 0;
+    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
     ctx->ibz_lo = ibz_lo;
     ctx->ibz_hi = ibz_hi;
@@ -355,11 +354,11 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
             ALT3_TRY(ctx->d_v33.alloc(nel));
         }
         ALT3_TRY(v3_prep_launch(ctx, ksub));
-        ALT3_TRY(v3_rows_run(ctx, ksub, fp32, derr.p));
+        ALT3_TRY(v3_rows_run(ctx, ksub, fp32, derr));
     } else if (prec == ALT3_V3_MIXED) {
         ctx->d_v33.release();
         ALT3_TRY(ctx->d_v33f.alloc(nel));
-        ALT3_TRY(v3_mixed_run(ctx, ksub, derr.p));
+        ALT3_TRY(v3_mixed_run(ctx, ksub, derr));
     } else {
         ctx->d_v33f.release();
         ALT3_TRY(ctx->d_v33.alloc(nel));
@@ -367,7 +366,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
             if (nb * nb * nb > 256) return fail(ALT3_E_ARG, "FP64_REF path supports nb^3 <= 256");
             hipLaunchKernelGGL(k_v3_ref, dim3(nk, ksub), dim3(256), 0, ctx->stream, md, nk, nb, ibz_lo,
                                ctx->d_mapinv.p, ctx->d_kpc.p, ctx->d_evec.p, ctx->d_eival.p, ctx->V.nterm,
-                               ctx->d_t_abc.p, ctx->d_t_psi.p, ctx->d_t_rj.p, ctx->d_t_rk.p, c33, ctx->d_v33.p, derr.p);
+                               ctx->d_t_abc.p, ctx->d_t_psi.p, ctx->d_t_rj.p, ctx->d_t_rk.p, c33, ctx->d_v33.p, derr);
             ALT3_HIP(hipGetLastError());
         } else {
             const int ndr = ctx->V.ndr, nb2 = nb * nb;
@@ -384,7 +383,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
     if (lds > 65536)                                                                                           \
         ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_main<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     hipLaunchKernelGGL(k_v3_main<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ibz_lo, ndr, ctx->d_mapinv.p, \
-                       ctx->d_eph.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr.p)
+                       ctx->d_eph.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr)
             switch (tq) {
                 case 64: LAUNCH(64); break;
                 case 32: LAUNCH(32); break;
@@ -399,7 +398,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
         }
     }
     int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     if (herr) return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
     ctx->have_v3 = true;

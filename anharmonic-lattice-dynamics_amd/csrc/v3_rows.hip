@@ -181,19 +181,19 @@ bool v3_rows_supported(const alt3_ctx *ctx, bool fp32)
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
 {
     const int nk = ctx->nk, nb = ctx->nb, ndr = ctx->V.ndr;
-    DBuf<double> wq;
-    ALT3_TRY(wq.alloc((size_t)nk * nb));
+    DBuf<double> &wq = ctx->d_wq;
+    ALT3_TRY(wq.reserve((size_t)nk * nb));
     hipLaunchKernelGGL(k_lam_quarter, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_eival.p,
                        wq.p);
     ALT3_HIP(hipGetLastError());
     if (!fp32) {
         ALT3_TRY(dispatch_rows<double>(ctx, ksub, ctx->d_eph.p, ctx->d_evec.p, wq.p, ctx->d_gp.p, ctx->d_v33.p, d_err));
     } else {
-        DBuf<float> ephf, evf, gpf;
+        DBuf<float> &ephf = ctx->d_ephf, &evf = ctx->d_evf, &gpf = ctx->d_gpf32;
         const size_t ne = (size_t)nk * nb * nb * 2, np = (size_t)nk * ndr * 2, ng = (size_t)ksub * ndr * nb * nb * nb * 2;
-        ALT3_TRY(ephf.alloc(np));
-        ALT3_TRY(evf.alloc(ne));
-        ALT3_TRY(gpf.alloc(ng));
+        ALT3_TRY(ephf.reserve(np));
+        ALT3_TRY(evf.reserve(ne));
+        ALT3_TRY(gpf.reserve(ng));
         hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, np,
                            ctx->d_eph.p, ephf.p);
         hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, ne,
@@ -202,7 +202,6 @@ int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
                            ctx->d_gp.p, gpf.p);
         ALT3_HIP(hipGetLastError());
         ALT3_TRY(dispatch_rows<float>(ctx, ksub, ephf.p, evf.p, wq.p, gpf.p, ctx->d_v33f.p, d_err));
-        ALT3_HIP(hipStreamSynchronize(ctx->stream));   // before the FP32 copies are freed
     }
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
