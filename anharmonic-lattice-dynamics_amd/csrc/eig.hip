@@ -3,7 +3,7 @@
 // Replaces get_frequencies (THERMACOND/rest.f90:2230-2405): for every mesh point the
 // dynamical matrix (set_dynamical_matrix, rest.f90:445-556) is built and diagonalised at q
 // and at q +- dq e_a (finitedif_vel, rest.f90:2587-2631, dq = REAL*4 0.001), i.e. 7
-// Hermitian eigenproblems per q.  One wavefront owns one matrix in LDS and runs cyclic
+// Hermitian eigenproblems per q.  A group of lanes owns one matrix in LDS and runs cyclic
 // complex Jacobi rotations (lane-parallel row/column updates); Jacobi converges to full
 // FP64 accuracy on the small (3*natoms)^2 matrices, including degenerate branches.
 // Eigenvalue ordering follows zheev (ascending), then the reference's post-processing:
@@ -231,6 +231,200 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *
     }
 }
 
+// The same solve with G lanes per matrix and 64/G matrices per wavefront, for cells of up to
+// 32 branches (G >= nb).  Every matrix goes through the same operations in the same order as
+// in k_eig_solve (the norm sums are taken by the group's first lane in k_eig_solve's
+// summation order), so the results are bit-identical; a wavefront now carries up to eight
+// rotation chains at once instead of one.
+template <int G>
+__global__ void __launch_bounds__(64) k_eig_group(int nk, int nb, const double *__restrict__ kpc,
+                                                  const int *__restrict__ ent_ptr,
+                                                  const double *__restrict__ term_amp,
+                                                  const double *__restrict__ term_rr, double dq,
+                                                  double *__restrict__ evals7, double *__restrict__ evec0)
+{
+    constexpr int M = 64 / G;
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x, g = lane / G, sub = lane % G;
+    const int nn = nb * nb;
+    zd *A = reinterpret_cast<zd *>(lds) + (size_t)g * 2 * nn;   // [nb][nb] row-major
+    zd *V = A + nn;                                              // [nb][nb], V[k][col]
+    __shared__ double s_rot[M][5];                               // c, s, cos(phi), sin(phi), skip
+    __shared__ double s_sum[M][2];
+    __shared__ int s_ord[M][G];
+    const int mat = blockIdx.x * M + g;
+    const bool valid = mat < nk * kDq;
+    const int q = valid ? mat / kDq : 0, var = valid ? mat % kDq : 0;
+    // norm sum of k_eig_solve: lane l of 64 sums entries l, l+64, ..., then the lane sums in order
+    auto sum2 = [&](auto f, double &s0, double &s1) {
+        s0 = 0;
+        s1 = 0;
+        for (int l = 0; l < 64 && l < nn; l++) {
+            double p0 = 0, p1 = 0;
+            for (int e = l; e < nn; e += 64) f(e, p0, p1);
+            s0 += p0;
+            s1 += p1;
+        }
+    };
+
+    double kq[3] = {kpc[3 * q], kpc[3 * q + 1], kpc[3 * q + 2]};
+    if (var > 0) {
+        int a = (var - 1) / 2;
+        kq[a] = (var & 1) ? kq[a] + dq : kq[a] - dq;
+    }
+    // 1. D(q)
+    for (int e = sub; e < nn; e += G) {
+        double re = 0, im = 0;
+        for (int t = ent_ptr[e]; t < ent_ptr[e + 1]; t++) {
+            double ph = kq[0] * term_rr[3 * t] + kq[1] * term_rr[3 * t + 1] + kq[2] * term_rr[3 * t + 2];
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            re = re + term_amp[t] * cs;
+            im = im + term_amp[t] * sn;
+        }
+        A[e] = {re, im};
+        V[e] = {(e / nb == e % nb) ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    // 2. Hermitisation as rest.f90:514-543 (column nb untouched)
+    for (int e = sub; e < nn; e += G) {
+        int t = e / nb, j = e % nb;
+        if (t == j) {
+            zd d = A[e];
+            if (fabs(d.i) > 9e-4 * fabs(d.r)) A[e] = {d.r, 0.0};
+        }
+    }
+    __syncthreads();
+    for (int e = sub; e < nn; e += G) {
+        int t = e / nb, j = e % nb;
+        if (j > t && j < nb - 1) {
+            zd a = A[t * nb + j], b = A[j * nb + t];
+            double mi = (a.i - b.i) / 2, mj = (a.r + b.r) / 2;
+            A[t * nb + j] = {mj, mi};
+            A[j * nb + t] = {mj, -mi};
+        }
+    }
+    __syncthreads();
+    if (sub == 0) {
+        double si, sr;
+        sum2([&](int e, double &a, double &b) { a += fabs(A[e].i); b += fabs(A[e].r); }, si, sr);
+        s_sum[g][0] = si;
+        s_sum[g][1] = sr;
+    }
+    __syncthreads();
+    {
+        bool realify = (s_sum[g][0] / nn) < (s_sum[g][1] / nn) * 1e-10;
+        if (realify)
+            for (int e = sub; e < nn; e += G) A[e] = {A[e].r, 0.0};
+    }
+    __syncthreads();
+    for (int e = sub; e < nn; e += G) {
+        int r = e / nb, c = e % nb;
+        if (r > c) A[e] = {A[c * nb + r].r, -A[c * nb + r].i};
+        if (r == c) A[e] = {A[e].r, 0.0};
+    }
+    __syncthreads();
+    // 3. cyclic complex Jacobi; a finished matrix idles while the others of its wavefront go on
+    bool done = !valid;
+    for (int sweep = 0; sweep < 40; sweep++) {
+        if (sub == 0 && !done) {
+            double off, dia;
+            sum2([&](int e, double &o, double &d) {
+                int r = e / nb, c = e % nb;
+                double v2 = A[e].r * A[e].r + A[e].i * A[e].i;
+                if (r != c) o += v2; else d += v2;
+            }, off, dia);
+            s_sum[g][0] = (!(off > 1e-32 * dia) || off == 0.0) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        done = done || s_sum[g][0] != 0.0;
+        if (__ballot(!done) == 0) break;   // one wavefront per block: uniform
+        for (int p = 0; p < nb - 1; p++)
+            for (int qq = p + 1; qq < nb; qq++) {
+                if (sub == 0 && !done) {
+                    zd apq = A[p * nb + qq];
+                    double mag = sqrt(apq.r * apq.r + apq.i * apq.i);
+                    double app = A[p * nb + p].r, aqq = A[qq * nb + qq].r;
+                    if (mag <= 1e-300 || mag * mag <= 1e-36 * (app * app + aqq * aqq)) {
+                        s_rot[g][4] = 1.0;
+                        A[p * nb + qq] = {0.0, 0.0};
+                        A[qq * nb + p] = {0.0, 0.0};
+                    } else {
+                        double tau = (aqq - app) / (2.0 * mag);
+                        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        double c = 1.0 / sqrt(1.0 + t * t);
+                        s_rot[g][0] = c;
+                        s_rot[g][1] = t * c;
+                        s_rot[g][2] = apq.r / mag;
+                        s_rot[g][3] = apq.i / mag;
+                        s_rot[g][4] = 0.0;
+                    }
+                }
+                __syncthreads();
+                const bool act = !done && s_rot[g][4] == 0.0;
+                const double c = s_rot[g][0], sn = s_rot[g][1];
+                const zd em = {s_rot[g][2], -s_rot[g][3]};  // e^{-i phi}
+                const zd ep = {s_rot[g][2], s_rot[g][3]};   // e^{+i phi}
+                if (act && sub < nb) {  // columns of A and V
+                    const int k = sub;
+                    zd x = A[k * nb + p], y = zmul(em, A[k * nb + qq]);
+                    A[k * nb + p] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
+                    A[k * nb + qq] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
+                    zd vx = V[k * nb + p], vy = zmul(em, V[k * nb + qq]);
+                    V[k * nb + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
+                    V[k * nb + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                }
+                __syncthreads();
+                if (act && sub < nb) {  // rows of A
+                    const int k = sub;
+                    zd x = A[p * nb + k], y = zmul(ep, A[qq * nb + k]);
+                    A[p * nb + k] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
+                    A[qq * nb + k] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
+                }
+                __syncthreads();
+                if (act && sub == 0) {
+                    A[p * nb + qq] = {0.0, 0.0};
+                    A[qq * nb + p] = {0.0, 0.0};
+                    A[p * nb + p].i = 0.0;
+                    A[qq * nb + qq].i = 0.0;
+                }
+                __syncthreads();
+            }
+    }
+    // 4. ascending order of the signed eigenvalues (zheev), ties keep index order
+    if (sub == 0) {
+        int *ord = s_ord[g];
+        for (int k = 0; k < nb; k++) ord[k] = k;
+        for (int a = 1; a < nb; a++) {
+            int x = ord[a];
+            double vx = A[x * nb + x].r;
+            int b = a - 1;
+            while (b >= 0 && A[ord[b] * nb + ord[b]].r > vx) {
+                ord[b + 1] = ord[b];
+                b--;
+            }
+            ord[b + 1] = x;
+        }
+    }
+    __syncthreads();
+    if (!valid || sub >= nb) return;
+    const int src = s_ord[g][sub];
+    evals7[((size_t)q * kDq + var) * nb + sub] = fabs(A[src * nb + src].r);
+    if (var == 0) {
+        // column j = sub (sorted) of V; "pure imaginary" columns (sum |Re| < 1e-3) times i
+        double sre = 0;
+        for (int k = 0; k < nb; k++) sre += fabs(V[k * nb + src].r);
+        const bool rot = sre < 1e-3;
+        for (int k = 0; k < nb; k++) {
+            zd v = V[k * nb + src];
+            if (rot) v = {-v.i, v.r};
+            size_t o = 2 * (((size_t)q * nb + sub) * nb + k);
+            evec0[o] = v.r;
+            evec0[o + 1] = v.i;
+        }
+    }
+}
+
 // finitedif_vel velocities + get_frequencies' second sort
 __global__ void k_eig_finish(int nk, int nb, double dq, double cnst, const double *__restrict__ evals7,
                              const double *__restrict__ evec0, double *__restrict__ eival,
@@ -331,12 +525,33 @@ int eig_run(alt3_ctx *ctx)
     ALT3_TRY(ctx->d_vel.alloc((size_t)nk * nb * 3));
     ALT3_TRY(ctx->d_evec.alloc((size_t)nk * nb * nb * 2));
     const double dq = (double)0.001f;
-    size_t lds = (size_t)2 * nb * nb * sizeof(zd) + 64 * sizeof(double);
-    if (lds > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
-    if (lds > 65536)   // cells with more than 44 branches
-        ALT3_HIP(hipFuncSetAttribute((const void *)k_eig_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_eig_solve, dim3(nk * kDq), dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,
-                       ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+    const int G = nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
+    if (G < 64) {   // 64/G matrices per wavefront
+        const int M = 64 / G, nmat = nk * kDq;
+        const size_t lds = (size_t)M * 2 * nb * nb * sizeof(zd);
+        const void *kern = G == 8 ? (const void *)k_eig_group<8> : G == 16 ? (const void *)k_eig_group<16>
+                                                                           : (const void *)k_eig_group<32>;
+        if (lds + 4096 > 65536) ALT3_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        dim3 grid((unsigned)((nmat + M - 1) / M));
+#define ALT3_EIG_GROUP(GG)                                                                                     \
+        hipLaunchKernelGGL(k_eig_group<GG>, grid, dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,         \
+                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+        if (G == 8) {
+            ALT3_EIG_GROUP(8)
+        } else if (G == 16) {
+            ALT3_EIG_GROUP(16)
+        } else {
+            ALT3_EIG_GROUP(32)
+        }
+#undef ALT3_EIG_GROUP
+    } else {
+        size_t lds = (size_t)2 * nb * nb * sizeof(zd) + 64 * sizeof(double);
+        if (lds > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
+        if (lds > 65536)   // cells with more than 44 branches
+            ALT3_HIP(hipFuncSetAttribute((const void *)k_eig_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_eig_solve, dim3(nk * kDq), dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,
+                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+    }
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_eig_finish, dim3((nk + 63) / 64), dim3(64), 0, ctx->stream, nk, nb, dq, cnst(), ev7.p,
                        evec0.p, ctx->d_eival.p, ctx->d_evec.p, ctx->d_vel.p, ctx->d_freq.p);

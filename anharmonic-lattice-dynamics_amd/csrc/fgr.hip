@@ -129,8 +129,10 @@ __device__ void weight_tet1(const float *w, double e, double *c)
 }
 
 // The same weights entirely in FP32, for the mixed path (whose |V3|^2 and P are FP32): the
-// same sort, case split and 1e-10 clamp, with single-precision quotients.
-__device__ void weight_tet1f(const float *w, float e, float *c)
+// same sort, case split and 1e-10 clamp.  The quotients are rewritten over the six hardware
+// reciprocals 1/e_ij (v_rcp_f32, 1 ulp), grouped so that every partial product stays bounded
+// (d/e_ij <= 1 inside its case), instead of up to twenty IEEE divisions.
+__device__ __forceinline__ void weight_tet1f(const float *w, float e, float *c)
 {
     float en[4];
     int uns[4];
@@ -149,29 +151,31 @@ __device__ void weight_tet1f(const float *w, float e, float *c)
     if (e < e1 || e > e4) {
         c[0] = c[1] = c[2] = c[3] = 0;
         return;
-    } else if (e1 <= e && e <= e2) {
-        const float d1 = e - e1, d13 = d1 * d1 * d1;
-        cc[3] = ((e2 - e) / e21 + (e3 - e) / e31 + (e4 - e) / e41) * (d1 * d1) / (e41 * e31 * e21);
-        cc[2] = d13 / (e21 * e21 * e31 * e41);
-        cc[1] = d13 / (e21 * (e31 * e31) * e41);
-        cc[0] = d13 / (e21 * e31 * (e41 * e41));
-    } else if (e2 <= e && e <= e3) {
-        const float r31 = 1.0f / (e31 * e31), r41 = 1.0f / (e41 * e41), r32 = 1.0f / (e32 * e32), r42 = 1.0f / (e42 * e42);
+    } else if (e <= e2) {
+        const float q21 = __builtin_amdgcn_rcpf(e21), q31 = __builtin_amdgcn_rcpf(e31), q41 = __builtin_amdgcn_rcpf(e41);
+        const float d1 = e - e1, g = (d1 * q21) * (d1 * q31) * q41;   // d1^2 / (e21 e31 e41)
+        cc[3] = ((e2 - e) * q21 + (e3 - e) * q31 + (e4 - e) * q41) * g;
+        cc[2] = g * (d1 * q21);
+        cc[1] = g * (d1 * q31);
+        cc[0] = g * (d1 * q41);
+    } else if (e <= e3) {
+        const float q31 = __builtin_amdgcn_rcpf(e31), q41 = __builtin_amdgcn_rcpf(e41);
+        const float q32 = __builtin_amdgcn_rcpf(e32), q42 = __builtin_amdgcn_rcpf(e42);
         const float a32 = (e3 - e) * (e - e2), a41 = (e4 - e) * (e - e1), a31 = (e3 - e) * (e - e1), a42 = (e4 - e) * (e - e2);
-        const float i1 = 1.0f / (e42 * e32), i2 = 1.0f / (e41 * e42), i3 = 1.0f / (e32 * e41);
-        const float i4 = 1.0f / (e42 * e31), i5 = 1.0f / (e31 * e32), i6 = 1.0f / (e31 * e41);
+        const float i1 = q42 * q32, i2 = q41 * q42, i3 = q32 * q41, i4 = q42 * q31, i5 = q31 * q32, i6 = q31 * q41;
         const float b11 = a32 * i1 + a41 * i2 + a31 * i3, b12 = a41 * i4 + a42 * i1 + a31 * i5;
         const float b21 = a32 * i4 + a42 * i2 + a31 * i6, b22 = a32 * i5 + a41 * i6 + a42 * i3;
-        cc[3] = 0.5f * ((e3 - e) * r31 * b11 + (e4 - e) * r41 * b12);
-        cc[2] = 0.5f * ((e3 - e) * r32 * b21 + (e4 - e) * r42 * b22);
-        cc[1] = 0.5f * ((e - e2) * r32 * b21 + (e - e1) * r31 * b11);
-        cc[0] = 0.5f * ((e - e2) * r42 * b22 + (e - e1) * r41 * b12);
+        cc[3] = 0.5f * (((e3 - e) * q31) * q31 * b11 + ((e4 - e) * q41) * q41 * b12);
+        cc[2] = 0.5f * (((e3 - e) * q32) * q32 * b21 + ((e4 - e) * q42) * q42 * b22);
+        cc[1] = 0.5f * (((e - e2) * q32) * q32 * b21 + ((e - e1) * q31) * q31 * b11);
+        cc[0] = 0.5f * (((e - e2) * q42) * q42 * b22 + ((e - e1) * q41) * q41 * b12);
     } else {
-        const float d4 = e4 - e, d43 = d4 * d4 * d4;
-        cc[3] = d43 / (e41 * e41 * e42 * e43);
-        cc[2] = d43 / (e41 * (e42 * e42) * e43);
-        cc[1] = d43 / (e41 * e42 * (e43 * e43));
-        cc[0] = ((e - e3) / e43 + (e - e2) / e42 + (e - e1) / e41) * (d4 * d4) / (e41 * e42 * e43);
+        const float q41 = __builtin_amdgcn_rcpf(e41), q42 = __builtin_amdgcn_rcpf(e42), q43 = __builtin_amdgcn_rcpf(e43);
+        const float d4 = e4 - e, h = (d4 * q41) * (d4 * q42) * q43;   // d4^2 / (e41 e42 e43)
+        cc[3] = h * (d4 * q41);
+        cc[2] = h * (d4 * q42);
+        cc[1] = h * (d4 * q43);
+        cc[0] = ((e - e3) * q43 + (e - e2) * q42 + (e - e1) * q41) * h;
     }
     float out[4] = {0, 0, 0, 0};
     for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
@@ -197,7 +201,8 @@ __device__ __forceinline__ float pick(const float (&a)[N], int j)
 // One workgroup per (s, l2, l3, process).  The corner energies e(q2) depend on l2, l3 and
 // the process but not on l1, so they are built once in LDS (with the energy range of every
 // cube kept in registers) and reused for the nb values omega(q1, l1); per l1 the LDS weight
-// sums W are accumulated and then written out as P = W * F.
+// sums W are accumulated (FP64 on both paths: LDS ds_add_f32 measured slower than ds_add_f64
+// here) and then written out as P = W * F.
 constexpr int kFgrThreads = 512;   // 8 waves; LDS (12 B per mesh point) allows 2 groups per CU
 constexpr int kFgrCase = 4096;     // LDS list per weight_tet case; one chunk of cubes (<= 6*512) always fits
 constexpr int kFgrList = 3 * kFgrCase;
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
                                                      VT *__restrict__ P2)
 {
     extern __shared__ double lds[];
-    double *W = lds;                                   // [nk]
+    double *W = lds;               // [nk]
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
     __shared__ int s_cnt[3];
@@ -399,7 +404,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             const int w = tid + j * kFgrThreads;
             if (w < nk) {
                 double F = (double)fv[j] * tpi2 / nk;
-                if (vzero >> j & 1u) F = 0.0;
+                                if (vzero >> j & 1u) F = 0.0;
                 P[base + w] = (VT)(W[w] * F);
                 W[w] = 0.0;
             }
@@ -502,6 +507,7 @@ int fgr_run(alt3_ctx *ctx)
     if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr needs alt3_v3sq_ibz first");
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
+    const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
     if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 9 * kFgrThreads)
         return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4608");
@@ -509,7 +515,6 @@ int fgr_run(alt3_ctx *ctx)
     const MeshDev md = MeshDev::make(ctx->nc);
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
-    const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
     const bool big = lds + kFgrList * sizeof(unsigned short) + 256 > 65536;   // static + dynamic LDS
 #define ALT3_FGR_LAUNCH(PER)                                                                                     \
