@@ -180,6 +180,8 @@ struct alt3_ctx {
     alt3::DBuf<double> d_dist;                     // [t][nk][nb]
     alt3::DBuf<double> d_q, d_qn, d_qu;            // [t][ksub][nb]
     alt3::DBuf<double> d_r1, d_r2;                 // [t][ksub][nb][nb][nk]  sum_l3 P1, P2
+    alt3::DBuf<float> d_r1f, d_r2f;                // same in FP32 (mixed path, R has the precision of P)
+    bool r_fp32 = false;
     size_t ksub_nb() const { return (size_t)(ibz_hi - ibz_lo + 1) * nb; }
     double *dist() const { return d_dist.p + (size_t)tau_cur * nk * nb; }
     double *q() const { return d_q.p + (size_t)tau_cur * ksub_nb(); }
@@ -187,6 +189,8 @@ struct alt3_ctx {
     double *qu() const { return d_qu.p + (size_t)tau_cur * ksub_nb(); }
     double *r1() const { return d_r1.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
     double *r2() const { return d_r2.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
+    float *r1f() const { return d_r1f.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
+    float *r2f() const { return d_r2f.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
     bool have_tau = false;
     alt3::DBuf<int> d_star_n, d_star_op, d_star_n0, d_fstar_n, d_fstar_op, d_fstar_n0;
     alt3::DBuf<int> d_cm_slot_n2, d_cm_slot_cnt, d_cm_tab;   // collision star table, see k_collision_c

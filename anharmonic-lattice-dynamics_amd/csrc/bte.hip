@@ -12,6 +12,8 @@
 #include <algorithm>
 
 #include "alt3_impl.h"
+
+#include <type_traits>
 #include "mesh_dev.h"
 
 namespace {
@@ -55,13 +57,14 @@ __device__ double block_sum(double v, double *sh)
 }
 
 // one workgroup per (s, l1, l2): the l3-reduced R1/R2 rows and partial Q, QN, QU of NT
-// temperatures at once, so that P1sm/P2sm are read once for all of them
+// temperatures at once, so that P1sm/P2sm are read once for all of them.  R is stored in the
+// precision of P (FP32 on the mixed path); the sums are FP64.
 template <int NT, typename PT>
 __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                              const int *__restrict__ mapinv, const double *__restrict__ dist,
                                              size_t dstr, const PT *__restrict__ P1sm,
                                              const PT *__restrict__ P2sm, double *__restrict__ qpart, size_t qstr,
-                                             double *__restrict__ R1, double *__restrict__ R2, size_t rstr)
+                                             PT *__restrict__ R1, PT *__restrict__ R2, size_t rstr)
 {
     __shared__ double sh[256];
     const int l2 = blockIdx.x % nb, sl = blockIdx.x / nb, s = sl / nb, l1 = sl % nb;
@@ -108,8 +111,8 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
         const size_t o = ((size_t)sl * nb + l2) * (size_t)nk + j;
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            R1[t * rstr + o] = r1[t];
-            R2[t * rstr + o] = r2[t];
+            R1[t * rstr + o] = (PT)r1[t];
+            R2[t * rstr + o] = (PT)r2[t];
         }
     }
 #pragma unroll
@@ -131,8 +134,8 @@ template <int NT, typename PT, int NB>
 __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz_lo, const int *__restrict__ mapinv,
                                                 const double *__restrict__ dist, size_t dstr,
                                                 const PT *__restrict__ P1sm, const PT *__restrict__ P2sm,
-                                                double *__restrict__ qpart, size_t qstr, double *__restrict__ R1,
-                                                double *__restrict__ R2, size_t rstr)
+                                                double *__restrict__ qpart, size_t qstr, PT *__restrict__ R1,
+                                                PT *__restrict__ R2, size_t rstr)
 {
     __shared__ double sh[256];
     const int s = blockIdx.y, nch = gridDim.x, ch = blockIdx.x;
@@ -194,8 +197,8 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
                 }
                 if (valid) {
                     const size_t o = (((size_t)s * NB + l1) * NB + l2) * (size_t)nk + j;
-                    R1[t * rstr + o] = r1;
-                    R2[t * rstr + o] = r2;
+                    R1[t * rstr + o] = (PT)r1;
+                    R2[t * rstr + o] = (PT)r2;
                 }
             }
         }
@@ -323,9 +326,10 @@ __global__ void __launch_bounds__(64) k_kappa_sum(int nibz, int nb, const double
 // column blocks (n2,l2), whose star gathers stay inside the (s,l1) slab of R1/R2.  The
 // device matrix is row-major (CMr[row*ncol + col]) so the three rows of a block are written
 // contiguously.
+template <typename RT>
 __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz,
                                                    const int *__restrict__ mapinv, const double *__restrict__ Q,
-                                                   const double *__restrict__ R1, const double *__restrict__ R2,
+                                                   const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                    const int *__restrict__ star_n, const int *__restrict__ star_op,
                                                    const int *__restrict__ star_n0, const double *__restrict__ opk,
                                                    double *__restrict__ CMr)
@@ -351,7 +355,8 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
                 m.coords(n0, c0, c1, c2);
                 const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q2
                 const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q2-q1
-                C = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+                C = -(double)R1[rbase + n0] + (double)R1[rbase + p2] + 0.5 * (double)R2[rbase + n0] +
+                    0.5 * (double)R2[rbase + p1];
             }
             const double *op = opk + 9 * star_op[48 * n2 + ns];
             for (int i = 0; i < 3; i++)
@@ -369,9 +374,10 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
 // each thread one irreducible q2 (slots ordered by decreasing star size, so that the lanes of
 // a wave run the same number of star members) and sums C(q0) op over its star from LDS.  The
 // star table is stored member-major (tab[ns][slot] = q0 << 6 | op) for coalesced reads.
+template <typename RT>
 __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
                                                      const int *__restrict__ mapinv, const double *__restrict__ Q,
-                                                     const double *__restrict__ R1, const double *__restrict__ R2,
+                                                     const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                      const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
                                                      const int *__restrict__ tab, const double *__restrict__ opk,
                                                      double *__restrict__ CMr)
@@ -391,7 +397,8 @@ __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, in
         m.coords(n0, c0, c1, c2);
         const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q0
         const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q0-q1
-        C[n0] = -R1[rbase + n0] + R1[rbase + p2] + 0.5 * R2[rbase + n0] + 0.5 * R2[rbase + p1];
+        C[n0] = -(double)R1[rbase + n0] + (double)R1[rbase + p2] + 0.5 * (double)R2[rbase + n0] +
+                0.5 * (double)R2[rbase + p1];
     }
     __syncthreads();
     if (threadIdx.x == 0 && l1 == l2) C[k1] = -Q[(size_t)s * nb + l1];
@@ -696,8 +703,18 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     ALT3_TRY(ctx->d_q.alloc(nt * qstr));
     ALT3_TRY(ctx->d_qn.alloc(nt * qstr));
     ALT3_TRY(ctx->d_qu.alloc(nt * qstr));
-    ALT3_TRY(ctx->d_r1.alloc(nt * rstr));
-    ALT3_TRY(ctx->d_r2.alloc(nt * rstr));
+    ctx->r_fp32 = ctx->p_fp32;
+    if (ctx->r_fp32) {
+        ctx->d_r1.release();
+        ctx->d_r2.release();
+        ALT3_TRY(ctx->d_r1f.alloc(nt * rstr));
+        ALT3_TRY(ctx->d_r2f.alloc(nt * rstr));
+    } else {
+        ctx->d_r1f.release();
+        ctx->d_r2f.release();
+        ALT3_TRY(ctx->d_r1.alloc(nt * rstr));
+        ALT3_TRY(ctx->d_r2.alloc(nt * rstr));
+    }
     for (int t = 0; t < nt; t++)
         hipLaunchKernelGGL(k_dist, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_freq.p,
                            temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr);
@@ -712,27 +729,29 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
         const int g = std::min(4, nt - t0);
         if (rows) {
             const dim3 grid((unsigned)nch, (unsigned)ksub);
-#define ALT3_TAU_Q2(G, PT, P1, P2)                                                                                 \
+#define ALT3_TAU_Q2(G, PT, P1, P2, R1, R2)                                                                         \
     hipLaunchKernelGGL((k_tau_q2<G, PT, 6>), grid, dim3(256), 0, ctx->stream, md, nk, ctx->ibz_lo, ctx->d_mapinv.p, \
-                       ctx->d_dist.p + t0 * dstr, dstr, P1, P2, qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr,  \
-                       ctx->d_r2.p + t0 * rstr, rstr)
+                       ctx->d_dist.p + t0 * dstr, dstr, P1, P2, qpart.p + t0 * pstr, pstr, (R1) + t0 * rstr,         \
+                       (R2) + t0 * rstr, rstr)
             if (ctx->p_fp32) {
-                if (g == 1) ALT3_TAU_Q2(1, float, ctx->d_p1f.p, ctx->d_p2f.p);
-                else if (g == 2) ALT3_TAU_Q2(2, float, ctx->d_p1f.p, ctx->d_p2f.p);
-                else if (g == 3) ALT3_TAU_Q2(3, float, ctx->d_p1f.p, ctx->d_p2f.p);
-                else ALT3_TAU_Q2(4, float, ctx->d_p1f.p, ctx->d_p2f.p);
+                float *r1 = ctx->d_r1f.p, *r2 = ctx->d_r2f.p;
+                if (g == 1) ALT3_TAU_Q2(1, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
+                else if (g == 2) ALT3_TAU_Q2(2, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
+                else if (g == 3) ALT3_TAU_Q2(3, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
+                else ALT3_TAU_Q2(4, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
             } else {
-                if (g == 1) ALT3_TAU_Q2(1, double, ctx->d_p1.p, ctx->d_p2.p);
-                else if (g == 2) ALT3_TAU_Q2(2, double, ctx->d_p1.p, ctx->d_p2.p);
-                else if (g == 3) ALT3_TAU_Q2(3, double, ctx->d_p1.p, ctx->d_p2.p);
-                else ALT3_TAU_Q2(4, double, ctx->d_p1.p, ctx->d_p2.p);
+                double *r1 = ctx->d_r1.p, *r2 = ctx->d_r2.p;
+                if (g == 1) ALT3_TAU_Q2(1, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
+                else if (g == 2) ALT3_TAU_Q2(2, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
+                else if (g == 3) ALT3_TAU_Q2(3, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
+                else ALT3_TAU_Q2(4, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
             }
 #undef ALT3_TAU_Q2
         } else if (ctx->p_fp32) {
             auto *kern = g == 1 ? k_tau<1, float> : g == 2 ? k_tau<2, float> : g == 3 ? k_tau<3, float> : k_tau<4, float>;
             hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1f.p, ctx->d_p2f.p,
-                               qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+                               qpart.p + t0 * pstr, pstr, ctx->d_r1f.p + t0 * rstr, ctx->d_r2f.p + t0 * rstr, rstr);
         } else {
             auto *kern = g == 1 ? k_tau<1, double> : g == 2 ? k_tau<2, double> : g == 3 ? k_tau<3, double> : k_tau<4, double>;
             hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
@@ -836,14 +855,21 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
     const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
-    if (lds <= 65536)
-        hipLaunchKernelGGL(k_collision_c, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
-                           nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), ctx->r1(), ctx->r2(), ctx->d_cm_slot_n2.p,
-                           ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, ctx->d_cm.p);
+    auto launch = [&](auto *R1, auto *R2) {
+        using RT = std::remove_const_t<std::remove_pointer_t<decltype(R1)>>;
+        if (lds <= 65536)
+            hipLaunchKernelGGL(k_collision_c<RT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb,
+                               ctx->ibz_lo, nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_cm_slot_n2.p,
+                               ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, ctx->d_cm.p);
+        else
+            hipLaunchKernelGGL(k_collision<RT>, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                               nibz, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_star_n.p, ctx->d_star_op.p,
+                               ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+    };
+    if (ctx->r_fp32)
+        launch((const float *)ctx->r1f(), (const float *)ctx->r2f());
     else
-        hipLaunchKernelGGL(k_collision, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, nibz,
-                           ctx->d_mapinv.p, ctx->q(), ctx->r1(), ctx->r2(), ctx->d_star_n.p, ctx->d_star_op.p,
-                           ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+        launch((const double *)ctx->r1(), (const double *)ctx->r2());
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
                        ctx->d_freq.p, ctx->dist(), ctx->d_vel.p, ctx->d_rhs.p);

@@ -143,7 +143,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
     ctx->d_eph.release(); ctx->d_gp.release(); ctx->d_gpf.release(); ctx->d_mapinv.release(); ctx->d_v33.release(); ctx->d_v33f.release();
     ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_p1f.release(); ctx->d_p2f.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
-    ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release();
+    ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release(); ctx->d_r1f.release(); ctx->d_r2f.release();
     ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
     ctx->d_cm_slot_n2.release(); ctx->d_cm_slot_cnt.release(); ctx->d_cm_tab.release();
     ctx->d_fstar_n.release(); ctx->d_fstar_op.release(); ctx->d_fstar_n0.release();
@@ -166,6 +166,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     b += ctx->d_evec.bytes() + ctx->d_eival.bytes() + ctx->d_freq.bytes() + ctx->d_vel.bytes();
     b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
     b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
+    b += ctx->d_r1f.bytes() + ctx->d_r2f.bytes();
     b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
     b += ctx->d_ephf.bytes() + ctx->d_evf.bytes() + ctx->d_gpf32.bytes();
     *bytes = b;
@@ -189,7 +190,8 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
         b += nel * 8;
     }
     b += 2 * nel * (precision == ALT3_V3_MIXED ? 4 : 8);                                       // P1sm, P2sm
-    b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * 8 + 3 * ni * nb * nb * 8);  // tau pass
+    const long long rb = precision == ALT3_V3_MIXED ? 4 : 8;                                  // R in the precision of P
+    b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * rb + 3 * ni * nb * nb * 8);  // tau pass
     b += n * n * 8 + n * 8 * 8 + ni * nb * 9 * 8 * 2;                                           // CM, RHS, CG, kappa
     b += (long long)ni * 48 * 4 * 7 + 9 * 48 * 8 * 2;                                           // star tables
     b += 16ll << 20;                                                                            // small buffers

@@ -8,8 +8,8 @@ conjugate gradients at T = 100, 300, 800 K.  One step = the whole path on the de
 eigen solve on the mesh, V3 over the irreducible zone, tetrahedron FGR, then per
 temperature tau, kappa_RTA, collision matrix, CG and kappa_BTE.
 
-value = |V3|^2 elements per second of the whole step with the mixed-precision V3 (FP32
-MFMA phase GEMM); the same step with FP64 V3 is timed beside it ("fp64"), together with
+value = |V3|^2 elements per second of the whole step with the mixed-precision path (FP32
+V3, scattering weights and P; FP64 from the scattering rates on); the same step with FP64 V3 is timed beside it ("fp64"), together with
 the kappa difference between the two ("mixed_vs_fp64").  The line also carries the
 V3-kernel-only rates, stage times, p50/p99 kappa-solve latency, the roofline of the
 dominant kernel (HIP-event time of k_fgr over the timed steps) and the CPU baseline (the
@@ -204,7 +204,7 @@ def main():
         "unit": "V3 elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"mixed": "fp32 (V3 phase GEMM on MFMA) + f64", "fp64": "f64", "fp64ref": "f64"}[head_name],
+        "dtype": {"mixed": "fp32 (V3, P) + f64", "fp64": "f64", "fp64ref": "f64"}[head_name],
         "data": "synthetic (Stillinger-Weber Si force constants by finite differences)",
         "config": {"workload": f"SW-Si 2-atom diamond, {n}^3 mesh, tetrahedron FGR + CG BTE at {temps} K",
                    **sizes, "v3_elements": nel, "precision": head_name},
