@@ -1063,6 +1063,84 @@ void or_collision(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const in
     free(narm); free(sop); free(sn0); free(sq);
 }
 
+/* update_mpi (sub.f90:2295-2449): one update of the iterative BTE for the irreducible points
+ * ibz_lo..hi,  F2(ibz,l1,x) = F_RTA(n1,l1,x) + A(ibz,l1,x) / Q(n1,l1)  with
+ *   A = sum_{n2 in IBZ} sum_{q2 in star(n2) on the mesh} sum_{l2,l3}
+ *         P1(ibz,q2,l1,l2,l3) (F(q1+q2,l3) - F(q2,l2))_x + 0.5 P2(ibz,q2,l1,l2,l3) (F(q2,l2) + F(q1-q2,l3))_x
+ * where F at a mesh point q is op_kmatrix(s) F(IBZ point of q): s = kvecop of q2 in the star of
+ * n2 (kstarsinfbz1), and for q3 = q1 +- q2 the op get_kvecopn2 (sub.f90:2045-2111) finds, the
+ * first member of the star of mapibz(q3) that is the mesh point q3.  P1, P2 (ksub,nk,nb,nb,nb)
+ * carry the occupations (calculate_tauRTA_mpi2); FFF, FRTA (nibz,nb,3); Q (nibz,nb);
+ * F2 (ksub,nb,3); mapibz 0-based.  Returns -1 if some q3 is not a member of its star (the
+ * reference would then reuse the op of an earlier q3), else 0. */
+int or_update(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const int *mapibz, int nk,
+              const double *kp, const int *nc, const double *g, const double *primlatt, int nops,
+              const double *opk, int nb, const double *P1, const double *P2, const double *FFF,
+              const double *FRTA, const double *Q, double *F2)
+{
+    int ksub = ibz_hi - ibz_lo + 1, bad = 0;
+    int *narm = malloc(sizeof(int) * nibz), *sop = malloc(sizeof(int) * 48 * nibz), *sn0 = malloc(sizeof(int) * 48 * nibz);
+    double *sq = malloc(sizeof(double) * 144 * nibz);
+    for (int n2 = 0; n2 < nibz; n2++)
+        narm[n2] = or_kstarsinfbz1(kibz + 3 * n2, nk, kp, primlatt, nops, opk, sq + 144 * n2, sop + 48 * n2, sn0 + 48 * n2);
+    /* get_kvecopn2 for every mesh point: first star member of its IBZ point that lands on it */
+    int *opq = malloc(sizeof(int) * nk);
+    for (int q = 0; q < nk; q++) {
+        int n = mapibz[q];
+        opq[q] = -1;
+        for (int s = 0; s < narm[n]; s++)
+            if (sn0[48 * n + s] == q) {
+                opq[q] = sop[48 * n + s];
+                break;
+            }
+    }
+#define PX(A, n1, n2, l1, l2, l3) A[(size_t)(n1) + (size_t)ksub * ((size_t)(n2) + (size_t)nk * ((l1) + (size_t)nb * ((l2) + (size_t)nb * (l3))))]
+#define FX(n, l, x) FFF[(size_t)(n) + (size_t)nibz * ((l) + (size_t)nb * (x))]
+#pragma omp parallel for collapse(2) schedule(dynamic) reduction(| : bad)
+    for (int ibz = 0; ibz < ksub; ibz++)
+        for (int l1 = 0; l1 < nb; l1++) {
+            const int n1 = ibz + ibz_lo;
+            const double *q1 = kibz + 3 * n1;
+            for (int xyz = 0; xyz < 3; xyz++) {
+                double A = 0;
+                for (int n2 = 0; n2 < nibz; n2++)
+                    for (int s = 0; s < narm[n2]; s++) {
+                        const double *q2 = sq + 144 * n2 + 3 * s;
+                        double a[3] = {q1[0] + q2[0], q1[1] + q2[1], q1[2] + q2[2]};
+                        double b[3] = {q1[0] - q2[0], q1[1] - q2[1], q1[2] - q2[2]};
+                        int ns, p1, p2;
+                        or_get_k_info(q2, nc, g, &ns, NULL);
+                        or_get_k_info(a, nc, g, &p1, NULL);
+                        or_get_k_info(b, nc, g, &p2, NULL);
+                        const int i1 = mapibz[p1], i2 = mapibz[p2], sq2 = sop[48 * n2 + s];
+                        const int sp1f = opq[p1], sp2f = opq[p2];
+                        if (sp1f < 0 || sp2f < 0) {
+                            bad = 1;
+                            continue;
+                        }
+                        for (int l2 = 0; l2 < nb; l2++)
+                            for (int l3 = 0; l3 < nb; l3++) {
+                                double f[3], fq2[3], fp1[3], fp2[3];
+                                for (int c = 0; c < 3; c++) f[c] = FX(n2, l2, c);
+                                opmul(opk, sq2, f, fq2);
+                                for (int c = 0; c < 3; c++) f[c] = FX(i1, l3, c);
+                                opmul(opk, sp1f, f, fp1);
+                                for (int c = 0; c < 3; c++) f[c] = FX(i2, l3, c);
+                                opmul(opk, sp2f, f, fp2);
+                                A = A + PX(P1, ibz, ns, l1, l2, l3) * (fp1[xyz] - fq2[xyz]) +
+                                    0.5 * PX(P2, ibz, ns, l1, l2, l3) * (fq2[xyz] + fp2[xyz]);
+                            }
+                    }
+                F2[ibz + (size_t)ksub * (l1 + (size_t)nb * xyz)] =
+                    FRTA[n1 + (size_t)nibz * (l1 + (size_t)nb * xyz)] + A / Q[n1 + (size_t)nibz * l1];
+            }
+        }
+#undef PX
+#undef FX
+    free(narm); free(sop); free(sn0); free(sq); free(opq);
+    return bad ? -1 : 0;
+}
+
 /* cal_RHS_IBZ (sub.f90:2618-2691) */
 void or_rhs(int nibz, int nb, int nk, const int *mapinv, const double *freq, const double *dist,
             const double *veloc, double tempk, double *RHS)

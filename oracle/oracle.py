@@ -317,6 +317,41 @@ def collision(ibz_lo, ibz_hi, kibz, mapinv, kp, nc, g, primlatt, opk, P1, P2):
     return CM
 
 
+def update(ibz_lo, ibz_hi, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, FFF, FRTA, Q):
+    """update_mpi: F2 (ksub, nb, 3) = F_RTA + A(FFF)/Q for the irreducible points ibz_lo..hi."""
+    nibz = len(kibz); nk = len(kp); nb = P1.shape[2]; ksub = ibz_hi - ibz_lo + 1
+    F2 = np.zeros((ksub, nb, 3), order="F")
+    rc = lib().or_update(ibz_lo, ibz_hi, nibz, _p(np.ascontiguousarray(kibz)), _p(_i32(mapibz)), nk,
+                         _p(np.ascontiguousarray(kp)), _p(_i32(np.asarray(nc))), _p(np.ascontiguousarray(g)),
+                         _p(_f64(primlatt)), len(opk), _p(np.ascontiguousarray(opk)), nb, _p(_f64(P1)), _p(_f64(P2)),
+                         _p(_f64(FFF)), _p(_f64(FRTA)), _p(_f64(Q)), _p(F2))
+    if rc != 0:
+        raise ValueError("update: a mesh point q1 +- q2 is not a member of its star")
+    return F2
+
+
+def bte_iter(kibz, mapinv, mapibz, kp, nc, g, primlatt, opk, eigenval, veloc, P1, P2, FRTA, Q, tempk, volume_r,
+             max_iter=150, update_mix=float(np.float32(0.4)), tol=1e-4, classical=0):
+    """The iterative BTE of job 3 (kappa.f90:1373-1487) on the whole irreducible zone:
+    F <- F + update_mix (update(F) - F), kappa by mode_kappa after every update, stop once
+    sqrt(sum (kappa_new - kappa_old)^2 / (9 nb)) < tol.  Returns (F, kappa history (nb,3,3,n)).
+    The defaults are read_params' (rest.f90:36,44): max_iter = 150 and update_mix = 0.4, a
+    REAL*4 literal stored in a REAL*8 (0.4000000059604645)."""
+    nibz = len(kibz); nb = eigenval.shape[0]
+    F = np.array(FRTA, order="F", copy=True)
+    kap, _ = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
+    hist = []
+    for _ in range(max_iter):
+        old = kap
+        F2 = update(0, nibz - 1, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, F, FRTA, Q)
+        F = np.asfortranarray(F + update_mix * (F2 - F))
+        kap, _ = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
+        hist.append(kap)
+        if np.sqrt(np.sum((kap - old) ** 2 / (nb * 9))) < tol:
+            break
+    return F, np.stack(hist, axis=-1)
+
+
 def rhs(mapinv, freq, dist, veloc, tempk):
     nk, nb = freq.shape; nibz = len(mapinv)
     R = np.zeros(nibz * nb * 3)

@@ -8,6 +8,12 @@
 ! Extra input: probe_temps.txt (optional) = list of temperatures in K, one per line; the
 ! temperature-dependent part (kappa.f90:557-856) is repeated for each.  Without it the
 ! single temperature tmin is used, as kappa.f90:565 does.
+!
+! REF_PROBE_ITER=n (environment, optional): after kappa_RTA, also run the iterative BTE of
+! job 3 (kappa.f90:1373-1487: update_mpi, F <- F + update_mix*(F2 - F), mode_kappa, stop when
+! the kappa change norm is below 1e-4) for at most min(n, max_iter) iterations, dumping the
+! first update F2, kappa after every iteration and the final F; F is then reset to F_RTA so
+! that the job-1 CG below starts from the same guess as before.
 program ref_probe
  use ios
  use kpoints
@@ -34,6 +40,9 @@ program ref_probe
  character(len=256) :: envs
  integer(8) :: c0, c1, crate
  logical :: ex
+ integer :: maxit, iter_, niter
+ real(8) :: normk
+ real(8), allocatable :: kiter(:,:,:,:)
 
  const33 = 1d30*(hbar/uma)**1.5d0/(sqrt(ee*1d20/uma))**1.5d0*ee/h_plank/100/c_light
  call system('mkdir -p dump')
@@ -168,6 +177,34 @@ program ref_probe
     call dumpr(trim(tag)//'_taueff', tauinv_eff, nibz*ndyn)
     call dumpr(trim(tag)//'_FRTA', FFF, nibz*ndyn*3)
     call dumpr(trim(tag)//'_kappaRTA', kappa, ndyn*9)
+    maxit = 0
+    call get_environment_variable('REF_PROBE_ITER', envs, status=ios_)
+    if (ios_ == 0) read(envs,*) maxit
+    maxit = min(maxit, max_iter)
+    if (maxit > 0) then
+       allocate(F_RTA(nibz,ndyn,3), F2(nibz,ndyn,3), F2mpi(ksub,ndyn,3))
+       allocate(kappa_old(ndyn,3,3), kappa_new(ndyn,3,3), e(ndyn,3,3), kiter(ndyn,3,3,maxit))
+       F_RTA = FFF
+       niter = 0
+       do iter_ = 1, maxit
+          kappa_old = kappa
+          call update_mpi(ndyn,ksubset)
+          F2 = F2mpi
+          if (iter_ == 1) call dumpr(trim(tag)//'_F2first', F2, nibz*ndyn*3)
+          FFF(:,:,:) = FFF(:,:,:) + update_mix*(F2(:,:,:)-FFF(:,:,:))
+          call mode_kappa(ndyn,tempk,veloc,eigenval)
+          kappa_new = kappa
+          niter = iter_
+          kiter(:,:,:,iter_) = kappa
+          e = (kappa_new-kappa_old)**2/(ndyn*3*3)
+          normk = sqrt(sum(e(:,:,:)))
+          if (normk .lt. 0.0001) exit
+       enddo
+       call dumpr(trim(tag)//'_kappaIter', kiter, ndyn*9*niter)
+       call dumpr(trim(tag)//'_Fiter', FFF, nibz*ndyn*3)
+       FFF = F_RTA
+       deallocate(F_RTA, F2, F2mpi, kappa_old, kappa_new, e, kiter)
+    endif
     call CollisionMatrix_mpi(ksubset,ndyn)
     call dumpr(trim(tag)//'_CM', Collision_Matrixmpi, size(Collision_Matrixmpi))
     allocate(Collision_Matrix(nibz*ndyn*3,nibz*ndyn*3))

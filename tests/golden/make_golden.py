@@ -6,6 +6,7 @@ the reference's file formats, the reference routines are run through oracle/_ref
 dumps are stored as compressed .npz next to this script.
 
     python tests/golden/make_golden.py            # si4 + si8
+    python tests/golden/make_golden.py iter       # si4_iter: the iterative BTE of job 3
 """
 from __future__ import annotations
 
@@ -28,12 +29,13 @@ PROBE = os.path.join(REPO, "oracle", "_ref", "ref_probe")
 TEMPS = (100.0, 300.0, 800.0)
 
 
-def run_reference(n: int, workdir: str):
+def run_reference(n: int, workdir: str, max_iter: int = 0):
     fc = models.sw_silicon()
     refio.write_inputs(workdir, fc, refio.RunParams(nc=(n, n, n), tmin=300.0, tmax=300.0))
     with open(os.path.join(workdir, "probe_temps.txt"), "w") as f:
         f.write("\n".join(str(t) for t in TEMPS) + "\n")
-    subprocess.run([PROBE], cwd=workdir, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    env = dict(os.environ, REF_PROBE_ITER=str(max_iter))
+    subprocess.run([PROBE], cwd=workdir, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env)
     return fc, load_dump(os.path.join(workdir, "dump"))
 
 
@@ -61,7 +63,24 @@ def pack(fc, D, ibz_subset=None, cm_rows=None):
     return out
 
 
+def main_iter():
+    """Job 3 on the 4^3 SW-Si case (the inputs of ref_si4.npz): update F2 of the first
+    iteration, kappa after every iteration (up to the reference's max_iter = 150 and its
+    1e-4 stop) and the final F, per temperature."""
+    with tempfile.TemporaryDirectory() as wd:
+        _, D = run_reference(4, wd, max_iter=150)
+        out = {"temps": np.array(D["temps"])}
+        for T in D["temps"]:
+            R = D[f"T{T}"]
+            for k in ("F2first", "Fiter", "kappaIter"):
+                out[f"T{T}_{k}"] = R[k]
+        np.savez_compressed(os.path.join(HERE, "ref_si4_iter.npz"), **out)
+        print("wrote si4_iter")
+
+
 def main():
+    if sys.argv[1:] == ["iter"]:
+        return main_iter()
     cases = [("si4", 4, None, None), ("si8", 8, [0, 1, 7, 23, 49], None)]
     for name, n, sub, rows in cases:
         with tempfile.TemporaryDirectory() as wd:

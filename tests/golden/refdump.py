@@ -53,4 +53,10 @@ def load_dump(d: str) -> dict:
             FRTA=r(f"{t}_FRTA", (nibz, nb, 3)), kappaRTA=r(f"{t}_kappaRTA", (nb, 3, 3)),
             CM=r(f"{t}_CM", (N, N)), RHS=r(f"{t}_RHS", (N,)), x=r(f"{t}_x", (N,)),
             kappaBTE=r(f"{t}_kappaBTE", (nb, 3, 3)), taueffd=r(f"{t}_taueffd", (nibz, nb)))
+        # iterative BTE (REF_PROBE_ITER): first update F2, kappa after each iteration, final F
+        kit = os.path.join(d, f"{t}_kappaIter.bin")
+        if os.path.exists(kit):
+            k = np.fromfile(kit, dtype=np.float64)
+            out[t].update(F2first=r(f"{t}_F2first", (nibz, nb, 3)), Fiter=r(f"{t}_Fiter", (nibz, nb, 3)),
+                          kappaIter=k.reshape((nb, 3, 3, k.size // (9 * nb)), order="F"))
     return out

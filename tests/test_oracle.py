@@ -252,3 +252,54 @@ def test_nbe_limits():
     assert O.lib().or_nbe(100 * t, t, 0) == 0.0
     assert abs(O.lib().or_nbe(2 * t, t, 0) - 1 / (np.exp(2) - 1)) < 1e-15
     assert abs(O.lib().or_nbe(3 * t, t, 1) - 1 / 3) < 1e-15
+
+
+# ------------------------------------------------------------------ iterative BTE (job 3)
+@pytest.fixture(scope="module")
+def ref4_iter():
+    from conftest import load_fixture
+    return load_fixture("si4_iter")
+
+
+def test_update_and_bte_iter_vs_reference(geo4, ref4, ref4_iter):
+    """update_mpi and the job-3 loop (kappa.f90:1373-1487) against the reference build: the
+    first update F2, kappa after every iteration and the iteration count, per temperature."""
+    G = geo4
+    E, V = ref4["eigenval"], ref4["veloc"]
+    for T in ref4["temps"]:
+        t = f"T{T}_"
+        freq, dist = O.distribution(E, float(T))
+        P1, P2, Q, QN, QU = O.tau(0, G["nibz"] - 1, G["mapinv"], G["kp"], G["nc"], G["g"], dist, ref4["P1sm"], ref4["P2sm"])
+        F = O.frta(G["mapinv"], freq, dist, V, Q, float(T))
+        F2 = O.update(0, G["nibz"] - 1, G["kibz"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt, G["opk"],
+                      P1, P2, F, F, Q)
+        assert rel(F2, ref4_iter[t + "F2first"]) < 1e-13
+        Fi, hist = O.bte_iter(G["kibz"], G["mapinv"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt, G["opk"],
+                              E, V, P1, P2, F, Q, float(T), G["vol"])
+        ref_hist = ref4_iter[t + "kappaIter"]
+        assert hist.shape == ref_hist.shape, (T, hist.shape, ref_hist.shape)
+        assert rel(hist, ref_hist) < 1e-11
+        assert rel(Fi, ref4_iter[t + "Fiter"]) < 1e-11
+
+
+def test_update_fixed_point_is_the_bte_solution(geo4, ref4):
+    """A planted check on the update's algebra: with F_RTA = F - A(F)/Q for an arbitrary F,
+    F is a fixed point of the update, and the map is affine in F (update(F1) - update(F0) is
+    linear in F1 - F0).  A sign or index slip in either process breaks the second property
+    only if it breaks linearity, so both are checked with random F."""
+    G = geo4
+    T = 300.0
+    freq, dist = O.distribution(ref4["eigenval"], T)
+    P1, P2, Q, _, _ = O.tau(0, G["nibz"] - 1, G["mapinv"], G["kp"], G["nc"], G["g"], dist, ref4["P1sm"], ref4["P2sm"])
+    rng = np.random.default_rng(3)
+    args = (0, G["nibz"] - 1, G["kibz"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt, G["opk"], P1, P2)
+    Z = np.zeros((G["nibz"], 6, 3), order="F")
+    Fa = np.asfortranarray(rng.standard_normal((G["nibz"], 6, 3)))
+    Fb = np.asfortranarray(rng.standard_normal((G["nibz"], 6, 3)))
+    Aa = O.update(*args, Fa, Z, Q)
+    Ab = O.update(*args, Fb, Z, Q)
+    Aab = O.update(*args, np.asfortranarray(2 * Fa - 3 * Fb), Z, Q)
+    assert rel(Aab, 2 * Aa - 3 * Ab) < 1e-12
+    # fixed point: F_RTA := F - A(F)/Q  =>  update(F) = F
+    Frta = np.asfortranarray(Fa - Aa)
+    assert rel(O.update(*args, Fa, Frta, Q), Fa) < 1e-12
