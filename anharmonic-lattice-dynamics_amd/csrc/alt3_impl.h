@@ -199,8 +199,15 @@ struct alt3_ctx {
     alt3::DBuf<double> d_cm, d_rhs;
     bool have_cm = false;
     alt3::DBuf<double> d_cg;                       // CG work vectors
+    // iterative BTE: S rows (sum over l2 of P with occupations, [ksub][nb][nb][nk], in the
+    // precision of R) and the update matrix U (rows (s,l1,alpha), columns (i,l,beta)) of upd_temp
+    alt3::DBuf<double> d_s1, d_s2, d_upd, d_iter;
+    alt3::DBuf<float> d_s1f, d_s2f;
+    bool have_upd = false;
+    double upd_temp = -1;
     // per-call scratch kept across calls (no hipMalloc/hipFree on the solve path)
-    alt3::DBuf<int> d_flags;                       // [0] V3 den=0, [1] tau, [2] kappa_rta, [3] kappa_bte, [4,5] CG
+    alt3::DBuf<int> d_flags;                       // [0] V3 den=0, [1] tau, [2] kappa_rta, [3] kappa_bte, [4,5] CG,
+                                                   // [6,7,8] iterative BTE (converged, iterations, N_BE error)
     alt3::DBuf<int> d_negs;                        // subst_eivecs pairs
     alt3::DBuf<double> d_qpart, d_wq, d_ev7, d_evec0;
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;

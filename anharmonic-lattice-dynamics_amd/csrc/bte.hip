@@ -5,6 +5,7 @@
 //   CollisionMatrix_mpi    sub.f90:3034-3210, cal_RHS_IBZ :2618-2691
 //   cg_quadratic           cg.f90:19-69
 //   mode_kappa_d           sub.f90:1407-1530 with veloc_avergeIBZ_invsyFBZ :5479-5537
+//   update_mpi + job 3     sub.f90:2295-2449, kappa.f90:1373-1487 (iterative BTE)
 //
 // Layouts on the device: P1sm/P2sm [s][l1][l2][l3][q2]; the tau kernel also writes
 // R1/R2[s][l1][l2][q2] = sum_l3 P1, P2 (occupations applied), which is all the collision
@@ -264,6 +265,35 @@ __device__ __forceinline__ void opmul(const double *op, const double *f, double 
     }
 }
 
+// mode_kappa (sub.f90:1292-1401) for one irreducible mode (i, ii) with distribution F and
+// velocity v at the IBZ point: the partial kappa over the mesh star and the tauinv_eff sum
+__device__ void mode_kappa_mode(int i, int ii, int nb, const double *F, const double *v,
+                                const double *__restrict__ eival, double temp, int classical, double const1,
+                                double const2, double cnst, const int *__restrict__ star_n,
+                                const int *__restrict__ star_op, const int *__restrict__ star_n0,
+                                const double *__restrict__ opk, double *kp, double &te, int *__restrict__ err)
+{
+    for (int e = 0; e < 9; e++) kp[e] = 0;
+    te = 0;
+    for (int s = 0; s < star_n[i]; s++) {
+        const double *op = opk + 9 * star_op[48 * i + s];
+        double F12[3], vc[3];
+        opmul(op, F, F12);
+        opmul(op, v, vc);
+        int n0 = star_n0[48 * i + s];
+        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
+        if (!(om > 0)) atomicExch(err, 1);
+        double nb0 = nbe(om, temp, classical);
+        for (int j = 0; j < 3; j++) {
+            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
+            if (vc[j] == 0.0)
+                te = te - const2 * F12[j] / om / 0.01;
+            else
+                te = te - const2 * F12[j] / om / vc[j] / (star_n[i] * 3.0);
+        }
+    }
+}
+
 // F_RTA (nibz,nb,3) Fortran layout, then per (i,ii) partial kappa (9) over the mesh star
 __global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ mapinv, const double *__restrict__ freq,
                             const double *__restrict__ dist, const double *__restrict__ eival,
@@ -286,25 +316,9 @@ __global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ ma
                (alt3::K_B * tempk * tempk) / (Q[(size_t)i * nb + ii] * alt3::C_LIGHT * 100);
         FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)] = F[a];
     }
-    double kp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double te = 0;
-    for (int s = 0; s < star_n[i]; s++) {
-        const double *op = opk + 9 * star_op[48 * i + s];
-        double F12[3], vc[3];
-        opmul(op, F, F12);
-        opmul(op, v, vc);
-        int n0 = star_n0[48 * i + s];
-        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
-        if (!(om > 0)) atomicExch(err, 1);
-        double nb0 = nbe(om, temp, classical);
-        for (int j = 0; j < 3; j++) {
-            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
-            if (vc[j] == 0.0)
-                te = te - const2 * F12[j] / om / 0.01;
-            else
-                te = te - const2 * F12[j] / om / vc[j] / (star_n[i] * 3.0);
-        }
-    }
+    double kp[9], te;
+    mode_kappa_mode(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, kp, te,
+                    err);
     for (int e = 0; e < 9; e++) kpart[(size_t)x * 9 + e] = kp[e];
     // mode_kappa zeroes the whole tauinv_eff array for every (i,ii): only the last survives
     taueff[i + (size_t)nibz * ii] = (i == nibz - 1 && ii == nb - 1) ? te : 0.0;
@@ -313,8 +327,9 @@ __global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ ma
 // kappa(ii, j, jj) Fortran (nb,3,3) = sum over i of the partials: one wavefront per output,
 // lane partial sums over i = lane, lane+64, ... then a shuffle tree
 __global__ void __launch_bounds__(64) k_kappa_sum(int nibz, int nb, const double *__restrict__ kpart,
-                                                  double *__restrict__ kappa)
+                                                  double *__restrict__ kappa, const int *__restrict__ skip)
 {
+    if (skip && *skip) return;
     const int x = blockIdx.x, ii = x / 9, e = x % 9;
     double acc = 0;
     for (int i = threadIdx.x; i < nibz; i += 64) acc += kpart[((size_t)i * nb + ii) * 9 + e];
@@ -368,6 +383,34 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
     }
 }
 
+// Phase 2 of the LDS collision kernels: each thread takes one irreducible q2 (slot) and sums
+// C(q0) op over the mesh points q0 of its star; row block rb, column branch l2 of the
+// row-major matrix M (ncol = nibz*nb*3).
+__device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, const double *C, const double *ops,
+                                          const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
+                                          const int *__restrict__ tab, double *__restrict__ M)
+{
+    const size_t ncol = (size_t)nibz * nb * 3;
+    for (int x = threadIdx.x; x < nibz; x += blockDim.x) {
+        const int n2 = slot_n2[x], cnt = slot_cnt[x];
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ns = 0; ns < cnt; ns++) {
+            const int e = tab[(size_t)ns * nibz + x];
+            const double c = C[e >> 6];
+            const double *op = ops + 9 * (e & 63);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (c * op[3 * j + i]);
+        }
+        const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) M[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+    }
+}
+
 // Collision matrix with the per-point collision term in LDS.  One workgroup per (row block
 // (s,l1), l2).  Phase 1 builds C(q0) = -R1(q0) + R1(q0-q1) + R2(q0)/2 + R2(q1-q0)/2 (or -Q
 // on the diagonal) for every mesh point q0 from row-contiguous reads of R1/R2; phase 2 gives
@@ -403,25 +446,141 @@ __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, in
     __syncthreads();
     if (threadIdx.x == 0 && l1 == l2) C[k1] = -Q[(size_t)s * nb + l1];
     __syncthreads();
-    const size_t ncol = (size_t)nibz * nb * 3;
-    for (int x = threadIdx.x; x < nibz; x += blockDim.x) {
-        const int n2 = slot_n2[x], cnt = slot_cnt[x];
-        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int ns = 0; ns < cnt; ns++) {
-            const int e = tab[(size_t)ns * nibz + x];
-            const double c = C[e >> 6];
-            const double *op = ops + 9 * (e & 63);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (c * op[3 * j + i]);
-        }
-        const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+    star_sums(nibz, nb, rb, l2, C, ops, slot_n2, slot_cnt, tab, CMr);
+}
+
+// update_mpi (sub.f90:2295-2449) as a matrix on the irreducible F, built like the collision
+// matrix: one workgroup per (row block (s,l1), branch l), with the coefficient of F(q0, l)
+//   U(q0) = -R1(q0) + R2(q0)/2 + S1(q0-q1) + S2(q1-q0)/2
+// (R: the -P1 F(q2,l2) and P2/2 F(q2,l2) terms; S: the P1 F(q1+q2,l3) and P2/2 F(q1-q2,l3)
+// terms, re-indexed by q0) and no diagonal replacement.  A(F) = U F.
+template <typename RT>
+__global__ void __launch_bounds__(256) k_update_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
+                                                  const int *__restrict__ mapinv, const RT *__restrict__ R1,
+                                                  const RT *__restrict__ R2, const RT *__restrict__ S1,
+                                                  const RT *__restrict__ S2, const int *__restrict__ slot_n2,
+                                                  const int *__restrict__ slot_cnt, const int *__restrict__ tab,
+                                                  const double *__restrict__ opk, double *__restrict__ U)
+{
+    extern __shared__ double lds[];
+    double *C = lds;            // [nk]
+    double *ops = lds + nk;     // [nops][9]
+    const int l = blockIdx.x % nb, rb = blockIdx.x / nb;  // rb = (s, l1)
+    const int s = rb / nb;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2;
+    m.coords(k1, a0, a1, a2);
+    const size_t rbase = ((size_t)rb * nb + l) * (size_t)nk;
+    for (int x = threadIdx.x; x < 9 * nops; x += blockDim.x) ops[x] = opk[x];
+    for (int n0 = threadIdx.x; n0 < nk; n0 += blockDim.x) {
+        int c0, c1, c2;
+        m.coords(n0, c0, c1, c2);
+        const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q0
+        const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q0-q1
+        C[n0] = -(double)R1[rbase + n0] + (double)S1[rbase + p2] + 0.5 * (double)R2[rbase + n0] +
+                0.5 * (double)S2[rbase + p1];
     }
+    __syncthreads();
+    star_sums(nibz, nb, rb, l, C, ops, slot_n2, slot_cnt, tab, U);
+}
+
+// S1/S2[s][l1][l3][q2] = sum over l2 of P1sm, P2sm times the occupations of
+// calculate_tauRTA_mpi2 (sub.f90:8054-8062): the weights of F(q1+q2, l3) and F(q1-q2, l3) in
+// update_mpi.  One thread per (s, q2).
+template <typename PT>
+__global__ void __launch_bounds__(256) k_tau_s(alt3::MeshDev m, int nk, int nb, int ibz_lo,
+                                               const int *__restrict__ mapinv, const double *__restrict__ dist,
+                                               const PT *__restrict__ P1sm, const PT *__restrict__ P2sm,
+                                               PT *__restrict__ S1, PT *__restrict__ S2)
+{
+    const int s = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nk) return;
+    const int k1 = mapinv[ibz_lo + s];
+    int a0, a1, a2, c0, c1, c2;
+    m.coords(k1, a0, a1, a2);
+    m.coords(j, c0, c1, c2);
+    const int np = m.sum(a0, a1, a2, c0, c1, c2), nm = m.diff(a0, a1, a2, c0, c1, c2);
+    for (int l1 = 0; l1 < nb; l1++) {
+        const double d1 = dist[(size_t)k1 * nb + l1];
+        for (int l3 = 0; l3 < nb; l3++) {
+            const double n3p = dist[(size_t)np * nb + l3] + 1, n3m = dist[(size_t)nm * nb + l3] + 1;
+            double s1 = 0, s2 = 0;
+            for (int l2 = 0; l2 < nb; l2++) {
+                const double d2 = dist[(size_t)j * nb + l2];
+                const size_t x = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk + j;
+                s1 += (double)P1sm[x] * ((d1 * d2) * n3p);
+                s2 += (double)P2sm[x] * ((d1 * (d2 + 1)) * n3m);
+            }
+            const size_t o = (((size_t)s * nb + l1) * nb + l3) * (size_t)nk + j;
+            S1[o] = (PT)s1;
+            S2[o] = (PT)s2;
+        }
+    }
+}
+
+// F2 = F_RTA + A/Q (update_mpi, sub.f90:2437) and, with mix > 0, F <- F + mix (F2 - F)
+// (kappa.f90:1456).  A and F in the vector layout 3*(i*nb+l)+alpha over rows i = ibz_lo..;
+// F_RTA Fortran (nibz,nb,3); Q [s][nb].  F2 (optional) Fortran (ksub,nb,3).
+__global__ void k_iter_mix(int ksub, int nibz, int nb, int ibz_lo, double mix, const double *__restrict__ FRTA,
+                           const double *__restrict__ Q, const double *__restrict__ A, double *__restrict__ F,
+                           double *__restrict__ F2out, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ksub * nb * 3) return;
+    const int a = e % 3, sl = e / 3, s = sl / nb, ii = sl % nb, i = ibz_lo + s;
+    const double F2 = FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)] + A[e] / Q[sl];
+    if (F2out) F2out[s + (size_t)ksub * (ii + (size_t)nb * a)] = F2;
+    if (F) F[e] = F[e] + mix * (F2 - F[e]);
+}
+
+// mode_kappa on F in the vector layout (the iterated FFF)
+__global__ void k_mode_kappa(int nibz, int nb, const int *__restrict__ mapinv, const double *__restrict__ x,
+                             const double *__restrict__ eival, const double *__restrict__ vel, double temp,
+                             int classical, double const1, double const2, double cnst, const int *__restrict__ star_n,
+                             const int *__restrict__ star_op, const int *__restrict__ star_n0,
+                             const double *__restrict__ opk, double *__restrict__ kpart, int *__restrict__ err,
+                             const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nibz * nb) return;
+    const int i = t / nb, ii = t % nb, m = mapinv[i];
+    const double F[3] = {x[3 * (size_t)t], x[3 * (size_t)t + 1], x[3 * (size_t)t + 2]};
+    const double v[3] = {vel[((size_t)m * nb + ii) * 3], vel[((size_t)m * nb + ii) * 3 + 1],
+                         vel[((size_t)m * nb + ii) * 3 + 2]};
+    double kp[9], te;
+    mode_kappa_mode(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, kp, te,
+                    err);
+    for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
+}
+
+// convergence test of kappa.f90:1466-1476: normk = sqrt(sum((kappa - kappa_old)^2 / (9 nb)));
+// kappa is appended to the history and becomes kappa_old; state[0] = converged,
+// state[1] = iterations done
+__global__ void k_iter_check(int nb, int it, double tol, const double *__restrict__ kappa, double *__restrict__ kold,
+                             double *__restrict__ hist, int *__restrict__ state)
+{
+    if (state[0] || threadIdx.x != 0) return;
+    const int n = 9 * nb;
+    double e = 0;
+    for (int x = 0; x < n; x++) {
+        const double d = kappa[x] - kold[x];
+        e = e + d * d / n;
+        hist[(size_t)it * n + x] = kappa[x];
+        kold[x] = kappa[x];
+    }
+    state[1] = it + 1;
+    if (sqrt(e) < tol) state[0] = 1;
+}
+
+// F_RTA Fortran (nibz,nb,3) -> vector layout 3*(i*nb+l)+alpha
+__global__ void k_frta_to_vec(int nibz, int nb, const double *__restrict__ FRTA, double *__restrict__ x)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nibz * nb * 3) return;
+    const int a = e % 3, il = e / 3, i = il / nb, ii = il % nb;
+    x[e] = FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)];
 }
 
 // row-major -> Fortran column-major copy (nrow x ncol)
@@ -433,17 +592,18 @@ __global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restric
     out[o] = in[r * ncol + c];
 }
 
-// y = A x for a row-major n x n matrix: one wavefront per row, columns in order per lane
-__global__ void __launch_bounds__(256) k_matvec_rows(int n, const double *__restrict__ A, const double *__restrict__ x,
-                                                     double *__restrict__ y, const int *__restrict__ skip)
+// y = A x for a row-major nrow x ncol matrix: one wavefront per row, columns in order per lane
+__global__ void __launch_bounds__(256) k_matvec_rows(int nrow, int ncol, const double *__restrict__ A,
+                                                     const double *__restrict__ x, double *__restrict__ y,
+                                                     const int *__restrict__ skip)
 {
-    if (skip && *skip) return;   // CG already converged
+    if (skip && *skip) return;   // iteration already converged
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    const double *a = A + (size_t)row * n;
+    if (row >= nrow) return;
+    const double *a = A + (size_t)row * ncol;
     double acc = 0;
-    for (int j = lane; j < n; j += 64) acc += a[j] * x[j];
+    for (int j = lane; j < ncol; j += 64) acc += a[j] * x[j];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) y[row] = acc;
 }
@@ -694,7 +854,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
         if (!(temps[t] > 0)) return fail(ALT3_E_ARG, "ERROR in N_BE: t=<0");
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const MeshDev md = MeshDev::make(ctx->nc);
-    ctx->have_tau = ctx->have_cm = false;
+    ctx->have_tau = ctx->have_cm = ctx->have_upd = false;
     ctx->tau_temps.clear();
     int *derr = ctx->d_flags.p + 1;
     ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
@@ -834,7 +994,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
                        temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
                        ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr);
     ALT3_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p, nullptr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -896,7 +1056,7 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
 {
     auto matvec = [&](const double *xin, double *yout, const int *skip) {
         if (rowmajor)
-            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
+            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, n, dA, xin, yout, skip);
         else
             hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
     };
@@ -964,13 +1124,156 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
                        ctx->d_eival.p, temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p,
                        ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr);
-    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p);
+    hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p, nullptr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    return ALT3_OK;
+}
+
+// ------------------------------------------------------------------ iterative BTE (job 3)
+// The update of update_mpi is linear in F: A(F) = U F with U built once per temperature from
+// the R rows of the tau pass and the S rows (k_tau_s), in the collision matrix's layout.  It
+// relies on the stars of the irreducible points (kstarsinfbz1) covering every mesh point once,
+// with mapibz naming the star a point belongs to: then the op update_mpi applies to F at
+// q1 +- q2 (get_kvecopn2) is the op of that point in the star table.
+static int update_build(alt3_ctx *ctx, double tempk, const char *who)
+{
+    ALT3_TRY(tau_select(ctx, tempk, who));
+    if (ctx->have_upd && ctx->upd_temp == tempk) return ALT3_OK;
+    const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    {
+        std::vector<int> seen(nk, 0);
+        for (int n = 0; n < nibz; n++)
+            for (int s = 0; s < ctx->T.star_n[n]; s++) {
+                const int q = ctx->T.star_n0[48 * n + s];
+                if (q < 0 || q >= nk || ctx->T.mapibz[q] != n || seen[q]++)
+                    return fail(ALT3_E_MESH, std::string(who) + ": the stars of the irreducible points do not partition the mesh");
+            }
+        for (int q = 0; q < nk; q++)
+            if (!seen[q]) return fail(ALT3_E_MESH, std::string(who) + ": a mesh point is in no star of an irreducible point");
+    }
+    const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
+    if (lds > ctx->lds_max) return fail(ALT3_E_ARG, std::string(who) + ": the mesh does not fit in LDS (nk too large)");
+    const size_t rstr = (size_t)ksub * nb * nb * nk, nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
+    ALT3_TRY(ctx->d_upd.alloc(nrow * ncol));
+    const MeshDev md = MeshDev::make(ctx->nc);
+    const dim3 gs((unsigned)((nk + 255) / 256), (unsigned)ksub);
+    auto launch = [&](auto *P1, auto *P2, auto *R1, auto *R2, auto &S1, auto &S2) {
+        using PT = std::remove_const_t<std::remove_pointer_t<decltype(P1)>>;
+        int rc = S1.alloc(rstr);
+        if (rc == ALT3_OK) rc = S2.alloc(rstr);
+        if (rc != ALT3_OK) return rc;
+        hipLaunchKernelGGL(k_tau_s<PT>, gs, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
+                           ctx->dist(), P1, P2, S1.p, S2.p);
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void *)k_update_c<PT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return fail(ALT3_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(k_update_c<PT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
+                           nibz, ctx->nops, ctx->d_mapinv.p, (const PT *)R1, (const PT *)R2, (const PT *)S1.p,
+                           (const PT *)S2.p, ctx->d_cm_slot_n2.p, ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p,
+                           ctx->d_upd.p);
+        return ALT3_OK;
+    };
+    if (ctx->r_fp32) {
+        ctx->d_s1.release();
+        ctx->d_s2.release();
+        ALT3_TRY(launch((const float *)ctx->d_p1f.p, (const float *)ctx->d_p2f.p, ctx->r1f(), ctx->r2f(), ctx->d_s1f, ctx->d_s2f));
+    } else {
+        ctx->d_s1f.release();
+        ctx->d_s2f.release();
+        ALT3_TRY(launch((const double *)ctx->d_p1.p, (const double *)ctx->d_p2.p, ctx->r1(), ctx->r2(), ctx->d_s1, ctx->d_s2));
+    }
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_upd = true;
+    ctx->upd_temp = tempk;
+    return ALT3_OK;
+}
+
+int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2)
+{
+    ALT3_TRY(update_build(ctx, tempk, "alt3_update"));
+    const int nb = ctx->nb, nibz = ctx->nibz, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
+    const size_t N = (size_t)nibz * nb * 3, nrow = (size_t)ksub * nb * 3;
+    std::vector<double> x(N);
+    for (int i = 0; i < nibz; i++)   // Fortran (nibz,nb,3) -> vector layout
+        for (int l = 0; l < nb; l++)
+            for (int a = 0; a < 3; a++) x[3 * ((size_t)i * nb + l) + a] = FFF[i + (size_t)nibz * (l + (size_t)nb * a)];
+    ALT3_TRY(ctx->d_iter.reserve(2 * N + nrow + N));
+    double *dx = ctx->d_iter.p, *dfr = dx + N, *dA = dfr + N, *dF2 = dA + nrow;
+    ALT3_HIP(hipMemcpyAsync(dx, x.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    ALT3_HIP(hipMemcpyAsync(dfr, FRTA, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_matvec_rows, dim3((unsigned)((nrow + 3) / 4)), dim3(256), 0, ctx->stream, (int)nrow, (int)N,
+                       ctx->d_upd.p, dx, dA, nullptr);
+    hipLaunchKernelGGL(k_iter_mix, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, ctx->stream, ksub, nibz, nb,
+                       ctx->ibz_lo, 0.0, dfr, ctx->q(), dA, nullptr, dF2, nullptr);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipMemcpyAsync(F2, dF2, sizeof(double) * nrow, hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    return ALT3_OK;
+}
+
+int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
+                   double *FFF, int *iters)
+{
+    if (max_iter < 1) return fail(ALT3_E_ARG, "max_iter must be at least 1");
+    ALT3_TRY(need_full_ibz(ctx, tempk, "alt3_kappa_iter"));
+    ALT3_TRY(kappa_rta_run(ctx, tempk, nullptr, nullptr, nullptr));   // F_RTA, kappa_RTA = first kappa_old
+    ALT3_TRY(update_build(ctx, tempk, "alt3_kappa_iter"));
+    const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, N = nibz * nb * 3;
+    ALT3_TRY(ctx->d_iter.reserve(2 * (size_t)N + 9 * (size_t)nb * (1 + (size_t)max_iter)));
+    double *x = ctx->d_iter.p, *y = x + N, *kold = y + N, *hist = kold + 9 * nb;
+    int *state = ctx->d_flags.p + 6, *derr = ctx->d_flags.p + 8;   // converged, iterations; N_BE error
+    ALT3_HIP(hipMemsetAsync(state, 0, 3 * sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_frta_to_vec, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_frta.p, x);
+    ALT3_HIP(hipMemcpyAsync(kold, ctx->d_kappa.p, sizeof(double) * 9 * nb, hipMemcpyDeviceToDevice, ctx->stream));
+    const double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
+    const double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
+    // iterations run on the device; the host looks at the converged flag every 8 of them
+    for (int it = 0; it < max_iter; it++) {
+        hipLaunchKernelGGL(k_matvec_rows, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, N, N, ctx->d_upd.p, x, y, state);
+        hipLaunchKernelGGL(k_iter_mix, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nibz, nb, 0, mix,
+                           ctx->d_frta.p, ctx->q(), y, x, nullptr, state);
+        hipLaunchKernelGGL(k_mode_kappa, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb,
+                           ctx->d_mapinv.p, x, ctx->d_eival.p, ctx->d_vel.p, temp_cm(tempk), ctx->classical, const1,
+                           const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p,
+                           ctx->d_kpart.p, derr, state);
+        hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p,
+                           state);
+        hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(64), 0, ctx->stream, nb, it, tol, ctx->d_kappa.p, kold, hist, state);
+        ALT3_HIP(hipGetLastError());
+        if (it % 8 == 7 || it == max_iter - 1) {
+            int st[3];
+            ALT3_HIP(hipMemcpyAsync(st, state, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+            ALT3_HIP(hipStreamSynchronize(ctx->stream));
+            if (st[2]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+            if (st[0]) break;
+        }
+    }
+    int st[3];
+    ALT3_HIP(hipMemcpyAsync(st, state, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (st[2]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    const int done = st[1];
+    if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * 9 * nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (khist && done > 0)
+        ALT3_HIP(hipMemcpyAsync(khist, hist, sizeof(double) * 9 * nb * done, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> xv;
+    if (FFF) {
+        xv.resize(N);
+        ALT3_HIP(hipMemcpyAsync(xv.data(), x, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (FFF)
+        for (int i = 0; i < nibz; i++)
+            for (int l = 0; l < nb; l++)
+                for (int a = 0; a < 3; a++) FFF[i + (size_t)nibz * (l + (size_t)nb * a)] = xv[3 * ((size_t)i * nb + l) + a];
+    if (iters) *iters = done;
     return ALT3_OK;
 }
 

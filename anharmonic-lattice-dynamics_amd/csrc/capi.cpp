@@ -32,6 +32,9 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
               double *funcmin, int *iters, bool rowmajor);
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
+int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
+int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
+                   double *FFF, int *iters);
 
 namespace {
 int copy_p(alt3_ctx *ctx, double *P1sm, double *P2sm)
@@ -152,6 +155,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_cm.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
+    ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -167,6 +171,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
     b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
     b += ctx->d_r1f.bytes() + ctx->d_r2f.bytes();
+    b += ctx->d_s1.bytes() + ctx->d_s2.bytes() + ctx->d_s1f.bytes() + ctx->d_s2f.bytes() + ctx->d_upd.bytes();
     b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
     b += ctx->d_ephf.bytes() + ctx->d_evf.bytes() + ctx->d_gpf32.bytes();
     *bytes = b;
@@ -397,6 +402,27 @@ int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, 
     ALT3_TRY(kappa_bte_run(ctx, tempk, dx.p, kappa, dte.p));
     tm.stop();
     if (taueff) ALT3_HIP(hipMemcpy(taueff, dte.p, sizeof(double) * ctx->nibz * ctx->nb, hipMemcpyDeviceToHost));
+    return ALT3_OK;
+}
+
+int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2)
+{
+    if (!ctx || !FFF || !FRTA || !F2) return fail(ALT3_E_ARG, "null argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(update_run(ctx, tempk, FFF, FRTA, F2));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix, double tol, double *kappa,
+                    double *kappa_hist, double *FFF, int *iters)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(kappa_iter_run(ctx, tempk, max_iter, update_mix, tol, kappa, kappa_hist, FFF, iters));
+    tm.stop();
     return ALT3_OK;
 }
 

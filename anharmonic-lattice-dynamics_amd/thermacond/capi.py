@@ -45,7 +45,11 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta", "alt3_footprint"]
+           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter"]
+
+# read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4 as a REAL*4 literal
+ITER_MAX = 150
+UPDATE_MIX = float(np.float32(0.4))
 
 _lib = None
 
@@ -211,6 +215,23 @@ class Context:
         te = np.zeros((self.nibz, self.nb), order="F")
         _check(lib().alt3_kappa_rta(self.h, C.c_double(tempk), _p(k), _p(F), _p(te)))
         return k, F, te
+
+    def update(self, tempk: float, FFF, FRTA):
+        """update_mpi: F2 (ksub, nb, 3) = F_RTA + A(FFF)/Q for the irreducible points of the context."""
+        FFF = np.asfortranarray(FFF, dtype=np.float64); FRTA = np.asfortranarray(FRTA, dtype=np.float64)
+        F2 = np.zeros((self._ksub(), self.nb, 3), order="F")
+        _check(lib().alt3_update(self.h, C.c_double(tempk), _p(FFF), _p(FRTA), _p(F2)))
+        return F2
+
+    def kappa_iter(self, tempk: float, max_iter: int = ITER_MAX, update_mix: float = UPDATE_MIX, tol: float = 1e-4):
+        """The iterative BTE of job 3: final kappa (nb,3,3), kappa after every iteration
+        (nb,3,3,iters), final F (nibz,nb,3) and the iteration count."""
+        k = np.zeros((self.nb, 3, 3), order="F"); F = np.zeros((self.nibz, self.nb, 3), order="F")
+        hist = np.zeros(max_iter * self.nb * 9); it = C.c_int()
+        _check(lib().alt3_kappa_iter(self.h, C.c_double(tempk), max_iter, C.c_double(update_mix), C.c_double(tol),
+                                     _p(k), _p(hist), _p(F), C.byref(it)))
+        n = it.value
+        return dict(kappa=k, hist=hist[:n * self.nb * 9].reshape((self.nb, 3, 3, n), order="F"), F=F, iters=n)
 
     def collision(self, tempk: float, want: bool = True):
         n = self.nibz * self.nb * 3

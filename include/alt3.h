@@ -150,6 +150,24 @@ int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, i
  * reference module array (pass the alt3_kappa_rta output). */
 int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, double *taueff);
 
+/* One update of the iterative BTE, update_mpi (sub.f90:2295-2449), for the irreducible points
+ * of the context:  F2 = F_RTA + A(FFF)/Q  with A summed over the mesh as the reference does
+ * (P1 (F(q1+q2,l3) - F(q2,l2)) + P2/2 (F(q2,l2) + F(q1-q2,l3)), F on the mesh rotated from the
+ * IBZ by the star operations).  Needs alt3_tau at tempk.  FFF and FRTA Fortran (nibz,nb,3)
+ * (the module arrays FFF and F_RTA); F2 Fortran (ksub,nb,3) (F2mpi).  A is held on the device
+ * as a matrix built once per temperature. */
+int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
+
+/* The iterative BTE of job 3 (kappa.f90:1373-1487) at tempk on the device, after alt3_tau over
+ * the whole IBZ: F starts at F_RTA (calculate_RTA_distributionfunction), then per iteration
+ * F <- F + update_mix (update_mpi(F) - F) and mode_kappa (sub.f90:1292-1401), until
+ * sqrt(sum (kappa - kappa_old)^2 / (9 nb)) < tol or max_iter iterations.  The reference uses
+ * max_iter = 150, update_mix = 0.4 as a REAL*4 literal (pass (double)0.4f) and tol = 1e-4.
+ * kappa Fortran (nb,3,3) of the last iteration; kappa_hist [iters][nb*9] or NULL (room for
+ * max_iter); FFF Fortran (nibz,nb,3) final or NULL; iters = iterations done. */
+int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix, double tol, double *kappa,
+                    double *kappa_hist, double *FFF, int *iters);
+
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
  * IBZ and FGR (unless the context still holds them from an earlier call with the same
  * precision and delta integration; alt3_eig/alt3_set_eig clear them), then per T tau (one pass over P1sm/P2sm per group of four temperatures),
