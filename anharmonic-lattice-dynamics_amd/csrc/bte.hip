@@ -265,63 +265,83 @@ __device__ __forceinline__ void opmul(const double *op, const double *f, double 
     }
 }
 
-// mode_kappa (sub.f90:1292-1401) for one irreducible mode (i, ii) with distribution F and
-// velocity v at the IBZ point: the partial kappa over the mesh star and the tauinv_eff sum
-__device__ void mode_kappa_mode(int i, int ii, int nb, const double *F, const double *v,
+// The per-mode star sums below run one wavefront per irreducible mode: lane s evaluates the
+// terms of star member s, and lane 0 adds them in member order, which is the reference's
+// summation order.  kStarWave modes per 256-thread block; sh holds 12 doubles per lane.
+constexpr int kStarWave = 4;
+
+// mode_kappa (sub.f90:1292-1401) for irreducible mode (i, ii) with distribution F and velocity
+// v at the IBZ point: the partial kappa over the mesh star and the tauinv_eff sum (lane 0)
+__device__ void mode_kappa_wave(int i, int ii, int nb, const double *F, const double *v,
                                 const double *__restrict__ eival, double temp, int classical, double const1,
                                 double const2, double cnst, const int *__restrict__ star_n,
                                 const int *__restrict__ star_op, const int *__restrict__ star_n0,
-                                const double *__restrict__ opk, double *kp, double &te, int *__restrict__ err)
+                                const double *__restrict__ opk, double *sh, double *kp, double &te,
+                                int *__restrict__ err, bool valid)
 {
-    for (int e = 0; e < 9; e++) kp[e] = 0;
-    te = 0;
-    for (int s = 0; s < star_n[i]; s++) {
-        const double *op = opk + 9 * star_op[48 * i + s];
+    const int lane = threadIdx.x & 63, narms = star_n[i];
+    double *my = sh + 12 * lane;
+    if (valid && lane < narms) {
+        const double *op = opk + 9 * star_op[48 * i + lane];
         double F12[3], vc[3];
         opmul(op, F, F12);
         opmul(op, v, vc);
-        int n0 = star_n0[48 * i + s];
-        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
+        const int n0 = star_n0[48 * i + lane];
+        const double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
         if (!(om > 0)) atomicExch(err, 1);
-        double nb0 = nbe(om, temp, classical);
+        const double nb0 = nbe(om, temp, classical);
         for (int j = 0; j < 3; j++) {
-            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
-            if (vc[j] == 0.0)
-                te = te - const2 * F12[j] / om / 0.01;
-            else
-                te = te - const2 * F12[j] / om / vc[j] / (star_n[i] * 3.0);
+            my[j] = const1 * om * vc[j] * nb0 * (nb0 + 1);
+            my[3 + j] = F12[j];
+            my[6 + j] = vc[j] == 0.0 ? const2 * F12[j] / om / (double)0.01f   // REAL*4 literal (sub.f90:1362)
+                                     : const2 * F12[j] / om / vc[j] / (narms * 3.0);
+        }
+    }
+    __syncthreads();   // every thread of the block reaches it (invalid modes carry valid = false)
+    if (valid && lane == 0) {
+        for (int e = 0; e < 9; e++) kp[e] = 0;
+        te = 0;
+        for (int m = 0; m < narms; m++) {
+            const double *t = sh + 12 * m;
+            for (int j = 0; j < 3; j++) {
+                for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - t[j] * t[3 + jj];
+                te = te - t[6 + j];
+            }
         }
     }
 }
 
-// F_RTA (nibz,nb,3) Fortran layout, then per (i,ii) partial kappa (9) over the mesh star
-__global__ void k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ mapinv, const double *__restrict__ freq,
-                            const double *__restrict__ dist, const double *__restrict__ eival,
-                            const double *__restrict__ vel, const double *__restrict__ Q /*[i][nb]*/, double tempk,
-                            double temp, int classical, double const1, double const2, double cnst,
-                            const int *__restrict__ star_n, const int *__restrict__ star_op,
-                            const int *__restrict__ star_n0, const double *__restrict__ opk,
-                            double *__restrict__ FRTA, double *__restrict__ kpart, double *__restrict__ taueff,
-                            int *__restrict__ err)
+// F_RTA (nibz,nb,3) Fortran layout, then per (i,ii) partial kappa (9) over the mesh star; one
+// wavefront per mode
+__global__ void __launch_bounds__(64 * kStarWave) k_kappa_rta(int nibz, int nb, int nk, const int *__restrict__ mapinv,
+                            const double *__restrict__ freq, const double *__restrict__ dist,
+                            const double *__restrict__ eival, const double *__restrict__ vel,
+                            const double *__restrict__ Q /*[i][nb]*/, double tempk, double temp, int classical,
+                            double const1, double const2, double cnst, const int *__restrict__ star_n,
+                            const int *__restrict__ star_op, const int *__restrict__ star_n0,
+                            const double *__restrict__ opk, double *__restrict__ FRTA, double *__restrict__ kpart,
+                            double *__restrict__ taueff, int *__restrict__ err)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nibz * nb) return;
-    int i = x / nb, ii = x % nb;
-    int m = mapinv[i];
-    double omega = freq[(size_t)m * nb + ii], nbv = dist[(size_t)m * nb + ii];
+    __shared__ double sh[kStarWave][64 * 12];
+    const int w = threadIdx.x >> 6, x0 = blockIdx.x * kStarWave + w;
+    const bool valid = x0 < nibz * nb;
+    const int x = valid ? x0 : 0, i = x / nb, ii = x % nb, m = mapinv[i], lane = threadIdx.x & 63;
+    const double omega = freq[(size_t)m * nb + ii], nbv = dist[(size_t)m * nb + ii];
     double F[3], v[3];
     for (int a = 0; a < 3; a++) {
         v[a] = vel[((size_t)m * nb + ii) * 3 + a];
         F[a] = -(alt3::C_LIGHT * v[a]) * alt3::H_PLANK * (omega * alt3::C_LIGHT * 100) * nbv * (nbv + 1) /
                (alt3::K_B * tempk * tempk) / (Q[(size_t)i * nb + ii] * alt3::C_LIGHT * 100);
-        FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)] = F[a];
     }
     double kp[9], te;
-    mode_kappa_mode(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, kp, te,
-                    err);
-    for (int e = 0; e < 9; e++) kpart[(size_t)x * 9 + e] = kp[e];
-    // mode_kappa zeroes the whole tauinv_eff array for every (i,ii): only the last survives
-    taueff[i + (size_t)nibz * ii] = (i == nibz - 1 && ii == nb - 1) ? te : 0.0;
+    mode_kappa_wave(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, sh[w],
+                    kp, te, err, valid);
+    if (valid && lane == 0) {
+        for (int a = 0; a < 3; a++) FRTA[i + (size_t)nibz * (ii + (size_t)nb * a)] = F[a];
+        for (int e = 0; e < 9; e++) kpart[(size_t)x * 9 + e] = kp[e];
+        // mode_kappa zeroes the whole tauinv_eff array for every (i,ii): only the last survives
+        taueff[i + (size_t)nibz * ii] = (i == nibz - 1 && ii == nb - 1) ? te : 0.0;
+    }
 }
 
 // kappa(ii, j, jj) Fortran (nb,3,3) = sum over i of the partials: one wavefront per output,
@@ -534,34 +554,42 @@ __global__ void k_iter_mix(int ksub, int nibz, int nb, int ibz_lo, double mix, c
     if (F) F[e] = F[e] + mix * (F2 - F[e]);
 }
 
-// mode_kappa on F in the vector layout (the iterated FFF)
-__global__ void k_mode_kappa(int nibz, int nb, const int *__restrict__ mapinv, const double *__restrict__ x,
-                             const double *__restrict__ eival, const double *__restrict__ vel, double temp,
-                             int classical, double const1, double const2, double cnst, const int *__restrict__ star_n,
-                             const int *__restrict__ star_op, const int *__restrict__ star_n0,
-                             const double *__restrict__ opk, double *__restrict__ kpart, int *__restrict__ err,
+// mode_kappa on F in the vector layout (the iterated FFF); one wavefront per mode
+__global__ void __launch_bounds__(64 * kStarWave) k_mode_kappa(int nibz, int nb, const int *__restrict__ mapinv,
+                             const double *__restrict__ x, const double *__restrict__ eival,
+                             const double *__restrict__ vel, double temp, int classical, double const1, double const2,
+                             double cnst, const int *__restrict__ star_n, const int *__restrict__ star_op,
+                             const int *__restrict__ star_n0, const double *__restrict__ opk,
+                             double *__restrict__ kpart, double *__restrict__ te_last, int *__restrict__ err,
                              const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nibz * nb) return;
-    const int i = t / nb, ii = t % nb, m = mapinv[i];
+    __shared__ double sh[kStarWave][64 * 12];
+    const int w = threadIdx.x >> 6, t0 = blockIdx.x * kStarWave + w;
+    const bool valid = t0 < nibz * nb;
+    const int t = valid ? t0 : 0, i = t / nb, ii = t % nb, m = mapinv[i];
     const double F[3] = {x[3 * (size_t)t], x[3 * (size_t)t + 1], x[3 * (size_t)t + 2]};
     const double v[3] = {vel[((size_t)m * nb + ii) * 3], vel[((size_t)m * nb + ii) * 3 + 1],
                          vel[((size_t)m * nb + ii) * 3 + 2]};
     double kp[9], te;
-    mode_kappa_mode(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, kp, te,
-                    err);
-    for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
+    mode_kappa_wave(i, ii, nb, F, v, eival, temp, classical, const1, const2, cnst, star_n, star_op, star_n0, opk, sh[w],
+                    kp, te, err, valid);
+    if (valid && (threadIdx.x & 63) == 0) {
+        for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
+        // mode_kappa clears tauinv_eff for every mode: only the last mode's value survives
+        if (t == nibz * nb - 1) *te_last = te;
+    }
 }
 
 // convergence test of kappa.f90:1466-1476: normk = sqrt(sum((kappa - kappa_old)^2 / (9 nb)));
-// kappa is appended to the history and becomes kappa_old; state[0] = converged,
-// state[1] = iterations done
+// kappa (and the surviving tauinv_eff value) is appended to the history and kappa becomes
+// kappa_old; state[0] = converged, state[1] = iterations done
 __global__ void k_iter_check(int nb, int it, double tol, const double *__restrict__ kappa, double *__restrict__ kold,
-                             double *__restrict__ hist, int *__restrict__ state)
+                             double *__restrict__ hist, const double *__restrict__ te_last,
+                             double *__restrict__ te_hist, int *__restrict__ state)
 {
     if (state[0] || threadIdx.x != 0) return;
+    te_hist[it] = *te_last;
     const int n = 9 * nb;
     double e = 0;
     for (int x = 0; x < n; x++) {
@@ -738,58 +766,78 @@ __global__ void k_axpy_init(int n, const double *__restrict__ ax, const double *
     conj[i] = -g;
 }
 
-// star-averaged velocities with inverse operations (veloc_avergeIBZ_invsyFBZ)
-__global__ void k_veloci(int nibz, int nb, const double *__restrict__ vel, const int *__restrict__ fstar_n,
-                         const int *__restrict__ fstar_op, const int *__restrict__ fstar_n0,
-                         const double *__restrict__ opk_inv, double *__restrict__ veloci)
+// star-averaged velocities with inverse operations (veloc_avergeIBZ_invsyFBZ); one wavefront per
+// mode, the members' v/narms added in member order by lane 0
+__global__ void __launch_bounds__(64 * kStarWave) k_veloci(int nibz, int nb, const double *__restrict__ vel,
+                         const int *__restrict__ fstar_n, const int *__restrict__ fstar_op,
+                         const int *__restrict__ fstar_n0, const double *__restrict__ opk_inv,
+                         double *__restrict__ veloci)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nibz * nb) return;
-    int i = x / nb, j = x % nb;
-    int narms = fstar_n[i];
-    double va[3] = {0, 0, 0};
-    for (int n = 0; n < narms; n++) {
-        int n0 = fstar_n0[48 * i + n];
+    __shared__ double sh[kStarWave][64 * 3];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, x0 = blockIdx.x * kStarWave + w;
+    const bool valid = x0 < nibz * nb;
+    const int x = valid ? x0 : 0, i = x / nb, j = x % nb, narms = fstar_n[i];
+    if (valid && lane < narms) {
+        const int n0 = fstar_n0[48 * i + lane];
         double vi[3];
-        opmul(opk_inv + 9 * fstar_op[48 * i + n], vel + ((size_t)n0 * nb + j) * 3, vi);
-        for (int r = 0; r < 3; r++) va[r] = va[r] + (vi[r] / narms);
+        opmul(opk_inv + 9 * fstar_op[48 * i + lane], vel + ((size_t)n0 * nb + j) * 3, vi);
+        for (int r = 0; r < 3; r++) sh[w][3 * lane + r] = vi[r] / narms;
     }
-    for (int r = 0; r < 3; r++) veloci[(size_t)x * 3 + r] = va[r];
+    __syncthreads();
+    if (valid && lane == 0) {
+        double va[3] = {0, 0, 0};
+        for (int n = 0; n < narms; n++)
+            for (int r = 0; r < 3; r++) va[r] = va[r] + sh[w][3 * n + r];
+        for (int r = 0; r < 3; r++) veloci[(size_t)x * 3 + r] = va[r];
+    }
 }
 
-__global__ void k_kappa_bte(int nibz, int nb, const double *__restrict__ x, const double *__restrict__ veloci,
-                            const double *__restrict__ eival, double temp, int classical, double const1,
-                            double const2, double cnst, const int *__restrict__ star_n,
-                            const int *__restrict__ star_op, const int *__restrict__ star_n0,
-                            const double *__restrict__ opk, double *__restrict__ kpart, double *__restrict__ taueff,
-                            int *__restrict__ err)
+// mode_kappa_d per mode, one wavefront per mode (terms per star member, ordered sums by lane 0)
+__global__ void __launch_bounds__(64 * kStarWave) k_kappa_bte(int nibz, int nb, const double *__restrict__ x,
+                            const double *__restrict__ veloci, const double *__restrict__ eival, double temp,
+                            int classical, double const1, double const2, double cnst,
+                            const int *__restrict__ star_n, const int *__restrict__ star_op,
+                            const int *__restrict__ star_n0, const double *__restrict__ opk,
+                            double *__restrict__ kpart, double *__restrict__ taueff, int *__restrict__ err)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nibz * nb) return;
-    int i = t / nb, ii = t % nb;
-    double f[3] = {x[3 * (size_t)t], x[3 * (size_t)t + 1], x[3 * (size_t)t + 2]};
-    const double *vi = veloci + (size_t)t * 3;
-    double kp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double te = taueff[i + (size_t)nibz * ii];
-    int narms = star_n[i];
-    for (int s = 0; s < narms; s++) {
-        const double *op = opk + 9 * star_op[48 * i + s];
+    __shared__ double sh[kStarWave][64 * 10];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, t0 = blockIdx.x * kStarWave + w;
+    const bool valid = t0 < nibz * nb;
+    const int t = valid ? t0 : 0, i = t / nb, ii = t % nb, narms = star_n[i];
+    if (valid && lane < narms) {
+        const double f[3] = {x[3 * (size_t)t], x[3 * (size_t)t + 1], x[3 * (size_t)t + 2]};
+        const double *vi = veloci + (size_t)t * 3;
+        const double *op = opk + 9 * star_op[48 * i + lane];
         double F12[3], vc[3];
         opmul(op, f, F12);
         opmul(op, vi, vc);
-        int n0 = star_n0[48 * i + s];
-        double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
+        const int n0 = star_n0[48 * i + lane];
+        const double om = sqrt(eival[(size_t)n0 * nb + ii]) * cnst;
         if (!(om > 0)) atomicExch(err, 1);
-        double nb0 = nbe(om, temp, classical);
+        const double nb0 = nbe(om, temp, classical);
+        double *my = sh[w] + 10 * lane;
         for (int j = 0; j < 3; j++) {
-            for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - const1 * om * vc[j] * nb0 * (nb0 + 1) * F12[jj];
-            double fv = F12[0] * vc[0] + F12[1] * vc[1] + F12[2] * vc[2];
-            double vv = vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2];
-            te = te - const2 * fv / om / vv / (narms * 3.0);
+            my[j] = const1 * om * vc[j] * nb0 * (nb0 + 1);
+            my[3 + j] = F12[j];
         }
+        const double fv = F12[0] * vc[0] + F12[1] * vc[1] + F12[2] * vc[2];
+        const double vv = vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2];
+        my[9] = const2 * fv / om / vv / (narms * 3.0);
     }
-    for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
-    taueff[i + (size_t)nibz * ii] = te;
+    __syncthreads();
+    if (valid && lane == 0) {
+        double kp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double te = taueff[i + (size_t)nibz * ii];
+        for (int m = 0; m < narms; m++) {
+            const double *tm = sh[w] + 10 * m;
+            for (int j = 0; j < 3; j++) {   // the reference subtracts the same term once per j
+                for (int jj = 0; jj < 3; jj++) kp[j + 3 * jj] = kp[j + 3 * jj] - tm[j] * tm[3 + jj];
+                te = te - tm[9];
+            }
+        }
+        for (int e = 0; e < 9; e++) kpart[(size_t)t * 9 + e] = kp[e];
+        taueff[i + (size_t)nibz * ii] = te;
+    }
 }
 
 }  // namespace
@@ -989,7 +1037,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
-    hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, nk,
+    hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb, nk,
                        ctx->d_mapinv.p, ctx->d_freq.p, ctx->dist(), ctx->d_eival.p, ctx->d_vel.p, ctx->q(), tempk,
                        temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p,
                        ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_frta.p, ctx->d_kpart.p, ctx->d_taueff.p, derr);
@@ -1115,13 +1163,13 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     int *derr = ctx->d_flags.p + 3;
     ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     if (!ctx->have_veloci) {   // star-averaged IBZ velocities depend on the eigen data only
-        hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
+        hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
                            ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
         ctx->have_veloci = true;
     }
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
-    hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
+    hipLaunchKernelGGL(k_kappa_bte, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb, dx, ctx->d_veloci.p,
                        ctx->d_eival.p, temp_cm(tempk), ctx->classical, const1, const2, cnst(), ctx->d_star_n.p,
                        ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr);
     hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p, nullptr);
@@ -1219,15 +1267,16 @@ int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRT
 }
 
 int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
-                   double *FFF, int *iters)
+                   double *tehist, double *FFF, int *iters)
 {
     if (max_iter < 1) return fail(ALT3_E_ARG, "max_iter must be at least 1");
     ALT3_TRY(need_full_ibz(ctx, tempk, "alt3_kappa_iter"));
     ALT3_TRY(kappa_rta_run(ctx, tempk, nullptr, nullptr, nullptr));   // F_RTA, kappa_RTA = first kappa_old
     ALT3_TRY(update_build(ctx, tempk, "alt3_kappa_iter"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, N = nibz * nb * 3;
-    ALT3_TRY(ctx->d_iter.reserve(2 * (size_t)N + 9 * (size_t)nb * (1 + (size_t)max_iter)));
+    ALT3_TRY(ctx->d_iter.reserve(2 * (size_t)N + (9 * (size_t)nb + 1) * (1 + (size_t)max_iter)));
     double *x = ctx->d_iter.p, *y = x + N, *kold = y + N, *hist = kold + 9 * nb;
+    double *te_hist = hist + 9 * (size_t)nb * max_iter, *te_last = te_hist + max_iter;
     int *state = ctx->d_flags.p + 6, *derr = ctx->d_flags.p + 8;   // converged, iterations; N_BE error
     ALT3_HIP(hipMemsetAsync(state, 0, 3 * sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_frta_to_vec, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_frta.p, x);
@@ -1239,13 +1288,14 @@ int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double
         hipLaunchKernelGGL(k_matvec_rows, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, N, N, ctx->d_upd.p, x, y, state);
         hipLaunchKernelGGL(k_iter_mix, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nibz, nb, 0, mix,
                            ctx->d_frta.p, ctx->q(), y, x, nullptr, state);
-        hipLaunchKernelGGL(k_mode_kappa, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb,
+        hipLaunchKernelGGL(k_mode_kappa, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb,
                            ctx->d_mapinv.p, x, ctx->d_eival.p, ctx->d_vel.p, temp_cm(tempk), ctx->classical, const1,
                            const2, cnst(), ctx->d_star_n.p, ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p,
-                           ctx->d_kpart.p, derr, state);
+                           ctx->d_kpart.p, te_last, derr, state);
         hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p,
                            state);
-        hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(64), 0, ctx->stream, nb, it, tol, ctx->d_kappa.p, kold, hist, state);
+        hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(64), 0, ctx->stream, nb, it, tol, ctx->d_kappa.p, kold, hist,
+                           te_last, te_hist, state);
         ALT3_HIP(hipGetLastError());
         if (it % 8 == 7 || it == max_iter - 1) {
             int st[3];
@@ -1263,6 +1313,8 @@ int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * 9 * nb, hipMemcpyDeviceToHost, ctx->stream));
     if (khist && done > 0)
         ALT3_HIP(hipMemcpyAsync(khist, hist, sizeof(double) * 9 * nb * done, hipMemcpyDeviceToHost, ctx->stream));
+    if (tehist && done > 0)
+        ALT3_HIP(hipMemcpyAsync(tehist, te_hist, sizeof(double) * done, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> xv;
     if (FFF) {
         xv.resize(N);

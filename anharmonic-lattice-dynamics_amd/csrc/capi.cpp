@@ -34,7 +34,7 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
 int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
-                   double *FFF, int *iters);
+                   double *tehist, double *FFF, int *iters);
 
 namespace {
 int copy_p(alt3_ctx *ctx, double *P1sm, double *P2sm)
@@ -416,12 +416,12 @@ int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FR
 }
 
 int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix, double tol, double *kappa,
-                    double *kappa_hist, double *FFF, int *iters)
+                    double *kappa_hist, double *taueff_hist, double *FFF, int *iters)
 {
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(kappa_iter_run(ctx, tempk, max_iter, update_mix, tol, kappa, kappa_hist, FFF, iters));
+    ALT3_TRY(kappa_iter_run(ctx, tempk, max_iter, update_mix, tol, kappa, kappa_hist, taueff_hist, FFF, iters));
     tm.stop();
     return ALT3_OK;
 }

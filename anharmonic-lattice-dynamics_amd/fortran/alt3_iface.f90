@@ -100,6 +100,20 @@ module alt3_iface
        real(c_double), value :: tempk
      end function alt3_kappa_bte
 
+     integer(c_int) function alt3_kappa_iter(ctx, tempk, max_iter, update_mix, tol, kappa, kappa_hist, taueff_hist, &
+          fff, iters) bind(C, name='alt3_kappa_iter')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, kappa, kappa_hist, taueff_hist, fff, iters
+       real(c_double), value :: tempk, update_mix, tol
+       integer(c_int), value :: max_iter
+     end function alt3_kappa_iter
+
+     integer(c_int) function alt3_update(ctx, tempk, fff, frta, f2) bind(C, name='alt3_update')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, fff, frta, f2
+       real(c_double), value :: tempk
+     end function alt3_update
+
      integer(c_int) function alt3_solve(ctx, ntemps, temps, precision, kappa_rta, kappa_bte, cg_iters, stage_ms) &
           bind(C, name='alt3_solve')
        import :: c_int, c_ptr

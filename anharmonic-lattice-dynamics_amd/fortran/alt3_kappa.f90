@@ -1,12 +1,13 @@
-!  THERMACOND job 1 (kappa.f90:283-851) driven through the alt3 C ABI.
+!  THERMACOND jobs 1 and 3 (kappa.f90:283-851, :1046-1487) driven through the alt3 C ABI.
 !
 !  Reads the reference's input files from the working directory (latdyn.params,
 !  structure.params, lat_fc.dat, fc2.dat, fc3.dat; readers rest.f90:2-443), runs the
-!  three-phonon path on the GPU stage by stage in the order of kappa.f90 job 1, and writes
-!  the reference's job-1 outputs with write_kappa's formats (sub.f90:1818-1916):
-!    tau_RTA.dat, kappa_RTA.dat                 (RTA,      kappa.f90:562-678)
-!    tau_RTAandeff-cg.dat, kappa_direct-cg.dat  (CG / BTE, kappa.f90:703-851)
-!  As in job 1, the temperature is tmin of latdyn.params (kappa.f90:565).
+!  three-phonon path on the GPU stage by stage in the order of kappa.f90, and writes the
+!  reference's outputs with write_kappa's formats (sub.f90:1818-1916):
+!    tau_RTA.dat, kappa_RTA.dat                 (RTA,      kappa.f90:562-678, :1190-1304)
+!    job 1: tau_RTAandeff-cg.dat, kappa_direct-cg.dat   (CG / BTE, kappa.f90:703-851)
+!    job 3: tau_RTAandeff_iter.dat, kappa_iter.dat       (iterative BTE, kappa.f90:1373-1487)
+!  The job is the one latdyn.params names (3, else 1); the temperature is tmin (kappa.f90:565).
 !
 !  usage: alt3_kappa [fp64 | mixed | fp64ref] [--host-only]
 !    --host-only builds the mesh and symmetry tables only (no GPU) and prints their sizes.
@@ -46,12 +47,12 @@ program alt3_kappa
   integer(c_int), allocatable, target :: mapinv(:)
   real(c_double), allocatable, target :: q(:, :), qn(:, :), qu(:, :), taun(:, :), tauu(:, :), tautot(:, :)
   real(c_double), allocatable, target :: frta(:, :, :), taueff(:, :), x(:)
-  real(c_double), allocatable, target :: kappa(:)
+  real(c_double), allocatable, target :: kappa(:), khist(:), tehist(:)
   real(c_double), target :: fmin
   real(c_double) :: tempk
   character(len=32) :: arg
   logical :: host_only
-  integer :: i, l, a, ia
+  integer :: i, l, a, ia, it
 
   prec = ALT3_V3_FP64
   host_only = .false.
@@ -129,6 +130,29 @@ program alt3_kappa
   call write_kappa(10, 11)
   close(10)
   close(11)
+
+  if (job == 3) then
+     ! iterative BTE (kappa.f90:1373-1487): max_iter = 150, update_mix = 0.4 and the stop
+     ! normk < 0.0001, the last two REAL*4 literals (rest.f90:36,44)
+     allocate(khist(150 * nb * 9), tehist(150))
+     call alt3_check(alt3_kappa_iter(ctx, tempk, 150, real(0.4, c_double), real(0.0001, c_double), c_loc(kappa), &
+          c_loc(khist), c_loc(tehist), c_null_ptr, c_loc(iters)), 'alt3_kappa_iter')
+     open(10, file='kappa_iter.dat', status='unknown')
+     open(11, file='tau_RTAandeff_iter.dat', status='unknown')
+     do it = 1, iters                  ! (write_kappa loops over the host variable i)
+        ! mode_kappa of iteration it: kappa, and tauinv_eff cleared but for its last mode
+        kappa = khist((it - 1) * nb * 9 + 1:it * nb * 9)
+        taueff = 0
+        taueff(nibz, nb) = tehist(it)
+        write(10, *) '================= iteration', it
+        call write_kappa(10, 11)
+     end do
+     close(10)
+     close(11)
+     write(*, '(a,i0)') 'alt3_kappa: job 3 iterations ', iters
+     call alt3_check(alt3_destroy(ctx), 'alt3_destroy')
+     stop
+  end if
 
   ! CollisionMatrix_mpi + cal_RHS_IBZ (kappa.f90:719-822); the matrix stays on the device
   call alt3_check(alt3_collision(ctx, tempk, c_null_ptr, c_null_ptr), 'alt3_collision')

@@ -47,9 +47,11 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter"]
 
-# read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4 as a REAL*4 literal
+# read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
+# (kappa.f90:1476).  Both constants are REAL*4 literals.
 ITER_MAX = 150
 UPDATE_MIX = float(np.float32(0.4))
+ITER_TOL = float(np.float32(1e-4))
 
 _lib = None
 
@@ -223,15 +225,18 @@ class Context:
         _check(lib().alt3_update(self.h, C.c_double(tempk), _p(FFF), _p(FRTA), _p(F2)))
         return F2
 
-    def kappa_iter(self, tempk: float, max_iter: int = ITER_MAX, update_mix: float = UPDATE_MIX, tol: float = 1e-4):
+    def kappa_iter(self, tempk: float, max_iter: int = ITER_MAX, update_mix: float = UPDATE_MIX,
+                   tol: float = ITER_TOL):
         """The iterative BTE of job 3: final kappa (nb,3,3), kappa after every iteration
-        (nb,3,3,iters), final F (nibz,nb,3) and the iteration count."""
+        (nb,3,3,iters), the surviving tauinv_eff(nibz,nb) per iteration, final F (nibz,nb,3)
+        and the iteration count."""
         k = np.zeros((self.nb, 3, 3), order="F"); F = np.zeros((self.nibz, self.nb, 3), order="F")
-        hist = np.zeros(max_iter * self.nb * 9); it = C.c_int()
+        hist = np.zeros(max_iter * self.nb * 9); te = np.zeros(max_iter); it = C.c_int()
         _check(lib().alt3_kappa_iter(self.h, C.c_double(tempk), max_iter, C.c_double(update_mix), C.c_double(tol),
-                                     _p(k), _p(hist), _p(F), C.byref(it)))
+                                     _p(k), _p(hist), _p(te), _p(F), C.byref(it)))
         n = it.value
-        return dict(kappa=k, hist=hist[:n * self.nb * 9].reshape((self.nb, 3, 3, n), order="F"), F=F, iters=n)
+        return dict(kappa=k, hist=hist[:n * self.nb * 9].reshape((self.nb, 3, 3, n), order="F"), taueff=te[:n],
+                    F=F, iters=n)
 
     def collision(self, tempk: float, want: bool = True):
         n = self.nibz * self.nb * 3

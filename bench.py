@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--precision", choices=["both", "mixed", "fp64", "fp64ref"], default="both")
     ap.add_argument("--temps", default="100,300,800")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-iter-bte", action="store_true", help="skip the iterative BTE (job 3) line")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving"], default="step",
                     help="step: the SW-Si 16^3 headline step (default); rocksalt: the 8-atom rock-salt-type "
@@ -104,6 +105,22 @@ def timed(ctx, prec, temps, warmup, steps, label, quiet):
     lat_s = sorted(lat)
     return dict(ms=1e3 * sum(lat) / steps, p50=1e3 * lat_s[len(lat) // 2],
                 p99=1e3 * lat_s[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], stage=stage, out=out)
+
+
+def timed_iter(ctx, prec, T, quiet):
+    """The iterative BTE of job 3 at one temperature after a solve in the head precision: the
+    first call builds the update matrix (S rows + star sums), the later ones reuse it."""
+    ctx.eig(want=False)
+    ctx.solve([T], prec)
+    times, r = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = ctx.kappa_iter(T)
+        times.append((time.perf_counter() - t0) * 1e3)
+    log(f"iterative BTE at {T} K: {r['iters']} iterations, {times[0]:.2f} ms first call, {times[-1]:.2f} ms after", quiet)
+    return {"tempk": T, "iterations": r["iters"], "ms_first_call": times[0], "ms": times[-1],
+            "ms_per_iteration": times[-1] / max(r["iters"], 1), "kappa": kappa_iso(r["kappa"]),
+            "note": "kappa.f90 job 3: update_mpi as a device matrix, mixing 0.4, stop at normk < 1e-4"}
 
 
 def profiled_traffic(kernel_prefix: str):
@@ -181,10 +198,11 @@ def main():
     precs = {"both": [("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)], "mixed": [("mixed", capi.V3_MIXED)],
              "fp64": [("fp64", capi.V3_FP64)], "fp64ref": [("fp64ref", capi.V3_FP64_REF)]}[args.precision]
     runs = {name: timed(ctx, p, temps, args.warmup, args.steps, name, args.quiet) for name, p in precs}
-    ctx.close()
     head_name = precs[0][0]
     head = runs[head_name]
     ti = min(1, len(temps) - 1)
+    iter_bte = None if args.no_iter_bte else timed_iter(ctx, precs[0][1], temps[ti], args.quiet)
+    ctx.close()
 
     def summary(r):
         return {"value": nel / (r["ms"] * 1e-3), "ms_per_step": r["ms"],
@@ -217,6 +235,8 @@ def main():
     }
     for name, r in runs.items():
         res[name] = summary(r)
+    if iter_bte is not None:
+        res["iter_bte"] = iter_bte
     if "mixed" in runs and "fp64" in runs:
         m, f = runs["mixed"], runs["fp64"]
         errs = {}

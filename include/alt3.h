@@ -162,11 +162,14 @@ int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FR
  * the whole IBZ: F starts at F_RTA (calculate_RTA_distributionfunction), then per iteration
  * F <- F + update_mix (update_mpi(F) - F) and mode_kappa (sub.f90:1292-1401), until
  * sqrt(sum (kappa - kappa_old)^2 / (9 nb)) < tol or max_iter iterations.  The reference uses
- * max_iter = 150, update_mix = 0.4 as a REAL*4 literal (pass (double)0.4f) and tol = 1e-4.
- * kappa Fortran (nb,3,3) of the last iteration; kappa_hist [iters][nb*9] or NULL (room for
- * max_iter); FFF Fortran (nibz,nb,3) final or NULL; iters = iterations done. */
+ * max_iter = 150 and update_mix = 0.4 (rest.f90:36,44) and stops at normk < 0.0001
+ * (kappa.f90:1476); the two constants are REAL*4 literals: pass (double)0.4f and (double)1e-4f.
+ * kappa Fortran (nb,3,3) of the last iteration; kappa_hist [iters][nb*9] or NULL and
+ * taueff_hist [iters] or NULL (room for max_iter): after each iteration kappa and
+ * tauinv_eff(nibz,nb), the only element of tauinv_eff mode_kappa leaves nonzero (it clears
+ * the array for every mode); FFF Fortran (nibz,nb,3) final or NULL; iters = iterations done. */
 int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix, double tol, double *kappa,
-                    double *kappa_hist, double *FFF, int *iters);
+                    double *kappa_hist, double *taueff_hist, double *FFF, int *iters);
 
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
  * IBZ and FGR (unless the context still holds them from an earlier call with the same

@@ -85,7 +85,7 @@ static void comp1(const double *q, const double *rr, int n, int *i, int *insid)
     double aux = qdr - floor(qdr);
     double y = 1 + n * aux;
     *i = (int)lround(y);
-    if (fabs(*i - y) > 0.01) *insid = 2;
+    if (fabs(*i - y) > (double)0.01f) *insid = 2;   /* REAL*4 literal, extratools.f90:609 */
     if (*i == n + 1) *i = 1;
 }
 
@@ -1000,7 +1000,7 @@ void or_mode_kappa(int nibz, int nb, int nk, const double *kibz, const int *mapi
                         kappa[ii + nb * (j + 3 * jj)] = kappa[ii + nb * (j + 3 * jj)] - const1 * omega * velocc[j] * nb0 * (nb0 + 1) * F12[jj];
                     size_t x = i + (size_t)nibz * ii;
                     if (velocc[j] == 0.0)
-                        taueff[x] = taueff[x] - const2 * F12[j] / omega / 0.01;
+                        taueff[x] = taueff[x] - const2 * F12[j] / omega / (double)0.01f;   /* REAL*4 literal */
                     else
                         taueff[x] = taueff[x] - const2 * F12[j] / omega / velocc[j] / (narms * 3.0);
                 }

@@ -331,25 +331,28 @@ def update(ibz_lo, ibz_hi, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, FFF, 
 
 
 def bte_iter(kibz, mapinv, mapibz, kp, nc, g, primlatt, opk, eigenval, veloc, P1, P2, FRTA, Q, tempk, volume_r,
-             max_iter=150, update_mix=float(np.float32(0.4)), tol=1e-4, classical=0):
+             max_iter=150, update_mix=float(np.float32(0.4)), tol=float(np.float32(1e-4)), classical=0):
     """The iterative BTE of job 3 (kappa.f90:1373-1487) on the whole irreducible zone:
     F <- F + update_mix (update(F) - F), kappa by mode_kappa after every update, stop once
-    sqrt(sum (kappa_new - kappa_old)^2 / (9 nb)) < tol.  Returns (F, kappa history (nb,3,3,n)).
+    sqrt(sum (kappa_new - kappa_old)^2 / (9 nb)) < tol.  Returns (F, kappa history (nb,3,3,n),
+    tauinv_eff(nibz,nb) after each iteration (n,)).
     The defaults are read_params' (rest.f90:36,44): max_iter = 150 and update_mix = 0.4, a
-    REAL*4 literal stored in a REAL*8 (0.4000000059604645)."""
+    REAL*4 literal stored in a REAL*8 (0.4000000059604645); the stop test normk < 0.0001
+    (kappa.f90:1476) compares with a REAL*4 literal too."""
     nibz = len(kibz); nb = eigenval.shape[0]
     F = np.array(FRTA, order="F", copy=True)
     kap, _ = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
-    hist = []
+    hist, tes = [], []
     for _ in range(max_iter):
         old = kap
         F2 = update(0, nibz - 1, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, F, FRTA, Q)
         F = np.asfortranarray(F + update_mix * (F2 - F))
-        kap, _ = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
+        kap, te = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
         hist.append(kap)
+        tes.append(te[-1, -1])
         if np.sqrt(np.sum((kap - old) ** 2 / (nb * 9))) < tol:
             break
-    return F, np.stack(hist, axis=-1)
+    return F, np.stack(hist, axis=-1), np.array(tes)
 
 
 def rhs(mapinv, freq, dist, veloc, tempk):

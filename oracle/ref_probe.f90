@@ -12,7 +12,8 @@
 ! REF_PROBE_ITER=n (environment, optional): after kappa_RTA, also run the iterative BTE of
 ! job 3 (kappa.f90:1373-1487: update_mpi, F <- F + update_mix*(F2 - F), mode_kappa, stop when
 ! the kappa change norm is below 1e-4) for at most min(n, max_iter) iterations, dumping the
-! first update F2, kappa after every iteration and the final F; F is then reset to F_RTA so
+! first update F2, kappa and tauinv_eff(nibz,ndyn) after every iteration and the final F
+! (mode_kappa clears tauinv_eff for every mode, so no other element survives); F is then reset to F_RTA so
 ! that the job-1 CG below starts from the same guess as before.
 program ref_probe
  use ios
@@ -42,7 +43,7 @@ program ref_probe
  logical :: ex
  integer :: maxit, iter_, niter
  real(8) :: normk
- real(8), allocatable :: kiter(:,:,:,:)
+ real(8), allocatable :: kiter(:,:,:,:), teiter(:)
 
  const33 = 1d30*(hbar/uma)**1.5d0/(sqrt(ee*1d20/uma))**1.5d0*ee/h_plank/100/c_light
  call system('mkdir -p dump')
@@ -183,7 +184,7 @@ program ref_probe
     maxit = min(maxit, max_iter)
     if (maxit > 0) then
        allocate(F_RTA(nibz,ndyn,3), F2(nibz,ndyn,3), F2mpi(ksub,ndyn,3))
-       allocate(kappa_old(ndyn,3,3), kappa_new(ndyn,3,3), e(ndyn,3,3), kiter(ndyn,3,3,maxit))
+       allocate(kappa_old(ndyn,3,3), kappa_new(ndyn,3,3), e(ndyn,3,3), kiter(ndyn,3,3,maxit), teiter(maxit))
        F_RTA = FFF
        niter = 0
        do iter_ = 1, maxit
@@ -196,14 +197,16 @@ program ref_probe
           kappa_new = kappa
           niter = iter_
           kiter(:,:,:,iter_) = kappa
+          teiter(iter_) = tauinv_eff(nibz,ndyn)
           e = (kappa_new-kappa_old)**2/(ndyn*3*3)
           normk = sqrt(sum(e(:,:,:)))
           if (normk .lt. 0.0001) exit
        enddo
        call dumpr(trim(tag)//'_kappaIter', kiter, ndyn*9*niter)
        call dumpr(trim(tag)//'_Fiter', FFF, nibz*ndyn*3)
+       call dumpr(trim(tag)//'_taueffIter', teiter, niter)
        FFF = F_RTA
-       deallocate(F_RTA, F2, F2mpi, kappa_old, kappa_new, e, kiter)
+       deallocate(F_RTA, F2, F2mpi, kappa_old, kappa_new, e, kiter, teiter)
     endif
     call CollisionMatrix_mpi(ksubset,ndyn)
     call dumpr(trim(tag)//'_CM', Collision_Matrixmpi, size(Collision_Matrixmpi))

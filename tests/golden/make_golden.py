@@ -65,14 +65,14 @@ def pack(fc, D, ibz_subset=None, cm_rows=None):
 
 def main_iter():
     """Job 3 on the 4^3 SW-Si case (the inputs of ref_si4.npz): update F2 of the first
-    iteration, kappa after every iteration (up to the reference's max_iter = 150 and its
-    1e-4 stop) and the final F, per temperature."""
+    iteration, kappa and the surviving tauinv_eff after every iteration (up to the
+    reference's max_iter = 150 and its 1e-4 stop) and the final F, per temperature."""
     with tempfile.TemporaryDirectory() as wd:
         _, D = run_reference(4, wd, max_iter=150)
         out = {"temps": np.array(D["temps"])}
         for T in D["temps"]:
             R = D[f"T{T}"]
-            for k in ("F2first", "Fiter", "kappaIter"):
+            for k in ("F2first", "Fiter", "kappaIter", "taueffIter"):
                 out[f"T{T}_{k}"] = R[k]
         np.savez_compressed(os.path.join(HERE, "ref_si4_iter.npz"), **out)
         print("wrote si4_iter")

@@ -58,5 +58,6 @@ def load_dump(d: str) -> dict:
         if os.path.exists(kit):
             k = np.fromfile(kit, dtype=np.float64)
             out[t].update(F2first=r(f"{t}_F2first", (nibz, nb, 3)), Fiter=r(f"{t}_Fiter", (nibz, nb, 3)),
-                          kappaIter=k.reshape((nb, 3, 3, k.size // (9 * nb)), order="F"))
+                          kappaIter=k.reshape((nb, 3, 3, k.size // (9 * nb)), order="F"),
+                          taueffIter=r(f"{t}_taueffIter", (k.size // (9 * nb),)))
     return out

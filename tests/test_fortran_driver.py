@@ -78,3 +78,33 @@ def test_driver_job1_matches_stage_entry_points(si_inputs, tmp_path):
     ok = np.isfinite(want)
     assert ok.sum() > nibz * nb // 2
     assert np.allclose(tab[ok, 10], want[ok], rtol=1e-7, atol=0)
+
+
+def _kappa_blocks(path):
+    """tot_kap of every write_kappa block of kappa_iter.dat, in order."""
+    return [float(l.split("=")[1].replace("D", "E")) for l in open(path) if "tot_kap=" in l]
+
+
+@pytest.mark.gpu
+def test_driver_job3_matches_kappa_iter(si_inputs, tmp_path):
+    """job = 3 in latdyn.params: kappa_RTA.dat plus one kappa_iter.dat / tau_RTAandeff_iter.dat
+    block per iteration; tot_kap of each block equals the alt3_kappa_iter history."""
+    print("case: Fortran driver job 3, SW-Si 4^3 fp64", flush=True)
+    from thermacond import refio
+    inp = si_inputs[4]
+    refio.write_inputs(str(tmp_path), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), job=3))
+    out = _run(tmp_path, "fp64")
+    T = float(inp.params.tmin)
+    with capi.Context(inp) as c:
+        c.eig(want=False)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        c.fgr(want=False)
+        c.tau(T)
+        r = c.kappa_iter(T)
+    assert f"job 3 iterations {r['iters']}" in out
+    tots = _kappa_blocks(tmp_path / "kappa_iter.dat")
+    want = [np.trace(r["hist"][..., i].sum(0)) / 3 for i in range(r["iters"])]
+    assert len(tots) == r["iters"]
+    assert np.allclose(tots, want, rtol=1e-10, atol=0)
+    assert (tmp_path / "tau_RTAandeff_iter.dat").exists()
+    assert not (tmp_path / "kappa_direct-cg.dat").exists()

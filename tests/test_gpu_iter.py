@@ -86,6 +86,7 @@ def test_kappa_iter_vs_reference_4(si_inputs, ref4, ref4_iter):
             assert r["iters"] == ref_hist.shape[-1], (T, r["iters"], ref_hist.shape[-1])
             check(r["hist"], ref_hist, 1e-7, t + "kappa per iteration")
             check(r["kappa"], ref_hist[..., -1], 1e-7, t + "final kappa")
+            check(r["taueff"], ref4_iter[t + "taueffIter"], 1e-7, t + "tauinv_eff per iteration")
             check(r["F"], ref4_iter[t + "Fiter"], 1e-7, t + "final F")
 
 

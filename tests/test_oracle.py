@@ -274,11 +274,12 @@ def test_update_and_bte_iter_vs_reference(geo4, ref4, ref4_iter):
         F2 = O.update(0, G["nibz"] - 1, G["kibz"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt, G["opk"],
                       P1, P2, F, F, Q)
         assert rel(F2, ref4_iter[t + "F2first"]) < 1e-13
-        Fi, hist = O.bte_iter(G["kibz"], G["mapinv"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt, G["opk"],
-                              E, V, P1, P2, F, Q, float(T), G["vol"])
+        Fi, hist, tes = O.bte_iter(G["kibz"], G["mapinv"], G["mapibz"], G["kp"], G["nc"], G["g"], G["inp"].primlatt,
+                                   G["opk"], E, V, P1, P2, F, Q, float(T), G["vol"])
         ref_hist = ref4_iter[t + "kappaIter"]
         assert hist.shape == ref_hist.shape, (T, hist.shape, ref_hist.shape)
         assert rel(hist, ref_hist) < 1e-11
+        assert rel(tes, ref4_iter[t + "taueffIter"]) < 1e-11
         assert rel(Fi, ref4_iter[t + "Fiter"]) < 1e-11
 
 
