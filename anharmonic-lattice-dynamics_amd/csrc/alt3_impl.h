@@ -117,6 +117,10 @@ struct V3Tables {
 };
 int build_v3_tables(const alt3_system *sys, V3Tables &v);
 int v3_prep_launch(alt3_ctx *ctx, int ksub);
+int v3_prep_points(alt3_ctx *ctx, int nq, const double *q1, const double *e1, double *gp);
+int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, double *evals, double *evec);
+int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, const double *w3, const double *w1,
+                   double *out, int *d_err);
 bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 

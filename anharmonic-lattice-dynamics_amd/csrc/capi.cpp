@@ -33,6 +33,8 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
               double *funcmin, int *iters, bool rowmajor);
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
+int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, double *omega0,
+             double *nself, double *uself);
 int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
                    double *tehist, double *FFF, int *iters);
 
@@ -411,6 +413,17 @@ int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FR
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
     ALT3_TRY(update_run(ctx, tempk, FFF, FRTA, F2));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
+                 double *nself, double *uself)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    Timer tm(ctx);
+    ALT3_TRY(self_run(ctx, nq, q, tempk, eta, v3_threshold, omega0, nself, uself));
     tm.stop();
     return ALT3_OK;
 }

@@ -22,7 +22,7 @@ __device__ __forceinline__ zd zmul(zd a, zd b) { return {a.r * b.r - a.i * b.i, 
 constexpr int kDq = 7;  // q, q+dx, q-dx, q+dy, q-dy, q+dz, q-dz
 
 // entries: e = row*nb + col; per entry a CSR range of FC2 terms (reference term order)
-__global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *__restrict__ kpc,
+__global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, const double *__restrict__ kpc,
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
                                                   const double *__restrict__ term_rr, double dq,
@@ -30,8 +30,8 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *
                                                   double *__restrict__ evec0)    // [nk][nb col][nb comp] complex
 {
     extern __shared__ double lds[];
-    const int m = blockIdx.x;  // matrix id = q*7 + variant
-    const int q = m / kDq, var = m % kDq;
+    const int m = blockIdx.x;  // matrix id = q*nvar + variant (nvar 7: q and q +- dq e_a; 1: q only)
+    const int q = m / nvar, var = m % nvar;
     if (q >= nk) return;
     const int lane = threadIdx.x;
     zd *A = reinterpret_cast<zd *>(lds);  // [nb][nb] row-major
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *
     __syncthreads();
     if (lane < nb) {
         int src = s_ord[lane];
-        evals7[((size_t)q * kDq + var) * nb + lane] = fabs(A[src * nb + src].r);
+        evals7[((size_t)q * nvar + var) * nb + lane] = fabs(A[src * nb + src].r);
     }
     if (var == 0) {
         // column j (sorted) of V; "pure imaginary" columns (sum |Re| < 1e-3) times i
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nb, const double *
 // summation order), so the results are bit-identical; a wavefront now carries up to eight
 // rotation chains at once instead of one.
 template <int G>
-__global__ void __launch_bounds__(64) k_eig_group(int nk, int nb, const double *__restrict__ kpc,
+__global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, const double *__restrict__ kpc,
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
                                                   const double *__restrict__ term_rr, double dq,
@@ -253,8 +253,8 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nb, const double *
     __shared__ double s_sum[M][2];
     __shared__ int s_ord[M][G];
     const int mat = blockIdx.x * M + g;
-    const bool valid = mat < nk * kDq;
-    const int q = valid ? mat / kDq : 0, var = valid ? mat % kDq : 0;
+    const bool valid = mat < nk * nvar;
+    const int q = valid ? mat / nvar : 0, var = valid ? mat % nvar : 0;
     // norm sum of k_eig_solve: lane l of 64 sums entries l, l+64, ..., then the lane sums in order
     auto sum2 = [&](auto f, double &s0, double &s1) {
         s0 = 0;
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nb, const double *
     __syncthreads();
     if (!valid || sub >= nb) return;
     const int src = s_ord[g][sub];
-    evals7[((size_t)q * kDq + var) * nb + sub] = fabs(A[src * nb + src].r);
+    evals7[((size_t)q * nvar + var) * nb + sub] = fabs(A[src * nb + src].r);
     if (var == 0) {
         // column j = sub (sorted) of V; "pure imaginary" columns (sum |Re| < 1e-3) times i
         double sre = 0;
@@ -513,29 +513,24 @@ int eig_prepare(alt3_ctx *ctx, const alt3_system *sys)
     return ALT3_OK;
 }
 
-int eig_run(alt3_ctx *ctx)
+// Hermitian eigensolves of D(q) for np points kq (Cartesian) and nvar variants each (7: q and
+// q +- dq e_a; 1: q only): evals [np][nvar][nb] = |lambda| in zheev order, evec [np][nb col][nb]
+// (variant 0, with the x i fix of pure-imaginary columns)
+int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, double *evals, double *evec)
 {
-    const int nk = ctx->nk, nb = ctx->nb;
-    if (nb > 64) return fail(ALT3_E_ARG, "more than 64 branches per cell");
-    DBuf<double> &ev7 = ctx->d_ev7, &evec0 = ctx->d_evec0;
-    ALT3_TRY(ev7.reserve((size_t)nk * kDq * nb));
-    ALT3_TRY(evec0.reserve((size_t)nk * nb * nb * 2));
-    ALT3_TRY(ctx->d_eival.alloc((size_t)nk * nb));
-    ALT3_TRY(ctx->d_freq.alloc((size_t)nk * nb));
-    ALT3_TRY(ctx->d_vel.alloc((size_t)nk * nb * 3));
-    ALT3_TRY(ctx->d_evec.alloc((size_t)nk * nb * nb * 2));
-    const double dq = (double)0.001f;
+    const int nb = ctx->nb;
     const int G = nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
     if (G < 64) {   // 64/G matrices per wavefront
-        const int M = 64 / G, nmat = nk * kDq;
+        const int M = 64 / G;
+        const size_t nmat = (size_t)np * nvar;
         const size_t lds = (size_t)M * 2 * nb * nb * sizeof(zd);
         const void *kern = G == 8 ? (const void *)k_eig_group<8> : G == 16 ? (const void *)k_eig_group<16>
                                                                            : (const void *)k_eig_group<32>;
         if (lds + 4096 > 65536) ALT3_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         dim3 grid((unsigned)((nmat + M - 1) / M));
 #define ALT3_EIG_GROUP(GG)                                                                                     \
-        hipLaunchKernelGGL(k_eig_group<GG>, grid, dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,         \
-                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+        hipLaunchKernelGGL(k_eig_group<GG>, grid, dim3(64), lds, ctx->stream, np, nvar, nb, kq, ctx->d_fc2_rc.p, \
+                           ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec);
         if (G == 8) {
             ALT3_EIG_GROUP(8)
         } else if (G == 16) {
@@ -549,9 +544,26 @@ int eig_run(alt3_ctx *ctx)
         if (lds > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
         if (lds > 65536)   // cells with more than 44 branches
             ALT3_HIP(hipFuncSetAttribute((const void *)k_eig_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_eig_solve, dim3(nk * kDq), dim3(64), lds, ctx->stream, nk, nb, ctx->d_kpc_eig.p,
-                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, ev7.p, evec0.p);
+        hipLaunchKernelGGL(k_eig_solve, dim3((unsigned)((size_t)np * nvar)), dim3(64), lds, ctx->stream, np, nvar, nb,
+                           kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec);
     }
+    ALT3_HIP(hipGetLastError());
+    return ALT3_OK;
+}
+
+int eig_run(alt3_ctx *ctx)
+{
+    const int nk = ctx->nk, nb = ctx->nb;
+    if (nb > 64) return fail(ALT3_E_ARG, "more than 64 branches per cell");
+    DBuf<double> &ev7 = ctx->d_ev7, &evec0 = ctx->d_evec0;
+    ALT3_TRY(ev7.reserve((size_t)nk * kDq * nb));
+    ALT3_TRY(evec0.reserve((size_t)nk * nb * nb * 2));
+    ALT3_TRY(ctx->d_eival.alloc((size_t)nk * nb));
+    ALT3_TRY(ctx->d_freq.alloc((size_t)nk * nb));
+    ALT3_TRY(ctx->d_vel.alloc((size_t)nk * nb * 3));
+    ALT3_TRY(ctx->d_evec.alloc((size_t)nk * nb * nb * 2));
+    const double dq = (double)0.001f;
+    ALT3_TRY(eig_launch(ctx, nk, kDq, ctx->d_kpc_eig.p, dq, ev7.p, evec0.p));
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_eig_finish, dim3((nk + 63) / 64), dim3(64), 0, ctx->stream, nk, nb, dq, cnst(), ev7.p,
                        evec0.p, ctx->d_eival.p, ctx->d_evec.p, ctx->d_vel.p, ctx->d_freq.p);

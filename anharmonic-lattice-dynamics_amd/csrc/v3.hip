@@ -125,15 +125,18 @@ __global__ void k_v3_prep(int nb, int ibz_lo, int nslot, int ndr, const int *__r
                           const int *__restrict__ slot_dr, const int *__restrict__ slot_bc,
                           const int *__restrict__ slot_ptr, const int *__restrict__ sterm_a,
                           const double *__restrict__ sterm_psi, const double *__restrict__ sterm_rk,
+                          const double *__restrict__ q1pts, const double *__restrict__ e1pts,
                           double *__restrict__ gp)
 {
     const int s = blockIdx.y;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nslot * nb) return;
     const int slot = id / nb, l1 = id % nb;
-    const int k1 = mapinv[ibz_lo + s];
-    const double *q1 = kpc + 3 * k1;
-    const double *e1 = evec + 2 * ((size_t)k1 * nb + l1) * nb;
+    // q1 = the irreducible mesh point s, or (q1pts given) the point s of a list with its own
+    // eigenvectors e1pts [s][nb col][nb]
+    const int k1 = q1pts ? 0 : mapinv[ibz_lo + s];
+    const double *q1 = q1pts ? q1pts + 3 * s : kpc + 3 * k1;
+    const double *e1 = q1pts ? e1pts + 2 * ((size_t)s * nb + l1) * nb : evec + 2 * ((size_t)k1 * nb + l1) * nb;
     double gr = 0, gi = 0;
     for (int t = slot_ptr[slot]; t < slot_ptr[slot + 1]; t++) {
         double a = -(q1[0] * sterm_rk[3 * t] + q1[1] * sterm_rk[3 * t + 1] + q1[2] * sterm_rk[3 * t + 2]);
@@ -315,7 +318,23 @@ int v3_prep_launch(alt3_ctx *ctx, int ksub)
         hipLaunchKernelGGL(k_v3_prep, dim3((nth + 127) / 128, ksub), dim3(128), 0, ctx->stream, nb, ctx->ibz_lo,
                            ctx->V.nslot, ndr, ctx->d_mapinv.p, ctx->d_kpc.p, ctx->d_evec.p, ctx->d_slot_dr.p,
                            ctx->d_slot_bc.p, ctx->d_slot_ptr.p, ctx->d_sterm_a.p, ctx->d_sterm_psi.p,
-                           ctx->d_sterm_rk.p, ctx->d_gp.p);
+                           ctx->d_sterm_rk.p, nullptr, nullptr, ctx->d_gp.p);
+        ALT3_HIP(hipGetLastError());
+    }
+    return ALT3_OK;
+}
+
+// G' of nq arbitrary points q1 (Cartesian) with eigenvectors e1 [q][nb col][nb] into gp
+// [q][ndr][nb][nb*nb] complex
+int v3_prep_points(alt3_ctx *ctx, int nq, const double *q1, const double *e1, double *gp)
+{
+    const int nb = ctx->nb, ndr = ctx->V.ndr;
+    ALT3_HIP(hipMemsetAsync(gp, 0, sizeof(double) * 2 * (size_t)nq * ndr * nb * nb * nb, ctx->stream));
+    const int nth = ctx->V.nslot * nb;
+    if (nth > 0) {
+        hipLaunchKernelGGL(k_v3_prep, dim3((nth + 127) / 128, nq), dim3(128), 0, ctx->stream, nb, 0, ctx->V.nslot, ndr,
+                           ctx->d_mapinv.p, ctx->d_kpc.p, ctx->d_evec.p, ctx->d_slot_dr.p, ctx->d_slot_bc.p,
+                           ctx->d_slot_ptr.p, ctx->d_sterm_a.p, ctx->d_sterm_psi.p, ctx->d_sterm_rk.p, q1, e1, gp);
         ALT3_HIP(hipGetLastError());
     }
     return ALT3_OK;

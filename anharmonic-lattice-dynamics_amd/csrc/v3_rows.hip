@@ -33,12 +33,17 @@ __global__ void k_copy_real(size_t n, const double *__restrict__ in, T *__restri
     if (i < n) out[i] = (T)in[i];
 }
 
-template <typename T, int NB, int TQ>
+// EXT: q1 is point s of a list (G' of that point in gp, w1x [s][NB]) and e3 / w3 at
+// q3 = -q1-q2 come per (s, q2) from ev3x [s][q2][NB col][NB] and w3x [s][q2][NB] (the
+// self-energy's q3 is not a mesh point); otherwise q1 is the irreducible point s and e3, w3
+// are read at the mesh point -q1-q2.
+template <typename T, int NB, int TQ, bool EXT>
 __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, int ibz_lo, int ndr,
                                                      const int *__restrict__ mapinv, const T *__restrict__ eph,
                                                      const T *__restrict__ evec, const double *__restrict__ wq,
                                                      const T *__restrict__ gp, double c33sq8, T *__restrict__ out,
-                                                     int *__restrict__ err)
+                                                     int *__restrict__ err, const T *__restrict__ ev3x,
+                                                     const double *__restrict__ w3x, const double *__restrict__ w1x)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *E = reinterpret_cast<T *>(smem);     // [r][q] re, then im        (2 ndr TQ)
@@ -49,14 +54,13 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     constexpr int NT = TQ * NB, NN = NB * NB;
     const int tid = threadIdx.x, q = tid % TQ, j = tid / TQ;
     const int s = blockIdx.y, q0 = blockIdx.x * TQ;
-    const int k1 = mapinv[ibz_lo + s];
-    int a0, a1, a2;
-    m.coords(k1, a0, a1, a2);
+    const int k1 = EXT ? 0 : mapinv[ibz_lo + s];
     const int n2 = q0 + q;
     const bool valid = n2 < nk;
     const int n2c = valid ? n2 : nk - 1;
-    if (j == 0) {
-        int b0, b1, b2;
+    if (!EXT && j == 0) {
+        int a0, a1, a2, b0, b1, b2;
+        m.coords(k1, a0, a1, a2);
         m.coords(n2c, b0, b1, b2);
         k3s[q] = m.neg_sum(a0, a1, a2, b0, b1, b2);
     }
@@ -69,13 +73,20 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     __syncthreads();
     for (int x = tid; x < NN * TQ; x += NT) {      // x = (c*NB + l3)*TQ + qq
         const int cl = x / TQ, qq = x - cl * TQ, c = cl / NB, l3 = cl - c * NB;
-        const T *e = evec + 2 * (((size_t)k3s[qq] * NB + l3) * NB + c);
+        const T *e;
+        if constexpr (EXT)
+            e = ev3x + 2 * ((((size_t)s * nk + min(q0 + qq, nk - 1)) * NB + l3) * NB + c);
+        else
+            e = evec + 2 * (((size_t)k3s[qq] * NB + l3) * NB + c);
         E3[x] = e[0];
         E3[NN * TQ + x] = e[1];
     }
     for (int x = tid; x < NB * TQ; x += NT) {
         const int l3 = x / TQ, qq = x - l3 * TQ;
-        w3s[l3][qq] = wq[(size_t)k3s[qq] * NB + l3];
+        if constexpr (EXT)
+            w3s[l3][qq] = w3x[((size_t)s * nk + min(q0 + qq, nk - 1)) * NB + l3];
+        else
+            w3s[l3][qq] = wq[(size_t)k3s[qq] * NB + l3];
     }
     const double w2 = wq[(size_t)n2c * NB + j];
     const T *e2 = evec + 2 * ((size_t)n2c * NB + j) * NB;   // e(q2) column l2 = j
@@ -124,7 +135,7 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
                     vi[l3] += er * xi + ei * xr;
                 }
             }
-            const double w12 = wq[(size_t)k1 * NB + l1] * w2;
+            const double w12 = (EXT ? w1x[(size_t)s * NB + l1] : wq[(size_t)k1 * NB + l1]) * w2;
             T *o = out + ((((size_t)s * NB + l1) * NB + j) * NB) * (size_t)nk + n2;
 #pragma unroll
             for (int l3 = 0; l3 < NB; l3++) {
@@ -137,32 +148,33 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     }
 }
 
-template <typename T, int NB, int TQ>
+template <typename T, int NB, int TQ, bool EXT>
 int launch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const double *wq, const T *gp, T *out,
-                int *d_err)
+                int *d_err, const T *ev3x, const double *w3x, const double *w1x)
 {
     const int ndr = ctx->V.ndr;
     const size_t lds = (size_t)2 * (ndr + 2 * NB * NB) * TQ * sizeof(T);
     if (lds > ctx->lds_max) return alt3::fail(ALT3_E_ARG, "V3 tile does not fit in LDS");
     if (lds > 65536)
-        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_rows<T, NB, TQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds));
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_rows<T, NB, TQ, EXT>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const double c33 = alt3::const33();
     dim3 grid((unsigned)((ctx->nk + TQ - 1) / TQ), (unsigned)ksub);
-    hipLaunchKernelGGL((k_v3_rows<T, NB, TQ>), grid, dim3(TQ * NB), lds, ctx->stream, alt3::MeshDev::make(ctx->nc),
-                       ctx->nk, ctx->ibz_lo, ndr, ctx->d_mapinv.p, eph, evec, wq, gp, c33 * c33 / 8, out, d_err);
+    hipLaunchKernelGGL((k_v3_rows<T, NB, TQ, EXT>), grid, dim3(TQ * NB), lds, ctx->stream,
+                       alt3::MeshDev::make(ctx->nc), ctx->nk, ctx->ibz_lo, ndr, ctx->d_mapinv.p, eph, evec, wq, gp,
+                       c33 * c33 / 8, out, d_err, ev3x, w3x, w1x);
     ALT3_HIP(hipGetLastError());
     return ALT3_OK;
 }
 
-template <typename T>
+template <typename T, bool EXT = false>
 int dispatch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const double *wq, const T *gp, T *out,
-                  int *d_err)
+                  int *d_err, const T *ev3x = nullptr, const double *w3x = nullptr, const double *w1x = nullptr)
 {
     switch (ctx->nb) {
-        case 6: return launch_rows<T, 6, 32>(ctx, ksub, eph, evec, wq, gp, out, d_err);
-        case 12: return launch_rows<T, 12, 16>(ctx, ksub, eph, evec, wq, gp, out, d_err);
-        default: return launch_rows<T, 24, 4>(ctx, ksub, eph, evec, wq, gp, out, d_err);
+        case 6: return launch_rows<T, 6, 32, EXT>(ctx, ksub, eph, evec, wq, gp, out, d_err, ev3x, w3x, w1x);
+        case 12: return launch_rows<T, 12, 16, EXT>(ctx, ksub, eph, evec, wq, gp, out, d_err, ev3x, w3x, w1x);
+        default: return launch_rows<T, 24, 4, EXT>(ctx, ksub, eph, evec, wq, gp, out, d_err, ev3x, w3x, w1x);
     }
 }
 
@@ -205,6 +217,22 @@ int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
     }
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
+}
+
+// |V3|^2 [s][l1][l2][l3][q2] (FP64) for nq list points q1 with G' in gp, e3 / w3 per (s, q2)
+// and w1 per s (the self-energy); cells of 6, 12 or 24 branches
+int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, const double *w3, const double *w1,
+                   double *out, int *d_err)
+{
+    const int nk = ctx->nk, nb = ctx->nb;
+    if (!(ctx->V.ndr > 0 && (nb == 6 || nb == 12 || nb == 24)))
+        return fail(ALT3_E_ARG, "the self-energy kernel handles cells of 6, 12 or 24 branches");
+    DBuf<double> &wq = ctx->d_wq;
+    ALT3_TRY(wq.reserve((size_t)nk * nb));
+    hipLaunchKernelGGL(k_lam_quarter, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_eival.p,
+                       wq.p);
+    ALT3_HIP(hipGetLastError());
+    return dispatch_rows<double, true>(ctx, nq, ctx->d_eph.p, ctx->d_evec.p, wq.p, gp, out, d_err, ev3, w3, w1);
 }
 
 }  // namespace alt3

@@ -45,13 +45,15 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter"]
+           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
 ITER_MAX = 150
 UPDATE_MIX = float(np.float32(0.4))
 ITER_TOL = float(np.float32(1e-4))
+# read_params (rest.f90:47): v3_threshold = 1E-7, a REAL*4 literal
+V3_THRESHOLD = float(np.float32(1e-7))
 
 _lib = None
 
@@ -237,6 +239,17 @@ class Context:
         n = it.value
         return dict(kappa=k, hist=hist[:n * self.nb * 9].reshape((self.nb, 3, 3, n), order="F"), taueff=te[:n],
                     F=F, iters=n)
+
+    def self_w3(self, q, tempk: float, eta: float, v3_threshold: float = V3_THRESHOLD):
+        """Three-phonon self-energy at points q (nq,3) Cartesian: omega0 (nb,nq), nself and
+        uself (complex (nb,nq)); the linewidth of jobs 2/6/7 is 2 Im(nself + uself)."""
+        q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+        nq = len(q)
+        om = np.zeros((self.nb, nq), order="F")
+        ns = np.zeros((self.nb, nq), dtype=np.complex128, order="F"); us = np.zeros_like(ns)
+        _check(lib().alt3_self_w3(self.h, nq, _p(q), C.c_double(tempk), C.c_double(eta), C.c_double(v3_threshold),
+                                  _p(om), _p(ns), _p(us)))
+        return om, ns, us
 
     def collision(self, tempk: float, want: bool = True):
         n = self.nibz * self.nb * 3

@@ -171,6 +171,19 @@ int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FR
 int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix, double tol, double *kappa,
                     double *kappa_hist, double *taueff_hist, double *FFF, int *iters);
 
+/* Three-phonon self-energy of jobs 2/6/7 (kappa.f90:900-925) at nq points q off the mesh
+ * (Cartesian, the units of the mesh points, [nq][3]): for every branch la, function_self_w3
+ * (rest.f90:4012-4124) at omega = omega(q, la), with matrix_elt_full (:3664-3719) and get_freq
+ * (:4319-4445) at q and at -q-k for every mesh point k, at tempk (K) and Lorentzian width eta
+ * (etaz of latdyn.params, cm^-1).  The reference keeps |V3| >= v3_threshold (1E-7, a REAL*4
+ * literal: pass (double)1e-7f).  Needs the mesh eigen data (alt3_eig or alt3_set_eig); cells
+ * of 6, 12 or 24 branches.  omega0 (nb,nq) cm^-1; nself, uself the normal and umklapp parts,
+ * complex (nb,nq) as (re,im) pairs; any may be NULL.  The linewidth of jobs 2/6/7 is
+ * 2 Im(nself + uself).  Points q where -q-k is Gamma for a mesh k give a zero omega3 and the
+ * reference's den = 0 stop (ALT3_E_DEN0) or, with round-off, an ill-conditioned sum. */
+int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
+                 double *nself, double *uself);
+
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
  * IBZ and FGR (unless the context still holds them from an earlier call with the same
  * precision and delta integration; alt3_eig/alt3_set_eig clear them), then per T tau (one pass over P1sm/P2sm per group of four temperatures),

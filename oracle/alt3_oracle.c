@@ -1141,6 +1141,105 @@ int or_update(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const int *m
     return bad ? -1 : 0;
 }
 
+/* check_inside_bz (extratools.f90:325-341): 1 if q + (g1+g2+g3)/2 has reduced coordinates in
+ * [0,1) (get_direct_components, :561-592, epsl = 5e-10), i.e. a normal process */
+static int check_inside_bz(const double *q, const double *g)
+{
+    double rr[9], q2[3];
+    recip_nopi(g, rr);
+    for (int c = 0; c < 3; c++) q2[c] = q[c] + 0.5 * (g[c] + g[3 + c] + g[6 + c]);
+    int inside = 1;
+    for (int i = 0; i < 3; i++) {
+        double x = dot3(q2, rr + 3 * i) + 5e-10;
+        if (x < 0 || x >= 1) inside = 0;
+    }
+    return inside;
+}
+
+/* function_self_w3 (rest.f90:4012-4124) with matrix_elt_full (:3664-3719): the three-phonon
+ * self-energy of the mode (q, branch with eigenvector evq and frequency omq) at omega and at
+ * temp (both cm^-1), summed over the mesh k2 with k3 = -q-k2 unfolded.  Mesh: kp (nk,3),
+ * eigenval (nb,nk), eigenvec (nb,nb,nk) [component, branch, k]; at every k3 the get_freq data
+ * eival3 (nb,nk) and eivec3 (nb,nb,nk).  eta = etaz; v3thr = v3_threshold.  nself, uself:
+ * complex (re, im) of the normal and umklapp parts. */
+void or_self_w3(const double *q, int nb, double omq, const double *evq_ri, double omega, double temp,
+                int classical, double eta, double v3thr, int nk, const double *kp, const double *g,
+                const double *eigenval, const double *eigenvec_ri, const double *eival3, const double *eivec3_ri,
+                const double *mass0, const double *atompos, const int *iatomcell0, int nt3, const int *at3,
+                const int *xyz3, const double *fc3, double *nself, double *uself)
+{
+    const zc *evq = (const zc *)evq_ri, *ev = (const zc *)eigenvec_ri, *ev3a = (const zc *)eivec3_ri;
+    const double etacut = eta * 3000, c33 = or_const33();
+    const float cf = 521.1098918f;
+    const double cnst3 = (double)(cf * cf * cf);   /* cnst*cnst*cnst in REAL*4 (rest.f90:3713) */
+    zc ns = 0, us = 0;
+    for (int ik = 0; ik < nk; ik++) {
+        const double *k2 = kp + 3 * ik;
+        double k3[3] = {-q[0] - k2[0], -q[1] - k2[1], -q[2] - k2[2]};
+        const int inside = check_inside_bz(k3, g);
+        for (int l2 = 0; l2 < nb; l2++) {
+            const double om2 = sqrt(fabs(eigenval[l2 + (size_t)nb * ik])) * cnst_();
+            const double nb2 = or_nbe(om2, temp, classical);
+            for (int l3 = 0; l3 < nb; l3++) {
+                const double om3 = sqrt(fabs(eival3[l3 + (size_t)nb * ik])) * cnst_();
+                const double nb3 = or_nbe(om3, temp, classical);
+                double ires = 0, rres = 0, arg;
+                arg = omega - om3 - om2;
+                if (fabs(arg) < etacut) {
+                    ires = (nb3 + nb2 + 1) * (eta / (arg * arg + eta * eta) / PI);
+                    rres = (nb3 + nb2 + 1) * arg / (arg * arg + eta * eta);
+                }
+                arg = omega + om3 + om2;
+                if (fabs(arg) < etacut) {
+                    ires = ires - (nb3 + nb2 + 1) * (eta / (arg * arg + eta * eta) / PI);
+                    rres = rres - (nb3 + nb2 + 1) * arg / (arg * arg + eta * eta);
+                }
+                arg = omega - om3 + om2;
+                if (fabs(arg) < etacut) {
+                    ires = ires + (nb2 - nb3) * (eta / (arg * arg + eta * eta) / PI);
+                    rres = rres + (nb2 - nb3) * arg / (arg * arg + eta * eta);
+                }
+                arg = omega - om2 + om3;
+                if (fabs(arg) < etacut) {
+                    ires = ires + (nb3 - nb2) * (eta / (arg * arg + eta * eta) / PI);
+                    rres = rres + (nb3 - nb2) * arg / (arg * arg + eta * eta);
+                }
+                const zc self = rres + I * (ires * PI);
+                /* matrix_elt_full */
+                zc xx = 0;
+                for (int t = 0; t < nt3; t++) {
+                    int i3 = at3[3 * t], j1 = at3[3 * t + 1], k1 = at3[3 * t + 2];
+                    int i0 = iatomcell0[i3 - 1];
+                    if (i0 != i3) continue;
+                    int j0 = iatomcell0[j1 - 1], k0 = iatomcell0[k1 - 1];
+                    double mi = mass0[i0 - 1], mj = mass0[j0 - 1], mk = mass0[k0 - 1];
+                    int ta1 = xyz3[3 * t] - 1 + 3 * (i0 - 1), ta2 = xyz3[3 * t + 1] - 1 + 3 * (j0 - 1),
+                        ta3 = xyz3[3 * t + 2] - 1 + 3 * (k0 - 1);
+                    double ph = 0, ph3 = 0;
+                    for (int c = 0; c < 3; c++) {
+                        ph += k2[c] * (atompos[3 * (j1 - 1) + c] - atompos[3 * (j0 - 1) + c]);
+                        ph3 += k3[c] * (atompos[3 * (k1 - 1) + c] - atompos[3 * (k0 - 1) + c]);
+                    }
+                    zc eiqr = cexp(I * (ph + ph3));
+                    xx = xx + fc3[t] / sqrt(mi * mj * mk) * eiqr * evq[ta1] * ev[ta2 + (size_t)nb * (l2 + (size_t)nb * ik)] *
+                                  ev3a[ta3 + (size_t)nb * (l3 + (size_t)nb * ik)];
+                }
+                const double den = sqrt(8 * omq * om2 * om3 / cnst3);
+                if (den == 0) continue;   /* the reference stops here (MATRIX_ELT: den=0) */
+                const zc w33 = xx / den * c33;
+                if (cabs(w33) < v3thr) continue;
+                const zc term = w33 * conj(w33);
+                if (inside == 1) ns = ns + term * self;
+                else us = us + term * self;
+            }
+        }
+    }
+    ns = ns / nk * PI;
+    us = us / nk * PI;
+    nself[0] = creal(ns); nself[1] = cimag(ns);
+    uself[0] = creal(us); uself[1] = cimag(us);
+}
+
 /* cal_RHS_IBZ (sub.f90:2618-2691) */
 void or_rhs(int nibz, int nb, int nk, const int *mapinv, const double *freq, const double *dist,
             const double *veloc, double tempk, double *RHS)

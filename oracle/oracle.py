@@ -355,6 +355,39 @@ def bte_iter(kibz, mapinv, mapibz, kp, nc, g, primlatt, opk, eigenval, veloc, P1
     return F, np.stack(hist, axis=-1), np.array(tes)
 
 
+def self_w3(qs, model, kp, g, eigenval, eigenvec, fc3_terms, tempk, eta, v3thr=float(np.float32(1e-7)), classical=0):
+    """function_self_w3 at every branch of each q (jobs 2/6/7, kappa.f90:900-925): get_freq at q
+    and at -q-k2 for every mesh k2 (LAPACK here), omega = omega(q, la), T in K.  Returns
+    omega0 (nb, nq), nself and uself (complex (nb, nq)).  fc3_terms = (masses, atompos,
+    iatomcell0, at3, xyz3, fc3).  v3_threshold is 1E-7, a REAL*4 literal (rest.f90:47)."""
+    qs = np.atleast_2d(qs)
+    masses, atompos, iatomcell0, at3, xyz3, fc3 = fc3_terms
+    nb = eigenval.shape[0]; nk = len(kp)
+    temp = float(lib().or_temp_cm(C.c_double(tempk)))
+    wq, vq = get_freq_batch(qs, model)
+    om0 = np.zeros((nb, len(qs)), order="F")
+    ns = np.zeros((nb, len(qs)), dtype=np.complex128, order="F"); us = np.zeros_like(ns)
+    ev = np.asfortranarray(eigenvec, dtype=np.complex128)
+    for iq, q in enumerate(qs):
+        w3, v3 = get_freq_batch(-q[None, :] - kp, model)          # (nk, nb), (nk, nb, nb) [k, comp, branch]
+        e3v = np.asfortranarray(w3.T)
+        e3c = np.asfortranarray(np.transpose(v3, (1, 2, 0)))      # [comp, branch, k]
+        for la in range(nb):
+            omq = np.sqrt(abs(wq[iq, la])) * CNST
+            om0[la, iq] = omq
+            evq = np.ascontiguousarray(vq[iq, :, la])
+            n2 = np.zeros(2); u2 = np.zeros(2)
+            lib().or_self_w3(_p(np.ascontiguousarray(q, dtype=np.float64)), nb, C.c_double(omq), _p(evq), C.c_double(omq),
+                             C.c_double(temp), classical, C.c_double(eta), C.c_double(v3thr), nk,
+                             _p(np.ascontiguousarray(kp)), _p(np.ascontiguousarray(g)), _p(_f64(eigenval)), _p(ev),
+                             _p(e3v), _p(e3c), _p(_f64(masses)), _p(np.ascontiguousarray(atompos)), _p(_i32(iatomcell0)),
+                             len(fc3), _p(np.ascontiguousarray(at3, dtype=np.int32)),
+                             _p(np.ascontiguousarray(xyz3, dtype=np.int32)), _p(_f64(fc3)), _p(n2), _p(u2))
+            ns[la, iq] = n2[0] + 1j * n2[1]
+            us[la, iq] = u2[0] + 1j * u2[1]
+    return om0, ns, us
+
+
 def rhs(mapinv, freq, dist, veloc, tempk):
     nk, nb = freq.shape; nibz = len(mapinv)
     R = np.zeros(nibz * nb * 3)

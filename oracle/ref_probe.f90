@@ -9,6 +9,12 @@
 ! temperature-dependent part (kappa.f90:557-856) is repeated for each.  Without it the
 ! single temperature tmin is used, as kappa.f90:565 does.
 !
+! probe_selfq.txt (optional): nq, then nq points q (Cartesian, the units of kpc), one per
+! line: the three-phonon self-energy of jobs 2/6/7 (kappa.f90:900-925) at every branch of
+! each q, function_self_w3 (rest.f90:4012-4124) at omega = omega(q,la) from get_freq and
+! T = tmin, dumped as selfE_omega (ndyn,nq), selfE_n and selfE_u (complex (ndyn,nq)); with
+! REF_PROBE_SELF_ONLY set the probe stops there.
+!
 ! REF_PROBE_ITER=n (environment, optional): after kappa_RTA, also run the iterative BTE of
 ! job 3 (kappa.f90:1373-1487: update_mpi, F <- F + update_mix*(F2 - F), mode_kappa, stop when
 ! the kappa change norm is below 1e-4) for at most min(n, max_iter) iterations, dumping the
@@ -41,7 +47,11 @@ program ref_probe
  character(len=256) :: envs
  integer(8) :: c0, c1, crate
  logical :: ex
- integer :: maxit, iter_, niter
+ integer :: maxit, iter_, niter, nsq, la
+ real(8), allocatable :: selfq(:,:), seom(:,:), eivq(:)
+ complex(8), allocatable :: sen(:,:), seu(:,:), evq(:,:)
+ complex(8) :: nself, uself
+ real(8) :: tcm
  real(8) :: normk
  real(8), allocatable :: kiter(:,:,:,:), teiter(:)
 
@@ -104,6 +114,32 @@ program ref_probe
     enddo
  enddo
  call dumpi('star_n', starn, nibz)
+ ! ---------------- self-energy at given q (jobs 2/6/7) ----------------
+ inquire(file='probe_selfq.txt', exist=ex)
+ if (ex) then
+    open(78, file='probe_selfq.txt', status='old')
+    read(78,*) nsq
+    allocate(selfq(3,nsq), seom(ndyn,nsq), sen(ndyn,nsq), seu(ndyn,nsq), eivq(ndyn), evq(ndyn,ndyn))
+    do i = 1, nsq
+       read(78,*) selfq(:,i)
+    enddo
+    close(78)
+    tcm = tmin*k_b/(100*h_plank*c_light)
+    do i = 1, nsq
+       call get_freq(selfq(:,i), ndyn, eivq, evq)
+       do la = 1, ndyn
+          seom(la,i) = sqrt(abs(eivq(la)))*cnst
+          call function_self_w3(selfq(:,i), la, seom(la,i), tcm, nself, uself)
+          sen(la,i) = nself
+          seu(la,i) = uself
+       enddo
+    enddo
+    call dumpr('selfE_omega', seom, ndyn*nsq)
+    call dumpc('selfE_n', sen, ndyn*nsq)
+    call dumpc('selfE_u', seu, ndyn*nsq)
+    call get_environment_variable('REF_PROBE_SELF_ONLY', envs, status=ios_)
+    if (ios_ == 0) stop
+ endif
  call dumpi('star_op', starop, 48*nibz)
  call dumpi('star_n0', starn0, 48*nibz)
 

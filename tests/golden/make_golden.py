@@ -7,6 +7,7 @@ dumps are stored as compressed .npz next to this script.
 
     python tests/golden/make_golden.py            # si4 + si8
     python tests/golden/make_golden.py iter       # si4_iter: the iterative BTE of job 3
+    python tests/golden/make_golden.py self       # si4_self, si8_self: self-energy at given q
 """
 from __future__ import annotations
 
@@ -78,9 +79,48 @@ def main_iter():
         print("wrote si4_iter")
 
 
+def self_points(r123):
+    """q points for the self-energy fixtures, in reduced units of the reciprocal vectors:
+    generic points, points on two symmetry lines (degenerate branches) and points near the
+    zone boundary.  None is a point of the 4^3 or 8^3 mesh: there -q-k2 hits Gamma for some
+    mesh k2, omega3 is zero up to round-off, and matrix_elt_full divides by it (the reference
+    then returns round-off-dominated values)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    from oracle import reciprocal
+    g, _ = reciprocal(r123)
+    red = np.array([[0.13, 0.27, -0.31], [0.1, 0.1, 0.0], [0.2, 0.05, 0.15], [0.45, 0.45, 0.0],
+                    [0.48, 0.02, 0.01], [-0.37, 0.21, 0.05], [0.3, 0.3, 0.3]])
+    return red @ g
+
+
+def main_self():
+    """Self-energy at the points of self_points (function_self_w3 with get_freq, T = tmin =
+    300 K, etaz of latdyn.params) on the 4^3 and 8^3 SW-Si meshes."""
+    for name, n in (("si4", 4), ("si8", 8)):
+        with tempfile.TemporaryDirectory() as wd:
+            fc = models.sw_silicon()
+            refio.write_inputs(wd, fc, refio.RunParams(nc=(n, n, n), tmin=300.0, tmax=300.0))
+            inp = refio.read_inputs(wd)
+            q = self_points(inp.r123)
+            with open(os.path.join(wd, "probe_selfq.txt"), "w") as f:
+                f.write(f"{len(q)}\n" + "".join(f"{float(a)!r} {float(b)!r} {float(c)!r}\n" for a, b, c in q))
+            env = dict(os.environ, REF_PROBE_SELF_ONLY="1")
+            subprocess.run([PROBE], cwd=wd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env)
+            d = os.path.join(wd, "dump")
+            om = np.fromfile(os.path.join(d, "selfE_omega.bin")).reshape((-1, len(q)), order="F")
+            nb = om.shape[0]
+            sn = np.fromfile(os.path.join(d, "selfE_n.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
+            su = np.fromfile(os.path.join(d, "selfE_u.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
+            np.savez_compressed(os.path.join(HERE, f"ref_{name}_self.npz"), q=q, omega=om, nself=sn, uself=su,
+                                tempk=np.array(300.0), etaz=np.array(refio.RunParams().etaz))
+            print("wrote", name + "_self")
+
+
 def main():
     if sys.argv[1:] == ["iter"]:
         return main_iter()
+    if sys.argv[1:] == ["self"]:
+        return main_self()
     cases = [("si4", 4, None, None), ("si8", 8, [0, 1, 7, 23, 49], None)]
     for name, n, sub, rows in cases:
         with tempfile.TemporaryDirectory() as wd:

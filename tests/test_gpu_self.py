@@ -1,0 +1,62 @@
+"""GPU: the three-phonon self-energy of jobs 2/6/7 (alt3_self_w3) against the reference build
+(tests/golden/ref_si{4,8}_self.npz: function_self_w3 with get_freq at points off the mesh,
+T = 300 K, etaz of latdyn.params) with the reference's mesh eigen data loaded.
+
+Tolerances: omega(q) 1e-7 (Jacobi vs ZHEEV); nself + uself summed over branches whose
+frequencies agree to 1e-3 (per-branch values inside such a group depend on the eigenvector
+gauge, the group sum does not) 2e-5, the spread the oracle shows against the reference with
+LAPACK eigenvectors at the same points; normal and umklapp parts separately 2e-5 of the total.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_fixture
+from thermacond import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def _groups(w, tol=1e-3):
+    out, l = [], 0
+    while l < len(w):
+        u = l + 1
+        while u < len(w) and abs(w[u] - w[l]) < tol * w.max():
+            u += 1
+        out.append((l, u))
+        l = u
+    return out
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_self_w3_vs_reference(si_inputs, n):
+    print(f"case: alt3_self_w3 SW-Si {n}^3 vs reference", flush=True)
+    R = load_fixture(f"si{n}")
+    S = load_fixture(f"si{n}_self")
+    with capi.Context(si_inputs[n]) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        om, ns, us = c.self_w3(S["q"], float(S["tempk"]), float(S["etaz"]))
+    assert np.max(np.abs(om - S["omega"])) / np.max(S["omega"]) < 1e-7
+    tot, rtot = ns + us, S["nself"] + S["uself"]
+    assert np.all(tot.imag > 0)
+    for i in range(len(S["q"])):
+        for a, b in _groups(S["omega"][:, i]):
+            y = rtot[a:b, i].sum()
+            assert abs(tot[a:b, i].sum() - y) / abs(y) < 2e-5, (i, a, b, tot[a:b, i].sum(), y)
+            yn = S["nself"][a:b, i].sum()
+            assert abs(ns[a:b, i].sum() - yn) / abs(y) < 2e-5, (i, a, b, "normal part")
+
+
+def test_self_w3_batches_agree(si_inputs):
+    """One call over many points equals one call per point (the batch split of the kernel)."""
+    print("case: alt3_self_w3 batches 4^3", flush=True)
+    S = load_fixture("si4_self")
+    rng = np.random.default_rng(5)
+    q = np.concatenate([S["q"], S["q"][:, :] * 0.93 + rng.uniform(-0.05, 0.05, S["q"].shape)] * 10)
+    with capi.Context(si_inputs[4]) as c:
+        c.eig(want=False)
+        om, ns, us = c.self_w3(q, 300.0, float(S["etaz"]))
+        for i in (0, 9, 77, len(q) - 1):
+            o1, n1, u1 = c.self_w3(q[i], 300.0, float(S["etaz"]))
+            assert np.allclose(o1[:, 0], om[:, i], rtol=1e-14, atol=0)
+            assert np.allclose(n1[:, 0], ns[:, i], rtol=1e-12, atol=0)
+            assert np.allclose(u1[:, 0], us[:, i], rtol=1e-12, atol=0)

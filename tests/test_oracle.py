@@ -304,3 +304,39 @@ def test_update_fixed_point_is_the_bte_solution(geo4, ref4):
     # fixed point: F_RTA := F - A(F)/Q  =>  update(F) = F
     Frta = np.asfortranarray(Fa - Aa)
     assert rel(O.update(*args, Fa, Frta, Q), Fa) < 1e-12
+
+
+# ------------------------------------------------------------------ self-energy (jobs 2/6/7)
+def _groups(w, tol=1e-3):
+    """Branches with frequencies within tol of the largest: per-branch values inside such a
+    group depend on the eigenvector gauge, their sum does not."""
+    out, l = [], 0
+    while l < len(w):
+        u = l + 1
+        while u < len(w) and abs(w[u] - w[l]) < tol * w.max():
+            u += 1
+        out.append((l, u))
+        l = u
+    return out
+
+
+def test_self_w3_vs_reference(geo4, ref4):
+    """function_self_w3 with matrix_elt_full and get_freq against the reference build at points
+    off the mesh (tests/golden/ref_si4_self.npz): omega(q) to 1e-7 (ZHEEV vs LAPACK), the
+    normal + umklapp self-energy summed over (near-)degenerate branches to 2e-5, Im > 0."""
+    from conftest import load_fixture
+    S = load_fixture("si4_self")
+    G = geo4
+    inp = G["inp"]
+    model = (inp.masses, inp.atompos, inp.iatomcell0, inp.fc2_atoms, inp.fc2_xyz, inp.fc2)
+    fc3 = (inp.masses, inp.atompos, inp.iatomcell0, inp.fc3_atoms, inp.fc3_xyz, inp.fc3)
+    sel = [0, 1, 4, 6]
+    om, ns, us = O.self_w3(S["q"][sel], model, G["kp"], G["g"], ref4["eigenval"], ref4["eigenvec"], fc3,
+                           float(S["tempk"]), float(S["etaz"]))
+    assert rel(om, S["omega"][:, sel]) < 1e-7
+    tot, rtot = ns + us, (S["nself"] + S["uself"])[:, sel]
+    assert np.all(rtot.imag > 0) and np.all(tot.imag > 0)
+    for i in range(len(sel)):
+        for a, b in _groups(S["omega"][:, sel[i]]):
+            y = rtot[a:b, i].sum()
+            assert abs(tot[a:b, i].sum() - y) / abs(y) < 2e-5, (sel[i], a, b)
