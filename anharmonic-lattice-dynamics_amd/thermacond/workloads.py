@@ -130,6 +130,50 @@ def run_sweep(count: int = 64, seed: int = 0, slots: int = 4, log=print) -> dict
             "generation_s": t_gen}
 
 
+# ---------------------------------------------------------------------------- band-path self-energy
+# The path of the reference's Ge example (THERMACOND/Ge-example/selfenergy/kpbs.params): 9
+# directions of 40 points in primitive reciprocal units, L-G-K-X-W-L-K-W-X-G, 360 points.
+FCC_PATH = """1
+40
+9
+L 0.5 0.5 0.5
+G 0. 0. 0
+K 0.375 0.375 0.75
+X 0.5 0.5 1
+W 0.5 0.25 0.75
+L 0.5 0.5 0.5
+K 0.375 0.375 0.75
+W 0.5 0.25 0.75
+X 0.5 0.0  0.5
+G 0. 0 0
+"""
+
+
+def run_selfenergy(n: int = 30, tempk: float = 300.0, eta: float = 30.0, log=print) -> dict:
+    """Job 2 on SW-Si (diamond, 6 branches) at the shape of the reference's Ge example: an n^3
+    mesh and the 360-point path above, every branch at its own frequency, T = 300 K, etaz = 30."""
+    from . import selfenergy
+    inp = cell_inputs("sw-si", n)
+    lp = np.array(inp.latparams, dtype=float)
+    lp[:3] *= inp.lattice_parameter
+    kp, dk, _ = selfenergy.make_kp_bs(lp, np.asarray(inp.primlatt).reshape(3, 3, order="F"), FCC_PATH)
+    with capi.Context(inp) as c:
+        t0 = time.perf_counter()
+        c.eig(want=False)
+        t_eig = time.perf_counter() - t0
+        c.self_w3(kp[:2], tempk, eta)   # warm-up (kernel loads)
+        t0 = time.perf_counter()
+        om, ns, us = c.self_w3(kp, tempk, eta)
+        dt = time.perf_counter() - t0
+        nk, nb = c.nk, c.nb
+    nel = len(kp) * nk * nb ** 3
+    lw = 2 * np.imag(ns + us)
+    log(f"self-energy {n}^3, {len(kp)} points: {dt:.3f} s ({nel / dt:.3e} V3 elements/s)")
+    return {"points": len(kp), "nk": nk, "nb": nb, "seconds": dt, "mesh_eig_s": t_eig,
+            "points_per_s": len(kp) / dt, "v3_elements": nel, "v3_elements_per_s": nel / dt,
+            "eigensolves": len(kp) * (nk + 1), "linewidth_cm1": {"min": float(lw.min()), "max": float(lw.max())}}
+
+
 # ---------------------------------------------------------------------------- serving stream
 def serving_requests(count: int, seed: int, precision: int) -> List[serving.Request]:
     """Cells uniform over CELLS, mesh an even size from 6^3 up to the cell's cap (4^3 where

@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-iter-bte", action="store_true", help="skip the iterative BTE (job 3) line")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving"], default="step",
+    ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving", "selfenergy"], default="step",
                     help="step: the SW-Si 16^3 headline step (default); rocksalt: the 8-atom rock-salt-type "
                          "cell at 12^3; sweep: 64 SW-Si instances through the server; serving: a Poisson "
                          "stream of 512 kappa requests over four cells")
@@ -158,6 +158,18 @@ def run_workload(args):
                "data": "synthetic (SW-Si, a ~ U[5.4,5.9] A, seed 0)",
                "config": {"workload": "64 SW-Si instances, meshes {8,10,12,14}^3, T = 100,200,300,500,800 K, "
                                       "RTA + CG, 4 server slots"}, **out}
+    elif args.workload == "selfenergy":
+        out = workloads.run_selfenergy(30, log=say)
+        ref_s = 97889.0   # Ge-example/selfenergy/times.dat: self-energy stage of the same shape
+        res = {"metric": "band-path self-energy: (q, branch) evaluations/s (job 2)", "unit": "evaluations/s",
+               "value": out["points"] * out["nb"] / out["seconds"], "higher_is_better": True, "n_gpus": 1,
+               "dtype": "f64", "data": "synthetic (SW-Si force constants; the reference's Ge path and mesh shape)",
+               "config": {"workload": "SW-Si diamond, 30^3 mesh, 360-point L-G-K-X-W-L-K-W-X-G path, 6 branches, "
+                                      "T = 300 K, etaz = 30 cm^-1"},
+               "reference_published": {"seconds": ref_s, "what": "Ge (diamond, 6 branches), 30^3 mesh, the same "
+                                       "360-point path, job 2 on CPU (Ge-example/selfenergy/times.dat); other "
+                                       "force constants, so a shape comparison only"},
+               **out}
     else:
         runs = {}
         for name, p in (("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)):

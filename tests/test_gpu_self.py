@@ -60,3 +60,25 @@ def test_self_w3_batches_agree(si_inputs):
             assert np.allclose(o1[:, 0], om[:, i], rtol=1e-14, atol=0)
             assert np.allclose(n1[:, 0], ns[:, i], rtol=1e-12, atol=0)
             assert np.allclose(u1[:, 0], us[:, i], rtol=1e-12, atol=0)
+
+
+def test_job2_band_path_files(si_inputs, tmp_path):
+    """run_job2: kpbs.params -> path (make_kp_bs) -> lw-selfenergy.dat / lw-selfenergy2.dat with
+    the linewidth 2 Im(nself + uself) of alt3_self_w3 at every (point, branch)."""
+    print("case: job 2 driver, SW-Si 4^3", flush=True)
+    from thermacond import refio, selfenergy
+    refio.write_inputs(str(tmp_path), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4)))
+    (tmp_path / "kpbs.params").write_text("1\n5\n2\nA 0.11 0.13 0.17\nB 0.41 0.07 0.33\nC 0.2 0.3 0.45\n")
+    r = selfenergy.run_job2(str(tmp_path))
+    rows = np.loadtxt(tmp_path / "lw-selfenergy.dat")
+    nb = r["omega"].shape[0]
+    assert rows.shape == (10 * nb, 12)
+    lw = 2 * np.imag(r["nself"] + r["uself"])
+    assert np.allclose(rows[:, 9], lw.T.ravel(), rtol=1e-12, atol=0)
+    assert np.allclose(rows[:, 7], r["omega"].T.ravel(), rtol=1e-12, atol=0)
+    rows2 = np.loadtxt(tmp_path / "lw-selfenergy2.dat")
+    assert rows2.shape == (10, 2 + 2 * nb)
+    with capi.Context(si_inputs[4]) as c:
+        c.eig(want=False)
+        om, ns, us = c.self_w3(r["kp"][3], 300.0, 30.0)
+    assert np.allclose(2 * np.imag(ns + us)[:, 0], lw[:, 3], rtol=1e-12, atol=0)

@@ -96,3 +96,20 @@ def test_fc_file_count_mismatch_raises(tmp_path, si_inputs):
     p.write_text("\n".join(lines[:-1]) + "\n")
     with pytest.raises(ValueError, match="READ_FC3"):
         refio.read_inputs(str(tmp_path))
+
+
+def test_make_kp_bs_vs_reference_ge_example():
+    """make_kp_bs (kpoints.f90:37-173) on the reference's Ge example: the 360 path points equal
+    those of its bs_freqs.dat (written to 3 decimals).  That file's path coordinate is |q| per
+    segment, while the source divides by |g01| (kpoints.f90:133): the two differ by exactly that
+    factor, which is checked too."""
+    from thermacond import selfenergy as se
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ge_bandpath.npz"),
+                 allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    kp, dk, names = se.make_kp_bs(d["latparams"], d["primlatt"], str(d["kpbs"]))
+    assert kp.shape == d["kp"].shape
+    assert np.abs(kp - d["kp"]).max() < 6e-4
+    g01 = np.linalg.norm(se.reciprocal_2pi(se.conventional_cell(d["latparams"]) @ d["primlatt"])[0])
+    assert np.abs(dk * g01 - d["dk"]).max() < 2e-3
+    assert names[0] == "L" and names[-1] == "G"
