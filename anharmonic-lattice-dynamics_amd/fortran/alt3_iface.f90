@@ -108,6 +108,14 @@ module alt3_iface
        integer(c_int), value :: max_iter
      end function alt3_kappa_iter
 
+     integer(c_int) function alt3_self_w3(ctx, nq, q, tempk, eta, v3_threshold, omega0, nself, uself) &
+          bind(C, name='alt3_self_w3')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q, omega0, nself, uself
+       integer(c_int), value :: nq
+       real(c_double), value :: tempk, eta, v3_threshold
+     end function alt3_self_w3
+
      integer(c_int) function alt3_update(ctx, tempk, fff, frta, f2) bind(C, name='alt3_update')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx, fff, frta, f2
