@@ -228,12 +228,22 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     r /= nb;
     const int l2 = r % nb;
     const int s = r / nb;
+    // Mixed path: the decay energies of (l2, l3) at q2 are those of (l3, l2) at q1-q2, and the
+    // mesh map q2 -> q1-q2 carries the tetrahedra of make_kp_reg_tet onto themselves (it
+    // inverts each cube through its centre, which keeps the diagonal all six share), so the
+    // decay group with l2 < l3 also writes the (l3, l2) rows from W(q1-q2) and the l2 > l3
+    // groups have nothing to do.  (Tied corner energies may then assign the tie's weights to
+    // the other corner, which the FP64 path, compared with the reference at 1e-8, avoids.)
+    constexpr bool kSym = sizeof(VT) == sizeof(float);
+    if (kSym && proc == 1 && l2 > l3) return;   // whole group
+    const bool mirror = kSym && proc == 1 && l2 < l3;
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
     m.coords(k1, a0, a1, a2);
     const int tid = threadIdx.x, gam = m.gamma();
     float lo = 3.4e38f, hi = -3.4e38f;
     unsigned vzero = 0;  // bit j: acoustic branch at Gamma in q2 or q3
+    unsigned vzerom = 0; // the same for the mirrored (l3, l2) row
 #pragma unroll
     for (int j = 0; j < kFgrPer; j++) {
         const int w = tid + j * kFgrThreads;
@@ -250,6 +260,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
                 e = freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
             }
             if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) vzero |= 1u << j;
+            if ((l3 < 3 && w == gam) || (l2 < 3 && p == gam)) vzerom |= 1u << j;
             const float ef = (float)e;
             en[w] = ef;
             W[w] = 0.0;
@@ -357,11 +368,12 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
         const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
         // |V3|^2 of this l1 (at q2 = q_w for coalescence, q2 = -q_w for decay), requested
         // before the weight phase so that the load latency overlaps it
-        VT fv[kFgrPer];
+        const size_t basem = ((((size_t)s * nb + l1) * nb + l3) * nb + l2) * (size_t)nk;   // mirrored row
+        VT fv[kFgrPer], fvm[kFgrPer];
 #pragma unroll
         for (int j = 0; j < kFgrPer; j++) {
             const int w = tid + j * kFgrThreads;
-            fv[j] = 0;
+            fv[j] = fvm[j] = 0;
             if (w < nk) {
                 int vo = w;
                 if (proc) {
@@ -370,6 +382,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
                     vo = m.neg(c0, c1, c2);
                 }
                 fv[j] = v33[base + vo];
+                if (mirror) fvm[j] = v33[basem + vo];
             }
         }
         if (omega_in(omega1, lo, hi)) {
@@ -398,16 +411,30 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             }
             __syncthreads();
         }
-        // P(q_w) = W(q_w) F(q_w); each thread then clears the weight sums it read
+        // P(q_w) = W(q_w) F(q_w) (and the mirrored row W(q1-q_w) F'(q_w)); the weight sums are
+        // cleared once every thread has read them
 #pragma unroll
         for (int j = 0; j < kFgrPer; j++) {
             const int w = tid + j * kFgrThreads;
             if (w < nk) {
                 double F = (double)fv[j] * tpi2 / nk;
-                                if (vzero >> j & 1u) F = 0.0;
+                if (vzero >> j & 1u) F = 0.0;
                 P[base + w] = (VT)(W[w] * F);
-                W[w] = 0.0;
+                if (mirror) {
+                    int c0, c1, c2;
+                    m.coords(w, c0, c1, c2);
+                    const int p = m.diff(a0, a1, a2, c0, c1, c2);
+                    double Fm = (double)fvm[j] * tpi2 / nk;
+                    if (vzerom >> j & 1u) Fm = 0.0;
+                    P[basem + w] = (VT)(W[p] * Fm);
+                }
             }
+        }
+        if (mirror) __syncthreads();   // uniform over the group
+#pragma unroll
+        for (int j = 0; j < kFgrPer; j++) {
+            const int w = tid + j * kFgrThreads;
+            if (w < nk) W[w] = 0.0;
         }
         __syncthreads();
     }

@@ -274,3 +274,30 @@ def test_gauss_rta_kappa_8(si_inputs, ref8):
     kg = np.trace(out["kappa_rta"][0].sum(0)) / 3
     kt = np.trace(ref8["T300_kappaRTA"].sum(0)) / 3
     assert 0.75 < kg / kt < 1.25, (kg, kt)
+
+
+def test_mixed_fgr_decay_mirror_vs_fp64(si_inputs):
+    """The mixed path builds the decay rows of (l2 > l3) from those of (l3, l2) at q1 - q2.
+    Against the FP64 kernel, which computes every row itself: P1sm and P2sm to 1e-4 of the
+    largest element, and the sums over q2 of every (q1, l1, l2, l3) row of P2sm, the quantity
+    the scattering rates integrate, to 1e-4 of the largest sum.  (Point by point a row can
+    differ more: on a mirrored row the corners of a tetrahedron come in another order, and
+    where two have the same REAL*4 energy weight_tet gives the tie's weights to the other
+    corner; and the FP32 weights of a nearly flat tetrahedron lose digits.  Neither moves the
+    row sums.)"""
+    print("case: mixed FGR mirror vs fp64 8^3", flush=True)
+    with capi.Context(si_inputs[8]) as c:
+        c.eig(want=False)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        P1d, P2d = c.fgr()
+        c.v3sq(0, c.nibz - 1, capi.V3_MIXED, want=False)
+        P1m, P2m = c.fgr()
+    check(P1m, P1d, 1e-4, "mixed P1sm")
+    check(P2m, P2d, 1e-4, "mixed P2sm")
+    sm, sd = P2m.sum(axis=1), P2d.sum(axis=1)            # (q1, l1, l2, l3)
+    check(sm, sd, 1e-4, "P2sm row sums")
+    nb = P2d.shape[2]
+    for l2 in range(nb):
+        for l3 in range(l2):                            # the mirrored rows on their own
+            r = float(np.max(np.abs(sm[:, :, l2, l3] - sd[:, :, l2, l3])) / np.max(np.abs(sd)))
+            assert r < 1e-4, (l2, l3, r)
