@@ -414,6 +414,7 @@ __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, cons
     for (int x = threadIdx.x; x < nibz; x += blockDim.x) {
         const int n2 = slot_n2[x], cnt = slot_cnt[x];
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4   // independent table loads in flight; the sums keep member order
         for (int ns = 0; ns < cnt; ns++) {
             const int e = tab[(size_t)ns * nibz + x];
             const double c = C[e >> 6];

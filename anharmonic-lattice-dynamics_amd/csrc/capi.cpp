@@ -33,6 +33,7 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
               double *funcmin, int *iters, bool rowmajor);
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
+int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev);
 int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, double *omega0,
              double *nself, double *uself);
 int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
@@ -302,6 +303,30 @@ int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm)
     ALT3_TRY(fgr_run(ctx));
     tm.stop();
     ALT3_TRY(copy_p(ctx, P1sm, P2sm));
+    return ALT3_OK;
+}
+
+int alt3_set_fgr(alt3_ctx *ctx, int ibz_lo, int ibz_hi, const double *P1sm, const double *P2sm)
+{
+    if (!ctx || !P1sm || !P2sm) return fail(ALT3_E_ARG, "null argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    if (!ctx->have_eig) return fail(ALT3_E_STATE, "eigen data missing: call alt3_eig or alt3_set_eig first");
+    if (ibz_lo < 0 || ibz_hi >= ctx->nibz || ibz_lo > ibz_hi) return fail(ALT3_E_ARG, "bad ibz range");
+    const int ksub = ibz_hi - ibz_lo + 1;
+    const size_t nel = (size_t)ksub * ctx->nb * ctx->nb * ctx->nb * ctx->nk;
+    ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
+    ctx->ibz_lo = ibz_lo;
+    ctx->ibz_hi = ibz_hi;
+    ctx->v3_prec = ctx->v3_req = ALT3_V3_FP64;   // P held in FP64; no |V3|^2 in the context
+    ctx->p_fp32 = false;
+    ctx->d_p1f.release();
+    ctx->d_p2f.release();
+    ALT3_TRY(ctx->d_p1.alloc(nel));
+    ALT3_TRY(ctx->d_p2.alloc(nel));
+    ALT3_TRY(load_fortran5(ctx, ksub, P1sm, ctx->d_p1.p));
+    ALT3_TRY(load_fortran5(ctx, ksub, P2sm, ctx->d_p2.p));
+    ctx->have_fgr = true;
+    ctx->fgr_kind = ALT3_DELTA_TETRA;
     return ALT3_OK;
 }
 

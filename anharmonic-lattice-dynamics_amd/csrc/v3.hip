@@ -260,6 +260,21 @@ __global__ void k_to_fortran5(int ksub, int nk, int nb, const T *__restrict__ in
     out[o] = (double)in[((((size_t)s * nb + l1) * nb + l2) * nb + l3) * nk + n2];
 }
 
+// Fortran (ksub,nk,nb,nb,nb) -> [s][l1][l2][l3][n2] (the inverse of k_to_fortran5)
+__global__ void k_from_fortran5(int ksub, int nk, int nb, const double *__restrict__ in, double *__restrict__ out)
+{
+    size_t tot = (size_t)ksub * nk * nb * nb * nb;
+    size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= tot) return;
+    size_t x = o;
+    int s = x % ksub; x /= ksub;
+    int n2 = x % nk; x /= nk;
+    int l1 = x % nb; x /= nb;
+    int l2 = x % nb; x /= nb;
+    int l3 = (int)x;
+    out[((((size_t)s * nb + l1) * nb + l2) * nb + l3) * nk + n2] = in[o];
+}
+
 }  // namespace
 
 namespace alt3 {
@@ -457,6 +472,21 @@ int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host)
 }
 
 template int copy_fortran5<double>(alt3_ctx *, const double *, double *);
+
+// host Fortran (ksub,nk,nb,nb,nb) into a device array of the kernels' layout
+int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev)
+{
+    const int nk = ctx->nk, nb = ctx->nb;
+    size_t tot = (size_t)ksub * nk * nb * nb * nb;
+    DBuf<double> tmp;
+    ALT3_TRY(tmp.alloc(tot));
+    ALT3_HIP(hipMemcpyAsync(tmp.p, host, tot * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_from_fortran5, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, ksub, nk, nb,
+                       tmp.p, dev);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    return ALT3_OK;
+}
 template int copy_fortran5<float>(alt3_ctx *, const float *, double *);
 
 }  // namespace alt3

@@ -7,7 +7,10 @@
 !    tau_RTA.dat, kappa_RTA.dat                 (RTA,      kappa.f90:562-678, :1190-1304)
 !    job 1: tau_RTAandeff-cg.dat, kappa_direct-cg.dat   (CG / BTE, kappa.f90:703-851)
 !    job 3: tau_RTAandeff_iter.dat, kappa_iter.dat       (iterative BTE, kappa.f90:1373-1487)
-!  The job is the one latdyn.params names (3, else 1); the temperature is tmin (kappa.f90:565).
+!    job 4: CM-1D.dat, RHS_IBZ.dat, DS.dat (the CG system, kappa.f90:2577-2625) after the RTA
+!    job 5: reads those three, CG and kappa_BTE as job 1 (kappa.f90:2635-2703)
+!  The job is the one latdyn.params names (3, 4, 5, else 1); the temperature is tmin
+!  (kappa.f90:565).
 !
 !  usage: alt3_kappa [fp64 | mixed | fp64ref] [--host-only]
 !    --host-only builds the mesh and symmetry tables only (no GPU) and prints their sizes.
@@ -47,7 +50,7 @@ program alt3_kappa
   integer(c_int), allocatable, target :: mapinv(:)
   real(c_double), allocatable, target :: q(:, :), qn(:, :), qu(:, :), taun(:, :), tauu(:, :), tautot(:, :)
   real(c_double), allocatable, target :: frta(:, :, :), taueff(:, :), x(:)
-  real(c_double), allocatable, target :: kappa(:), khist(:), tehist(:)
+  real(c_double), allocatable, target :: kappa(:), khist(:), tehist(:), cm(:, :), rhs(:)
   real(c_double), target :: fmin
   real(c_double) :: tempk
   character(len=32) :: arg
@@ -114,11 +117,34 @@ program alt3_kappa
   call alt3_check(alt3_eig(ctx, c_loc(eigenval), c_null_ptr, c_loc(veloc)), 'alt3_eig')
   call alt3_check(alt3_get_mesh(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(mapinv), c_loc(kibz), &
        c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'alt3_get_mesh')
+  tempk = tmin                                     ! kappa.f90:565, :2649
+
+  if (job == 5) then
+     ! read_CM_1D_mpi, read_RHS, read_DS (sub.f90:5401-5470), then cg_quadratic and
+     ! mode_kappa_d; the scattering rates of the tau file are not known here, as in the reference
+     allocate(cm(n, n), rhs(n))
+     call read_rows('CM-1D.dat', n, cm)
+     call read_column('RHS_IBZ.dat', n, rhs)
+     call read_column('DS.dat', n, x)
+     call alt3_check(alt3_cg(ctx, n, c_loc(cm), c_loc(rhs), c_loc(x), 100, 2, c_loc(fmin), c_loc(iters)), 'alt3_cg')
+     taueff = 0
+     taun = 0
+     tauu = 0
+     tautot = 0
+     call alt3_check(alt3_kappa_bte(ctx, tempk, c_loc(x), c_loc(kappa), c_loc(taueff)), 'alt3_kappa_bte')
+     open(10, file='kappa_direct-cg.dat', status='unknown')
+     open(11, file='tau_RTAandeff-cg.dat', status='unknown')
+     call write_kappa(10, 11)
+     close(10)
+     close(11)
+     write(*, '(a,i0,a,es14.6)') 'alt3_kappa: job 5 cg iterations ', iters, ', funcmin ', fmin
+     call alt3_check(alt3_destroy(ctx), 'alt3_destroy')
+     stop
+  end if
   ! calculate_w3_ibz_split_sq over the whole IBZ (kappa.f90:440), calculate_FGR_mpi2 (:485)
   call alt3_check(alt3_v3sq_ibz(ctx, 0, nibz - 1, prec, c_null_ptr), 'alt3_v3sq_ibz')
   call alt3_check(alt3_fgr(ctx, c_null_ptr, c_null_ptr), 'alt3_fgr')
 
-  tempk = tmin                                     ! kappa.f90:565
   ! distribution + calculate_tauRTA_mpi2 + Qandtau (kappa.f90:567-667)
   call alt3_check(alt3_tau(ctx, tempk, c_loc(q), c_loc(qn), c_loc(qu), c_loc(taun), c_loc(tauu), c_loc(tautot)), &
        'alt3_tau')
@@ -150,6 +176,37 @@ program alt3_kappa
      close(10)
      close(11)
      write(*, '(a,i0)') 'alt3_kappa: job 3 iterations ', iters
+     call alt3_check(alt3_destroy(ctx), 'alt3_destroy')
+     stop
+  end if
+
+  if (job == 4) then
+     ! the collision system for a later job 5: CM-1D.dat row by row, RHS_IBZ.dat, and DS.dat =
+     ! F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:2577-2625)
+     allocate(cm(n, n), rhs(n))
+     call alt3_check(alt3_collision(ctx, tempk, c_loc(cm), c_loc(rhs)), 'alt3_collision')
+     open(10, file='CM-1D.dat', status='unknown')
+     do i = 1, n
+        do ia = 1, n
+           write(10, *) cm(i, ia)
+        end do
+     end do
+     close(10)
+     open(10, file='RHS_IBZ.dat', status='unknown')
+     do i = 1, n
+        write(10, *) rhs(i)
+     end do
+     close(10)
+     open(10, file='DS.dat', status='unknown')
+     do i = 1, nibz
+        do l = 1, nb
+           do a = 1, 3
+              write(10, *) frta(i, l, a)
+           end do
+        end do
+     end do
+     close(10)
+     write(*, '(a,i0)') 'alt3_kappa: job 4 wrote the collision system, n = ', n
      call alt3_check(alt3_destroy(ctx), 'alt3_destroy')
      stop
   end if
@@ -287,6 +344,35 @@ contains
     end do
     close(u)
   end subroutine read_fc
+
+  subroutine read_column(fname, cnt, v)
+    ! one list-directed value per line (read_RHS / read_CM_1D_mpi / read_DS, sub.f90:5367-5470)
+    character(*), intent(in) :: fname
+    integer, intent(in) :: cnt
+    real(c_double), intent(out) :: v(*)
+    integer :: u, k
+    open(newunit=u, file=fname, status='old')
+    do k = 1, cnt
+       read(u, *) v(k)
+    end do
+    close(u)
+  end subroutine read_column
+
+  subroutine read_rows(fname, m, a)
+    ! CM-1D.dat holds the matrix row by row; read_CM_1D_mpi fills CM3D(i,j) with j fastest
+    ! (sub.f90:5424-5431)
+    character(*), intent(in) :: fname
+    integer, intent(in) :: m
+    real(c_double), intent(out) :: a(m, m)
+    integer :: u, r, c
+    open(newunit=u, file=fname, status='old')
+    do r = 1, m
+       do c = 1, m
+          read(u, *) a(r, c)
+       end do
+    end do
+    close(u)
+  end subroutine read_rows
 
   real(8) function nbe(w, t)
     ! nbe, extratools.f90:257-288

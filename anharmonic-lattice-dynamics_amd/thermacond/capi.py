@@ -45,7 +45,7 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3"]
+           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_set_fgr"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -192,6 +192,12 @@ class Context:
         P1 = np.zeros((ksub, self.nk, self.nb, self.nb, self.nb), order="F"); P2 = np.zeros_like(P1)
         _check(lib().alt3_fgr(self.h, _p(P1), _p(P2)))
         return P1, P2
+
+    def set_fgr(self, lo: int, P1sm, P2sm):
+        """Load P1sm / P2sm (ksub, nk, nb, nb, nb) of irreducible points lo.. (FP64)."""
+        P1 = np.asfortranarray(P1sm, dtype=np.float64); P2 = np.asfortranarray(P2sm, dtype=np.float64)
+        _check(lib().alt3_set_fgr(self.h, lo, lo + P1.shape[0] - 1, _p(P1), _p(P2)))
+        self._last_ksub = P1.shape[0]
 
     def _ksub(self):
         return getattr(self, "_last_ksub", self.nibz)

@@ -119,6 +119,12 @@ int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm);
  * Gaussian rates, cubic_rates rest.f90:1698-1775, are unreachable from kappa.f90).
  * P1sm/P2sm optional, Fortran (ksub,nk,nb,nb,nb). */
 int alt3_fgr_gauss(alt3_ctx *ctx, double sigma, double *P1sm, double *P2sm);
+/* Load P1sm/P2sm computed elsewhere (the reference's v33sq_delta.NNN.dat split files, read
+ * by calculate_tauRTA_iso sub.f90:7598-7613 and job 5's route) for irreducible points
+ * ibz_lo..ibz_hi, Fortran (ksub,nk,nb,nb,nb) FP64, in place of alt3_v3sq_ibz + alt3_fgr.
+ * Needs the eigen data (alt3_eig or alt3_set_eig); alt3_tau and what follows use them. */
+int alt3_set_fgr(alt3_ctx *ctx, int ibz_lo, int ibz_hi, const double *P1sm, const double *P2sm);
+
 /* Delta integration used by alt3_solve: ALT3_DELTA_TETRA (default) or ALT3_DELTA_GAUSS with
  * width sigma (cm^-1). */
 int alt3_set_delta(alt3_ctx *ctx, int kind, double sigma);

@@ -108,3 +108,40 @@ def test_driver_job3_matches_kappa_iter(si_inputs, tmp_path):
     assert np.allclose(tots, want, rtol=1e-10, atol=0)
     assert (tmp_path / "tau_RTAandeff_iter.dat").exists()
     assert not (tmp_path / "kappa_direct-cg.dat").exists()
+
+
+@pytest.mark.gpu
+def test_driver_job4_then_job5_equals_job1(si_inputs, tmp_path):
+    """job 4 writes CM-1D.dat, RHS_IBZ.dat and DS.dat (read back through thermacond.splitio
+    they equal the collision system of the binding to 1e-12 of each array's largest value);
+    job 5 in the same directory solves it and its kappa_direct-cg.dat equals job 1's (the
+    host-matrix CG path, 1e-9)."""
+    print("case: Fortran driver jobs 4 and 5, SW-Si 4^3 fp64", flush=True)
+    from thermacond import refio, splitio
+    inp = si_inputs[4]
+    j1 = tmp_path / "job1"
+    j45 = tmp_path / "job45"
+    refio.write_inputs(str(j1), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), job=1))
+    refio.write_inputs(str(j45), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), job=4))
+    _run(j1, "fp64")
+    out4 = _run(j45, "fp64")
+    assert "job 4 wrote the collision system" in out4
+    T = float(inp.params.tmin)
+    with capi.Context(inp) as c:
+        c.eig(want=False)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        c.fgr(want=False)
+        c.tau(T)
+        _, F, _ = c.kappa_rta(T)
+        CM, RHS = c.collision(T)
+        n = c.nibz * c.nb * 3
+    CMf, Rf, Df = splitio.read_cg_system(str(j45), n)
+    # the two runs differ only by the order of the FGR kernel's LDS atomics (1e-16 relative)
+    # and the 16 digits of list-directed output; compare on the scale of each array
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    assert rel(CMf, CM) < 1e-12 and rel(Rf, RHS) < 1e-12
+    assert rel(Df, np.array([F[i, l, a] for i in range(F.shape[0]) for l in range(F.shape[1]) for a in range(3)])) < 1e-12
+    refio.write_inputs(str(j45), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), job=5))
+    out5 = _run(j45, "fp64")
+    assert "job 5 cg iterations" in out5
+    assert abs(_tot_kap(j45 / "kappa_direct-cg.dat") / _tot_kap(j1 / "kappa_direct-cg.dat") - 1) < 1e-9

@@ -301,3 +301,28 @@ def test_mixed_fgr_decay_mirror_vs_fp64(si_inputs):
         for l3 in range(l2):                            # the mirrored rows on their own
             r = float(np.max(np.abs(sm[:, :, l2, l3] - sd[:, :, l2, l3])) / np.max(np.abs(sd)))
             assert r < 1e-4, (l2, l3, r)
+
+
+def test_set_fgr_from_split_files(si_inputs, ref4, tmp_path):
+    """P1sm / P2sm written to v33sq_delta.NNN.dat blocks (the reference's split format) and
+    loaded back with alt3_set_fgr give the same scattering rates and kappa_RTA as the FGR
+    kernel's own P (bit for bit: the same FP64 values)."""
+    print("case: alt3_set_fgr from split files 4^3", flush=True)
+    from thermacond import splitio
+    T = 300.0
+    with capi.Context(si_inputs[4]) as c:
+        _load_ref_eig(c, ref4)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        P1, P2 = c.fgr()
+        q0 = c.tau(T)["Q"]
+        k0, _, _ = c.kappa_rta(T)
+        cut = c.nibz // 2
+        for idx, (lo, hi) in enumerate(((0, cut), (cut, c.nibz)), 1):
+            splitio.write_v33sq_delta(str(tmp_path / splitio.block_name("v33sq_delta", idx)), P1[lo:hi], P2[lo:hi])
+        parts = [splitio.read_v33sq_delta(str(tmp_path / splitio.block_name("v33sq_delta", i)), n, c.nk, c.nb)
+                 for i, n in ((1, cut), (2, c.nibz - cut))]
+        c.set_fgr(0, np.concatenate([parts[0][1], parts[1][1]]), np.concatenate([parts[0][2], parts[1][2]]))
+        q1 = c.tau(T)["Q"]
+        k1, _, _ = c.kappa_rta(T)
+    assert np.array_equal(q1, q0)
+    assert np.array_equal(k1, k0)
