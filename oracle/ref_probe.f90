@@ -21,6 +21,15 @@
 ! first update F2, kappa and tauinv_eff(nibz,ndyn) after every iteration and the final F
 ! (mode_kappa clears tauinv_eff for every mode, so no other element survives); F is then reset to F_RTA so
 ! that the job-1 CG below starts from the same guess as before.
+!
+! REF_PROBE_ISO (environment, optional): isotope scattering, iso = 1, routine by routine:
+! cal_giso and P_iso_split (kappa.f90:522-543) after the FGR, dumped as giso and Piso; the
+! P1sm/P2sm blocks written as v33sq_delta.NNN.dat (the files calculate_tauRTA_iso reads);
+! per temperature calculate_tauRTA_iso after calculate_tauRTA_mpi2 (kappa.f90:574-578),
+! update_iso in place of update_mpi (kappa.f90:1401-1407) and CollisionMatrix_iso in place of
+! CollisionMatrix_mpi (kappa.f90:710-716), its Collision_Matrix then feeding RHS, CG and
+! mode_kappa_d as before.  (kappa.f90's own job-1 flow gathers Qvaluempi after the iso
+! branch, which calculate_tauRTA_iso never sets; the probe keeps the routine's Qvalue.)
 program ref_probe
  use ios
  use kpoints
@@ -54,6 +63,9 @@ program ref_probe
  real(8) :: tcm
  real(8) :: normk
  real(8), allocatable :: kiter(:,:,:,:), teiter(:)
+ logical :: iso_
+ integer :: i0, i1, n2, l2, l3
+ character(len=64) :: fname
 
  const33 = 1d30*(hbar/uma)**1.5d0/(sqrt(ee*1d20/uma))**1.5d0*ee/h_plank/100/c_light
  call system('mkdir -p dump')
@@ -173,6 +185,35 @@ program ref_probe
  call dumpr('P2sm', P2smpi, size(P2smpi))
  call dumpr('frequency', frequency, nkc*ndyn)
 
+ call get_environment_variable('REF_PROBE_ISO', envs, status=ios_)
+ iso_ = (ios_ == 0)
+ if (iso_) then
+    allocate(giso(natoms0), Pisos(nibz,ndyn))
+    call cal_giso(giso)
+    call P_iso_split(ksubset,giso,ndyn,eigenvec)
+    call dumpr('giso', giso, natoms0)
+    call dumpr('Piso', Pisos, nibz*ndyn)
+    do n = 1, (nibz + ksub_size - 1)/ksub_size
+       i0 = (n - 1)*ksub_size + 1
+       i1 = min(n*ksub_size, nibz)
+       write(fname,'(a,i3.3,a)') 'v33sq_delta.', n, '.dat'
+       open(80, file=trim(v3path)//trim(fname), form='unformatted', status='replace')
+       write(80) (i1 - i0 + 1)*nkc*ndyn**3
+       do n1 = i0, i1
+          do l1 = 1, ndyn
+             do n2 = 1, nkc
+                do l2 = 1, ndyn
+                   do l3 = 1, ndyn
+                      write(80) P1smpi(n1,n2,l1,l2,l3), P2smpi(n1,n2,l1,l2,l3)
+                   enddo
+                enddo
+             enddo
+          enddo
+       enddo
+       close(80)
+    enddo
+ endif
+
  ntemps = 1
  allocate(temps(64)); temps(1) = tmin
  inquire(file='probe_temps.txt', exist=ex)
@@ -197,6 +238,10 @@ program ref_probe
     call distribution(nkc,ndyn,tempk)
     call calculate_tauRTA_mpi2(ksubset,ndyn,tempk)
     Qvalue = Qvaluempi; Qvalue_N = Qvalue_Nmpi; Qvalue_U = Qvalue_Umpi
+    if (iso_) then
+       Piso = Pisos
+       call calculate_tauRTA_iso(ndyn,tempk)
+    endif
     call Qandtau(ndyn,tempk)
     call calculate_RTA_distributionfunction(nibz,nkc,kibz,ndyn,tempk,veloc,eigenval)
     call mode_kappa(ndyn,tempk,veloc,eigenval)
@@ -225,8 +270,12 @@ program ref_probe
        niter = 0
        do iter_ = 1, maxit
           kappa_old = kappa
-          call update_mpi(ndyn,ksubset)
-          F2 = F2mpi
+          if (iso_) then
+             call update_iso(ndyn)
+          else
+             call update_mpi(ndyn,ksubset)
+             F2 = F2mpi
+          endif
           if (iter_ == 1) call dumpr(trim(tag)//'_F2first', F2, nibz*ndyn*3)
           FFF(:,:,:) = FFF(:,:,:) + update_mix*(F2(:,:,:)-FFF(:,:,:))
           call mode_kappa(ndyn,tempk,veloc,eigenval)
@@ -244,10 +293,15 @@ program ref_probe
        FFF = F_RTA
        deallocate(F_RTA, F2, F2mpi, kappa_old, kappa_new, e, kiter, teiter)
     endif
-    call CollisionMatrix_mpi(ksubset,ndyn)
-    call dumpr(trim(tag)//'_CM', Collision_Matrixmpi, size(Collision_Matrixmpi))
     allocate(Collision_Matrix(nibz*ndyn*3,nibz*ndyn*3))
-    Collision_Matrix = Collision_Matrixmpi
+    if (iso_) then
+       call CollisionMatrix_iso(ndyn)
+       call dumpr(trim(tag)//'_CM', Collision_Matrix, size(Collision_Matrix))
+    else
+       call CollisionMatrix_mpi(ksubset,ndyn)
+       call dumpr(trim(tag)//'_CM', Collision_Matrixmpi, size(Collision_Matrixmpi))
+       Collision_Matrix = Collision_Matrixmpi
+    endif
     call cal_RHS_IBZ(ndyn,nkc,tempk,veloc,eigenval)
     call dumpr(trim(tag)//'_RHS', RHS, nibz*ndyn*3)
     s = 0
