@@ -123,6 +123,7 @@ int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, c
                    double *out, int *d_err);
 bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
+int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
 
 }  // namespace alt3
 
@@ -209,6 +210,10 @@ struct alt3_ctx {
     alt3::DBuf<float> d_s1f, d_s2f;
     bool have_upd = false;
     double upd_temp = -1;
+    // isotope scattering (iso = 1): per-atom g, Piso [nibz][nb] of the current eigenvectors
+    std::vector<double> giso;                      // empty: off
+    alt3::DBuf<double> d_giso, d_piso;
+    bool piso_ok = false;                          // d_piso matches the eigen data
     // per-call scratch kept across calls (no hipMalloc/hipFree on the solve path)
     alt3::DBuf<int> d_flags;                       // [0] V3 den=0, [1] tau, [2] kappa_rta, [3] kappa_bte, [4,5] CG,
                                                    // [6,7,8] iterative BTE (converged, iterations, N_BE error)

@@ -217,9 +217,11 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
     }
 }
 
-// Q, QN, QU of (s, l1) = sum of the m partial sums of that row
-__global__ void k_tau_sum(int n, int nb, const double *__restrict__ qpart, double *__restrict__ Q,
-                          double *__restrict__ QN, double *__restrict__ QU)
+// Q, QN, QU of (s, l1) = sum of the m partial sums of that row; with the isotope term
+// (piso [nibz][nb], else NULL) Piso is added to all three, as calculate_tauRTA_iso does
+// (sub.f90:8502-8504)
+__global__ void k_tau_sum(int n, int nb, int ibz_lo, const double *__restrict__ qpart, const double *__restrict__ piso,
+                          double *__restrict__ Q, double *__restrict__ QN, double *__restrict__ QU)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;   // (s, l1)
     if (x >= n) return;
@@ -229,6 +231,12 @@ __global__ void k_tau_sum(int n, int nb, const double *__restrict__ qpart, doubl
         q += p[0];
         qn += p[1];
         qu += p[2];
+    }
+    if (piso) {
+        const double pi = piso[(size_t)ibz_lo * nb + x];
+        q = q + pi;
+        qn = qn + pi;
+        qu = qu + pi;
     }
     Q[x] = q;
     QN[x] = qn;
@@ -367,7 +375,7 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
                                                    const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                    const int *__restrict__ star_n, const int *__restrict__ star_op,
                                                    const int *__restrict__ star_n0, const double *__restrict__ opk,
-                                                   double *__restrict__ CMr)
+                                                   const double *__restrict__ piso, double *__restrict__ CMr)
 {
     const int rb = blockIdx.x;  // (s, l1)
     const int s = rb / nb, l1 = rb % nb;
@@ -376,6 +384,7 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
     m.coords(k1, a0, a1, a2);
     const size_t ncol = (size_t)nibz * nb * 3;
     const double qd = Q[(size_t)s * nb + l1];
+    const double pis = piso ? piso[(size_t)(ibz_lo + s) * nb + l1] : 0.0;
     for (int cb = threadIdx.x; cb < nibz * nb; cb += blockDim.x) {
         const int n2 = cb / nb, l2 = cb % nb;
         const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
@@ -385,6 +394,11 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
             double C;
             if (n0 == k1 && l1 == l2) {
                 C = -qd;
+            } else if (piso) {
+                int c0, c1, c2;
+                m.coords(n0, c0, c1, c2);
+                const int p3 = m.sum(a0, a1, a2, c0, c1, c2);   // q1+q2
+                C = -(double)R1[rbase + n0] + (double)R1[rbase + p3] + (double)R2[rbase + n0] + pis;
             } else {
                 int c0, c1, c2;
                 m.coords(n0, c0, c1, c2);
@@ -444,7 +458,7 @@ __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, in
                                                      const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                      const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
                                                      const int *__restrict__ tab, const double *__restrict__ opk,
-                                                     double *__restrict__ CMr)
+                                                     const double *__restrict__ piso, double *__restrict__ CMr)
 {
     extern __shared__ double lds[];
     double *C = lds;            // [nk]
@@ -456,15 +470,26 @@ __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, in
     m.coords(k1, a0, a1, a2);
     const size_t rbase = ((size_t)rb * nb + l2) * (size_t)nk;
     for (int x = threadIdx.x; x < 9 * nops; x += blockDim.x) ops[x] = opk[x];
-    for (int n0 = threadIdx.x; n0 < nk; n0 += blockDim.x) {
-        int c0, c1, c2;
-        m.coords(n0, c0, c1, c2);
-        const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q0
-        const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q0-q1
-        C[n0] = -(double)R1[rbase + n0] + (double)R1[rbase + p2] + 0.5 * (double)R2[rbase + n0] +
-                0.5 * (double)R2[rbase + p1];
+    if (piso) {   // CollisionMatrix_iso: -R1(q0) + R1(q1+q0) + R2(q0) + Piso
+        const double pis = piso[(size_t)(ibz_lo + s) * nb + l1];
+        for (int n0 = threadIdx.x; n0 < nk; n0 += blockDim.x) {
+            int c0, c1, c2;
+            m.coords(n0, c0, c1, c2);
+            const int p3 = m.sum(a0, a1, a2, c0, c1, c2);   // q1+q0
+            C[n0] = -(double)R1[rbase + n0] + (double)R1[rbase + p3] + (double)R2[rbase + n0] + pis;
+        }
+    } else {
+        for (int n0 = threadIdx.x; n0 < nk; n0 += blockDim.x) {
+            int c0, c1, c2;
+            m.coords(n0, c0, c1, c2);
+            const int p1 = m.diff(a0, a1, a2, c0, c1, c2);  // q1-q0
+            const int p2 = m.diff(c0, c1, c2, a0, a1, a2);  // q0-q1
+            C[n0] = -(double)R1[rbase + n0] + (double)R1[rbase + p2] + 0.5 * (double)R2[rbase + n0] +
+                    0.5 * (double)R2[rbase + p1];
+        }
     }
     __syncthreads();
+    // the diagonal: -Q, which holds Piso when the isotope term is on (-Q3 - Piso, as there)
     if (threadIdx.x == 0 && l1 == l2) C[k1] = -Q[(size_t)s * nb + l1];
     __syncthreads();
     star_sums(nibz, nb, rb, l2, C, ops, slot_n2, slot_cnt, tab, CMr);
@@ -503,6 +528,21 @@ __global__ void __launch_bounds__(256) k_update_c(alt3::MeshDev m, int nk, int n
     }
     __syncthreads();
     star_sums(nibz, nb, rb, l, C, ops, slot_n2, slot_cnt, tab, U);
+}
+
+// update_iso (sub.f90:2456-2610) adds Piso(n1,l1) Fsq2 to A after its loops, Fsq2 being what
+// the loops left there: F of the last branch at the last irreducible point, rotated by the op
+// of the last member of that point's star.  In the matrix U: Piso(n1,l1) op(i,j) in column
+// (nibz-1, nb-1, j) of every row (s, l1, i).  One thread per row.
+__global__ void k_update_iso(int ksub, int nb, int nibz, int ibz_lo, const double *__restrict__ piso,
+                             const double *__restrict__ op, double *__restrict__ U)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;   // (s, l1, i)
+    if (r >= ksub * nb * 3) return;
+    const int i = r % 3, sl = r / 3;
+    const size_t ncol = (size_t)nibz * nb * 3, d2 = 3 * ((size_t)(nibz - 1) * nb + nb - 1);
+    const double p = piso[(size_t)ibz_lo * nb + sl];
+    for (int j = 0; j < 3; j++) U[(size_t)r * ncol + d2 + j] = U[(size_t)r * ncol + d2 + j] + p * op[3 * j + i];
 }
 
 // S1/S2[s][l1][l3][q2] = sum over l2 of P1sm, P2sm times the occupations of
@@ -905,6 +945,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     const MeshDev md = MeshDev::make(ctx->nc);
     ctx->have_tau = ctx->have_cm = ctx->have_upd = false;
     ctx->tau_temps.clear();
+    ALT3_TRY(piso_run(ctx));
     int *derr = ctx->d_flags.p + 1;
     ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     const size_t dstr = (size_t)nk * nb, qstr = (size_t)ksub * nb, rstr = (size_t)ksub * nb * nb * nk;
@@ -969,9 +1010,11 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
         }
         ALT3_HIP(hipGetLastError());
     }
+    const double *piso = ctx->giso.empty() ? nullptr : ctx->d_piso.p;
     for (int t = 0; t < nt; t++)
         hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, parts,
-                           qpart.p + t * pstr, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr, ctx->d_qu.p + t * qstr);
+                           ctx->ibz_lo, qpart.p + t * pstr, piso, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr,
+                           ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1064,16 +1107,17 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
     const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
+    const double *piso = ctx->giso.empty() ? nullptr : ctx->d_piso.p;   // CollisionMatrix_iso
     auto launch = [&](auto *R1, auto *R2) {
         using RT = std::remove_const_t<std::remove_pointer_t<decltype(R1)>>;
         if (lds <= 65536)
             hipLaunchKernelGGL(k_collision_c<RT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb,
                                ctx->ibz_lo, nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_cm_slot_n2.p,
-                               ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, ctx->d_cm.p);
+                               ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, piso, ctx->d_cm.p);
         else
             hipLaunchKernelGGL(k_collision<RT>, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                nibz, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_star_n.p, ctx->d_star_op.p,
-                               ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_cm.p);
+                               ctx->d_star_n0.p, ctx->d_opk.p, piso, ctx->d_cm.p);
     };
     if (ctx->r_fp32)
         launch((const float *)ctx->r1f(), (const float *)ctx->r2f());
@@ -1238,6 +1282,12 @@ static int update_build(alt3_ctx *ctx, double tempk, const char *who)
         ALT3_TRY(launch((const double *)ctx->d_p1.p, (const double *)ctx->d_p2.p, ctx->r1(), ctx->r2(), ctx->d_s1, ctx->d_s2));
     }
     ALT3_HIP(hipGetLastError());
+    if (!ctx->giso.empty()) {
+        const int nl = nibz - 1, op = ctx->T.star_op[48 * nl + ctx->T.star_n[nl] - 1];
+        hipLaunchKernelGGL(k_update_iso, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, ctx->stream, ksub, nb,
+                           nibz, ctx->ibz_lo, ctx->d_piso.p, ctx->d_opk.p + 9 * op, ctx->d_upd.p);
+        ALT3_HIP(hipGetLastError());
+    }
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_upd = true;
     ctx->upd_temp = tempk;

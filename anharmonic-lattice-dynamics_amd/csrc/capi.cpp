@@ -279,6 +279,7 @@ int alt3_set_eig(alt3_ctx *ctx, const double *eigenval, const double *eigenvec, 
     ALT3_HIP(hipMemcpy(ctx->d_evec.p, eigenvec, sizeof(double) * nk * nb * nb * 2, hipMemcpyHostToDevice));
     ALT3_HIP(hipMemcpy(ctx->d_vel.p, veloc, sizeof(double) * nk * nb * 3, hipMemcpyHostToDevice));
     ctx->have_eig = true;
+    ctx->piso_ok = false;
     ctx->have_veloci = false;
     ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
     return ALT3_OK;
@@ -439,6 +440,32 @@ int alt3_update(alt3_ctx *ctx, double tempk, const double *FFF, const double *FR
     Timer tm(ctx);
     ALT3_TRY(update_run(ctx, tempk, FFF, FRTA, F2));
     tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_set_iso(alt3_ctx *ctx, const double *giso, double *Piso)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    ctx->have_tau = ctx->have_cm = ctx->have_upd = false;
+    ctx->piso_ok = false;
+    if (!giso) {
+        ctx->giso.clear();
+        if (Piso) return fail(ALT3_E_ARG, "Piso needs giso");
+        return ALT3_OK;
+    }
+    for (int a = 0; a < ctx->natoms0; a++)
+        if (!(giso[a] >= 0)) return fail(ALT3_E_ARG, "giso must be >= 0");
+    ctx->giso.assign(giso, giso + ctx->natoms0);
+    if (!Piso) return ALT3_OK;
+    Timer tm(ctx);
+    ALT3_TRY(piso_run(ctx));
+    tm.stop();
+    const size_t nibz = ctx->nibz, nb = ctx->nb;
+    std::vector<double> h(nibz * nb);
+    ALT3_HIP(hipMemcpy(h.data(), ctx->d_piso.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nibz; i++)
+        for (size_t l = 0; l < nb; l++) Piso[i + nibz * l] = h[i * nb + l];
     return ALT3_OK;
 }
 

@@ -590,6 +590,7 @@ int eig_run(alt3_ctx *ctx)
     }
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_eig = true;
+    ctx->piso_ok = false;
     ctx->have_veloci = false;
     return ALT3_OK;
 }

@@ -575,4 +575,93 @@ int fgr_run(alt3_ctx *ctx)
     return ALT3_OK;
 }
 
+// Isotope term of P_iso_split (sub.f90:4825-4961).  One workgroup per (irreducible q1, l1);
+// each thread takes cubes of the mesh and, per branch l2, the tetrahedra of its cube whose
+// energy range (corner energies w(q,l2), REAL*4 as in weight_tet) contains w1 = w(q1,l1):
+//   Piso(q1,l1) = w1^2 pi/2 sum_l2 sum_tetra sum_corners gs(q,l2) c/6,
+//   gs(q,l2)    = sum_atoms g(a) |sum_{x of atom a} conj(e(x,l1,q1)) e(x,l2,q)|^2.
+// The reference repeats this sum inside a loop over n2 = 1..nk that changes nothing in it and
+// divides by nk at the end; the sum is formed once.  The thread sums are reduced in a fixed
+// order, so the result does not depend on timing.
+__global__ void __launch_bounds__(256) k_piso(MeshDev m, int nk, int nb, int natoms0, const int *__restrict__ mapinv,
+                                              const double *__restrict__ freq, const double *__restrict__ evec,
+                                              const double *__restrict__ giso, double *__restrict__ piso)
+{
+    __shared__ double e1[2 * 64];
+    __shared__ double ga[32];
+    __shared__ double red[4];
+    const int s = blockIdx.x / nb, l1 = blockIdx.x % nb, tid = threadIdx.x;
+    const int k1 = mapinv[s];
+    const double omega1 = freq[(size_t)k1 * nb + l1];
+    for (int x = tid; x < 2 * nb; x += blockDim.x) e1[x] = evec[2 * ((size_t)k1 * nb + l1) * nb + x];
+    for (int x = tid; x < natoms0; x += blockDim.x) ga[x] = giso[x];
+    __syncthreads();
+    double acc = 0;
+    for (int l2 = 0; l2 < nb; l2++)
+        for (int k = tid; k < nk; k += blockDim.x) {
+            int c0, c1, c2;
+            m.coords(k, c0, c1, c2);
+            const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                      d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+            const int ci[8] = {m.index(c0, c1, c2), m.index(d0, c1, c2), m.index(c0, d1, c2), m.index(d0, d1, c2),
+                               m.index(c0, c1, d2), m.index(d0, c1, d2), m.index(c0, d1, d2), m.index(d0, d1, d2)};
+            float ce[8], lo = 3.4e38f, hi = -3.4e38f;
+            for (int p = 0; p < 8; p++) {
+                ce[p] = (float)freq[(size_t)ci[p] * nb + l2];
+                lo = fminf(lo, ce[p]);
+                hi = fmaxf(hi, ce[p]);
+            }
+            if (!omega_in(omega1, lo, hi)) continue;
+            for (int l = 0; l < 6; l++) {
+                const float w4[4] = {ce[kTc[l][0]], ce[kTc[l][1]], ce[kTc[l][2]], ce[kTc[l][3]]};
+                if (!omega_in(omega1, fminf(fminf(w4[0], w4[1]), fminf(w4[2], w4[3])),
+                              fmaxf(fmaxf(w4[0], w4[1]), fmaxf(w4[2], w4[3]))))
+                    continue;
+                double c[4];
+                weight_tet1(w4, omega1, c);
+                for (int y = 0; y < 4; y++) {
+                    if (c[y] == 0.0) continue;
+                    const double *e2 = evec + 2 * ((size_t)ci[kTc[l][y]] * nb + l2) * nb;
+                    double gs = 0;
+                    for (int a = 0; a < natoms0; a++) {
+                        double re = 0, im = 0;
+                        for (int x = 3 * a; x < 3 * a + 3; x++) {
+                            const double ar = e1[2 * x], ai = e1[2 * x + 1], br = e2[2 * x], bi = e2[2 * x + 1];
+                            re = re + (ar * br + ai * bi);
+                            im = im + (ar * bi - ai * br);
+                        }
+                        gs = gs + ga[a] * (re * re + im * im);
+                    }
+                    acc = acc + gs * c[y] / 6.0;
+                }
+            }
+        }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+        piso[(size_t)s * nb + l1] = t * omega1 * omega1 * PI / 2;
+    }
+}
+
+int piso_run(alt3_ctx *ctx)
+{
+    if (ctx->giso.empty() || ctx->piso_ok) return ALT3_OK;
+    if (!ctx->have_eig) return fail(ALT3_E_STATE, "isotope scattering needs the eigen data (alt3_eig or alt3_set_eig)");
+    if (ctx->nb > 64) return fail(ALT3_E_ARG, "isotope term: at most 64 branches");
+    const int nb = ctx->nb, nibz = ctx->nibz;
+    ALT3_TRY(ctx->d_giso.alloc(ctx->giso.size()));
+    ALT3_TRY(ctx->d_piso.alloc((size_t)nibz * nb));
+    ALT3_HIP(hipMemcpyAsync(ctx->d_giso.p, ctx->giso.data(), sizeof(double) * ctx->giso.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    hipLaunchKernelGGL(k_piso, dim3((unsigned)(nibz * nb)), dim3(256), 0, ctx->stream, MeshDev::make(ctx->nc), ctx->nk,
+                       nb, ctx->natoms0, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_evec.p, ctx->d_giso.p, ctx->d_piso.p);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->piso_ok = true;
+    return ALT3_OK;
+}
+
 }  // namespace alt3

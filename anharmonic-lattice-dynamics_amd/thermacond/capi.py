@@ -45,7 +45,8 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_set_fgr"]
+           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_set_fgr",
+           "alt3_set_iso", "alt3_isotope_g"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -70,6 +71,20 @@ def lib():
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
+
+
+def isotope_g(element: str) -> float:
+    """cal_g (sub.f90:3766-4693): the natural-isotope mass variance of an element (0 if the
+    table has no data for it)."""
+    g = C.c_double()
+    _check(lib().alt3_isotope_g(element.encode(), C.byref(g)))
+    return g.value
+
+
+def giso_of(inp: InputSet):
+    """cal_giso (sub.f90:4695-4725): g of every atom of the primitive cell from the element
+    names and atom types of structure.params."""
+    return np.array([isotope_g(inp.names[t - 1]) for t in inp.types])
 
 
 def _check(rc: int):
@@ -198,6 +213,19 @@ class Context:
         P1 = np.asfortranarray(P1sm, dtype=np.float64); P2 = np.asfortranarray(P2sm, dtype=np.float64)
         _check(lib().alt3_set_fgr(self.h, lo, lo + P1.shape[0] - 1, _p(P1), _p(P2)))
         self._last_ksub = P1.shape[0]
+
+    def set_iso(self, giso=None, want: bool = True):
+        """Isotope scattering (iso = 1): giso per atom of the primitive cell, None to switch it
+        off.  Returns Piso (nibz, nb) of P_iso_split when want."""
+        if giso is None:
+            _check(lib().alt3_set_iso(self.h, None, None))
+            return None
+        g = np.ascontiguousarray(giso, dtype=np.float64)
+        if g.shape != (self.inp.natoms0,):
+            raise ValueError(f"giso needs {self.inp.natoms0} values")
+        out = np.zeros((self.nibz, self.nb), order="F") if want else None
+        _check(lib().alt3_set_iso(self.h, _p(g), _p(out)))
+        return out
 
     def _ksub(self):
         return getattr(self, "_last_ksub", self.nibz)

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import dataclasses
 import os
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -170,6 +170,7 @@ class InputSet:
     fc3_atoms: np.ndarray        # (nt3,3)
     fc3_xyz: np.ndarray          # (nt3,3)
     fc3: np.ndarray              # (nt3,)
+    names: Tuple[str, ...] = ()  # element label per atom type (structure.params)
 
     @property
     def natoms0(self) -> int:
@@ -203,6 +204,7 @@ def read_inputs(dirname: str) -> InputSet:
     scale = _flt(_toks(S[2])[0])
     ntype = int(_toks(S[7])[0])
     mas = [_flt(x) for x in _toks(S[8])[:ntype]]
+    names = tuple(_toks(S[9])[:ntype])
     nat0 = int(_toks(S[10])[0])
     types = np.zeros(nat0, dtype=np.int64)
     for r in range(nat0):
@@ -255,4 +257,4 @@ def read_inputs(dirname: str) -> InputSet:
 
     a2, x2, v2 = rdfc("fc2.dat", 2, nterms[1])
     a3, x3, v3 = rdfc("fc3.dat", 3, nterms[2])
-    return InputSet(rp, latp, prim, scale, masses, types, r123, atompos, i0, ic, a2, x2, v2, a3, x3, v3)
+    return InputSet(rp, latp, prim, scale, masses, types, r123, atompos, i0, ic, a2, x2, v2, a3, x3, v3, names)

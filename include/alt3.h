@@ -190,6 +190,24 @@ int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix
 int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
                  double *nself, double *uself);
 
+/* Isotope scattering, iso = 1 (kappa.f90:522-543).  giso (natoms0): the mass-variance factor
+ * of each atom of the primitive cell (cal_giso, sub.f90:4695-4725; alt3_isotope_g gives it per
+ * element).  The isotope term Piso (nibz,nb) of P_iso_split (sub.f90:4825-4961) is formed
+ * from the eigen data of the context (again after alt3_eig or alt3_set_eig) and, while set,
+ *   alt3_tau adds it to Q, QN and QU (calculate_tauRTA_iso, sub.f90:8311-8549),
+ *   alt3_collision builds CollisionMatrix_iso (sub.f90:3387-3570) in place of
+ *     CollisionMatrix_mpi: C = -P1(q0) + P1(q1+q0) + P2(q0) + Piso off the diagonal,
+ *   alt3_update and alt3_kappa_iter apply update_iso (sub.f90:2456-2610),
+ * and alt3_solve follows.  giso == NULL switches the term off.  Piso (optional) receives
+ * Piso, Fortran (nibz,nb); it needs the eigen data.  Clears tau and what depends on it. */
+int alt3_set_iso(alt3_ctx *ctx, const double *giso, double *Piso);
+
+/* g of an element from its natural isotopes, cal_g (sub.f90:3766-4693):
+ * g = sum_i f_i (1 - m_i/m_ave)^2 with abundances f_i and m_ave = sum_i f_i m_i; element is
+ * compared as the reference's CHARACTER*2 names are.  An element without isotope data gets
+ * g = 0, as in the reference.  Needs no context and no GPU. */
+int alt3_isotope_g(const char *element, double *g);
+
 /* The whole job-1 path on the device for each temperature: eig (unless set), V3 over the
  * IBZ and FGR (unless the context still holds them from an earlier call with the same
  * precision and delta integration; alt3_eig/alt3_set_eig clear them), then per T tau (one pass over P1sm/P2sm per group of four temperatures),

@@ -792,6 +792,49 @@ int or_fgr(int ibz_lo, int ibz_hi, const int *mapinv, int nk, const double *kp, 
     return 0;
 }
 
+/* P_iso_split (sub.f90:4825-4961): the isotope term of irreducible point n1, branch l1,
+ *   Piso(n1,l1) = w1^2 pi/2 sum_l2 sum_tetra sum_corners gs(q,l2) c/6,
+ *   gs(q,l2) = sum_atoms g(a) |dot_product(e(3a-2:3a, l1, q1), e(3a-2:3a, l2, q))|^2
+ * with c the weight_tet weights of delta(w1 - w(q,l2)) over all 6 nk tetrahedra (corner
+ * energies frequency(q,l2)).  The reference wraps this in a loop over n2 = 1..nk that changes
+ * nothing inside, then divides by nk: nk identical copies and the division cancel, so the sum
+ * is formed once.  evec: complex (nb,nb,nk) as interleaved doubles; g (natoms0); tet (4,6nk)
+ * 0-based; Piso (ksub,nb). */
+void or_piso(int ibz_lo, int ibz_hi, const int *mapinv, int nk, int nb, int natoms0, const double *freq,
+             const int *tet, const double *evec, const double *gat, double *Piso)
+{
+    int ksub = ibz_hi - ibz_lo + 1;
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (int s = 0; s < ksub; s++)
+        for (int l1 = 0; l1 < nb; l1++) {
+            const int i = mapinv[ibz_lo + s];
+            const double omega1 = freq[i + (size_t)nk * l1];
+            const double *e1 = evec + 2 * ((size_t)i * nb + l1) * nb;
+            double S = 0;
+            for (int l2 = 0; l2 < nb; l2++)
+                for (int t = 0; t < 6 * nk; t++) {
+                    double w[4], c[4];
+                    for (int k = 0; k < 4; k++) w[k] = freq[tet[4 * t + k] + (size_t)nk * l2];
+                    or_weight_tet1(w, omega1, c);
+                    for (int k = 0; k < 4; k++) {
+                        const double *e2 = evec + 2 * ((size_t)tet[4 * t + k] * nb + l2) * nb;
+                        double gs = 0;
+                        for (int a = 0; a < natoms0; a++) {
+                            double re = 0, im = 0;
+                            for (int x = 3 * a; x < 3 * a + 3; x++) {
+                                re += e1[2 * x] * e2[2 * x] + e1[2 * x + 1] * e2[2 * x + 1];
+                                im += e1[2 * x] * e2[2 * x + 1] - e1[2 * x + 1] * e2[2 * x];
+                            }
+                            double m = sqrt(re * re + im * im);
+                            gs = gs + gat[a] * (m * m);
+                        }
+                        S = S + gs * c[k] / 6.0;
+                    }
+                }
+            Piso[s + (size_t)ksub * l1] = S * omega1 * omega1 * PI / 2;
+        }
+}
+
 /* Gaussian-smeared form of or_fgr: delta_g(x, sigma) = exp(-x^2/2/sigma^2)/sqrt(2 pi)/sigma
  * (extratools.f90:239-246) at each mesh point instead of the tetrahedron weights; same
  * processes, energies and acoustic masks as calculate_FGR_mpi2 (sub.f90:7199-7499). */
@@ -1011,9 +1054,13 @@ void or_mode_kappa(int nibz, int nb, int nk, const double *kibz, const int *mapi
 
 /* CollisionMatrix_mpi (sub.f90:3034-3210): CM (ksub*nb*3, nibz*nb*3) column-major;
  * P1,P2 (ksub,nk,nb,nb,nb) with occupations applied. */
+/* CollisionMatrix_mpi (sub.f90:3034-3210), or with Piso != NULL CollisionMatrix_iso
+ * (sub.f90:3387-3570), the older form used when iso = 1: off the diagonal
+ *   C = sum_l3 -P1(q0) + P1(q1+q0) + P2(q0)   plus Piso(n1,l1),
+ * on it (q0 = q1, l2 = l1) the same diagonal sum minus Piso(n1,l1).  Piso (nibz,nb). */
 void or_collision(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const int *mapinv, int nk,
                   const double *kp, const int *nc, const double *g, const double *primlatt, int nops,
-                  const double *opk, int nb, const double *P1, const double *P2, double *CM)
+                  const double *opk, int nb, const double *P1, const double *P2, const double *Piso, double *CM)
 {
     int ksub = ibz_hi - ibz_lo + 1;
     size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
@@ -1041,6 +1088,15 @@ void or_collision(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const in
                                 for (int l3 = 0; l3 < nb; l3++)
                                     for (int n22 = 0; n22 < nk; n22++)
                                         C = C + (-PX(P1, n1, n22, l1, nl2, l3) - 0.5 * PX(P2, n1, n22, l1, nl2, l3));
+                            if (Piso) C = C - Piso[n1 + ibz_lo + (size_t)nibz * l1];
+                        } else if (Piso) {
+                            const double *q1 = kibz + 3 * (n1 + ibz_lo), *q2 = sq + 144 * n2 + 3 * ns;
+                            double a[3] = {q1[0] + q2[0], q1[1] + q2[1], q1[2] + q2[2]};
+                            int p1;
+                            or_get_k_info(a, nc, g, &p1, NULL);
+                            for (int nl3 = 0; nl3 < nb; nl3++)
+                                C = C + (-PX(P1, n1, n0, l1, l2, nl3) + PX(P1, n1, p1, l1, l2, nl3) + PX(P2, n1, n0, l1, l2, nl3));
+                            C = C + Piso[n1 + ibz_lo + (size_t)nibz * l1];
                         } else {
                             const double *q1 = kibz + 3 * (n1 + ibz_lo), *q2 = sq + 144 * n2 + 3 * ns;
                             double a[3] = {q1[0] - q2[0], q1[1] - q2[1], q1[2] - q2[2]};
@@ -1076,7 +1132,7 @@ void or_collision(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const in
 int or_update(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const int *mapibz, int nk,
               const double *kp, const int *nc, const double *g, const double *primlatt, int nops,
               const double *opk, int nb, const double *P1, const double *P2, const double *FFF,
-              const double *FRTA, const double *Q, double *F2)
+              const double *FRTA, const double *Q, const double *Piso, double *F2)
 {
     int ksub = ibz_hi - ibz_lo + 1, bad = 0;
     int *narm = malloc(sizeof(int) * nibz), *sop = malloc(sizeof(int) * 48 * nibz), *sn0 = malloc(sizeof(int) * 48 * nibz);
@@ -1131,6 +1187,16 @@ int or_update(int ibz_lo, int ibz_hi, int nibz, const double *kibz, const int *m
                                     0.5 * PX(P2, ibz, ns, l1, l2, l3) * (fq2[xyz] + fp2[xyz]);
                             }
                     }
+                if (Piso) {
+                    /* update_iso (sub.f90:2456-2610) adds Piso(n1,l1) Fsq2(xyz) after the loops,
+                     * Fsq2 as the loops left it: F(nibz, ndn) rotated by the op of the last
+                     * star member of the last irreducible point */
+                    const int nl = nibz - 1;
+                    double f[3], fq2[3];
+                    for (int c = 0; c < 3; c++) f[c] = FX(nl, nb - 1, c);
+                    opmul(opk, sop[48 * nl + narm[nl] - 1], f, fq2);
+                    A = A + Piso[n1 + (size_t)nibz * l1] * fq2[xyz];
+                }
                 F2[ibz + (size_t)ksub * (l1 + (size_t)nb * xyz)] =
                     FRTA[n1 + (size_t)nibz * (l1 + (size_t)nb * xyz)] + A / Q[n1 + (size_t)nibz * l1];
             }

@@ -308,44 +308,62 @@ def mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, FFF, tem
     return kap, te
 
 
-def collision(ibz_lo, ibz_hi, kibz, mapinv, kp, nc, g, primlatt, opk, P1, P2):
+def collision(ibz_lo, ibz_hi, kibz, mapinv, kp, nc, g, primlatt, opk, P1, P2, Piso=None):
+    """CollisionMatrix_mpi, or CollisionMatrix_iso when Piso (nibz, nb) is given."""
     nibz = len(kibz); nk = len(kp); nb = P1.shape[2]; ksub = ibz_hi - ibz_lo + 1
     CM = np.zeros((ksub * nb * 3, nibz * nb * 3), order="F")
+    pi = None if Piso is None else _f64(Piso)
     lib().or_collision(ibz_lo, ibz_hi, nibz, _p(np.ascontiguousarray(kibz)), _p(_i32(mapinv)), nk,
                        _p(np.ascontiguousarray(kp)), _p(_i32(np.asarray(nc))), _p(np.ascontiguousarray(g)),
-                       _p(_f64(primlatt)), len(opk), _p(np.ascontiguousarray(opk)), nb, _p(_f64(P1)), _p(_f64(P2)), _p(CM))
+                       _p(_f64(primlatt)), len(opk), _p(np.ascontiguousarray(opk)), nb, _p(_f64(P1)), _p(_f64(P2)),
+                       None if pi is None else _p(pi), _p(CM))
     return CM
 
 
-def update(ibz_lo, ibz_hi, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, FFF, FRTA, Q):
-    """update_mpi: F2 (ksub, nb, 3) = F_RTA + A(FFF)/Q for the irreducible points ibz_lo..hi."""
+def piso(ibz_lo, ibz_hi, mapinv, freq, tet, eigenvec, giso):
+    """P_iso_split: Piso (ksub, nb) from the mesh eigenvectors (nb, nb, nk), the tetrahedra
+    (4, 6 nk, 0-based) and the per-atom g factors."""
+    nk, nb = freq.shape; ksub = ibz_hi - ibz_lo + 1
+    out = np.zeros((ksub, nb), order="F")
+    ev = np.require(eigenvec, dtype=np.complex128, requirements=["F_CONTIGUOUS", "ALIGNED"])
+    gi = np.ascontiguousarray(giso, dtype=np.float64)
+    lib().or_piso(ibz_lo, ibz_hi, _p(_i32(mapinv)), nk, nb, len(gi), _p(_f64(freq)),
+                  _p(np.ascontiguousarray(tet, dtype=np.int32)), _p(ev), _p(gi), _p(out))
+    return out
+
+
+def update(ibz_lo, ibz_hi, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, FFF, FRTA, Q, Piso=None):
+    """update_mpi: F2 (ksub, nb, 3) = F_RTA + A(FFF)/Q for the irreducible points ibz_lo..hi;
+    with Piso (nibz, nb) update_iso."""
     nibz = len(kibz); nk = len(kp); nb = P1.shape[2]; ksub = ibz_hi - ibz_lo + 1
     F2 = np.zeros((ksub, nb, 3), order="F")
+    pi = None if Piso is None else _f64(Piso)
     rc = lib().or_update(ibz_lo, ibz_hi, nibz, _p(np.ascontiguousarray(kibz)), _p(_i32(mapibz)), nk,
                          _p(np.ascontiguousarray(kp)), _p(_i32(np.asarray(nc))), _p(np.ascontiguousarray(g)),
                          _p(_f64(primlatt)), len(opk), _p(np.ascontiguousarray(opk)), nb, _p(_f64(P1)), _p(_f64(P2)),
-                         _p(_f64(FFF)), _p(_f64(FRTA)), _p(_f64(Q)), _p(F2))
+                         _p(_f64(FFF)), _p(_f64(FRTA)), _p(_f64(Q)), None if pi is None else _p(pi), _p(F2))
     if rc != 0:
         raise ValueError("update: a mesh point q1 +- q2 is not a member of its star")
     return F2
 
 
 def bte_iter(kibz, mapinv, mapibz, kp, nc, g, primlatt, opk, eigenval, veloc, P1, P2, FRTA, Q, tempk, volume_r,
-             max_iter=150, update_mix=float(np.float32(0.4)), tol=float(np.float32(1e-4)), classical=0):
+             max_iter=150, update_mix=float(np.float32(0.4)), tol=float(np.float32(1e-4)), classical=0, Piso=None):
     """The iterative BTE of job 3 (kappa.f90:1373-1487) on the whole irreducible zone:
     F <- F + update_mix (update(F) - F), kappa by mode_kappa after every update, stop once
     sqrt(sum (kappa_new - kappa_old)^2 / (9 nb)) < tol.  Returns (F, kappa history (nb,3,3,n),
     tauinv_eff(nibz,nb) after each iteration (n,)).
     The defaults are read_params' (rest.f90:36,44): max_iter = 150 and update_mix = 0.4, a
     REAL*4 literal stored in a REAL*8 (0.4000000059604645); the stop test normk < 0.0001
-    (kappa.f90:1476) compares with a REAL*4 literal too."""
+    (kappa.f90:1476) compares with a REAL*4 literal too.  With Piso the update is update_iso
+    (kappa.f90:1401-1407)."""
     nibz = len(kibz); nb = eigenval.shape[0]
     F = np.array(FRTA, order="F", copy=True)
     kap, _ = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
     hist, tes = [], []
     for _ in range(max_iter):
         old = kap
-        F2 = update(0, nibz - 1, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, F, FRTA, Q)
+        F2 = update(0, nibz - 1, kibz, mapibz, kp, nc, g, primlatt, opk, P1, P2, F, FRTA, Q, Piso)
         F = np.asfortranarray(F + update_mix * (F2 - F))
         kap, te = mode_kappa(kibz, mapinv, kp, nc, g, primlatt, opk, eigenval, veloc, F, tempk, volume_r, classical)
         hist.append(kap)

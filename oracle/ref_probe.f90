@@ -22,6 +22,9 @@
 ! (mode_kappa clears tauinv_eff for every mode, so no other element survives); F is then reset to F_RTA so
 ! that the job-1 CG below starts from the same guess as before.
 !
+! probe_calg.txt (optional): n, then n element names, one per line: cal_g (sub.f90:3766-4693)
+! for each, dumped as calg (n); the probe stops there.
+!
 ! REF_PROBE_ISO (environment, optional): isotope scattering, iso = 1, routine by routine:
 ! cal_giso and P_iso_split (kappa.f90:522-543) after the FGR, dumped as giso and Piso; the
 ! P1sm/P2sm blocks written as v33sq_delta.NNN.dat (the files calculate_tauRTA_iso reads);
@@ -66,8 +69,24 @@ program ref_probe
  logical :: iso_
  integer :: i0, i1, n2, l2, l3
  character(len=64) :: fname
+ character(len=2) :: elname
+ real(8), allocatable :: calg(:)
 
  const33 = 1d30*(hbar/uma)**1.5d0/(sqrt(ee*1d20/uma))**1.5d0*ee/h_plank/100/c_light
+ inquire(file='probe_calg.txt', exist=ex)
+ if (ex) then
+    open(78, file='probe_calg.txt', status='old')
+    read(78,*) n
+    allocate(calg(n))
+    do i = 1, n
+       read(78,'(a)') elname
+       call cal_g(elname, calg(i))
+    enddo
+    close(78)
+    call system('mkdir -p dump')
+    call dumpr('calg', calg, n)
+    stop
+ endif
  call system('mkdir -p dump')
 
  call read_params
