@@ -575,67 +575,140 @@ int fgr_run(alt3_ctx *ctx)
     return ALT3_OK;
 }
 
-// Isotope term of P_iso_split (sub.f90:4825-4961).  One workgroup per (irreducible q1, l1);
-// each thread takes cubes of the mesh and, per branch l2, the tetrahedra of its cube whose
-// energy range (corner energies w(q,l2), REAL*4 as in weight_tet) contains w1 = w(q1,l1):
-//   Piso(q1,l1) = w1^2 pi/2 sum_l2 sum_tetra sum_corners gs(q,l2) c/6,
-//   gs(q,l2)    = sum_atoms g(a) |sum_{x of atom a} conj(e(x,l1,q1)) e(x,l2,q)|^2.
-// The reference repeats this sum inside a loop over n2 = 1..nk that changes nothing in it and
-// divides by nk at the end; the sum is formed once.  The thread sums are reduced in a fixed
-// order, so the result does not depend on timing.
-__global__ void __launch_bounds__(256) k_piso(MeshDev m, int nk, int nb, int natoms0, const int *__restrict__ mapinv,
-                                              const double *__restrict__ freq, const double *__restrict__ evec,
+// Isotope term of P_iso_split (sub.f90:4825-4961).  One workgroup per (irreducible q1, l1):
+//   Piso(q1,l1) = w1^2 pi/2 sum_l2 sum_q W(q,l2) gs(q,l2),
+//   W(q,l2)     = sum over the tetrahedra at q of c/6,   c the weight_tet weights of
+//                 delta(w1 - w(q,l2)) (corner energies REAL*4, as there),
+//   gs(q,l2)    = sum_atoms g(a) |sum_{x of atom a} conj(e(x,l1,q1)) e(x,l2,q)|^2,
+// which is the reference's sum over tetrahedra and corners of gs c/6, grouped by mesh point.
+// Per branch l2 the mesh energies go to LDS; the threads scan a chunk of cubes and push the
+// tetrahedra whose energy range contains w1 onto an LDS list, which all threads then work
+// through (few tetrahedra are active, so the FP64 weight formula runs with every lane busy),
+// adding c/6 to W in LDS; gs is then formed once per mesh point with a weight.  The reference
+// repeats the whole sum inside a loop over n2 = 1..nk that changes nothing in it and divides by
+// nk at the end; the sum is formed once.
+constexpr int kIsoList = 4096;   // tetrahedra per chunk of cubes (kIsoList / 6 cubes)
+constexpr int kIsoThreads = 1024;
+
+// the mesh energies as REAL*4, branch-major [l][q], for coalesced per-branch loads
+__global__ void k_freq_f32t(int nk, int nb, const double *__restrict__ freq, float *__restrict__ enT)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;   // (l, q)
+    if (x >= nk * nb) return;
+    const int l = x / nk, q = x - l * nk;
+    enT[x] = (float)freq[(size_t)q * nb + l];
+}
+
+__global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb, int natoms0, const int *__restrict__ mapinv,
+                                              const double *__restrict__ freq, const float *__restrict__ enT,
+                                              const double *__restrict__ evec,
                                               const double *__restrict__ giso, double *__restrict__ piso)
 {
+    extern __shared__ double Wl[];                          // [nk] weight sums of branch l2
+    float *ens = reinterpret_cast<float *>(Wl + nk);        // [nk] energies of branch l2
     __shared__ double e1[2 * 64];
     __shared__ double ga[32];
-    __shared__ double red[4];
+    __shared__ double red[kIsoThreads / 64];
+    __shared__ int s_cnt;
+    __shared__ unsigned s_list[kIsoList];   // cube << 3 | tetrahedron
     const int s = blockIdx.x / nb, l1 = blockIdx.x % nb, tid = threadIdx.x;
     const int k1 = mapinv[s];
     const double omega1 = freq[(size_t)k1 * nb + l1];
     for (int x = tid; x < 2 * nb; x += blockDim.x) e1[x] = evec[2 * ((size_t)k1 * nb + l1) * nb + x];
     for (int x = tid; x < natoms0; x += blockDim.x) ga[x] = giso[x];
-    __syncthreads();
+    for (int w = tid; w < nk; w += blockDim.x) Wl[w] = 0.0;
     double acc = 0;
-    for (int l2 = 0; l2 < nb; l2++)
-        for (int k = tid; k < nk; k += blockDim.x) {
-            int c0, c1, c2;
-            m.coords(k, c0, c1, c2);
-            const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
-                      d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
-            const int ci[8] = {m.index(c0, c1, c2), m.index(d0, c1, c2), m.index(c0, d1, c2), m.index(d0, d1, c2),
-                               m.index(c0, c1, d2), m.index(d0, c1, d2), m.index(c0, d1, d2), m.index(d0, d1, d2)};
-            float ce[8], lo = 3.4e38f, hi = -3.4e38f;
-            for (int p = 0; p < 8; p++) {
-                ce[p] = (float)freq[(size_t)ci[p] * nb + l2];
+    constexpr int kChunk = kIsoList / 6;
+    // tetrahedra of cubes [c0, c1) whose energy range contains w1 onto the list (entries past
+    // its end are counted but not stored)
+    auto scan = [&](int c0, int c1) {
+        for (int k = c0 + tid; k < c1; k += blockDim.x) {
+            int a0, a1, a2;
+            m.coords(k, a0, a1, a2);
+            const int d0 = (a0 + 1 == m.n0) ? 0 : a0 + 1, d1 = (a1 + 1 == m.n1) ? 0 : a1 + 1,
+                      d2 = (a2 + 1 == m.n2) ? 0 : a2 + 1;
+            const float ce[8] = {ens[m.index(a0, a1, a2)], ens[m.index(d0, a1, a2)], ens[m.index(a0, d1, a2)],
+                                 ens[m.index(d0, d1, a2)], ens[m.index(a0, a1, d2)], ens[m.index(d0, a1, d2)],
+                                 ens[m.index(a0, d1, d2)], ens[m.index(d0, d1, d2)]};
+            float lo = ce[0], hi = ce[0];
+            for (int p = 1; p < 8; p++) {
                 lo = fminf(lo, ce[p]);
                 hi = fmaxf(hi, ce[p]);
             }
             if (!omega_in(omega1, lo, hi)) continue;
             for (int l = 0; l < 6; l++) {
-                const float w4[4] = {ce[kTc[l][0]], ce[kTc[l][1]], ce[kTc[l][2]], ce[kTc[l][3]]};
-                if (!omega_in(omega1, fminf(fminf(w4[0], w4[1]), fminf(w4[2], w4[3])),
-                              fmaxf(fmaxf(w4[0], w4[1]), fmaxf(w4[2], w4[3]))))
-                    continue;
-                double c[4];
-                weight_tet1(w4, omega1, c);
-                for (int y = 0; y < 4; y++) {
-                    if (c[y] == 0.0) continue;
-                    const double *e2 = evec + 2 * ((size_t)ci[kTc[l][y]] * nb + l2) * nb;
-                    double gs = 0;
-                    for (int a = 0; a < natoms0; a++) {
-                        double re = 0, im = 0;
-                        for (int x = 3 * a; x < 3 * a + 3; x++) {
-                            const double ar = e1[2 * x], ai = e1[2 * x + 1], br = e2[2 * x], bi = e2[2 * x + 1];
-                            re = re + (ar * br + ai * bi);
-                            im = im + (ar * bi - ai * br);
-                        }
-                        gs = gs + ga[a] * (re * re + im * im);
-                    }
-                    acc = acc + gs * c[y] / 6.0;
+                const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
+                if (omega_in(omega1, fminf(fminf(a, b), fminf(c, d)), fmaxf(fmaxf(a, b), fmaxf(c, d)))) {
+                    const int x = atomicAdd(&s_cnt, 1);
+                    if (x < kIsoList) s_list[x] = ((unsigned)k << 3) | (unsigned)l;
                 }
             }
         }
+    };
+    // weight_tet of the n listed tetrahedra, c/6 added to W at their corners
+    auto process = [&](int n) {
+        for (int x = tid; x < n; x += blockDim.x) {
+            const int k = (int)(s_list[x] >> 3), l = (int)(s_list[x] & 7u);
+            int a0, a1, a2;
+            m.coords(k, a0, a1, a2);
+            int v[4];
+            for (int y = 0; y < 4; y++) {
+                const int p = kTc[l][y];
+                int b0 = a0 + (p & 1), b1 = a1 + ((p >> 1) & 1), b2 = a2 + ((p >> 2) & 1);
+                if (b0 == m.n0) b0 = 0;
+                if (b1 == m.n1) b1 = 0;
+                if (b2 == m.n2) b2 = 0;
+                v[y] = m.index(b0, b1, b2);
+            }
+            const float w4[4] = {ens[v[0]], ens[v[1]], ens[v[2]], ens[v[3]]};
+            double c[4];
+            weight_tet1(w4, omega1, c);
+            for (int y = 0; y < 4; y++)
+                if (c[y] != 0.0) atomicAdd(&Wl[v[y]], c[y] / 6.0);
+        }
+    };
+    for (int l2 = 0; l2 < nb; l2++) {
+        __syncthreads();   // the previous branch's energies and weights are no longer read
+        for (int w = tid; w < nk; w += blockDim.x) ens[w] = enT[(size_t)l2 * nk + w];
+        // all cubes at once when their active tetrahedra fit the list (the usual case), else
+        // chunk by chunk (kIsoList / 6 cubes cannot overflow it); the branch is uniform
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        scan(0, nk);
+        __syncthreads();
+        const int nall = s_cnt;
+        if (nall <= kIsoList) {
+            process(nall);
+        } else {
+            for (int c0 = 0; c0 < nk; c0 += kChunk) {
+                __syncthreads();
+                if (tid == 0) s_cnt = 0;
+                __syncthreads();
+                scan(c0, min(nk, c0 + kChunk));
+                __syncthreads();
+                process(s_cnt);
+            }
+        }
+        __syncthreads();   // W is complete
+        // gs at the mesh points with a weight; the weights are cleared for the next branch
+        for (int w = tid; w < nk; w += blockDim.x) {
+            const double W = Wl[w];
+            if (W == 0.0) continue;
+            Wl[w] = 0.0;
+            const double *e2 = evec + 2 * ((size_t)w * nb + l2) * nb;
+            double gs = 0;
+            for (int at = 0; at < natoms0; at++) {
+                double re = 0, im = 0;
+                for (int x3 = 3 * at; x3 < 3 * at + 3; x3++) {
+                    const double ar = e1[2 * x3], ai = e1[2 * x3 + 1], br = e2[2 * x3], bi = e2[2 * x3 + 1];
+                    re = re + (ar * br + ai * bi);
+                    im = im + (ar * bi - ai * br);
+                }
+                gs = gs + ga[at] * (re * re + im * im);
+            }
+            acc = acc + gs * W;
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = acc;
     __syncthreads();
@@ -656,8 +729,18 @@ int piso_run(alt3_ctx *ctx)
     ALT3_TRY(ctx->d_piso.alloc((size_t)nibz * nb));
     ALT3_HIP(hipMemcpyAsync(ctx->d_giso.p, ctx->giso.data(), sizeof(double) * ctx->giso.size(), hipMemcpyHostToDevice,
                             ctx->stream));
-    hipLaunchKernelGGL(k_piso, dim3((unsigned)(nibz * nb)), dim3(256), 0, ctx->stream, MeshDev::make(ctx->nc), ctx->nk,
-                       nb, ctx->natoms0, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_evec.p, ctx->d_giso.p, ctx->d_piso.p);
+    const size_t lds = (size_t)ctx->nk * (sizeof(double) + sizeof(float));
+    if (lds + kIsoList * sizeof(unsigned) + 2048 > ctx->lds_max)
+        return fail(ALT3_E_ARG, "isotope term: the mesh energies do not fit in LDS (nk too large)");
+    if (lds + kIsoList * sizeof(unsigned) + 2048 > 65536)
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_piso, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DBuf<double> &enT = ctx->d_red;   // reduction scratch, nk*nb floats here
+    ALT3_TRY(enT.reserve(((size_t)ctx->nk * nb + 1) / 2));
+    hipLaunchKernelGGL(k_freq_f32t, dim3((unsigned)((ctx->nk * nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->nk, nb,
+                       ctx->d_freq.p, reinterpret_cast<float *>(enT.p));
+    hipLaunchKernelGGL(k_piso, dim3((unsigned)(nibz * nb)), dim3(kIsoThreads), lds, ctx->stream, MeshDev::make(ctx->nc),
+                       ctx->nk, nb, ctx->natoms0, ctx->d_mapinv.p, ctx->d_freq.p,
+                       reinterpret_cast<const float *>(enT.p), ctx->d_evec.p, ctx->d_giso.p, ctx->d_piso.p);
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->piso_ok = true;

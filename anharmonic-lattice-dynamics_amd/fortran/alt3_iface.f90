@@ -122,6 +122,17 @@ module alt3_iface
        real(c_double), value :: tempk
      end function alt3_update
 
+     integer(c_int) function alt3_set_iso(ctx, giso, piso) bind(C, name='alt3_set_iso')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, giso, piso
+     end function alt3_set_iso
+
+     integer(c_int) function alt3_isotope_g(element, g) bind(C, name='alt3_isotope_g')
+       import :: c_int, c_char, c_double
+       character(kind=c_char), dimension(*) :: element
+       real(c_double) :: g
+     end function alt3_isotope_g
+
      integer(c_int) function alt3_solve(ctx, ntemps, temps, precision, kappa_rta, kappa_bte, cg_iters, stage_ms) &
           bind(C, name='alt3_solve')
        import :: c_int, c_ptr

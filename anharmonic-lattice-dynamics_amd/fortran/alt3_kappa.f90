@@ -10,7 +10,9 @@
 !    job 4: CM-1D.dat, RHS_IBZ.dat, DS.dat (the CG system, kappa.f90:2577-2625) after the RTA
 !    job 5: reads those three, CG and kappa_BTE as job 1 (kappa.f90:2635-2703)
 !  The job is the one latdyn.params names (3, 4, 5, else 1); the temperature is tmin
-!  (kappa.f90:565).
+!  (kappa.f90:565).  iso = 1 in latdyn.params adds isotope scattering (cal_giso, P_iso_split,
+!  calculate_tauRTA_iso, CollisionMatrix_iso, update_iso; kappa.f90:522-543, :574-578,
+!  :710-716, :1401-1407).
 !
 !  usage: alt3_kappa [fp64 | mixed | fp64ref] [--host-only]
 !    --host-only builds the mesh and symmetry tables only (no GPU) and prints their sizes.
@@ -31,6 +33,8 @@ program alt3_kappa
   real(8) :: scale
   integer :: natom_type, natoms0
   real(8), allocatable :: mas(:)
+  character(len=2), allocatable :: atname(:)
+  real(c_double), allocatable, target :: giso(:)
   integer, allocatable :: atom_type(:)
   real(c_double), allocatable, target :: mass(:)
   ! lat_fc.dat
@@ -118,6 +122,16 @@ program alt3_kappa
   call alt3_check(alt3_get_mesh(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(mapinv), c_loc(kibz), &
        c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'alt3_get_mesh')
   tempk = tmin                                     ! kappa.f90:565, :2649
+  if (iso == 1) then
+     ! cal_giso and P_iso_split (kappa.f90:522-543): g of each atom from its element, the
+     ! isotope term on the device; tau, collision and update then take their iso forms
+     allocate(giso(natoms0))
+     do i = 1, natoms0
+        call alt3_check(alt3_isotope_g(trim(atname(atom_type(i)))//c_null_char, giso(i)), 'alt3_isotope_g')
+     end do
+     call alt3_check(alt3_set_iso(ctx, c_loc(giso), c_null_ptr), 'alt3_set_iso')
+     write(*, '(a,99es12.4)') 'alt3_kappa: iso = 1, g per atom ', giso
+  end if
 
   if (job == 5) then
      ! read_CM_1D_mpi, read_RHS, read_DS (sub.f90:5401-5470), then cg_quadratic and
@@ -271,7 +285,8 @@ contains
     read(u, *) natom_type
     allocate(mas(natom_type))
     read(u, *) mas
-    read(u, '(a)') junk                          ! element names
+    allocate(atname(natom_type))
+    read(u, *) atname                            ! element names
     read(u, *) natoms0
     do k = 1, 8
        read(u, '(a)') junk

@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--temps", default="100,300,800")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-iter-bte", action="store_true", help="skip the iterative BTE (job 3) line")
+    ap.add_argument("--no-iso", action="store_true", help="skip the isotope-scattering (iso = 1) line")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving", "selfenergy"], default="step",
                     help="step: the SW-Si 16^3 headline step (default); rocksalt: the 8-atom rock-salt-type "
@@ -121,6 +122,31 @@ def timed_iter(ctx, prec, T, quiet):
     return {"tempk": T, "iterations": r["iters"], "ms_first_call": times[0], "ms": times[-1],
             "ms_per_iteration": times[-1] / max(r["iters"], 1), "kappa": kappa_iso(r["kappa"]),
             "note": "kappa.f90 job 3: update_mpi as a device matrix, mixing 0.4, stop at normk < 1e-4"}
+
+
+def timed_iso(ctx, prec, temps, giso, quiet):
+    """Isotope scattering (iso = 1) on the same workload: the P_iso_split kernel alone (its
+    HIP-event time), then whole steps with the isotope term in tau, collision and CG."""
+    ctx.eig(want=False)
+    ctx.set_iso(giso, want=False)
+    t0 = time.perf_counter()
+    ctx.set_iso(giso)                 # forms Piso now (the eigen data are current)
+    piso_wall = (time.perf_counter() - t0) * 1e3
+    piso_ms = ctx.last_ms()
+    lat, out = [], None
+    for it in range(3):
+        t0 = time.perf_counter()
+        ctx.eig(want=False)
+        out = ctx.solve(temps, prec)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    ctx.set_iso(None)
+    ti = min(1, len(temps) - 1)
+    log(f"iso: Piso kernel {piso_ms:.3f} ms, step with iso {min(lat[1:]):.2f} ms", quiet)
+    return {"g_per_atom": [float(x) for x in giso], "piso_kernel_ms": piso_ms, "set_iso_wall_ms": piso_wall,
+            "ms_per_step": sum(lat[1:]) / len(lat[1:]),
+            "kappa_300K": {"rta": kappa_iso(out["kappa_rta"][ti]), "bte": kappa_iso(out["kappa_bte"][ti])},
+            "note": "iso = 1: P_iso_split on the device (natural Si g), calculate_tauRTA_iso, CollisionMatrix_iso; "
+                    "the step recomputes Piso after its eigen solve"}
 
 
 def profiled_traffic(kernel_prefix: str):
@@ -214,6 +240,7 @@ def main():
     head = runs[head_name]
     ti = min(1, len(temps) - 1)
     iter_bte = None if args.no_iter_bte else timed_iter(ctx, precs[0][1], temps[ti], args.quiet)
+    iso = None if args.no_iso else timed_iso(ctx, precs[0][1], temps, capi.giso_of(inp), args.quiet)
     ctx.close()
 
     def summary(r):
@@ -249,6 +276,8 @@ def main():
         res[name] = summary(r)
     if iter_bte is not None:
         res["iter_bte"] = iter_bte
+    if iso is not None:
+        res["iso"] = iso
     if "mixed" in runs and "fp64" in runs:
         m, f = runs["mixed"], runs["fp64"]
         errs = {}
