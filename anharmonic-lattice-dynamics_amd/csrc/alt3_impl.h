@@ -221,4 +221,5 @@ struct alt3_ctx {
     alt3::DBuf<double> d_qpart, d_wq, d_ev7, d_evec0;
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
     alt3::DBuf<double> d_red;                      // reduction scratch
+    alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
 };

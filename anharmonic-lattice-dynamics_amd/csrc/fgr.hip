@@ -204,6 +204,15 @@ __device__ __forceinline__ float pick(const float (&a)[N], int j)
 // sums W are accumulated (FP64 on both paths: LDS ds_add_f32 measured slower than ds_add_f64
 // here) and then written out as P = W * F.
 constexpr int kFgrThreads = 512;   // 8 waves; LDS (12 B per mesh point) allows 2 groups per CU
+// the frequencies branch-major [l][q], so that the energy gathers of a wave are contiguous
+__global__ void k_freq_t(int nk, int nb, const double *__restrict__ freq, double *__restrict__ freqT)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;   // (l, q)
+    if (x >= nk * nb) return;
+    const int l = x / nk, q = x - l * nk;
+    freqT[x] = freq[(size_t)q * nb + l];
+}
+
 constexpr int kFgrCase = 4096;     // LDS list per weight_tet case; one chunk of cubes (<= 6*512) always fits
 constexpr int kFgrList = 3 * kFgrCase;
 
@@ -213,6 +222,7 @@ __device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return 
 template <typename VT, int kFgrPer>   // kFgrPer: mesh points (and cubes) per thread
 __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
+                                                     const double *__restrict__ freqT,
                                                      const VT *__restrict__ v33, VT *__restrict__ P1,
                                                      VT *__restrict__ P2)
 {
@@ -222,8 +232,13 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
     __shared__ int s_cnt[3];
     __shared__ unsigned short s_list[kFgrList];  // [case][kFgrCase] of (cube << 3 | tetrahedron), nk <= 8192
-    const int proc = blockIdx.x & 1;
-    int r = blockIdx.x >> 1;
+    // The two processes of one (s, l2, l3) read the same |V3|^2 rows.  Workgroups are dealt to
+    // the eight XCDs round robin, so the pair is placed 8 apart (blocks b and b+8 of each run of
+    // 16 share an XCD and its L2) and the second read hits the cache; the tail of the grid that
+    // does not fill a run of 16 keeps the plain pairing.
+    const int b = blockIdx.x, full = (int)(gridDim.x & ~15u);
+    const int proc = b < full ? (b >> 3) & 1 : b & 1;
+    int r = b < full ? ((b >> 4) << 3) | (b & 7) : b >> 1;
     const int l3 = r % nb;
     r /= nb;
     const int l2 = r % nb;
@@ -254,10 +269,10 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             double e;
             if (proc == 0) {  // -w(q2,l2) + w(q1+q2,l3)
                 p = m.sum(a0, a1, a2, c0, c1, c2);
-                e = -freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+                e = -freqT[(size_t)l2 * nk + w] + freqT[(size_t)l3 * nk + p];
             } else {          // w(q2,l2) + w(q1-q2,l3)
                 p = m.diff(a0, a1, a2, c0, c1, c2);
-                e = freq[(size_t)w * nb + l2] + freq[(size_t)p * nb + l3];
+                e = freqT[(size_t)l2 * nk + w] + freqT[(size_t)l3 * nk + p];
             }
             if ((l2 < 3 && w == gam) || (l3 < 3 && p == gam)) vzero |= 1u << j;
             if ((l3 < 3 && w == gam) || (l2 < 3 && p == gam)) vzerom |= 1u << j;
@@ -540,6 +555,9 @@ int fgr_run(alt3_ctx *ctx)
         return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4608");
     ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
+    ALT3_TRY(ctx->d_freqT.reserve((size_t)nk * nb));
+    hipLaunchKernelGGL(k_freq_t, dim3((unsigned)((nk * nb + 255) / 256)), dim3(256), 0, ctx->stream, nk, nb, ctx->d_freq.p,
+                       ctx->d_freqT.p);
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
@@ -551,10 +569,12 @@ int fgr_run(alt3_ctx *ctx)
     }                                                                                                            \
     if (mixed)                                                                                                   \
         hipLaunchKernelGGL((k_fgr<float, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
-                           ctx->d_mapinv.p, ctx->d_freq.p, (const float *)v, ctx->d_p1f.p, ctx->d_p2f.p);        \
+                           ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const float *)v, ctx->d_p1f.p,         \
+                           ctx->d_p2f.p);                                                                        \
     else                                                                                                         \
         hipLaunchKernelGGL((k_fgr<double, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,            \
-                           ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, (const double *)v, ctx->d_p1.p, ctx->d_p2.p);
+                           ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const double *)v, ctx->d_p1.p, \
+                           ctx->d_p2.p);
     if (per <= 2) {
         ALT3_FGR_LAUNCH(2)
     } else if (per <= 4) {
