@@ -124,6 +124,7 @@ int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, c
 bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
+int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
 
 }  // namespace alt3
 
@@ -222,4 +223,5 @@ struct alt3_ctx {
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
     alt3::DBuf<double> d_red;                      // reduction scratch
     alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
+    alt3::DBuf<double> d_solve_x, d_solve_te;      // alt3_solve: CG vector, tauinv_eff
 };

@@ -1195,6 +1195,17 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
     return ALT3_OK;
 }
 
+// x (device, vector layout) = F_RTA of the last alt3_kappa_rta, the CG start of job 1
+// (kappa.f90:835-843), and te (device) = its tauinv_eff, both on the context stream
+int cg_start_launch(alt3_ctx *ctx, double *x, double *te)
+{
+    const int nibz = ctx->nibz, nb = ctx->nb, N = nibz * nb * 3;
+    hipLaunchKernelGGL(k_frta_to_vec, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nb, ctx->d_frta.p, x);
+    ALT3_HIP(hipGetLastError());
+    ALT3_HIP(hipMemcpyAsync(te, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToDevice, ctx->stream));
+    return ALT3_OK;
+}
+
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev)
 {
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;

@@ -516,9 +516,9 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         ALT3_TRY(stage(2, [&] {
             return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
         }));
-    DBuf<double> dx, dte;
-    ALT3_TRY(dx.alloc(n));
-    ALT3_TRY(dte.alloc((size_t)nibz * nb));
+    DBuf<double> &dx = ctx->d_solve_x, &dte = ctx->d_solve_te;
+    ALT3_TRY(dx.reserve(n));
+    ALT3_TRY(dte.reserve((size_t)nibz * nb));
     for (int it = 0; it < ntemps; it++) {
         double T = temps[it];
         // one tau pass (one read of P1sm/P2sm) per group of four temperatures
@@ -528,14 +528,8 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         }
         ALT3_TRY(stage(4, [&] { return kappa_rta_run(ctx, T, kappa_rta ? kappa_rta + (size_t)it * nb * 9 : nullptr, nullptr, nullptr); }));
         ALT3_TRY(stage(5, [&] { return collision_run(ctx, T, nullptr, nullptr); }));
-        // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843)
-        std::vector<double> fr((size_t)nibz * nb * 3), x0(n);
-        ALT3_HIP(hipMemcpy(fr.data(), ctx->d_frta.p, sizeof(double) * fr.size(), hipMemcpyDeviceToHost));
-        for (int i = 0; i < nibz; i++)
-            for (int l = 0; l < nb; l++)
-                for (int a = 0; a < 3; a++) x0[3 * ((size_t)i * nb + l) + a] = fr[i + (size_t)nibz * (l + (size_t)nb * a)];
-        ALT3_HIP(hipMemcpy(dx.p, x0.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-        ALT3_HIP(hipMemcpy(dte.p, ctx->d_taueff.p, sizeof(double) * nibz * nb, hipMemcpyDeviceToDevice));
+        // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843), on the device
+        ALT3_TRY(cg_start_launch(ctx, dx.p, dte.p));
         int its = 0;
         ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its, true); }));
         if (cg_iters) cg_iters[it] = its;
