@@ -224,4 +224,6 @@ struct alt3_ctx {
     alt3::DBuf<double> d_red;                      // reduction scratch
     alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
     alt3::DBuf<double> d_solve_x, d_solve_te;      // alt3_solve: CG vector, tauinv_eff
+    alt3::DBuf<double> d_solve_k;                  // alt3_solve: kappa_RTA, kappa_BTE per temperature
+    std::vector<hipEvent_t> evpool;                // alt3_solve: stage events
 };

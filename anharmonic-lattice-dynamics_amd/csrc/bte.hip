@@ -935,7 +935,9 @@ int upload_bte_tables(alt3_ctx *ctx)
 // Q, QN, QU and the l3-reduced R1/R2 for each of nt temperatures, P1sm/P2sm read once per
 // group of up to four temperatures.  Temperature t lands in slot t of the per-temperature
 // arrays; slot 0 is selected afterwards.
-int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
+// sync = false (alt3_solve): no wait at the end; an N_BE error stays in the device flag, which
+// the caller cleared before and reads after its last stage
+int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
 {
     if (!ctx->have_fgr) return fail(ALT3_E_STATE, "alt3_tau needs alt3_fgr first");
     if (nt < 1) return fail(ALT3_E_ARG, "no temperature");
@@ -947,7 +949,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
     ctx->tau_temps.clear();
     ALT3_TRY(piso_run(ctx));
     int *derr = ctx->d_flags.p + 1;
-    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
+    if (sync) ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     const size_t dstr = (size_t)nk * nb, qstr = (size_t)ksub * nb, rstr = (size_t)ksub * nb * nb * nk;
     ALT3_TRY(ctx->d_dist.alloc(nt * dstr));
     ALT3_TRY(ctx->d_q.alloc(nt * qstr));
@@ -1016,10 +1018,12 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt)
                            ctx->ibz_lo, qpart.p + t * pstr, piso, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr,
                            ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());
-    int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    ALT3_HIP(hipStreamSynchronize(ctx->stream));
-    if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
+    if (sync) {
+        int herr = 0;
+        ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        if (herr) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
+    }
     ctx->tau_temps.assign(temps, temps + nt);
     ctx->tau_cur = 0;
     ctx->tempk = temps[0];
@@ -1043,7 +1047,7 @@ int tau_select(alt3_ctx *ctx, double tempk, const char *who)
 
 int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT)
 {
-    ALT3_TRY(tau_multi_run(ctx, &tempk, 1));
+    ALT3_TRY(tau_multi_run(ctx, &tempk, 1, true));
     const int nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     if (Q || QN || QU || tN || tU || tT) {
         DBuf<double> o;
@@ -1069,7 +1073,7 @@ static int need_full_ibz(alt3_ctx *ctx, double tempk, const char *who)
     return ALT3_OK;
 }
 
-int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff)
+int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff, bool sync)
 {
     ALT3_TRY(need_full_ibz(ctx, tempk, "alt3_kappa_rta"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;
@@ -1078,7 +1082,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
     ALT3_TRY(ctx->d_taueff.alloc((size_t)nibz * nb));
     int *derr = ctx->d_flags.p + 2;
-    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
+    if (sync) ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     double const1 = H_PLANK * C_LIGHT * 100 * C_LIGHT / (nk * ctx->T.volume_r / 1e30);
     double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     hipLaunchKernelGGL(k_kappa_rta, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb, nk,
@@ -1088,6 +1092,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p, nullptr);
     ALT3_HIP(hipGetLastError());
+    if (!sync) return ALT3_OK;   // outputs stay on the device (d_kappa, d_frta, d_taueff)
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
@@ -1098,7 +1103,7 @@ int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, doub
     return ALT3_OK;
 }
 
-int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
+int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool sync)
 {
     ALT3_TRY(tau_select(ctx, tempk, "alt3_collision"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
@@ -1137,7 +1142,7 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
         ALT3_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (RHS) ALT3_HIP(hipMemcpyAsync(RHS, ctx->d_rhs.p, sizeof(double) * nibz * nb * 3, hipMemcpyDeviceToHost, ctx->stream));
-    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (sync || RHS) ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_cm = true;
     return ALT3_OK;
 }
@@ -1206,7 +1211,7 @@ int cg_start_launch(alt3_ctx *ctx, double *x, double *te)
     return ALT3_OK;
 }
 
-int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev)
+int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev, bool sync)
 {
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk;
     for (int j = 0; j < nibz; j++)
@@ -1217,7 +1222,7 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
     ALT3_TRY(ctx->d_kpart.alloc((size_t)nibz * nb * 9));
     ALT3_TRY(ctx->d_kappa.alloc((size_t)nb * 9));
     int *derr = ctx->d_flags.p + 3;
-    ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
+    if (sync) ALT3_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx->stream));
     if (!ctx->have_veloci) {   // star-averaged IBZ velocities depend on the eigen data only
         hipLaunchKernelGGL(k_veloci, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb, ctx->d_vel.p,
                            ctx->d_fstar_n.p, ctx->d_fstar_op.p, ctx->d_fstar_n0.p, ctx->d_opk_inv.p, ctx->d_veloci.p);
@@ -1230,6 +1235,7 @@ int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, 
                        ctx->d_star_op.p, ctx->d_star_n0.p, ctx->d_opk.p, ctx->d_kpart.p, taueff_dev, derr);
     hipLaunchKernelGGL(k_kappa_sum, dim3(nb * 9), dim3(64), 0, ctx->stream, nibz, nb, ctx->d_kpart.p, ctx->d_kappa.p, nullptr);
     ALT3_HIP(hipGetLastError());
+    if (!sync) return ALT3_OK;   // kappa stays in d_kappa
     int herr = 0;
     ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     if (kappa) ALT3_HIP(hipMemcpyAsync(kappa, ctx->d_kappa.p, sizeof(double) * nb * 9, hipMemcpyDeviceToHost, ctx->stream));
@@ -1333,7 +1339,7 @@ int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double
 {
     if (max_iter < 1) return fail(ALT3_E_ARG, "max_iter must be at least 1");
     ALT3_TRY(need_full_ibz(ctx, tempk, "alt3_kappa_iter"));
-    ALT3_TRY(kappa_rta_run(ctx, tempk, nullptr, nullptr, nullptr));   // F_RTA, kappa_RTA = first kappa_old
+    ALT3_TRY(kappa_rta_run(ctx, tempk, nullptr, nullptr, nullptr, true));   // F_RTA, kappa_RTA = first kappa_old
     ALT3_TRY(update_build(ctx, tempk, "alt3_kappa_iter"));
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, N = nibz * nb * 3;
     ALT3_TRY(ctx->d_iter.reserve(2 * (size_t)N + (9 * (size_t)nb + 1) * (1 + (size_t)max_iter)));

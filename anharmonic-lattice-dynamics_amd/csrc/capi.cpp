@@ -26,12 +26,12 @@ int fgr_run(alt3_ctx *ctx);
 int fgr_gauss_run(alt3_ctx *ctx, double sigma);
 int upload_bte_tables(alt3_ctx *ctx);
 int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT);
-int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt);
-int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff);
-int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
+int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync);
+int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff, bool sync);
+int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool sync);
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
               double *funcmin, int *iters, bool rowmajor);
-int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev);
+int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev, bool sync);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
 int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev);
 int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, double *omega0,
@@ -159,6 +159,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
+    for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -367,7 +368,7 @@ int alt3_kappa_rta(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, dou
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(kappa_rta_run(ctx, tempk, kappa, FRTA, taueff));
+    ALT3_TRY(kappa_rta_run(ctx, tempk, kappa, FRTA, taueff, true));
     tm.stop();
     return ALT3_OK;
 }
@@ -377,7 +378,7 @@ int alt3_collision(alt3_ctx *ctx, double tempk, double *CM, double *RHS)
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(collision_run(ctx, tempk, CM, RHS));
+    ALT3_TRY(collision_run(ctx, tempk, CM, RHS, true));
     tm.stop();
     return ALT3_OK;
 }
@@ -427,7 +428,7 @@ int alt3_kappa_bte(alt3_ctx *ctx, double tempk, const double *x, double *kappa, 
     else
         ALT3_HIP(hipMemset(dte.p, 0, sizeof(double) * ctx->nibz * ctx->nb));
     Timer tm(ctx);
-    ALT3_TRY(kappa_bte_run(ctx, tempk, dx.p, kappa, dte.p));
+    ALT3_TRY(kappa_bte_run(ctx, tempk, dx.p, kappa, dte.p, true));
     tm.stop();
     if (taueff) ALT3_HIP(hipMemcpy(taueff, dte.p, sizeof(double) * ctx->nibz * ctx->nb, hipMemcpyDeviceToHost));
     return ALT3_OK;
@@ -498,11 +499,31 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int nb = ctx->nb, nibz = ctx->nibz, n = nibz * nb * 3;
+    // Stage times come from events recorded on the stream and read once the last stage is
+    // done, so the temperature loop runs without waiting on the host (only CG looks at its
+    // convergence flag); the N_BE flags of tau, kappa_RTA and kappa_BTE are read at the end.
+    struct Mark {
+        int slot;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    auto event = [&](size_t i, hipEvent_t *e) -> int {
+        while (ctx->evpool.size() <= i) {
+            hipEvent_t x;
+            ALT3_HIP(hipEventCreate(&x));
+            ctx->evpool.push_back(x);
+        }
+        *e = ctx->evpool[i];
+        return ALT3_OK;
+    };
     auto stage = [&](int slot, auto &&fn) -> int {
-        Timer tm(ctx);
-        int rc = fn();
-        tm.stop();
-        ms[slot] += ctx->last_ms;
+        Mark m{slot, nullptr, nullptr};
+        ALT3_TRY(event(2 * marks.size(), &m.a));
+        ALT3_TRY(event(2 * marks.size() + 1, &m.b));
+        ALT3_HIP(hipEventRecord(m.a, ctx->stream));
+        const int rc = fn();
+        ALT3_HIP(hipEventRecord(m.b, ctx->stream));
+        marks.push_back(m);
         return rc;
     };
     if (!ctx->have_eig) ALT3_TRY(stage(0, [&] { return eig_run(ctx); }));
@@ -516,25 +537,50 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         ALT3_TRY(stage(2, [&] {
             return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
         }));
-    DBuf<double> &dx = ctx->d_solve_x, &dte = ctx->d_solve_te;
+    DBuf<double> &dx = ctx->d_solve_x, &dte = ctx->d_solve_te, &dk = ctx->d_solve_k;
     ALT3_TRY(dx.reserve(n));
     ALT3_TRY(dte.reserve((size_t)nibz * nb));
+    ALT3_TRY(dk.reserve((size_t)2 * ntemps * nb * 9));
+    ALT3_HIP(hipMemsetAsync(ctx->d_flags.p + 1, 0, 3 * sizeof(int), ctx->stream));   // N_BE flags
+    const size_t kb = (size_t)nb * 9 * sizeof(double);
     for (int it = 0; it < ntemps; it++) {
         double T = temps[it];
         // one tau pass (one read of P1sm/P2sm) per group of four temperatures
         if (it % 4 == 0) {
             const int g = std::min(4, ntemps - it);
-            ALT3_TRY(stage(3, [&] { return tau_multi_run(ctx, temps + it, g); }));
+            ALT3_TRY(stage(3, [&] { return tau_multi_run(ctx, temps + it, g, false); }));
         }
-        ALT3_TRY(stage(4, [&] { return kappa_rta_run(ctx, T, kappa_rta ? kappa_rta + (size_t)it * nb * 9 : nullptr, nullptr, nullptr); }));
-        ALT3_TRY(stage(5, [&] { return collision_run(ctx, T, nullptr, nullptr); }));
+        ALT3_TRY(stage(4, [&] { return kappa_rta_run(ctx, T, nullptr, nullptr, nullptr, false); }));
+        ALT3_HIP(hipMemcpyAsync(dk.p + (size_t)2 * it * nb * 9, ctx->d_kappa.p, kb, hipMemcpyDeviceToDevice, ctx->stream));
+        ALT3_TRY(stage(5, [&] { return collision_run(ctx, T, nullptr, nullptr, false); }));
         // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843), on the device
         ALT3_TRY(cg_start_launch(ctx, dx.p, dte.p));
         int its = 0;
         ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its, true); }));
         if (cg_iters) cg_iters[it] = its;
-        ALT3_TRY(stage(7, [&] { return kappa_bte_run(ctx, T, dx.p, kappa_bte ? kappa_bte + (size_t)it * nb * 9 : nullptr, dte.p); }));
+        ALT3_TRY(stage(7, [&] { return kappa_bte_run(ctx, T, dx.p, nullptr, dte.p, false); }));
+        ALT3_HIP(hipMemcpyAsync(dk.p + (size_t)(2 * it + 1) * nb * 9, ctx->d_kappa.p, kb, hipMemcpyDeviceToDevice,
+                                ctx->stream));
     }
+    int flags[3] = {0, 0, 0};
+    ALT3_HIP(hipMemcpyAsync(flags, ctx->d_flags.p + 1, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (flags[0]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
+    if (flags[1] || flags[2]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    std::vector<double> hk((size_t)2 * ntemps * nb * 9);
+    ALT3_HIP(hipMemcpy(hk.data(), dk.p, sizeof(double) * hk.size(), hipMemcpyDeviceToHost));
+    for (int it = 0; it < ntemps; it++) {
+        if (kappa_rta) std::memcpy(kappa_rta + (size_t)it * nb * 9, hk.data() + (size_t)2 * it * nb * 9, kb);
+        if (kappa_bte) std::memcpy(kappa_bte + (size_t)it * nb * 9, hk.data() + (size_t)(2 * it + 1) * nb * 9, kb);
+    }
+    double total = 0;
+    for (const Mark &m : marks) {
+        float t = 0;
+        ALT3_HIP(hipEventElapsedTime(&t, m.a, m.b));
+        ms[m.slot] += t;
+        total += t;
+    }
+    ctx->last_ms = total;
     if (stage_ms) std::memcpy(stage_ms, ms, sizeof(ms));
     return ALT3_OK;
 }
