@@ -18,11 +18,11 @@ int fail(int code, const std::string &msg)
 int eig_prepare(alt3_ctx *ctx, const alt3_system *sys);
 int eig_run(alt3_ctx *ctx);
 int v3_prepare(alt3_ctx *ctx);
-int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec);
+int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync);
 int v3_copy_out(alt3_ctx *ctx, double *host);
 template <typename T>
 int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host);
-int fgr_run(alt3_ctx *ctx);
+int fgr_run(alt3_ctx *ctx, bool sync);
 int fgr_gauss_run(alt3_ctx *ctx, double sigma);
 int upload_bte_tables(alt3_ctx *ctx);
 int tau_run(alt3_ctx *ctx, double tempk, double *Q, double *QN, double *QU, double *tN, double *tU, double *tT);
@@ -291,7 +291,7 @@ int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(v3_run(ctx, ibz_lo, ibz_hi, precision));
+    ALT3_TRY(v3_run(ctx, ibz_lo, ibz_hi, precision, true));
     tm.stop();
     if (v33s8) ALT3_TRY(v3_copy_out(ctx, v33s8));
     return ALT3_OK;
@@ -302,7 +302,7 @@ int alt3_fgr(alt3_ctx *ctx, double *P1sm, double *P2sm)
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(fgr_run(ctx));
+    ALT3_TRY(fgr_run(ctx, true));
     tm.stop();
     ALT3_TRY(copy_p(ctx, P1sm, P2sm));
     return ALT3_OK;
@@ -530,12 +530,15 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     // |V3|^2 and the scattering weights do not depend on temperature: a context that still
     // holds them for the whole IBZ at this precision and delta integration reuses them
     const bool v3_held = ctx->have_v3 && ctx->v3_req == precision && ctx->ibz_lo == 0 && ctx->ibz_hi == nibz - 1;
-    if (!v3_held) ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision); }));
+    if (!v3_held) {
+        ALT3_HIP(hipMemsetAsync(ctx->d_flags.p, 0, sizeof(int), ctx->stream));   // den = 0 flag
+        ALT3_TRY(stage(1, [&] { return v3_run(ctx, 0, nibz - 1, precision, false); }));
+    }
     const bool fgr_held = v3_held && ctx->have_fgr && ctx->fgr_kind == ctx->delta_kind &&
                           (ctx->delta_kind == ALT3_DELTA_TETRA || ctx->fgr_sigma == ctx->delta_sigma);
     if (!fgr_held)
         ALT3_TRY(stage(2, [&] {
-            return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx);
+            return ctx->delta_kind == ALT3_DELTA_GAUSS ? fgr_gauss_run(ctx, ctx->delta_sigma) : fgr_run(ctx, false);
         }));
     DBuf<double> &dx = ctx->d_solve_x, &dte = ctx->d_solve_te, &dk = ctx->d_solve_k;
     ALT3_TRY(dx.reserve(n));
@@ -562,11 +565,15 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         ALT3_HIP(hipMemcpyAsync(dk.p + (size_t)(2 * it + 1) * nb * 9, ctx->d_kappa.p, kb, hipMemcpyDeviceToDevice,
                                 ctx->stream));
     }
-    int flags[3] = {0, 0, 0};
-    ALT3_HIP(hipMemcpyAsync(flags, ctx->d_flags.p + 1, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    int flags[4] = {0, 0, 0, 0};
+    ALT3_HIP(hipMemcpyAsync(flags, ctx->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
-    if (flags[0]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
-    if (flags[1] || flags[2]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
+    if (flags[0] && !v3_held) {   // what was built from a den = 0 triple is not kept
+        ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
+        return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
+    }
+    if (flags[1]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero or imaginary mode frequency)");
+    if (flags[2] || flags[3]) return fail(ALT3_E_NBE, "ERROR in N_BE: w=<0 (a zero frequency on a star member)");
     std::vector<double> hk((size_t)2 * ntemps * nb * 9);
     ALT3_HIP(hipMemcpy(hk.data(), dk.p, sizeof(double) * hk.size(), hipMemcpyDeviceToHost));
     for (int it = 0; it < ntemps; it++) {

@@ -544,7 +544,7 @@ int fgr_gauss_run(alt3_ctx *ctx, double sigma)
     return ALT3_OK;
 }
 
-int fgr_run(alt3_ctx *ctx)
+int fgr_run(alt3_ctx *ctx, bool sync)
 {
     if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr needs alt3_v3sq_ibz first");
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
@@ -588,7 +588,7 @@ int fgr_run(alt3_ctx *ctx)
     }
 #undef ALT3_FGR_LAUNCH
     ALT3_HIP(hipGetLastError());
-    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    if (sync) ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;
     ctx->fgr_kind = ALT3_DELTA_TETRA;
     ctx->have_tau = ctx->have_cm = false;

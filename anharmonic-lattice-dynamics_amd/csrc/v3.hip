@@ -355,7 +355,8 @@ int v3_prep_points(alt3_ctx *ctx, int nq, const double *q1, const double *e1, do
     return ALT3_OK;
 }
 
-int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
+// sync = false (alt3_solve): no wait at the end; a den = 0 stays in the device flag
+int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync)
 {
     if (!ctx->have_eig) return fail(ALT3_E_STATE, "eigenvectors missing: call alt3_eig or alt3_set_eig first");
     if (ibz_lo < 0 || ibz_hi >= ctx->nibz || ibz_lo > ibz_hi) return fail(ALT3_E_ARG, "bad ibz range");
@@ -431,10 +432,12 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec)
             ALT3_HIP(hipGetLastError());
         }
     }
-    int herr = 0;
-    ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    ALT3_HIP(hipStreamSynchronize(ctx->stream));
-    if (herr) return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
+    if (sync) {
+        int herr = 0;
+        ALT3_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        if (herr) return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
+    }
     ctx->have_v3 = true;
     return ALT3_OK;
 }
