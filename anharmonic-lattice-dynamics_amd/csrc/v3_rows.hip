@@ -49,6 +49,11 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     T *E = reinterpret_cast<T *>(smem);     // [r][q] re, then im        (2 ndr TQ)
     T *E3 = E + 2 * ndr * TQ;                // [c][l3][q] re, then im    (2 NB NB TQ)
     T *U = E3 + 2 * NB * NB * TQ;            // [b][l3][q] re, then im    (2 NB NB TQ)
+    // G' rows of the current l1, [r][b][c] (re, im), staged in LDS for NB <= 12 so that the
+    // phase loop reads them as LDS broadcasts (two rows b per wave) instead of vector loads
+    constexpr bool kStageG = NB <= 12;
+    constexpr int kGr = kStageG ? NB * NB : NB * NB * NB;   // stride of r in the G' rows read
+    T *Gs = U + 2 * NB * NB * TQ;            // (2 ndr NB NB) when kStageG
     __shared__ double w3s[NB][TQ];
     __shared__ int k3s[TQ];
     constexpr int NT = TQ * NB, NN = NB * NB;
@@ -92,17 +97,27 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     const T *e2 = evec + 2 * ((size_t)n2c * NB + j) * NB;   // e(q2) column l2 = j
     __syncthreads();
     for (int l1 = 0; l1 < NB; l1++) {
+        if constexpr (kStageG) {
+            // (r, b, c) of this l1; the previous l1's reads ended at the barrier after V
+            for (int x = tid; x < ndr * NN; x += NT) {
+                const int r = x / NN, bc = x - r * NN;
+                const T *src = gp + 2 * ((((size_t)s * ndr + r) * NB + l1) * NN + bc);
+                Gs[2 * x] = src[0];
+                Gs[2 * x + 1] = src[1];
+            }
+            __syncthreads();
+        }
         // row b = j of U = Phi' e3, Phi' streamed column by column
         T ur[NB], ui[NB];
 #pragma unroll
         for (int l3 = 0; l3 < NB; l3++) ur[l3] = ui[l3] = 0;
-        const T *g = gp + 2 * (((size_t)s * ndr * NB + l1) * NN + j * NB);   // + r*NB*NN + c
+        const T *g = kStageG ? Gs + 2 * (j * NB) : gp + 2 * (((size_t)s * ndr * NB + l1) * NN + j * NB);   // + r*NB*NN + c
 #pragma unroll 1
         for (int c = 0; c < NB; c++) {
             T pr = 0, pi = 0;
 #pragma unroll 4
             for (int r = 0; r < ndr; r++) {
-                const T gr = g[2 * ((size_t)r * NB * NN + c)], gi = g[2 * ((size_t)r * NB * NN + c) + 1];
+                const T gr = g[2 * ((size_t)r * kGr + c)], gi = g[2 * ((size_t)r * kGr + c) + 1];
                 const T er = E[r * TQ + q], ei = E[ndr * TQ + r * TQ + q];
                 pr += er * gr - ei * gi;
                 pi += er * gi + ei * gr;
@@ -153,7 +168,7 @@ int launch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const doub
                 int *d_err, const T *ev3x, const double *w3x, const double *w1x)
 {
     const int ndr = ctx->V.ndr;
-    const size_t lds = (size_t)2 * (ndr + 2 * NB * NB) * TQ * sizeof(T);
+    const size_t lds = (size_t)2 * ((ndr + 2 * NB * NB) * TQ + (NB <= 12 ? ndr * NB * NB : 0)) * sizeof(T);
     if (lds > ctx->lds_max) return alt3::fail(ALT3_E_ARG, "V3 tile does not fit in LDS");
     if (lds > 65536)
         ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_rows<T, NB, TQ, EXT>,
