@@ -232,13 +232,14 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
     __shared__ int s_cnt[3];
     __shared__ unsigned short s_list[kFgrList];  // [case][kFgrCase] of (cube << 3 | tetrahedron), nk <= 8192
-    // The two processes of one (s, l2, l3) read the same |V3|^2 rows.  Workgroups are dealt to
-    // the eight XCDs round robin, so the pair is placed 8 apart (blocks b and b+8 of each run of
-    // 16 share an XCD and its L2) and the second read hits the cache; the tail of the grid that
-    // does not fill a run of 16 keeps the plain pairing.
-    const int b = blockIdx.x, full = (int)(gridDim.x & ~15u);
-    const int proc = b < full ? (b >> 3) & 1 : b & 1;
-    int r = b < full ? ((b >> 4) << 3) | (b & 7) : b >> 1;
+    // The groups of one q1 read the same |V3|^2 rows (both processes, and the decay mirror the
+    // (l3, l2) rows of the coalescence groups).  Workgroups are dealt to the eight XCDs round
+    // robin, so work item (s, l2, l3, proc) is taken from a contiguous eighth of the items per
+    // XCD: the groups of one q1 then run together on one XCD and share its L2.
+    const int G = (int)gridDim.x, b = blockIdx.x;
+    const int item = (G & 7) == 0 ? (b & 7) * (G >> 3) + (b >> 3) : b;
+    const int proc = item & 1;
+    int r = item >> 1;
     const int l3 = r % nb;
     r /= nb;
     const int l2 = r % nb;
