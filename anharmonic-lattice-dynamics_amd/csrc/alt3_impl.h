@@ -199,7 +199,9 @@ struct alt3_ctx {
     float *r2f() const { return d_r2f.p + (size_t)tau_cur * ksub_nb() * nb * nk; }
     bool have_tau = false;
     alt3::DBuf<int> d_star_n, d_star_op, d_star_n0, d_fstar_n, d_fstar_op, d_fstar_n0;
-    alt3::DBuf<int> d_cm_slot_n2, d_cm_slot_cnt, d_cm_tab;   // collision star table, see k_collision_c
+    alt3::DBuf<int> d_cm_slot_n2, d_cm_tab, d_cm_split_x;   // collision star table, see star_sums
+    alt3::DBuf<int4> d_cm_task;
+    int cm_ntask = 0, cm_nsplit = 0;
     alt3::DBuf<double> d_opk, d_opk_inv, d_kibz;
     alt3::DBuf<double> d_frta, d_kpart, d_kappa, d_taueff, d_veloci;
     alt3::DBuf<double> d_cm, d_rhs;

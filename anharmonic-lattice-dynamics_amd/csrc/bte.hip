@@ -57,6 +57,21 @@ __device__ double block_sum(double v, double *sh)
     return r;
 }
 
+// the star table of the collision and update kernels (see star_sums)
+struct StarTasks {
+    const int *slot_n2;   // [nibz] irreducible point of slot x (slots by decreasing star size)
+    const int4 *task;     // [ntask] (slot, first member, end, part): part -1 = the whole star
+    const int *split_x;   // [nsplit] slot of split star p (parts 2p and 2p+1)
+    const int *tab;       // [48][nibz] member-major q0 << 6 | op
+    int ntask, nsplit;
+};
+
+StarTasks star_tasks(const alt3_ctx *ctx)
+{
+    return StarTasks{ctx->d_cm_slot_n2.p, ctx->d_cm_task.p, ctx->d_cm_split_x.p, ctx->d_cm_tab.p, ctx->cm_ntask,
+                     ctx->cm_nsplit};
+}
+
 // one workgroup per (s, l1, l2): the l3-reduced R1/R2 rows and partial Q, QN, QU of NT
 // temperatures at once, so that P1sm/P2sm are read once for all of them.  R is stored in the
 // precision of P (FP32 on the mixed path); the sums are FP64.
@@ -417,20 +432,23 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
     }
 }
 
-// Phase 2 of the LDS collision kernels: each thread takes one irreducible q2 (slot) and sums
-// C(q0) op over the mesh points q0 of its star; row block rb, column branch l2 of the
-// row-major matrix M (ncol = nibz*nb*3).
+// Phase 2 of the LDS collision kernels: the sums of C(q0) op over the mesh points q0 of each
+// irreducible q2's star, into row block rb, column branch l2 of the row-major matrix M
+// (ncol = nibz*nb*3).  Each thread takes tasks of the star table (longest first): a task is a
+// whole star, or one half of a star longer than the split threshold, so that no thread walks
+// a 48-member star while the others wait; the two halves meet in LDS (part, 18 doubles per
+// split star) after a barrier.  Members are summed in order within a task.  Every thread of
+// the block calls this.
 __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, const double *C, const double *ops,
-                                          const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
-                                          const int *__restrict__ tab, double *__restrict__ M)
+                                          double *part, const StarTasks &st, double *__restrict__ M)
 {
-    const size_t ncol = (size_t)nibz * nb * 3;
-    for (int x = threadIdx.x; x < nibz; x += blockDim.x) {
-        const int n2 = slot_n2[x], cnt = slot_cnt[x];
+    const size_t ncol = (size_t)nibz * nb * 3, d1 = 3 * (size_t)rb;
+    for (int t = threadIdx.x; t < st.ntask; t += blockDim.x) {
+        const int4 tk = st.task[t];   // slot, first member, end, part (-1: whole star)
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll 4   // independent table loads in flight; the sums keep member order
-        for (int ns = 0; ns < cnt; ns++) {
-            const int e = tab[(size_t)ns * nibz + x];
+        for (int ns = tk.y; ns < tk.z; ns++) {
+            const int e = st.tab[(size_t)ns * nibz + tk.x];
             const double c = C[e >> 6];
             const double *op = ops + 9 * (e & 63);
 #pragma unroll
@@ -438,11 +456,27 @@ __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, cons
 #pragma unroll
                 for (int j = 0; j < 3; j++) acc[i + 3 * j] = acc[i + 3 * j] + (c * op[3 * j + i]);
         }
-        const size_t d1 = 3 * (size_t)rb, d2 = 3 * ((size_t)n2 * nb + l2);
+        if (tk.w < 0) {
+            const size_t d2 = 3 * ((size_t)st.slot_n2[tk.x] * nb + l2);
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+            for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) M[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+                for (int j = 0; j < 3; j++) M[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; k++) part[9 * tk.w + k] = acc[k];
+        }
+    }
+    if (st.nsplit > 0) {   // uniform over the block
+        __syncthreads();
+        for (int p = threadIdx.x; p < st.nsplit; p += blockDim.x) {
+            const size_t d2 = 3 * ((size_t)st.slot_n2[st.split_x[p]] * nb + l2);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    M[(d1 + i) * ncol + d2 + j] = part[9 * (2 * p) + i + 3 * j] + part[9 * (2 * p + 1) + i + 3 * j];
+        }
     }
 }
 
@@ -456,13 +490,13 @@ template <typename RT>
 __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
                                                      const int *__restrict__ mapinv, const double *__restrict__ Q,
                                                      const RT *__restrict__ R1, const RT *__restrict__ R2,
-                                                     const int *__restrict__ slot_n2, const int *__restrict__ slot_cnt,
-                                                     const int *__restrict__ tab, const double *__restrict__ opk,
+                                                     const StarTasks st, const double *__restrict__ opk,
                                                      const double *__restrict__ piso, double *__restrict__ CMr)
 {
     extern __shared__ double lds[];
     double *C = lds;            // [nk]
     double *ops = lds + nk;     // [nops][9]
+    double *part = ops + 9 * nops;   // [2 nsplit][9]
     const int l2 = blockIdx.x % nb, rb = blockIdx.x / nb;  // rb = (s, l1)
     const int s = rb / nb, l1 = rb % nb;
     const int k1 = mapinv[ibz_lo + s];
@@ -492,7 +526,7 @@ __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, in
     // the diagonal: -Q, which holds Piso when the isotope term is on (-Q3 - Piso, as there)
     if (threadIdx.x == 0 && l1 == l2) C[k1] = -Q[(size_t)s * nb + l1];
     __syncthreads();
-    star_sums(nibz, nb, rb, l2, C, ops, slot_n2, slot_cnt, tab, CMr);
+    star_sums(nibz, nb, rb, l2, C, ops, part, st, CMr);
 }
 
 // update_mpi (sub.f90:2295-2449) as a matrix on the irreducible F, built like the collision
@@ -504,13 +538,13 @@ template <typename RT>
 __global__ void __launch_bounds__(256) k_update_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
                                                   const int *__restrict__ mapinv, const RT *__restrict__ R1,
                                                   const RT *__restrict__ R2, const RT *__restrict__ S1,
-                                                  const RT *__restrict__ S2, const int *__restrict__ slot_n2,
-                                                  const int *__restrict__ slot_cnt, const int *__restrict__ tab,
+                                                  const RT *__restrict__ S2, const StarTasks st,
                                                   const double *__restrict__ opk, double *__restrict__ U)
 {
     extern __shared__ double lds[];
     double *C = lds;            // [nk]
     double *ops = lds + nk;     // [nops][9]
+    double *part = ops + 9 * nops;   // [2 nsplit][9]
     const int l = blockIdx.x % nb, rb = blockIdx.x / nb;  // rb = (s, l1)
     const int s = rb / nb;
     const int k1 = mapinv[ibz_lo + s];
@@ -527,7 +561,7 @@ __global__ void __launch_bounds__(256) k_update_c(alt3::MeshDev m, int nk, int n
                 0.5 * (double)S2[rbase + p1];
     }
     __syncthreads();
-    star_sums(nibz, nb, rb, l, C, ops, slot_n2, slot_cnt, tab, U);
+    star_sums(nibz, nb, rb, l, C, ops, part, st, U);
 }
 
 // update_iso (sub.f90:2456-2610) adds Piso(n1,l1) Fsq2 to A after its loops, Fsq2 being what
@@ -923,12 +957,33 @@ int upload_bte_tables(alt3_ctx *ctx)
             tab[(size_t)ns * T.nibz + x] = (n0 << 6) | op;
         }
     }
+    // tasks of the star sums: stars longer than kSplitAt members (up to kMaxSplit of them, the
+    // longest) are split in two halves; longest tasks first
+    constexpr int kSplitAt = 24, kMaxSplit = 128;
+    std::vector<int4> tasks;
+    std::vector<int> split_x;
+    for (int x = 0; x < T.nibz; x++) {
+        if (cnt[x] > kSplitAt && (int)split_x.size() < kMaxSplit) {
+            const int p = (int)split_x.size(), h = cnt[x] / 2;
+            split_x.push_back(x);
+            tasks.push_back(make_int4(x, 0, h, 2 * p));
+            tasks.push_back(make_int4(x, h, cnt[x], 2 * p + 1));
+        } else {
+            tasks.push_back(make_int4(x, 0, cnt[x], -1));
+        }
+    }
+    std::stable_sort(tasks.begin(), tasks.end(), [](const int4 &a, const int4 &b) { return a.z - a.y > b.z - b.y; });
     ALT3_TRY(ctx->d_cm_slot_n2.alloc(T.nibz));
-    ALT3_TRY(ctx->d_cm_slot_cnt.alloc(T.nibz));
     ALT3_TRY(ctx->d_cm_tab.alloc((size_t)48 * T.nibz));
+    ALT3_TRY(ctx->d_cm_task.alloc(tasks.size()));
+    ALT3_TRY(ctx->d_cm_split_x.alloc(std::max<size_t>(1, split_x.size())));
     ALT3_HIP(hipMemcpy(ctx->d_cm_slot_n2.p, order.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
-    ALT3_HIP(hipMemcpy(ctx->d_cm_slot_cnt.p, cnt.data(), sizeof(int) * T.nibz, hipMemcpyHostToDevice));
     ALT3_HIP(hipMemcpy(ctx->d_cm_tab.p, tab.data(), sizeof(int) * 48 * T.nibz, hipMemcpyHostToDevice));
+    ALT3_HIP(hipMemcpy(ctx->d_cm_task.p, tasks.data(), sizeof(int4) * tasks.size(), hipMemcpyHostToDevice));
+    if (!split_x.empty())
+        ALT3_HIP(hipMemcpy(ctx->d_cm_split_x.p, split_x.data(), sizeof(int) * split_x.size(), hipMemcpyHostToDevice));
+    ctx->cm_ntask = (int)tasks.size();
+    ctx->cm_nsplit = (int)split_x.size();
     return ALT3_OK;
 }
 
@@ -1111,14 +1166,14 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool syn
     const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
     ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
-    const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
+    const size_t lds = ((size_t)nk + 9 * ctx->nops + 18 * ctx->cm_nsplit) * sizeof(double);
     const double *piso = ctx->giso.empty() ? nullptr : ctx->d_piso.p;   // CollisionMatrix_iso
     auto launch = [&](auto *R1, auto *R2) {
         using RT = std::remove_const_t<std::remove_pointer_t<decltype(R1)>>;
         if (lds <= 65536)
             hipLaunchKernelGGL(k_collision_c<RT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb,
-                               ctx->ibz_lo, nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_cm_slot_n2.p,
-                               ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p, piso, ctx->d_cm.p);
+                               ctx->ibz_lo, nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), R1, R2, star_tasks(ctx),
+                               ctx->d_opk.p, piso, ctx->d_cm.p);
         else
             hipLaunchKernelGGL(k_collision<RT>, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                nibz, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_star_n.p, ctx->d_star_op.p,
@@ -1266,7 +1321,7 @@ static int update_build(alt3_ctx *ctx, double tempk, const char *who)
         for (int q = 0; q < nk; q++)
             if (!seen[q]) return fail(ALT3_E_MESH, std::string(who) + ": a mesh point is in no star of an irreducible point");
     }
-    const size_t lds = ((size_t)nk + 9 * ctx->nops) * sizeof(double);
+    const size_t lds = ((size_t)nk + 9 * ctx->nops + 18 * ctx->cm_nsplit) * sizeof(double);
     if (lds > ctx->lds_max) return fail(ALT3_E_ARG, std::string(who) + ": the mesh does not fit in LDS (nk too large)");
     const size_t rstr = (size_t)ksub * nb * nb * nk, nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
     ALT3_TRY(ctx->d_upd.alloc(nrow * ncol));
@@ -1285,8 +1340,7 @@ static int update_build(alt3_ctx *ctx, double tempk, const char *who)
         }
         hipLaunchKernelGGL(k_update_c<PT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo,
                            nibz, ctx->nops, ctx->d_mapinv.p, (const PT *)R1, (const PT *)R2, (const PT *)S1.p,
-                           (const PT *)S2.p, ctx->d_cm_slot_n2.p, ctx->d_cm_slot_cnt.p, ctx->d_cm_tab.p, ctx->d_opk.p,
-                           ctx->d_upd.p);
+                           (const PT *)S2.p, star_tasks(ctx), ctx->d_opk.p, ctx->d_upd.p);
         return ALT3_OK;
     };
     if (ctx->r_fp32) {

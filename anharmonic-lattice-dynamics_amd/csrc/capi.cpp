@@ -151,7 +151,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_p1.release(); ctx->d_p2.release(); ctx->d_p1f.release(); ctx->d_p2f.release(); ctx->d_dist.release(); ctx->d_q.release(); ctx->d_qn.release();
     ctx->d_qu.release(); ctx->d_r1.release(); ctx->d_r2.release(); ctx->d_r1f.release(); ctx->d_r2f.release();
     ctx->d_star_n.release(); ctx->d_star_op.release(); ctx->d_star_n0.release();
-    ctx->d_cm_slot_n2.release(); ctx->d_cm_slot_cnt.release(); ctx->d_cm_tab.release();
+    ctx->d_cm_slot_n2.release(); ctx->d_cm_tab.release(); ctx->d_cm_task.release(); ctx->d_cm_split_x.release();
     ctx->d_fstar_n.release(); ctx->d_fstar_op.release(); ctx->d_fstar_n0.release();
     ctx->d_opk.release(); ctx->d_opk_inv.release(); ctx->d_kibz.release(); ctx->d_frta.release();
     ctx->d_kpart.release(); ctx->d_kappa.release(); ctx->d_taueff.release(); ctx->d_veloci.release();
