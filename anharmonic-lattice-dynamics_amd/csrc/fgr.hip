@@ -608,8 +608,8 @@ int fgr_run(alt3_ctx *ctx, bool sync)
 // adding c/6 to W in LDS; gs is then formed once per mesh point with a weight.  The reference
 // repeats the whole sum inside a loop over n2 = 1..nk that changes nothing in it and divides by
 // nk at the end; the sum is formed once.
-constexpr int kIsoList = 4096;   // tetrahedra per chunk of cubes (kIsoList / 6 cubes)
-constexpr int kIsoThreads = 1024;
+constexpr int kIsoList = 2048;   // tetrahedra per case and chunk of cubes (kIsoList / 6 cubes)
+constexpr int kIsoThreads = 512;
 
 // the mesh energies as REAL*4, branch-major [l][q], for coalesced per-branch loads
 __global__ void k_freq_f32t(int nk, int nb, const double *__restrict__ freq, float *__restrict__ enT)
@@ -630,8 +630,8 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
     __shared__ double e1[2 * 64];
     __shared__ double ga[32];
     __shared__ double red[kIsoThreads / 64];
-    __shared__ int s_cnt;
-    __shared__ unsigned s_list[kIsoList];   // cube << 3 | tetrahedron
+    __shared__ int s_cnt[3];
+    __shared__ unsigned s_list[3 * kIsoList];   // [case][kIsoList] of cube << 3 | tetrahedron
     const int s = blockIdx.x / nb, l1 = blockIdx.x % nb, tid = threadIdx.x;
     const int k1 = mapinv[s];
     const double omega1 = freq[(size_t)k1 * nb + l1];
@@ -640,8 +640,10 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
     for (int w = tid; w < nk; w += blockDim.x) Wl[w] = 0.0;
     double acc = 0;
     constexpr int kChunk = kIsoList / 6;
-    // tetrahedra of cubes [c0, c1) whose energy range contains w1 onto the list (entries past
-    // its end are counted but not stored)
+    // tetrahedra of cubes [c0, c1) whose energy range contains w1 onto the list of their
+    // weight_tet case (w1 below the second-lowest corner, below the third, or above), so that
+    // the lanes of a wave take the same branch of the weight formula (entries past the end of
+    // a list are counted but not stored)
     auto scan = [&](int c0, int c1) {
         for (int k = c0 + tid; k < c1; k += blockDim.x) {
             int a0, a1, a2;
@@ -659,17 +661,22 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
             if (!omega_in(omega1, lo, hi)) continue;
             for (int l = 0; l < 6; l++) {
                 const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
-                if (omega_in(omega1, fminf(fminf(a, b), fminf(c, d)), fmaxf(fmaxf(a, b), fmaxf(c, d)))) {
-                    const int x = atomicAdd(&s_cnt, 1);
-                    if (x < kIsoList) s_list[x] = ((unsigned)k << 3) | (unsigned)l;
+                const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
+                if (omega_in(omega1, fminf(s1, s3), fmaxf(s2, s4))) {
+                    const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
+                    const int cs = (omega1 <= (double)fminf(m1, m2)) ? 0 : (omega1 <= (double)fmaxf(m1, m2)) ? 1 : 2;
+                    const int x = atomicAdd(&s_cnt[cs], 1);
+                    if (x < kIsoList) s_list[cs * kIsoList + x] = ((unsigned)k << 3) | (unsigned)l;
                 }
             }
         }
     };
-    // weight_tet of the n listed tetrahedra, c/6 added to W at their corners
-    auto process = [&](int n) {
-        for (int x = tid; x < n; x += blockDim.x) {
-            const int k = (int)(s_list[x] >> 3), l = (int)(s_list[x] & 7u);
+    // weight_tet of the listed tetrahedra (n0, n1, n2 of the three cases), c/6 added to W at
+    // their corners
+    auto process = [&](int n0, int n1, int n2) {
+        for (int x = tid; x < n0 + n1 + n2; x += blockDim.x) {
+            const int at = x < n0 ? x : x < n0 + n1 ? kIsoList + x - n0 : 2 * kIsoList + x - n0 - n1;
+            const int k = (int)(s_list[at] >> 3), l = (int)(s_list[at] & 7u);
             int a0, a1, a2;
             m.coords(k, a0, a1, a2);
             int v[4];
@@ -693,21 +700,21 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
         for (int w = tid; w < nk; w += blockDim.x) ens[w] = enT[(size_t)l2 * nk + w];
         // all cubes at once when their active tetrahedra fit the list (the usual case), else
         // chunk by chunk (kIsoList / 6 cubes cannot overflow it); the branch is uniform
-        if (tid == 0) s_cnt = 0;
+        if (tid < 3) s_cnt[tid] = 0;
         __syncthreads();
         scan(0, nk);
         __syncthreads();
-        const int nall = s_cnt;
-        if (nall <= kIsoList) {
-            process(nall);
+        const int n0 = s_cnt[0], n1 = s_cnt[1], n2 = s_cnt[2];
+        if (n0 <= kIsoList && n1 <= kIsoList && n2 <= kIsoList) {
+            process(n0, n1, n2);
         } else {
             for (int c0 = 0; c0 < nk; c0 += kChunk) {
                 __syncthreads();
-                if (tid == 0) s_cnt = 0;
+                if (tid < 3) s_cnt[tid] = 0;
                 __syncthreads();
                 scan(c0, min(nk, c0 + kChunk));
                 __syncthreads();
-                process(s_cnt);
+                process(s_cnt[0], s_cnt[1], s_cnt[2]);
             }
         }
         __syncthreads();   // W is complete
@@ -751,9 +758,9 @@ int piso_run(alt3_ctx *ctx)
     ALT3_HIP(hipMemcpyAsync(ctx->d_giso.p, ctx->giso.data(), sizeof(double) * ctx->giso.size(), hipMemcpyHostToDevice,
                             ctx->stream));
     const size_t lds = (size_t)ctx->nk * (sizeof(double) + sizeof(float));
-    if (lds + kIsoList * sizeof(unsigned) + 2048 > ctx->lds_max)
+    if (lds + 3 * kIsoList * sizeof(unsigned) + 2048 > ctx->lds_max)
         return fail(ALT3_E_ARG, "isotope term: the mesh energies do not fit in LDS (nk too large)");
-    if (lds + kIsoList * sizeof(unsigned) + 2048 > 65536)
+    if (lds + 3 * kIsoList * sizeof(unsigned) + 2048 > 65536)
         ALT3_HIP(hipFuncSetAttribute((const void *)k_piso, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     DBuf<double> &enT = ctx->d_red;   // reduction scratch, nk*nb floats here
     ALT3_TRY(enT.reserve(((size_t)ctx->nk * nb + 1) / 2));
