@@ -19,6 +19,10 @@
 
 namespace {
 
+// a*b + c as one fused multiply-add: each complex multiply-accumulate below is four of them
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // |lambda(k,l)|^(-1/4); +inf for a zero eigenvalue (the reference's den = 0 stop)
 __global__ void k_lam_quarter(int n, const double *__restrict__ eival, double *__restrict__ w)
 {
@@ -119,14 +123,18 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
             for (int r = 0; r < ndr; r++) {
                 const T gr = g[2 * ((size_t)r * kGr + c)], gi = g[2 * ((size_t)r * kGr + c) + 1];
                 const T er = E[r * TQ + q], ei = E[ndr * TQ + r * TQ + q];
-                pr += er * gr - ei * gi;
-                pi += er * gi + ei * gr;
+                pr = fmx(er, gr, pr);
+                pr = fmx(-ei, gi, pr);
+                pi = fmx(er, gi, pi);
+                pi = fmx(ei, gr, pi);
             }
 #pragma unroll
             for (int l3 = 0; l3 < NB; l3++) {
                 const T xr = E3[(c * NB + l3) * TQ + q], xi = E3[NN * TQ + (c * NB + l3) * TQ + q];
-                ur[l3] += pr * xr - pi * xi;
-                ui[l3] += pr * xi + pi * xr;
+                ur[l3] = fmx(pr, xr, ur[l3]);
+                ur[l3] = fmx(-pi, xi, ur[l3]);
+                ui[l3] = fmx(pr, xi, ui[l3]);
+                ui[l3] = fmx(pi, xr, ui[l3]);
             }
         }
 #pragma unroll
@@ -146,8 +154,10 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
 #pragma unroll
                 for (int l3 = 0; l3 < NB; l3++) {
                     const T xr = U[(b * NB + l3) * TQ + q], xi = U[NN * TQ + (b * NB + l3) * TQ + q];
-                    vr[l3] += er * xr - ei * xi;
-                    vi[l3] += er * xi + ei * xr;
+                    vr[l3] = fmx(er, xr, vr[l3]);
+                    vr[l3] = fmx(-ei, xi, vr[l3]);
+                    vi[l3] = fmx(er, xi, vi[l3]);
+                    vi[l3] = fmx(ei, xr, vi[l3]);
                 }
             }
             const double w12 = (EXT ? w1x[(size_t)s * NB + l1] : wq[(size_t)k1 * NB + l1]) * w2;
