@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
             T pr = 0, pi = 0;
 #pragma unroll 4
             for (int r = 0; r < ndr; r++) {
-                const T gr = g[2 * ((size_t)r * kGr + c)], gi = g[2 * ((size_t)r * kGr + c) + 1];
+                const T gr = g[2 * (r * kGr + c)], gi = g[2 * (r * kGr + c) + 1];
                 const T er = E[r * TQ + q], ei = E[ndr * TQ + r * TQ + q];
                 pr = fmx(er, gr, pr);
                 pr = fmx(-ei, gi, pr);
