@@ -205,6 +205,8 @@ struct alt3_ctx {
     alt3::DBuf<double> d_opk, d_opk_inv, d_kibz;
     alt3::DBuf<double> d_frta, d_kpart, d_kappa, d_taueff, d_veloci;
     alt3::DBuf<double> d_cm, d_rhs;
+    alt3::DBuf<float> d_cmf;                       // the collision matrix in FP32 (mixed path)
+    bool cm_fp32 = false;                          // which of d_cm / d_cmf holds it
     bool have_cm = false;
     alt3::DBuf<double> d_cg;                       // CG work vectors
     // iterative BTE: S rows (sum over l2 of P with occupations, [ksub][nb][nb][nk], in the

@@ -384,13 +384,13 @@ __global__ void __launch_bounds__(64) k_kappa_sum(int nibz, int nb, const double
 // column blocks (n2,l2), whose star gathers stay inside the (s,l1) slab of R1/R2.  The
 // device matrix is row-major (CMr[row*ncol + col]) so the three rows of a block are written
 // contiguously.
-template <typename RT>
+template <typename RT, typename MT>
 __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz,
                                                    const int *__restrict__ mapinv, const double *__restrict__ Q,
                                                    const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                    const int *__restrict__ star_n, const int *__restrict__ star_op,
                                                    const int *__restrict__ star_n0, const double *__restrict__ opk,
-                                                   const double *__restrict__ piso, double *__restrict__ CMr)
+                                                   const double *__restrict__ piso, MT *__restrict__ CMr)
 {
     const int rb = blockIdx.x;  // (s, l1)
     const int s = rb / nb, l1 = rb % nb;
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
         }
         const size_t d1 = 3 * (size_t)rb, d2 = 3 * (size_t)cb;
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+            for (int j = 0; j < 3; j++) CMr[(d1 + i) * ncol + d2 + j] = (MT)acc[i + 3 * j];
     }
 }
 
@@ -439,8 +439,9 @@ __global__ void __launch_bounds__(256) k_collision(alt3::MeshDev m, int nk, int 
 // a 48-member star while the others wait; the two halves meet in LDS (part, 18 doubles per
 // split star) after a barrier.  Members are summed in order within a task.  Every thread of
 // the block calls this.
+template <typename MT>
 __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, const double *C, const double *ops,
-                                          double *part, const StarTasks &st, double *__restrict__ M)
+                                          double *part, const StarTasks &st, MT *__restrict__ M)
 {
     const size_t ncol = (size_t)nibz * nb * 3, d1 = 3 * (size_t)rb;
     for (int t = threadIdx.x; t < st.ntask; t += blockDim.x) {
@@ -461,7 +462,7 @@ __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, cons
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) M[(d1 + i) * ncol + d2 + j] = acc[i + 3 * j];
+                for (int j = 0; j < 3; j++) M[(d1 + i) * ncol + d2 + j] = (MT)acc[i + 3 * j];
         } else {
 #pragma unroll
             for (int k = 0; k < 9; k++) part[9 * tk.w + k] = acc[k];
@@ -475,7 +476,7 @@ __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, cons
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++)
-                    M[(d1 + i) * ncol + d2 + j] = part[9 * (2 * p) + i + 3 * j] + part[9 * (2 * p + 1) + i + 3 * j];
+                    M[(d1 + i) * ncol + d2 + j] = (MT)(part[9 * (2 * p) + i + 3 * j] + part[9 * (2 * p + 1) + i + 3 * j]);
         }
     }
 }
@@ -486,12 +487,12 @@ __device__ __forceinline__ void star_sums(int nibz, int nb, int rb, int l2, cons
 // each thread one irreducible q2 (slots ordered by decreasing star size, so that the lanes of
 // a wave run the same number of star members) and sums C(q0) op over its star from LDS.  The
 // star table is stored member-major (tab[ns][slot] = q0 << 6 | op) for coalesced reads.
-template <typename RT>
+template <typename RT, typename MT>
 __global__ void __launch_bounds__(256) k_collision_c(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nibz, int nops,
                                                      const int *__restrict__ mapinv, const double *__restrict__ Q,
                                                      const RT *__restrict__ R1, const RT *__restrict__ R2,
                                                      const StarTasks st, const double *__restrict__ opk,
-                                                     const double *__restrict__ piso, double *__restrict__ CMr)
+                                                     const double *__restrict__ piso, MT *__restrict__ CMr)
 {
     extern __shared__ double lds[];
     double *C = lds;            // [nk]
@@ -687,7 +688,8 @@ __global__ void k_frta_to_vec(int nibz, int nb, const double *__restrict__ FRTA,
 }
 
 // row-major -> Fortran column-major copy (nrow x ncol)
-__global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restrict__ in, double *__restrict__ out)
+template <typename MT>
+__global__ void k_rows_to_cols(size_t nrow, size_t ncol, const MT *__restrict__ in, double *__restrict__ out)
 {
     size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nrow * ncol) return;
@@ -696,7 +698,8 @@ __global__ void k_rows_to_cols(size_t nrow, size_t ncol, const double *__restric
 }
 
 // y = A x for a row-major nrow x ncol matrix: one wavefront per row, columns in order per lane
-__global__ void __launch_bounds__(256) k_matvec_rows(int nrow, int ncol, const double *__restrict__ A,
+template <typename AT>
+__global__ void __launch_bounds__(256) k_matvec_rows(int nrow, int ncol, const AT *__restrict__ A,
                                                      const double *__restrict__ x, double *__restrict__ y,
                                                      const int *__restrict__ skip)
 {
@@ -704,9 +707,9 @@ __global__ void __launch_bounds__(256) k_matvec_rows(int nrow, int ncol, const d
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= nrow) return;
-    const double *a = A + (size_t)row * ncol;
+    const AT *a = A + (size_t)row * ncol;
     double acc = 0;
-    for (int j = lane; j < ncol; j += 64) acc += a[j] * x[j];
+    for (int j = lane; j < ncol; j += 64) acc += (double)a[j] * x[j];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) y[row] = acc;
 }
@@ -1164,25 +1167,35 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool syn
     const int nb = ctx->nb, nibz = ctx->nibz, nk = ctx->nk, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const MeshDev md = MeshDev::make(ctx->nc);
     const size_t nrow = (size_t)ksub * nb * 3, ncol = (size_t)nibz * nb * 3;
-    ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
+    // the mixed path (FP32 P and R) keeps the matrix in FP32: half the bytes for the kernel's
+    // stores and for every CG product; the products and CG vectors stay FP64
+    ctx->cm_fp32 = ctx->r_fp32;
+    if (ctx->cm_fp32) {
+        ctx->d_cm.release();
+        ALT3_TRY(ctx->d_cmf.alloc(nrow * ncol));
+    } else {
+        ctx->d_cmf.release();
+        ALT3_TRY(ctx->d_cm.alloc(nrow * ncol));
+    }
     ALT3_TRY(ctx->d_rhs.alloc((size_t)nibz * nb * 3));
     const size_t lds = ((size_t)nk + 9 * ctx->nops + 18 * ctx->cm_nsplit) * sizeof(double);
     const double *piso = ctx->giso.empty() ? nullptr : ctx->d_piso.p;   // CollisionMatrix_iso
-    auto launch = [&](auto *R1, auto *R2) {
+    auto launch = [&](auto *R1, auto *R2, auto *M) {
         using RT = std::remove_const_t<std::remove_pointer_t<decltype(R1)>>;
+        using MT = std::remove_pointer_t<decltype(M)>;
         if (lds <= 65536)
-            hipLaunchKernelGGL(k_collision_c<RT>, dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb,
+            hipLaunchKernelGGL((k_collision_c<RT, MT>), dim3(ksub * nb * nb), dim3(256), lds, ctx->stream, md, nk, nb,
                                ctx->ibz_lo, nibz, ctx->nops, ctx->d_mapinv.p, ctx->q(), R1, R2, star_tasks(ctx),
-                               ctx->d_opk.p, piso, ctx->d_cm.p);
+                               ctx->d_opk.p, piso, M);
         else
-            hipLaunchKernelGGL(k_collision<RT>, dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
-                               nibz, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_star_n.p, ctx->d_star_op.p,
-                               ctx->d_star_n0.p, ctx->d_opk.p, piso, ctx->d_cm.p);
+            hipLaunchKernelGGL((k_collision<RT, MT>), dim3(ksub * nb), dim3(256), 0, ctx->stream, md, nk, nb,
+                               ctx->ibz_lo, nibz, ctx->d_mapinv.p, ctx->q(), R1, R2, ctx->d_star_n.p, ctx->d_star_op.p,
+                               ctx->d_star_n0.p, ctx->d_opk.p, piso, M);
     };
     if (ctx->r_fp32)
-        launch((const float *)ctx->r1f(), (const float *)ctx->r2f());
+        launch((const float *)ctx->r1f(), (const float *)ctx->r2f(), ctx->d_cmf.p);
     else
-        launch((const double *)ctx->r1(), (const double *)ctx->r2());
+        launch((const double *)ctx->r1(), (const double *)ctx->r2(), ctx->d_cm.p);
     ALT3_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_rhs, dim3((nibz * nb + 127) / 128), dim3(128), 0, ctx->stream, nibz, nb, tempk, ctx->d_mapinv.p,
                        ctx->d_freq.p, ctx->dist(), ctx->d_vel.p, ctx->d_rhs.p);
@@ -1190,8 +1203,12 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool syn
     if (CM) {  // Fortran Collision_Matrixmpi(nrow, ncol) is column-major
         DBuf<double> t;
         ALT3_TRY(t.alloc(nrow * ncol));
-        hipLaunchKernelGGL(k_rows_to_cols, dim3((unsigned)((nrow * ncol + 255) / 256)), dim3(256), 0, ctx->stream, nrow,
-                           ncol, ctx->d_cm.p, t.p);
+        if (ctx->cm_fp32)
+            hipLaunchKernelGGL(k_rows_to_cols<float>, dim3((unsigned)((nrow * ncol + 255) / 256)), dim3(256), 0,
+                               ctx->stream, nrow, ncol, ctx->d_cmf.p, t.p);
+        else
+            hipLaunchKernelGGL(k_rows_to_cols<double>, dim3((unsigned)((nrow * ncol + 255) / 256)), dim3(256), 0,
+                               ctx->stream, nrow, ncol, ctx->d_cm.p, t.p);
         ALT3_HIP(hipGetLastError());
         ALT3_HIP(hipMemcpyAsync(CM, t.p, sizeof(double) * nrow * ncol, hipMemcpyDeviceToHost, ctx->stream));
         ALT3_HIP(hipStreamSynchronize(ctx->stream));
@@ -1204,12 +1221,17 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool syn
 
 // CG on a device matrix (dA, db); x is a device vector, overwritten.  rowmajor selects the
 // layout of dA (the collision matrix is kept row-major on the device).
+// dAf (FP32, row-major) replaces dA when given
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
-              double *funcmin, int *iters, bool rowmajor)
+              double *funcmin, int *iters, bool rowmajor, const float *dAf)
 {
     auto matvec = [&](const double *xin, double *yout, const int *skip) {
-        if (rowmajor)
-            hipLaunchKernelGGL(k_matvec_rows, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, n, dA, xin, yout, skip);
+        if (dAf)
+            hipLaunchKernelGGL(k_matvec_rows<float>, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, n, dAf, xin, yout,
+                               skip);
+        else if (rowmajor)
+            hipLaunchKernelGGL(k_matvec_rows<double>, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, n, dA, xin, yout,
+                               skip);
         else
             hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
     };
@@ -1378,7 +1400,7 @@ int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRT
     double *dx = ctx->d_iter.p, *dfr = dx + N, *dA = dfr + N, *dF2 = dA + nrow;
     ALT3_HIP(hipMemcpyAsync(dx, x.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
     ALT3_HIP(hipMemcpyAsync(dfr, FRTA, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_matvec_rows, dim3((unsigned)((nrow + 3) / 4)), dim3(256), 0, ctx->stream, (int)nrow, (int)N,
+    hipLaunchKernelGGL(k_matvec_rows<double>, dim3((unsigned)((nrow + 3) / 4)), dim3(256), 0, ctx->stream, (int)nrow, (int)N,
                        ctx->d_upd.p, dx, dA, nullptr);
     hipLaunchKernelGGL(k_iter_mix, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, ctx->stream, ksub, nibz, nb,
                        ctx->ibz_lo, 0.0, dfr, ctx->q(), dA, nullptr, dF2, nullptr);
@@ -1407,7 +1429,7 @@ int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double
     const double const2 = K_B * tempk * tempk / (H_PLANK * C_LIGHT * 100) / C_LIGHT;
     // iterations run on the device; the host looks at the converged flag every 8 of them
     for (int it = 0; it < max_iter; it++) {
-        hipLaunchKernelGGL(k_matvec_rows, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, N, N, ctx->d_upd.p, x, y, state);
+        hipLaunchKernelGGL(k_matvec_rows<double>, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, N, N, ctx->d_upd.p, x, y, state);
         hipLaunchKernelGGL(k_iter_mix, dim3((N + 255) / 256), dim3(256), 0, ctx->stream, nibz, nibz, nb, 0, mix,
                            ctx->d_frta.p, ctx->q(), y, x, nullptr, state);
         hipLaunchKernelGGL(k_mode_kappa, dim3((nibz * nb + kStarWave - 1) / kStarWave), dim3(64 * kStarWave), 0, ctx->stream, nibz, nb,

@@ -30,7 +30,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync);
 int kappa_rta_run(alt3_ctx *ctx, double tempk, double *kappa, double *FRTA, double *taueff, bool sync);
 int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool sync);
 int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
-              double *funcmin, int *iters, bool rowmajor);
+              double *funcmin, int *iters, bool rowmajor, const float *dAf);
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev, bool sync);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
 int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev);
@@ -155,7 +155,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_fstar_n.release(); ctx->d_fstar_op.release(); ctx->d_fstar_n0.release();
     ctx->d_opk.release(); ctx->d_opk_inv.release(); ctx->d_kibz.release(); ctx->d_frta.release();
     ctx->d_kpart.release(); ctx->d_kappa.release(); ctx->d_taueff.release(); ctx->d_veloci.release();
-    ctx->d_cm.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
+    ctx->d_cm.release(); ctx->d_cmf.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
@@ -173,7 +173,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     long long b = 0;
     b += ctx->d_evec.bytes() + ctx->d_eival.bytes() + ctx->d_freq.bytes() + ctx->d_vel.bytes();
     b += ctx->d_v33.bytes() + ctx->d_v33f.bytes() + ctx->d_gp.bytes() + ctx->d_gpf.bytes();
-    b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes();
+    b += ctx->d_p1.bytes() + ctx->d_p2.bytes() + ctx->d_p1f.bytes() + ctx->d_p2f.bytes() + ctx->d_r1.bytes() + ctx->d_r2.bytes() + ctx->d_cm.bytes() + ctx->d_cmf.bytes();
     b += ctx->d_r1f.bytes() + ctx->d_r2f.bytes();
     b += ctx->d_s1.bytes() + ctx->d_s2.bytes() + ctx->d_s1f.bytes() + ctx->d_s2f.bytes() + ctx->d_upd.bytes();
     b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
@@ -408,7 +408,8 @@ int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, i
     ALT3_TRY(dx.alloc(n));
     ALT3_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
     Timer tm(ctx);
-    ALT3_TRY(cg_device(ctx, n, pA, pb, dx.p, max_iter, ndig, funcmin, iters, A == nullptr));
+    ALT3_TRY(cg_device(ctx, n, pA, pb, dx.p, max_iter, ndig, funcmin, iters, A == nullptr,
+                       A == nullptr && ctx->cm_fp32 ? ctx->d_cmf.p : nullptr));
     tm.stop();
     ALT3_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     return ALT3_OK;
@@ -559,7 +560,10 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
         // x0 = F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:835-843), on the device
         ALT3_TRY(cg_start_launch(ctx, dx.p, dte.p));
         int its = 0;
-        ALT3_TRY(stage(6, [&] { return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its, true); }));
+        ALT3_TRY(stage(6, [&] {
+            return cg_device(ctx, n, ctx->d_cm.p, ctx->d_rhs.p, dx.p, 100, 2, nullptr, &its, true,
+                             ctx->cm_fp32 ? ctx->d_cmf.p : nullptr);
+        }));
         if (cg_iters) cg_iters[it] = its;
         ALT3_TRY(stage(7, [&] { return kappa_bte_run(ctx, T, dx.p, nullptr, dte.p, false); }));
         ALT3_HIP(hipMemcpyAsync(dk.p + (size_t)(2 * it + 1) * nb * 9, ctx->d_kappa.p, kb, hipMemcpyDeviceToDevice,
