@@ -155,3 +155,15 @@ def test_iso_mixed_vs_fp64(si_inputs):
             res[name] = c.solve([300.0], p)
     for k in ("kappa_rta", "kappa_bte"):
         check(res["mixed"][k], res["fp64"][k], 1e-5, k + " mixed vs FP64 with iso")
+
+
+def test_iso_zero_g_and_bad_g(si_inputs, ref4):
+    print("case: iso with g = 0 (Piso zero, Q bit-equal to the non-iso Q) and negative g", flush=True)
+    T = 300.0
+    with _ctx(si_inputs[4], ref4) as c:
+        q0 = c.tau(T)["Q"]
+        P = c.set_iso(np.zeros(2))
+        assert not np.any(P)
+        assert np.array_equal(c.tau(T)["Q"], q0)
+        with pytest.raises(capi.Alt3Error):
+            c.set_iso(np.array([2e-4, -1e-4]))

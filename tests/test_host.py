@@ -113,3 +113,14 @@ def test_make_kp_bs_vs_reference_ge_example():
     g01 = np.linalg.norm(se.reciprocal_2pi(se.conventional_cell(d["latparams"]) @ d["primlatt"])[0])
     assert np.abs(dk * g01 - d["dk"]).max() < 2e-3
     assert names[0] == "L" and names[-1] == "G"
+
+
+def test_set_iso_needs_device_state(si_inputs):
+    """alt3_set_iso on a host-only context is refused like every device stage; a wrong
+    number of g values is refused by the binding before any call."""
+    with capi.Context(si_inputs[4], host_only=True) as c:
+        with pytest.raises(capi.Alt3Error) as e:
+            c.set_iso(capi.giso_of(si_inputs[4]))
+        assert e.value.code == -4
+        with pytest.raises(ValueError):
+            c.set_iso([2e-4])
