@@ -34,7 +34,8 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
 int kappa_bte_run(alt3_ctx *ctx, double tempk, const double *dx, double *kappa, double *taueff_dev, bool sync);
 int update_run(alt3_ctx *ctx, double tempk, const double *FFF, const double *FRTA, double *F2);
 int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev);
-int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, double *omega0,
+int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, int kind,
+             double *omega0,
              double *nself, double *uself);
 int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double tol, double *kappa, double *khist,
                    double *tehist, double *FFF, int *iters);
@@ -477,7 +478,19 @@ int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double et
     if (!ctx) return fail(ALT3_E_ARG, "null context");
     if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
     Timer tm(ctx);
-    ALT3_TRY(self_run(ctx, nq, q, tempk, eta, v3_threshold, omega0, nself, uself));
+    ALT3_TRY(self_run(ctx, nq, q, tempk, eta, v3_threshold, 0, omega0, nself, uself));
+    tm.stop();
+    return ALT3_OK;
+}
+
+int alt3_self_w4(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
+                 double *nself, double *uself)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    if (!(eta > 0)) return fail(ALT3_E_ARG, "the Gaussian width eta must be positive");
+    Timer tm(ctx);
+    ALT3_TRY(self_run(ctx, nq, q, tempk, eta, v3_threshold, 1, omega0, nself, uself));
     tm.stop();
     return ALT3_OK;
 }

@@ -54,8 +54,17 @@ __global__ void k_self_w(size_t n, const double *__restrict__ lam, double *__res
     if (i < n) w[i] = 1.0 / sqrt(sqrt(fabs(lam[i])));
 }
 
-// Lorentzian sums of function_self_w3 for point s and 256 mesh points k2 per group:
-// part[s][chunk][l1][4] = (Re, Im) of the normal and of the umklapp sums (before / nk * pi)
+// Gaussian delta of width w (delta_g, extratools.f90:239-246)
+__device__ __forceinline__ double delta_gauss(double x, double w)
+{
+    return exp(-x * x / 2.0 / w / w) / sqrt(2 * alt3::PI) / w;
+}
+
+// Sums of the self-energy for point s and 256 mesh points k2 per group:
+// part[s][chunk][l1][4] = (Re, Im) of the normal and of the umklapp sums (before / nk * pi).
+// kind 0: function_self_w3 (rest.f90:4012-4124), four Lorentzian processes cut at 3000 eta;
+// kind 1: function_self_w4 (rest.f90:4126-4240), the imaginary part alone from the Gaussian
+// deltas of three processes cut at 6 eta (the real part stays zero)
 __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *__restrict__ v33,
                                                   const double *__restrict__ lamq, const double *__restrict__ lam3,
                                                   const double *__restrict__ freq, const double *__restrict__ q,
@@ -63,7 +72,7 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
                                                   double rr0, double rr1, double rr2, double rr3, double rr4,
                                                   double rr5, double rr6, double rr7, double rr8, double temp,
                                                   int classical, double eta, double v3thr, double cnst,
-                                                  double *__restrict__ part)
+                                                  int kind, double *__restrict__ part)
 {
     __shared__ double red[4][256];
     const int s = blockIdx.y, nch = gridDim.x, tid = threadIdx.x;
@@ -80,7 +89,7 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
         const double x = (sh[0] * rr[3 * i] + sh[1] * rr[3 * i + 1] + sh[2] * rr[3 * i + 2]) + 5e-10;
         if (x < 0 || x >= 1) inside = false;
     }
-    const double etacut = eta * 3000;
+    const double etacut = kind == 1 ? eta * 6 : eta * 3000;
     for (int l1 = 0; l1 < nb; l1++) {
         double nr = 0, ni = 0, ur = 0, ui = 0;
         if (valid) {
@@ -93,6 +102,15 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
                     const double om3 = sqrt(fabs(lam3[((size_t)s * nk + k2) * nb + l3])) * cnst;
                     const double nb3 = nbe_self(om3, temp, classical);
                     double ires = 0, rres = 0, arg;
+                    if (kind == 1) {
+                        arg = omega - om3 - om2;
+                        if (fabs(arg) < etacut) ires = (nb3 + nb2 + 1) * delta_gauss(arg, eta);
+                        arg = omega - om3 + om2;
+                        if (fabs(arg) < etacut) ires = ires + (nb2 - nb3) * delta_gauss(arg, eta);
+                        arg = omega - om2 + om3;
+                        if (fabs(arg) < etacut) ires = ires + (nb3 - nb2) * delta_gauss(arg, eta);
+                        goto summed;
+                    }
                     arg = omega - om3 - om2;
                     if (fabs(arg) < etacut) {
                         ires = (nb3 + nb2 + 1) * (eta / (arg * arg + eta * eta) / alt3::PI);
@@ -113,6 +131,7 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
                         ires = ires + (nb3 - nb2) * (eta / (arg * arg + eta * eta) / alt3::PI);
                         rres = rres + (nb3 - nb2) * arg / (arg * arg + eta * eta);
                     }
+                summed:
                     if (inside) {
                         nr += term * rres;
                         ni += term * (ires * alt3::PI);
@@ -158,8 +177,8 @@ __global__ void k_self_final(int nq, int nb, int nch, int nk, const double *__re
 
 namespace alt3 {
 
-int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, double *omega0,
-             double *nself, double *uself)
+int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3thr, int kind,
+             double *omega0, double *nself, double *uself)
 {
     if (nq < 1 || !q) return fail(ALT3_E_ARG, "no q point");
     if (!(tempk > 0)) return fail(ALT3_E_ARG, "ERROR in N_BE: t=<0");
@@ -222,7 +241,7 @@ int self_run(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, d
         ALT3_TRY(v3_rows_points(ctx, b, gp.p, ev3, w3, w.p, v33.p, derr));
         hipLaunchKernelGGL(k_self_sum, dim3((unsigned)nch, (unsigned)b), dim3(256), 0, ctx->stream, nk, nb, v33.p, lam.p,
                            lam3, ctx->d_freq.p, dq.p, ctx->d_kpc.p, gs[0], gs[1], gs[2], rr[0], rr[1], rr[2], rr[3],
-                           rr[4], rr[5], rr[6], rr[7], rr[8], temp, ctx->classical, eta, v3thr, cnst(), part.p);
+                           rr[4], rr[5], rr[6], rr[7], rr[8], temp, ctx->classical, eta, v3thr, cnst(), kind, part.p);
         hipLaunchKernelGGL(k_self_final, dim3((b * nb + 63) / 64), dim3(64), 0, ctx->stream, b, nb, nch, nk, part.p,
                            outs.p, outs.p + 2 * (size_t)b * nb);
         ALT3_HIP(hipGetLastError());

@@ -115,6 +115,13 @@ module alt3_iface
        integer(c_int), value :: nq
        real(c_double), value :: tempk, eta, v3_threshold
      end function alt3_self_w3
+     integer(c_int) function alt3_self_w4(ctx, nq, q, tempk, eta, v3_threshold, omega0, nself, uself) &
+          bind(C, name='alt3_self_w4')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx, q, omega0, nself, uself
+       integer(c_int), value :: nq
+       real(c_double), value :: tempk, eta, v3_threshold
+     end function alt3_self_w4
 
      integer(c_int) function alt3_update(ctx, tempk, fff, frta, f2) bind(C, name='alt3_update')
        import :: c_int, c_ptr, c_double

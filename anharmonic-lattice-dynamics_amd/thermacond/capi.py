@@ -45,7 +45,7 @@ class System(C.Structure):
 EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes", "alt3_sizes", "alt3_get_mesh",
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
-           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_set_fgr",
+           "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
            "alt3_set_iso", "alt3_isotope_g"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
@@ -282,6 +282,16 @@ class Context:
         om = np.zeros((self.nb, nq), order="F")
         ns = np.zeros((self.nb, nq), dtype=np.complex128, order="F"); us = np.zeros_like(ns)
         _check(lib().alt3_self_w3(self.h, nq, _p(q), C.c_double(tempk), C.c_double(eta), C.c_double(v3_threshold),
+                                  _p(om), _p(ns), _p(us)))
+        return om, ns, us
+
+    def self_w4(self, q, tempk: float, eta: float, v3_threshold: float = V3_THRESHOLD):
+        """The Gaussian form (function_self_w4): imaginary parts only, as self_w3 otherwise."""
+        q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+        nq = len(q)
+        om = np.zeros((self.nb, nq), order="F")
+        ns = np.zeros((self.nb, nq), dtype=np.complex128, order="F"); us = np.zeros_like(ns)
+        _check(lib().alt3_self_w4(self.h, nq, _p(q), C.c_double(tempk), C.c_double(eta), C.c_double(v3_threshold),
                                   _p(om), _p(ns), _p(us)))
         return om, ns, us
 

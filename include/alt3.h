@@ -190,6 +190,15 @@ int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix
 int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
                  double *nself, double *uself);
 
+/* The Gaussian form of the same self-energy, function_self_w4 (rest.f90:4126-4240): the
+ * imaginary part alone, sum over k, l2, l3 of |V3|^2 [(n2+n3+1) delta_g(omega-omega3-omega2)
+ * + (n2-n3) delta_g(omega-omega3+omega2) + (n3-n2) delta_g(omega-omega2+omega3)] pi / nk, with
+ * delta_g(x) = exp(-x^2/2/eta^2)/sqrt(2 pi)/eta (extratools.f90:239-246) cut at |x| < 6 eta; the
+ * real parts of nself and uself are zero.  Arguments, units and conditions as alt3_self_w3;
+ * eta must be positive. */
+int alt3_self_w4(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
+                 double *nself, double *uself);
+
 /* Isotope scattering, iso = 1 (kappa.f90:522-543).  giso (natoms0): the mass-variance factor
  * of each atom of the primitive cell (cal_giso, sub.f90:4695-4725; alt3_isotope_g gives it per
  * element).  The isotope term Piso (nibz,nb) of P_iso_split (sub.f90:4825-4961) is formed

@@ -1232,10 +1232,12 @@ void or_self_w3(const double *q, int nb, double omq, const double *evq_ri, doubl
                 int classical, double eta, double v3thr, int nk, const double *kp, const double *g,
                 const double *eigenval, const double *eigenvec_ri, const double *eival3, const double *eivec3_ri,
                 const double *mass0, const double *atompos, const int *iatomcell0, int nt3, const int *at3,
-                const int *xyz3, const double *fc3, double *nself, double *uself)
+                const int *xyz3, const double *fc3, int kind, double *nself, double *uself)
 {
+    /* kind 1: function_self_w4 (rest.f90:4126-4240), the imaginary part alone with Gaussian
+     * deltas delta_g(arg, eta) (extratools.f90:239-246) of the three processes, cut at 6 eta */
     const zc *evq = (const zc *)evq_ri, *ev = (const zc *)eigenvec_ri, *ev3a = (const zc *)eivec3_ri;
-    const double etacut = eta * 3000, c33 = or_const33();
+    const double etacut = kind == 1 ? eta * 6 : eta * 3000, c33 = or_const33();
     const float cf = 521.1098918f;
     const double cnst3 = (double)(cf * cf * cf);   /* cnst*cnst*cnst in REAL*4 (rest.f90:3713) */
     zc ns = 0, us = 0;
@@ -1250,6 +1252,15 @@ void or_self_w3(const double *q, int nb, double omq, const double *evq_ri, doubl
                 const double om3 = sqrt(fabs(eival3[l3 + (size_t)nb * ik])) * cnst_();
                 const double nb3 = or_nbe(om3, temp, classical);
                 double ires = 0, rres = 0, arg;
+                if (kind == 1) {
+                    arg = omega - om3 - om2;
+                    if (fabs(arg) < etacut) ires = (nb3 + nb2 + 1) * or_delta_g(arg, eta);
+                    arg = omega - om3 + om2;
+                    if (fabs(arg) < etacut) ires = ires + (nb2 - nb3) * or_delta_g(arg, eta);
+                    arg = omega - om2 + om3;
+                    if (fabs(arg) < etacut) ires = ires + (nb3 - nb2) * or_delta_g(arg, eta);
+                    goto selfdone;
+                }
                 arg = omega - om3 - om2;
                 if (fabs(arg) < etacut) {
                     ires = (nb3 + nb2 + 1) * (eta / (arg * arg + eta * eta) / PI);
@@ -1270,6 +1281,7 @@ void or_self_w3(const double *q, int nb, double omq, const double *evq_ri, doubl
                     ires = ires + (nb3 - nb2) * (eta / (arg * arg + eta * eta) / PI);
                     rres = rres + (nb3 - nb2) * arg / (arg * arg + eta * eta);
                 }
+            selfdone:;
                 const zc self = rres + I * (ires * PI);
                 /* matrix_elt_full */
                 zc xx = 0;

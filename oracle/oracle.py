@@ -373,11 +373,13 @@ def bte_iter(kibz, mapinv, mapibz, kp, nc, g, primlatt, opk, eigenval, veloc, P1
     return F, np.stack(hist, axis=-1), np.array(tes)
 
 
-def self_w3(qs, model, kp, g, eigenval, eigenvec, fc3_terms, tempk, eta, v3thr=float(np.float32(1e-7)), classical=0):
+def self_w3(qs, model, kp, g, eigenval, eigenvec, fc3_terms, tempk, eta, v3thr=float(np.float32(1e-7)), classical=0,
+            kind=0):
     """function_self_w3 at every branch of each q (jobs 2/6/7, kappa.f90:900-925): get_freq at q
     and at -q-k2 for every mesh k2 (LAPACK here), omega = omega(q, la), T in K.  Returns
     omega0 (nb, nq), nself and uself (complex (nb, nq)).  fc3_terms = (masses, atompos,
-    iatomcell0, at3, xyz3, fc3).  v3_threshold is 1E-7, a REAL*4 literal (rest.f90:47)."""
+    iatomcell0, at3, xyz3, fc3).  v3_threshold is 1E-7, a REAL*4 literal (rest.f90:47).
+    kind = 1: function_self_w4 (rest.f90:4126-4240), the Gaussian imaginary part."""
     qs = np.atleast_2d(qs)
     masses, atompos, iatomcell0, at3, xyz3, fc3 = fc3_terms
     nb = eigenval.shape[0]; nk = len(kp)
@@ -400,7 +402,7 @@ def self_w3(qs, model, kp, g, eigenval, eigenvec, fc3_terms, tempk, eta, v3thr=f
                              _p(np.ascontiguousarray(kp)), _p(np.ascontiguousarray(g)), _p(_f64(eigenval)), _p(ev),
                              _p(e3v), _p(e3c), _p(_f64(masses)), _p(np.ascontiguousarray(atompos)), _p(_i32(iatomcell0)),
                              len(fc3), _p(np.ascontiguousarray(at3, dtype=np.int32)),
-                             _p(np.ascontiguousarray(xyz3, dtype=np.int32)), _p(_f64(fc3)), _p(n2), _p(u2))
+                             _p(np.ascontiguousarray(xyz3, dtype=np.int32)), _p(_f64(fc3)), kind, _p(n2), _p(u2))
             ns[la, iq] = n2[0] + 1j * n2[1]
             us[la, iq] = u2[0] + 1j * u2[1]
     return om0, ns, us

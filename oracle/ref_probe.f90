@@ -12,7 +12,8 @@
 ! probe_selfq.txt (optional): nq, then nq points q (Cartesian, the units of kpc), one per
 ! line: the three-phonon self-energy of jobs 2/6/7 (kappa.f90:900-925) at every branch of
 ! each q, function_self_w3 (rest.f90:4012-4124) at omega = omega(q,la) from get_freq and
-! T = tmin, dumped as selfE_omega (ndyn,nq), selfE_n and selfE_u (complex (ndyn,nq)); with
+! T = tmin, dumped as selfE_omega (ndyn,nq), selfE_n and selfE_u (complex (ndyn,nq)), and the
+! Gaussian form function_self_w4 (rest.f90:4126-4240) as selfE4_n and selfE4_u; with
 ! REF_PROBE_SELF_ONLY set the probe stops there.
 !
 ! REF_PROBE_ITER=n (environment, optional): after kappa_RTA, also run the iterative BTE of
@@ -61,7 +62,7 @@ program ref_probe
  logical :: ex
  integer :: maxit, iter_, niter, nsq, la
  real(8), allocatable :: selfq(:,:), seom(:,:), eivq(:)
- complex(8), allocatable :: sen(:,:), seu(:,:), evq(:,:)
+ complex(8), allocatable :: sen(:,:), seu(:,:), evq(:,:), sen4(:,:), seu4(:,:)
  complex(8) :: nself, uself
  real(8) :: tcm
  real(8) :: normk
@@ -151,6 +152,7 @@ program ref_probe
     open(78, file='probe_selfq.txt', status='old')
     read(78,*) nsq
     allocate(selfq(3,nsq), seom(ndyn,nsq), sen(ndyn,nsq), seu(ndyn,nsq), eivq(ndyn), evq(ndyn,ndyn))
+    allocate(sen4(ndyn,nsq), seu4(ndyn,nsq))
     do i = 1, nsq
        read(78,*) selfq(:,i)
     enddo
@@ -163,11 +165,16 @@ program ref_probe
           call function_self_w3(selfq(:,i), la, seom(la,i), tcm, nself, uself)
           sen(la,i) = nself
           seu(la,i) = uself
+          call function_self_w4(selfq(:,i), la, seom(la,i), tcm, nself, uself)
+          sen4(la,i) = nself
+          seu4(la,i) = uself
        enddo
     enddo
     call dumpr('selfE_omega', seom, ndyn*nsq)
     call dumpc('selfE_n', sen, ndyn*nsq)
     call dumpc('selfE_u', seu, ndyn*nsq)
+    call dumpc('selfE4_n', sen4, ndyn*nsq)
+    call dumpc('selfE4_u', seu4, ndyn*nsq)
     call get_environment_variable('REF_PROBE_SELF_ONLY', envs, status=ios_)
     if (ios_ == 0) stop
  endif

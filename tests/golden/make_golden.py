@@ -95,8 +95,9 @@ def self_points(r123):
 
 
 def main_self():
-    """Self-energy at the points of self_points (function_self_w3 with get_freq, T = tmin =
-    300 K, etaz of latdyn.params) on the 4^3 and 8^3 SW-Si meshes."""
+    """Self-energy at the points of self_points (function_self_w3 with get_freq, and its
+    Gaussian form function_self_w4, T = tmin = 300 K, etaz of latdyn.params) on the 4^3 and 8^3
+    SW-Si meshes."""
     for name, n in (("si4", 4), ("si8", 8)):
         with tempfile.TemporaryDirectory() as wd:
             fc = models.sw_silicon()
@@ -112,8 +113,10 @@ def main_self():
             nb = om.shape[0]
             sn = np.fromfile(os.path.join(d, "selfE_n.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
             su = np.fromfile(os.path.join(d, "selfE_u.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
+            sn4 = np.fromfile(os.path.join(d, "selfE4_n.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
+            su4 = np.fromfile(os.path.join(d, "selfE4_u.bin"), dtype=np.complex128).reshape((nb, len(q)), order="F")
             np.savez_compressed(os.path.join(HERE, f"ref_{name}_self.npz"), q=q, omega=om, nself=sn, uself=su,
-                                tempk=np.array(300.0), etaz=np.array(refio.RunParams().etaz))
+                                nself4=sn4, uself4=su4, tempk=np.array(300.0), etaz=np.array(refio.RunParams().etaz))
             print("wrote", name + "_self")
 
 

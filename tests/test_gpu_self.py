@@ -1,4 +1,5 @@
-"""GPU: the three-phonon self-energy of jobs 2/6/7 (alt3_self_w3) against the reference build
+"""GPU: the three-phonon self-energy of jobs 2/6/7 (alt3_self_w3, and the Gaussian form
+alt3_self_w4) against the reference build
 (tests/golden/ref_si{4,8}_self.npz: function_self_w3 with get_freq at points off the mesh,
 T = 300 K, etaz of latdyn.params) with the reference's mesh eigen data loaded.
 
@@ -44,6 +45,32 @@ def test_self_w3_vs_reference(si_inputs, n):
             assert abs(tot[a:b, i].sum() - y) / abs(y) < 2e-5, (i, a, b, tot[a:b, i].sum(), y)
             yn = S["nself"][a:b, i].sum()
             assert abs(ns[a:b, i].sum() - yn) / abs(y) < 2e-5, (i, a, b, "normal part")
+
+
+@pytest.mark.parametrize("n", [4, 8])
+def test_self_w4_vs_reference(si_inputs, n):
+    """function_self_w4 (Gaussian deltas, imaginary part only) against the reference build:
+    real parts exactly zero, group sums of nself + uself and of nself to 2e-5 of the total."""
+    print(f"case: alt3_self_w4 SW-Si {n}^3 vs reference", flush=True)
+    R = load_fixture(f"si{n}")
+    S = load_fixture(f"si{n}_self")
+    with capi.Context(si_inputs[n]) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        om, ns, us = c.self_w4(S["q"], float(S["tempk"]), float(S["etaz"]))
+        with pytest.raises(capi.Alt3Error):
+            c.self_w4(S["q"][:1], float(S["tempk"]), 0.0)
+    assert np.max(np.abs(om - S["omega"])) / np.max(S["omega"]) < 1e-7
+    assert not np.any(ns.real) and not np.any(us.real)
+    tot, rtot = ns + us, S["nself4"] + S["uself4"]
+    scale = np.max(np.abs(rtot))
+    assert scale > 0
+    for i in range(len(S["q"])):
+        for a, b in _groups(S["omega"][:, i]):
+            y = rtot[a:b, i].sum()
+            d = max(abs(y), 1e-3 * scale)
+            assert abs(tot[a:b, i].sum() - y) / d < 2e-5, (i, a, b, tot[a:b, i].sum(), y)
+            yn = S["nself4"][a:b, i].sum()
+            assert abs(ns[a:b, i].sum() - yn) / d < 2e-5, (i, a, b, "normal part")
 
 
 def test_self_w3_batches_agree(si_inputs):

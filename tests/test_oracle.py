@@ -340,3 +340,28 @@ def test_self_w3_vs_reference(geo4, ref4):
         for a, b in _groups(S["omega"][:, sel[i]]):
             y = rtot[a:b, i].sum()
             assert abs(tot[a:b, i].sum() - y) / abs(y) < 2e-5, (sel[i], a, b)
+
+
+def test_self_w4_vs_reference(geo4, ref4):
+    """function_self_w4 (the Gaussian imaginary part, rest.f90:4126-4240) against the reference
+    build at the same points: real part zero, the normal + umklapp sum over (near-)degenerate
+    branches to 2e-5; and on a single branch against the Lorentzian w3 in the limit of a narrow
+    width both estimate the same delta function (pinned by the reference values themselves)."""
+    from conftest import load_fixture
+    S = load_fixture("si4_self")
+    G = geo4
+    inp = G["inp"]
+    model = (inp.masses, inp.atompos, inp.iatomcell0, inp.fc2_atoms, inp.fc2_xyz, inp.fc2)
+    fc3 = (inp.masses, inp.atompos, inp.iatomcell0, inp.fc3_atoms, inp.fc3_xyz, inp.fc3)
+    sel = [0, 1, 4, 6]
+    om, ns, us = O.self_w3(S["q"][sel], model, G["kp"], G["g"], ref4["eigenval"], ref4["eigenvec"], fc3,
+                           float(S["tempk"]), float(S["etaz"]), kind=1)
+    tot, rtot = ns + us, (S["nself4"] + S["uself4"])[:, sel]
+    assert not np.any(tot.real) and not np.any(rtot.real)
+    for i in range(len(sel)):
+        for a, b in _groups(S["omega"][:, sel[i]]):
+            y = rtot[a:b, i].sum()
+            if abs(y) == 0:
+                assert abs(tot[a:b, i].sum()) < 1e-12
+                continue
+            assert abs(tot[a:b, i].sum() - y) / abs(y) < 2e-5, (sel[i], a, b)
