@@ -190,7 +190,8 @@ int alt3_kappa_iter(alt3_ctx *ctx, double tempk, int max_iter, double update_mix
 int alt3_self_w3(alt3_ctx *ctx, int nq, const double *q, double tempk, double eta, double v3_threshold, double *omega0,
                  double *nself, double *uself);
 
-/* The Gaussian form of the same self-energy, function_self_w4 (rest.f90:4126-4240): the
+/* The Gaussian form of the same self-energy, function_self_w4 (rest.f90:4126-4240; self_energy
+ * calls it for iread = 4, kappa.f90:3311, and function_self_w3 for iread = 3): the
  * imaginary part alone, sum over k, l2, l3 of |V3|^2 [(n2+n3+1) delta_g(omega-omega3-omega2)
  * + (n2-n3) delta_g(omega-omega3+omega2) + (n3-n2) delta_g(omega-omega2+omega3)] pi / nk, with
  * delta_g(x) = exp(-x^2/2/eta^2)/sqrt(2 pi)/eta (extratools.f90:239-246) cut at |x| < 6 eta; the
