@@ -57,6 +57,31 @@ __device__ double block_sum(double v, double *sh)
     return r;
 }
 
+// (a, b) summed over the workgroup (blockDim.x a multiple of 64, at most 1024): butterflies
+// inside each wavefront, then the wavefront partials through LDS (sh: 2 doubles per
+// wavefront), added by every thread in the same order, so all threads hold the same totals
+__device__ void block_sum2(double &a, double &b, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[2 * w] = a;
+        sh[2 * w + 1] = b;
+    }
+    __syncthreads();
+    double x = 0, y = 0;
+    for (int i = 0; i < nw; i++) {
+        x += sh[2 * i];
+        y += sh[2 * i + 1];
+    }
+    __syncthreads();
+    a = x;
+    b = y;
+}
+
 // the star table of the collision and update kernels (see star_sums)
 struct StarTasks {
     const int *slot_n2;   // [nibz] irreducible point of slot x (slots by decreasing star size)
@@ -797,7 +822,7 @@ __global__ void __launch_bounds__(1024) k_cg_update(int n, int it, double tol, d
                                                     const double *__restrict__ aux, double *__restrict__ v0,
                                                     int *__restrict__ state)
 {
-    __shared__ double sh[1024];
+    __shared__ double sh[32];
     if (state[0]) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     double a = 0, b = 0;
@@ -805,7 +830,8 @@ __global__ void __launch_bounds__(1024) k_cg_update(int n, int it, double tol, d
         a += grad[i] * conj[i];
         b += aux[i] * conj[i];
     }
-    const double num = block_sum(a, sh), den = block_sum(b, sh);
+    block_sum2(a, b, sh);
+    const double num = a, den = b;
     const double r = num / den;
     a = b = 0;
     for (int i = tid; i < n; i += nt) {
@@ -815,7 +841,8 @@ __global__ void __launch_bounds__(1024) k_cg_update(int n, int it, double tol, d
         a += vmin[i] * vmin[i];
         b += conj[i] * conj[i];
     }
-    const double v2 = block_sum(a, sh), cc = block_sum(b, sh);
+    block_sum2(a, b, sh);
+    const double v2 = a, cc = b;
     const double err = (num * num / den / den) * cc / v2;
     if (err < tol) {
         if (tid == 0) {
@@ -829,7 +856,8 @@ __global__ void __launch_bounds__(1024) k_cg_update(int n, int it, double tol, d
         a += grad[i] * grad[i];
         b += v0[i] * v0[i];
     }
-    const double gg = block_sum(a, sh), vv = block_sum(b, sh);
+    block_sum2(a, b, sh);
+    const double gg = a, vv = b;
     for (int i = tid; i < n; i += nt) conj[i] = -grad[i] + gg / vv * conj[i];
     if (tid == 0) state[1] = it;
 }
