@@ -243,17 +243,34 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
                 }
             }
         }
+        // the group's 3*NT partial sums: butterflies inside each of the four wavefronts, then the
+        // wavefront partials through LDS, added in wavefront order by thread x < 3*NT
 #pragma unroll
         for (int t = 0; t < NT; t++) {
             if (!valid) q[t] = qn[t] = qu[t] = 0;
-            const double a = block_sum(q[t], sh), b = block_sum(qn[t], sh), c = block_sum(qu[t], sh);
-            if (threadIdx.x == 0) {
-                double *o = qpart + t * qstr + 3 * (((size_t)s * NB + l1) * nch + ch);
-                o[0] = a;
-                o[1] = b;
-                o[2] = c;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                q[t] += __shfl_xor(q[t], off, 64);
+                qn[t] += __shfl_xor(qn[t], off, 64);
+                qu[t] += __shfl_xor(qu[t], off, 64);
             }
         }
+        const int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                sh[(3 * t) * 4 + wv] = q[t];
+                sh[(3 * t + 1) * 4 + wv] = qn[t];
+                sh[(3 * t + 2) * 4 + wv] = qu[t];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 3 * NT) {
+            const int x = threadIdx.x, t = x / 3;
+            const double tot = ((sh[4 * x] + sh[4 * x + 1]) + sh[4 * x + 2]) + sh[4 * x + 3];
+            qpart[t * qstr + 3 * (((size_t)s * NB + l1) * nch + ch) + x % 3] = tot;
+        }
+        __syncthreads();
     }
 }
 
