@@ -46,13 +46,13 @@ __global__ void k_dist(int n, const double *__restrict__ freq, double temp, int 
 
 __device__ double block_sum(double v, double *sh)
 {
-    sh[threadIdx.x] = v;
+    // blockDim.x a multiple of 64: butterflies inside each wavefront, then the wavefront
+    // partials through LDS, added by every thread in wavefront order
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
     __syncthreads();
     return r;
 }

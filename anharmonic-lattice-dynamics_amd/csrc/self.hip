@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
                                                   int classical, double eta, double v3thr, double cnst,
                                                   int kind, double *__restrict__ part)
 {
-    __shared__ double red[4][256];
+    __shared__ double red[4][4];   // [sum][wavefront]
     const int s = blockIdx.y, nch = gridDim.x, tid = threadIdx.x;
     const int k2 = blockIdx.x * 256 + tid;
     const bool valid = k2 < nk;
@@ -142,17 +142,23 @@ __global__ void __launch_bounds__(256) k_self_sum(int nk, int nb, const double *
                 }
             }
         }
-        red[0][tid] = nr;
-        red[1][tid] = ni;
-        red[2][tid] = ur;
-        red[3][tid] = ui;
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (tid < st)
-                for (int c = 0; c < 4; c++) red[c][tid] += red[c][tid + st];
-            __syncthreads();
+        // butterflies inside each of the four wavefronts, then the wavefront partials in LDS
+        for (int off = 32; off > 0; off >>= 1) {
+            nr += __shfl_xor(nr, off, 64);
+            ni += __shfl_xor(ni, off, 64);
+            ur += __shfl_xor(ur, off, 64);
+            ui += __shfl_xor(ui, off, 64);
         }
-        if (tid < 4) part[(((size_t)s * nch + blockIdx.x) * nb + l1) * 4 + tid] = red[tid][0];
+        if ((tid & 63) == 0) {
+            red[0][tid >> 6] = nr;
+            red[1][tid >> 6] = ni;
+            red[2][tid >> 6] = ur;
+            red[3][tid >> 6] = ui;
+        }
+        __syncthreads();
+        if (tid < 4)
+            part[(((size_t)s * nch + blockIdx.x) * nb + l1) * 4 + tid] =
+                ((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3];
         __syncthreads();
     }
 }
