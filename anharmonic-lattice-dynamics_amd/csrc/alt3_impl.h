@@ -79,6 +79,16 @@ struct MeshDims {
     int n0, n1, n2;
 };
 
+// ---------------------------------------------------------------- Born charges
+// constants of nonanal (rest.f90:3517-3578) for the eigen kernels; on == 0: no term
+struct BornDev {
+    int on;
+    double epsil[9];     // row-major epsil(i,j)
+    double eps_scale;    // 8.8541878176e-12 / 1e10 / ee
+    double om0;          // |r3 . (r1 x r2)|
+    double rho2;         // ((|g1||g2||g3|)^0.33 / 4)^2
+};
+
 // ---------------------------------------------------------------- host tables
 struct HostTables {
     int nk = 0, nibz = 0, nops = 0, npos = 0;
@@ -122,6 +132,9 @@ int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, dou
 int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, const double *w3, const double *w1,
                    double *out, int *d_err);
 bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
+bool v3_chain_supported(const alt3_ctx *ctx);
+int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
+                 int *d_err);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
@@ -149,6 +162,8 @@ struct alt3_ctx {
     alt3::DBuf<double> d_eival, d_freq, d_vel;     // [nk][nb], [nk][nb], [nk][nb][3]
     alt3::DBuf<double> d_evec;                     // [nk][nb][nb] complex (interleaved)
     alt3::DBuf<int> d_mappos;
+    alt3::BornDev born{};                          // non-analytic term (born_flag != 0)
+    alt3::DBuf<double> d_born;                     // [natoms0][10]: charge(i,j) row-major, mass
     bool have_eig = false;
     bool have_veloci = false;                      // d_veloci holds the star averages of d_vel
 
@@ -225,6 +240,7 @@ struct alt3_ctx {
     alt3::DBuf<int> d_negs;                        // subst_eivecs pairs
     alt3::DBuf<double> d_qpart, d_wq, d_ev7, d_evec0;
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
+    alt3::DBuf<float> d_chain_e, d_chain_g;        // operand packs of k_v3_chain (eigenvectors, G')
     alt3::DBuf<double> d_red;                      // reduction scratch
     alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
     alt3::DBuf<double> d_solve_x, d_solve_te;      // alt3_solve: CG vector, tauinv_eff

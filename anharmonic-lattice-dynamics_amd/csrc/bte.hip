@@ -274,17 +274,18 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
     }
 }
 
-// Q, QN, QU of (s, l1) = sum of the m partial sums of that row; with the isotope term
-// (piso [nibz][nb], else NULL) Piso is added to all three, as calculate_tauRTA_iso does
-// (sub.f90:8502-8504)
-__global__ void k_tau_sum(int n, int nb, int ibz_lo, const double *__restrict__ qpart, const double *__restrict__ piso,
-                          double *__restrict__ Q, double *__restrict__ QN, double *__restrict__ QU)
+// Q, QN, QU of (s, l1) = sum of the parts partial sums of that row; with the isotope term
+// (piso [nibz][nb], else NULL) Piso of the irreducible point ibz_lo + s is added to all three, as
+// calculate_tauRTA_iso does (sub.f90:8502-8504)
+__global__ void k_tau_sum(int n, int parts, int nb, int ibz_lo, const double *__restrict__ qpart,
+                          const double *__restrict__ piso, double *__restrict__ Q, double *__restrict__ QN,
+                          double *__restrict__ QU)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;   // (s, l1)
     if (x >= n) return;
     double q = 0, qn = 0, qu = 0;
-    for (int l2 = 0; l2 < nb; l2++) {
-        const double *p = qpart + 3 * ((size_t)x * nb + l2);
+    for (int l2 = 0; l2 < parts; l2++) {
+        const double *p = qpart + 3 * ((size_t)x * parts + l2);
         q += p[0];
         qn += p[1];
         qu += p[2];
@@ -1117,7 +1118,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
     }
     const double *piso = ctx->giso.empty() ? nullptr : ctx->d_piso.p;
     for (int t = 0; t < nt; t++)
-        hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, parts,
+        hipLaunchKernelGGL(k_tau_sum, dim3((ksub * nb + 255) / 256), dim3(256), 0, ctx->stream, ksub * nb, parts, nb,
                            ctx->ibz_lo, qpart.p + t * pstr, piso, ctx->d_q.p + t * qstr, ctx->d_qn.p + t * qstr,
                            ctx->d_qu.p + t * qstr);
     ALT3_HIP(hipGetLastError());

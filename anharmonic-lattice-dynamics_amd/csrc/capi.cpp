@@ -144,7 +144,7 @@ int alt3_destroy(alt3_ctx *ctx)
     // DBuf members release in their destructors? they are PODs; release explicitly
     ctx->d_fc2_rc.release(); ctx->d_fc2_amp.release(); ctx->d_fc2_rr.release();
     ctx->d_kpc_eig.release(); ctx->d_kpc.release(); ctx->d_eival.release(); ctx->d_freq.release();
-    ctx->d_vel.release(); ctx->d_evec.release(); ctx->d_mappos.release();
+    ctx->d_vel.release(); ctx->d_evec.release(); ctx->d_mappos.release(); ctx->d_born.release();
     ctx->d_t_abc.release(); ctx->d_t_psi.release(); ctx->d_t_rj.release(); ctx->d_t_rk.release();
     ctx->d_dr.release(); ctx->d_slot_dr.release(); ctx->d_slot_bc.release(); ctx->d_slot_ptr.release();
     ctx->d_sterm_a.release(); ctx->d_sterm_psi.release(); ctx->d_sterm_rk.release();
@@ -159,12 +159,23 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_cm.release(); ctx->d_cmf.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
+    ctx->d_chain_e.release(); ctx->d_chain_g.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return ALT3_OK;
+}
+
+int alt3_device_free(long long *free_bytes, long long *total_bytes)
+{
+    if (!free_bytes) return fail(ALT3_E_ARG, "null argument");
+    size_t f = 0, t = 0;
+    ALT3_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = (long long)f;
+    if (total_bytes) *total_bytes = (long long)t;
     return ALT3_OK;
 }
 
@@ -179,6 +190,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     b += ctx->d_s1.bytes() + ctx->d_s2.bytes() + ctx->d_s1f.bytes() + ctx->d_s2f.bytes() + ctx->d_upd.bytes();
     b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
     b += ctx->d_ephf.bytes() + ctx->d_evf.bytes() + ctx->d_gpf32.bytes();
+    b += ctx->d_chain_e.bytes() + ctx->d_chain_g.bytes();
     *bytes = b;
     return ALT3_OK;
 }
@@ -196,6 +208,7 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
     if (precision == ALT3_V3_MIXED) {
         const long long Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb * nb + 15) / 16) * 16;
         b += nel * 4 + ni * nb * Kp * Np * 4 + nk * nb * nb * 8 + nk * ndr * 8;
+        if (nb == 24) b += nk * 60 * 64 * 4 + ni * nb * (Kp / 4) * 36 * 64 * 4;                // k_v3_chain packs
     } else {
         b += nel * 8;
     }

@@ -10,6 +10,8 @@
 // |lambda|, eigenvectors multiplied by i when "pure imaginary", the second sort of
 // get_frequencies that permutes eigenvalues/eigenvectors but not velocities, and
 // subst_eivecs (rest.f90:3262-3293).
+#include <cmath>
+
 #include "alt3_impl.h"
 
 namespace {
@@ -21,13 +23,39 @@ __device__ __forceinline__ zd zmul(zd a, zd b) { return {a.r * b.r - a.i * b.i, 
 
 constexpr int kDq = 7;  // q, q+dx, q-dx, q+dy, q-dy, q+dz, q-dz
 
+// nonanal (rest.f90:3517-3578), entry e = (row, col) of D at kq: the real term
+//   zag(i) zbg(j) / (qeq eps_scale) / om0 / sqrt(ma mb) exp(-q^2/rho^2)
+// with zag(i) = sum_k q_k charge_a(k,i), qeq = q.epsil.q, evaluated in the reference's order;
+// get_freq shifts q = 0 to (1e-10, 1e-10, 1e-10) first (rest.f90:4342-4346).  zm [natoms0][10]
+// holds charge(i,j) row-major and the mass of each atom.
+__device__ __forceinline__ double nonanal_term(const alt3::BornDev &b, const double *__restrict__ zm, const double *q,
+                                               double qeq, int nb, int e)
+{
+    const int r = e / nb, c = e - r * nb, na = r / 3, i = r - 3 * na, nbb = c / 3, j = c - 3 * nbb;
+    const double *za = zm + 10 * na, *zb = zm + 10 * nbb;
+    const double zag = q[0] * za[i] + q[1] * za[3 + i] + q[2] * za[6 + i];
+    const double zbg = q[0] * zb[j] + q[1] * zb[3 + j] + q[2] * zb[6 + j];
+    const double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    return zag * zbg / (qeq * b.eps_scale) / b.om0 / sqrt(za[9] * zb[9]) * exp(-q2 / b.rho2);
+}
+
+__device__ __forceinline__ void born_q(const double *kq, double *q, const alt3::BornDev &b, double &qeq)
+{
+    const bool z = kq[0] == 0.0 && kq[1] == 0.0 && kq[2] == 0.0;
+    for (int a = 0; a < 3; a++) q[a] = z ? kq[a] + 1.0e-10 : kq[a];
+    const double *E = b.epsil;
+    qeq = q[0] * (E[0] * q[0] + E[1] * q[1] + E[2] * q[2]) + q[1] * (E[3] * q[0] + E[4] * q[1] + E[5] * q[2]) +
+          q[2] * (E[6] * q[0] + E[7] * q[1] + E[8] * q[2]);
+}
+
 // entries: e = row*nb + col; per entry a CSR range of FC2 terms (reference term order)
 __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, const double *__restrict__ kpc,
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
                                                   const double *__restrict__ term_rr, double dq,
                                                   double *__restrict__ evals7,   // [nk][7][nb]
-                                                  double *__restrict__ evec0)    // [nk][nb col][nb comp] complex
+                                                  double *__restrict__ evec0,    // [nk][nb col][nb comp] complex
+                                                  alt3::BornDev born, const double *__restrict__ bzm)
 {
     extern __shared__ double lds[];
     const int m = blockIdx.x;  // matrix id = q*nvar + variant (nvar 7: q and q +- dq e_a; 1: q only)
@@ -105,6 +133,12 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, cons
         __syncthreads();
         if (realify)
             for (int e = lane; e < nn; e += 64) A[e] = {A[e].r, 0.0};
+        __syncthreads();
+    }
+    if (born.on) {   // nonanal after set_dynamical_matrix (rest.f90:4348)
+        double qb[3], qeq;
+        born_q(kq, qb, born, qeq);
+        for (int e = lane; e < nn; e += 64) A[e].r = A[e].r + nonanal_term(born, bzm, qb, qeq, nb, e);
         __syncthreads();
     }
     // zheev('U') reads the upper triangle and a real diagonal: make A exactly Hermitian
@@ -241,7 +275,8 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
                                                   const double *__restrict__ term_rr, double dq,
-                                                  double *__restrict__ evals7, double *__restrict__ evec0)
+                                                  double *__restrict__ evals7, double *__restrict__ evec0,
+                                                  alt3::BornDev born, const double *__restrict__ bzm)
 {
     constexpr int M = 64 / G;
     extern __shared__ double lds[];
@@ -318,6 +353,12 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
             for (int e = sub; e < nn; e += G) A[e] = {A[e].r, 0.0};
     }
     __syncthreads();
+    if (born.on) {   // nonanal after set_dynamical_matrix (rest.f90:4348)
+        double qb[3], qeq;
+        born_q(kq, qb, born, qeq);
+        for (int e = sub; e < nn; e += G) A[e].r = A[e].r + nonanal_term(born, bzm, qb, qeq, nb, e);
+        __syncthreads();
+    }
     for (int e = sub; e < nn; e += G) {
         int r = e / nb, c = e % nb;
         if (r > c) A[e] = {A[c * nb + r].r, -A[c * nb + r].i};
@@ -477,8 +518,36 @@ __global__ void k_subst(int nneg, int nb, const int *__restrict__ negs, const in
 
 namespace alt3 {
 
+// read_born's data and the constants nonanal derives from the lattice (rest.f90:3528-3538)
+int born_prepare(alt3_ctx *ctx, const alt3_system *sys)
+{
+    ctx->born = BornDev{};
+    if (!sys->born_flag) return ALT3_OK;
+    if (!sys->epsil || !sys->zeu) return fail(ALT3_E_ARG, "born_flag set without epsil and charges");
+    BornDev &b = ctx->born;
+    b.on = 1;
+    for (int k = 0; k < 9; k++) b.epsil[k] = sys->epsil[k];
+    b.eps_scale = 8.8541878176e-12 / 1e10 / EE;
+    const double *g = ctx->T.g, *r = sys->r123;
+    auto len = [](const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    const double gg = std::pow(len(g) * len(g + 3) * len(g + 6), (double)0.33f);   // **0.33, a REAL*4 literal
+    const double rho = gg / 4.0;
+    b.rho2 = rho * rho;
+    const double cx = r[1] * r[5] - r[2] * r[4], cy = r[2] * r[3] - r[0] * r[5], cz = r[0] * r[4] - r[1] * r[3];
+    b.om0 = std::fabs(r[6] * cx + r[7] * cy + r[8] * cz);   // calculate_volume: |r3 . (r1 x r2)|
+    std::vector<double> zm((size_t)10 * sys->natoms0);
+    for (int a = 0; a < sys->natoms0; a++) {
+        for (int k = 0; k < 9; k++) zm[10 * a + k] = sys->zeu[9 * a + k];
+        zm[10 * a + 9] = sys->mass[a];
+    }
+    ALT3_TRY(ctx->d_born.alloc(zm.size()));
+    ALT3_HIP(hipMemcpy(ctx->d_born.p, zm.data(), sizeof(double) * zm.size(), hipMemcpyHostToDevice));
+    return ALT3_OK;
+}
+
 int eig_prepare(alt3_ctx *ctx, const alt3_system *sys)
 {
+    ALT3_TRY(born_prepare(ctx, sys));
     const int nb = ctx->nb, nn = nb * nb;
     std::vector<std::vector<int>> per(nn);
     for (int t = 0; t < sys->nt2; t++) {
@@ -530,7 +599,7 @@ int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, dou
         dim3 grid((unsigned)((nmat + M - 1) / M));
 #define ALT3_EIG_GROUP(GG)                                                                                     \
         hipLaunchKernelGGL(k_eig_group<GG>, grid, dim3(64), lds, ctx->stream, np, nvar, nb, kq, ctx->d_fc2_rc.p, \
-                           ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec);
+                           ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born, ctx->d_born.p);
         if (G == 8) {
             ALT3_EIG_GROUP(8)
         } else if (G == 16) {
@@ -545,7 +614,8 @@ int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, dou
         if (lds > 65536)   // cells with more than 44 branches
             ALT3_HIP(hipFuncSetAttribute((const void *)k_eig_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_eig_solve, dim3((unsigned)((size_t)np * nvar)), dim3(64), lds, ctx->stream, np, nvar, nb,
-                           kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec);
+                           kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born,
+                           ctx->d_born.p);
     }
     ALT3_HIP(hipGetLastError());
     return ALT3_OK;

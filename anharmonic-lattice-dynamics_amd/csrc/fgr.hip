@@ -128,11 +128,15 @@ __device__ void weight_tet1(const float *w, double e, double *c)
     c[3] = out[3];
 }
 
-// The same weights entirely in FP32, for the mixed path (whose |V3|^2 and P are FP32): the
+// The same weights in FP32 arithmetic, for the mixed path (whose |V3|^2 and P are FP32): the
 // same sort, case split and 1e-10 clamp.  The quotients are rewritten over the six hardware
 // reciprocals 1/e_ij (v_rcp_f32, 1 ulp), grouped so that every partial product stays bounded
-// (d/e_ij <= 1 inside its case), instead of up to twenty IEEE divisions.
-__device__ __forceinline__ void weight_tet1f(const float *w, float e, float *c)
+// (d/e_ij <= 1 inside its case), instead of up to twenty IEEE divisions.  e stays REAL*8 as in
+// the reference: the case tests and the differences e - e_i are taken in FP64 and only then
+// rounded.  (Rounding e itself to REAL*4 makes it tie exactly with corner energies -- the
+// energy -w(Gamma, acoustic) + w(q1, l1) of a Gamma corner rounds to the same float as w(q1, l1)
+// -- and a tie inside a degenerate pair e_i = e_j gives 0 * (1/0).)
+__device__ __forceinline__ void weight_tet1f(const float *w, double e, float *c)
 {
     float en[4];
     int uns[4];
@@ -147,35 +151,40 @@ __device__ __forceinline__ void weight_tet1f(const float *w, float e, float *c)
     }
     CLAMPF(e21) CLAMPF(e31) CLAMPF(e41) CLAMPF(e42) CLAMPF(e43)
 #undef CLAMPF
+    const double E1 = e1, E2 = e2, E3 = e3, E4 = e4;
     float cc[4];
-    if (e < e1 || e > e4) {
+    if (e < E1 || e > E4) {
         c[0] = c[1] = c[2] = c[3] = 0;
         return;
-    } else if (e <= e2) {
+    }
+    // e - e_i (>= 0 for the corners below e) and e_i - e (above), each rounded once
+    const float d1 = (float)(e - E1), d2 = (float)(e - E2), u3 = (float)(E3 - e), u4 = (float)(E4 - e);
+    if (e <= E2) {
         const float q21 = __builtin_amdgcn_rcpf(e21), q31 = __builtin_amdgcn_rcpf(e31), q41 = __builtin_amdgcn_rcpf(e41);
-        const float d1 = e - e1, g = (d1 * q21) * (d1 * q31) * q41;   // d1^2 / (e21 e31 e41)
-        cc[3] = ((e2 - e) * q21 + (e3 - e) * q31 + (e4 - e) * q41) * g;
+        const float g = (d1 * q21) * (d1 * q31) * q41;   // d1^2 / (e21 e31 e41)
+        cc[3] = (-d2 * q21 + u3 * q31 + u4 * q41) * g;
         cc[2] = g * (d1 * q21);
         cc[1] = g * (d1 * q31);
         cc[0] = g * (d1 * q41);
-    } else if (e <= e3) {
+    } else if (e <= E3) {
         const float q31 = __builtin_amdgcn_rcpf(e31), q41 = __builtin_amdgcn_rcpf(e41);
         const float q32 = __builtin_amdgcn_rcpf(e32), q42 = __builtin_amdgcn_rcpf(e42);
-        const float a32 = (e3 - e) * (e - e2), a41 = (e4 - e) * (e - e1), a31 = (e3 - e) * (e - e1), a42 = (e4 - e) * (e - e2);
+        const float a32 = u3 * d2, a41 = u4 * d1, a31 = u3 * d1, a42 = u4 * d2;
         const float i1 = q42 * q32, i2 = q41 * q42, i3 = q32 * q41, i4 = q42 * q31, i5 = q31 * q32, i6 = q31 * q41;
         const float b11 = a32 * i1 + a41 * i2 + a31 * i3, b12 = a41 * i4 + a42 * i1 + a31 * i5;
         const float b21 = a32 * i4 + a42 * i2 + a31 * i6, b22 = a32 * i5 + a41 * i6 + a42 * i3;
-        cc[3] = 0.5f * (((e3 - e) * q31) * q31 * b11 + ((e4 - e) * q41) * q41 * b12);
-        cc[2] = 0.5f * (((e3 - e) * q32) * q32 * b21 + ((e4 - e) * q42) * q42 * b22);
-        cc[1] = 0.5f * (((e - e2) * q32) * q32 * b21 + ((e - e1) * q31) * q31 * b11);
-        cc[0] = 0.5f * (((e - e2) * q42) * q42 * b22 + ((e - e1) * q41) * q41 * b12);
+        cc[3] = 0.5f * ((u3 * q31) * q31 * b11 + (u4 * q41) * q41 * b12);
+        cc[2] = 0.5f * ((u3 * q32) * q32 * b21 + (u4 * q42) * q42 * b22);
+        cc[1] = 0.5f * ((d2 * q32) * q32 * b21 + (d1 * q31) * q31 * b11);
+        cc[0] = 0.5f * ((d2 * q42) * q42 * b22 + (d1 * q41) * q41 * b12);
     } else {
         const float q41 = __builtin_amdgcn_rcpf(e41), q42 = __builtin_amdgcn_rcpf(e42), q43 = __builtin_amdgcn_rcpf(e43);
-        const float d4 = e4 - e, h = (d4 * q41) * (d4 * q42) * q43;   // d4^2 / (e41 e42 e43)
-        cc[3] = h * (d4 * q41);
-        cc[2] = h * (d4 * q42);
-        cc[1] = h * (d4 * q43);
-        cc[0] = ((e - e3) * q43 + (e - e2) * q42 + (e - e1) * q41) * h;
+        const float d3 = (float)(e - E3);
+        const float h = (u4 * q41) * (u4 * q42) * q43;   // d4^2 / (e41 e42 e43)
+        cc[3] = h * (u4 * q41);
+        cc[2] = h * (u4 * q42);
+        cc[1] = h * (u4 * q43);
+        cc[0] = (d3 * q43 + d2 * q42 + d1 * q41) * h;
     }
     float out[4] = {0, 0, 0, 0};
     for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
@@ -368,7 +377,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
             if constexpr (sizeof(VT) == sizeof(float)) {
                 float c[4];
-                weight_tet1f(w4, (float)e, c);
+                weight_tet1f(w4, e, c);
                 for (int y = 0; y < 4; y++)
                     if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)(c[y] / 6.0f));
             } else {

@@ -230,17 +230,21 @@ int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
         const size_t ne = (size_t)nk * nb * nb * 2, np = (size_t)nk * ndr * 2, ng = (size_t)ksub * ndr * nb * nb * nb * 2;
         ALT3_TRY(ephf.reserve(np));
         ALT3_TRY(evf.reserve(ne));
-        ALT3_TRY(gpf.reserve(ng));
+        if (!v3_chain_supported(ctx)) ALT3_TRY(gpf.reserve(ng));
         hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, np,
                            ctx->d_eph.p, ephf.p);
         hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, ne,
                            ctx->d_evec.p, evf.p);
-        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ng,
-                           ctx->d_gp.p, gpf.p);
+        if (!v3_chain_supported(ctx))
+            hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ng,
+                               ctx->d_gp.p, gpf.p);
         ALT3_HIP(hipGetLastError());
-        ALT3_TRY(dispatch_rows<float>(ctx, ksub, ephf.p, evf.p, wq.p, gpf.p, ctx->d_v33f.p, d_err));
+        if (v3_chain_supported(ctx))   // 24 branches: every contraction on the matrix cores
+            ALT3_TRY(v3_chain_run(ctx, ksub, evf.p, ephf.p, wq.p, ctx->d_v33f.p, d_err));
+        else
+            ALT3_TRY(dispatch_rows<float>(ctx, ksub, ephf.p, evf.p, wq.p, gpf.p, ctx->d_v33f.p, d_err));
     }
-    ALT3_HIP(hipStreamSynchronize(ctx->stream));
+    // no wait: every buffer used here belongs to the context and outlives the kernels
     return ALT3_OK;
 }
 

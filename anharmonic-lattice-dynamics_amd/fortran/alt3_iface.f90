@@ -24,6 +24,8 @@ module alt3_iface
      type(c_ptr) :: fc3_atoms, fc3_xyz, fc3
      integer(c_int) :: nc(3)
      integer(c_int) :: classical, host_only
+     integer(c_int) :: born_flag
+     type(c_ptr) :: epsil, zeu
   end type alt3_system
 
   interface

@@ -45,6 +45,10 @@ program alt3_kappa
   ! fc2.dat / fc3.dat
   integer(c_int), allocatable, target :: fc2_atoms(:, :), fc2_xyz(:, :), fc3_atoms(:, :), fc3_xyz(:, :)
   real(c_double), allocatable, target :: fc2(:), fc3(:)
+  ! dielectric.params (read_born): epsil(i,j) and charge(i,j) of each atom, row-major for the C side
+  integer :: born_flag
+  real(c_double), target :: epsil_rm(9)
+  real(c_double), allocatable, target :: zeu_rm(:)
 
   type(alt3_system) :: sys
   type(c_ptr) :: ctx
@@ -81,6 +85,7 @@ program alt3_kappa
   call read_lat_fc()
   call read_fc('fc2.dat', 2, nterms(2), fc2_atoms, fc2_xyz, fc2)
   call read_fc('fc3.dat', 3, nterms(3), fc3_atoms, fc3_xyz, fc3)
+  call read_born()
 
   ! the module state of read_lattice/read_fc234, handed over as one struct
   latparams(1:3) = latparams(1:3) * scale          ! rest.f90:92
@@ -103,6 +108,9 @@ program alt3_kappa
   sys%nc = nc
   sys%classical = classical
   sys%host_only = merge(1, 0, host_only)
+  sys%born_flag = born_flag
+  sys%epsil = c_loc(epsil_rm)
+  sys%zeu = c_loc(zeu_rm)
   call alt3_check(alt3_create(sys, ctx), 'alt3_create')
   call alt3_check(alt3_sizes(ctx, nk, nibz, nb, nops), 'alt3_sizes')
   write(*, '(a,i0,a,i0,a,i0,a,i0)') 'alt3_kappa: nk=', nk, ' nibz=', nibz, ' nb=', nb, ' nops=', nops
@@ -340,6 +348,37 @@ contains
     end do
     close(u)
   end subroutine read_lat_fc
+
+  ! read_born (rest.f90:3580-3600): born_flag, epsil(i,1:3) per record, then charge(i,1:3) of
+  ! each atom of the primitive cell, three records per atom; the library adds nonanal's term
+  ! to the dynamical matrices when born_flag /= 0
+  subroutine read_born()
+    integer :: u, k, i, j
+    real(c_double) :: e(3, 3), z(3, 3)
+    allocate(zeu_rm(9 * natoms0))
+    zeu_rm = 0
+    open(newunit=u, file='dielectric.params', status='old')
+    read(u, *) born_flag
+    do i = 1, 3
+       read(u, *) (e(i, j), j = 1, 3)
+    end do
+    do k = 1, natoms0
+       do i = 1, 3
+          read(u, *) (z(i, j), j = 1, 3)
+       end do
+       do i = 1, 3
+          do j = 1, 3
+             zeu_rm(9 * (k - 1) + 3 * (i - 1) + j) = z(i, j)
+          end do
+       end do
+    end do
+    close(u)
+    do i = 1, 3
+       do j = 1, 3
+          epsil_rm(3 * (i - 1) + j) = e(i, j)
+       end do
+    end do
+  end subroutine read_born
 
   subroutine read_fc(fname, rank, nt, atoms, xyz, val)
     ! read_fc234, rest.f90:337-402

@@ -39,6 +39,7 @@ class System(C.Structure):
         ("nt2", C.c_int), ("fc2_atoms", C.c_void_p), ("fc2_xyz", C.c_void_p), ("fc2", C.c_void_p),
         ("nt3", C.c_int), ("fc3_atoms", C.c_void_p), ("fc3_xyz", C.c_void_p), ("fc3", C.c_void_p),
         ("nc", C.c_int * 3), ("classical", C.c_int), ("host_only", C.c_int),
+        ("born_flag", C.c_int), ("epsil", C.c_void_p), ("zeu", C.c_void_p),
     ]
 
 
@@ -46,7 +47,7 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
-           "alt3_set_iso", "alt3_isotope_g"]
+           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -79,6 +80,13 @@ def isotope_g(element: str) -> float:
     g = C.c_double()
     _check(lib().alt3_isotope_g(element.encode(), C.byref(g)))
     return g.value
+
+
+def device_free_bytes() -> int:
+    """Free memory of the current device (alt3_device_free: hipMemGetInfo)."""
+    f = C.c_longlong()
+    _check(lib().alt3_device_free(C.byref(f), None))
+    return int(f.value)
 
 
 def giso_of(inp: InputSet):
@@ -133,6 +141,11 @@ class Context:
         s.nc = (C.c_int * 3)(*nc)
         s.classical = int(inp.params.classical if classical is None else classical)
         s.host_only = 1 if host_only else 0
+        # dielectric.params (read_born): the non-analytic term when born_flag != 0
+        s.born_flag = int(inp.born_flag)
+        if inp.born_flag:
+            s.epsil = arr(inp.epsil, np.float64)
+            s.zeu = arr(inp.zeu, np.float64)
         self._sys = s
         h = C.c_void_p()
         _check(lib().alt3_create(C.byref(s), C.byref(h)))

@@ -48,7 +48,17 @@ def _fl(x: float) -> str:
     return repr(float(x))
 
 
-def write_inputs(dirname: str, fcs: ForceConstants, rp: RunParams, nsmax: int = 15) -> None:
+@dataclasses.dataclass
+class Born:
+    """dielectric.params (read_born, rest.f90:3580-3600): born_flag, the dielectric tensor
+    epsil(i,j) and the effective charges charge(i,j) of every atom of the primitive cell."""
+    born_flag: int = 0
+    epsil: np.ndarray = dataclasses.field(default_factory=lambda: np.eye(3))
+    zeu: Optional[np.ndarray] = None            # (natoms0, 3, 3)
+
+
+def write_inputs(dirname: str, fcs: ForceConstants, rp: RunParams, nsmax: int = 15,
+                 born: Optional[Born] = None) -> None:
     os.makedirs(dirname, exist_ok=True)
     c: Crystal = fcs.crystal
     with open(os.path.join(dirname, "structure.params"), "w") as f:
@@ -83,13 +93,15 @@ def write_inputs(dirname: str, fcs: ForceConstants, rp: RunParams, nsmax: int = 
         f.write(f"{rp.classical}   classical\n")
         f.write(f"{rp.calc_cross} " + " ".join(_fl(v) for v in rp.q_cross) + "   calc_cross q_cross\n")
         f.write(f"{_fl(rp.lmicron)}   lmicron\n")
+    b = born if born is not None else Born()
+    zeu = np.zeros((c.natoms0, 3, 3)) if b.zeu is None else np.asarray(b.zeu, dtype=float)
     with open(os.path.join(dirname, "dielectric.params"), "w") as f:
-        f.write("0   born_flag\n")
+        f.write(f"{int(b.born_flag)}   born_flag\n")
         for i in range(3):
-            f.write(" ".join("1.0" if i == j else "0.0" for j in range(3)) + "\n")
-        for _ in range(c.natoms0):
+            f.write(" ".join(_fl(b.epsil[i][j]) for j in range(3)) + "\n")
+        for k in range(c.natoms0):
             for i in range(3):
-                f.write(" ".join("0.0" for j in range(3)) + "\n")
+                f.write(" ".join(_fl(zeu[k, i, j]) for j in range(3)) + "\n")
     with open(os.path.join(dirname, "kpbs.params"), "w") as f:
         f.write("0\n10\n3\nG 0 0 0\nX 0 1 1\nL 0.5 0.5 0.5\n")
     P = c.prim_cart
@@ -171,6 +183,9 @@ class InputSet:
     fc3_xyz: np.ndarray          # (nt3,3)
     fc3: np.ndarray              # (nt3,)
     names: Tuple[str, ...] = ()  # element label per atom type (structure.params)
+    born_flag: int = 0           # dielectric.params (read_born)
+    epsil: Optional[np.ndarray] = None   # (3,3) epsil(i,j)
+    zeu: Optional[np.ndarray] = None     # (natoms0,3,3) charge(i,j) per atom
 
     @property
     def natoms0(self) -> int:
@@ -257,4 +272,30 @@ def read_inputs(dirname: str) -> InputSet:
 
     a2, x2, v2 = rdfc("fc2.dat", 2, nterms[1])
     a3, x3, v3 = rdfc("fc3.dat", 3, nterms[2])
-    return InputSet(rp, latp, prim, scale, masses, types, r123, atompos, i0, ic, a2, x2, v2, a3, x3, v3, names)
+    born_flag, epsil, zeu = read_born(P("dielectric.params"), nat0)
+    return InputSet(rp, latp, prim, scale, masses, types, r123, atompos, i0, ic, a2, x2, v2, a3, x3, v3, names,
+                    born_flag, epsil, zeu)
+
+
+def read_born(path: str, natoms0: int):
+    """read_born (rest.f90:3580-3600): born_flag on record 1, epsil(i,1:3) on records 2-4, then
+    charge(i,1:3) of atom k on three records per atom (list-directed: blank records are
+    skipped, as a Fortran list-directed read skips them).  A missing file stops the reference
+    (status='old'); here it means born_flag = 0."""
+    if not os.path.exists(path):
+        return 0, np.eye(3), np.zeros((natoms0, 3, 3))
+    with open(path) as f:
+        recs = [_toks(l) for l in f.read().splitlines()]
+    recs = [r for r in recs if r]
+    flag = int(recs[0][0])
+    epsil = np.array([[_flt(x) for x in recs[1 + i][:3]] for i in range(3)])
+    zeu = np.zeros((natoms0, 3, 3))
+    for k in range(natoms0):
+        for i in range(3):
+            r = recs[4 + 3 * k + i] if 4 + 3 * k + i < len(recs) else []
+            if len(r) < 3:
+                if flag:
+                    raise ValueError(f"dielectric.params: charges of atom {k + 1} incomplete")
+                continue
+            zeu[k, i] = [_flt(x) for x in r[:3]]
+    return flag, epsil, zeu

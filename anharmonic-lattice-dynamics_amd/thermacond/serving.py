@@ -53,6 +53,9 @@ class Request:
     reused: bool = False
     result: Optional[dict] = None
     error: Optional[str] = None
+    # the admission decision: the cached context this request was admitted on (its busy count
+    # raised, its reserve grown), or None when it was charged its own footprint
+    _entry: Optional["_Cached"] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def mesh_key(self) -> str:
@@ -74,9 +77,8 @@ class _Cached:
 
 
 def device_free_bytes() -> int:
-    import torch
-    free, _ = torch.cuda.mem_get_info()
-    return int(free)
+    """Free device memory, from the library (alt3_device_free, hipMemGetInfo)."""
+    return capi.device_free_bytes()
 
 
 class KappaServer:
@@ -171,10 +173,12 @@ class KappaServer:
             self._in_use += extra
             cached.busy += 1
             self._cache.move_to_end(req.mesh_key)
+            req._entry = cached
             return True
         if self.budget - self._in_use < req.footprint and not self._evict_for(req.footprint):
             return False
         self._in_use += req.footprint
+        req._entry = None
         return True
 
     def _schedule(self) -> None:
@@ -208,8 +212,10 @@ class KappaServer:
         ctx_entry: Optional[_Cached] = None
         own_ctx: Optional[capi.Context] = None
         try:
-            with self._cv:
-                ctx_entry = self._cache.get(req.mesh_key) if self.use_cache else None
+            # exactly what _try_admit decided: a cache entry inserted by another worker after
+            # this request was charged its own footprint is not used (its busy count and
+            # reserve were never raised for this request)
+            ctx_entry = req._entry
             if ctx_entry is None:
                 own_ctx = capi.Context(req.inputs, nc=req.nc)
                 ctx, lock = own_ctx, threading.Lock()

@@ -62,12 +62,22 @@ typedef struct {
     int classical;             /* 0 quantum, 1 classical occupations */
     int host_only;             /* 1: build the mesh/symmetry tables only (no device needed);
                                   such a context answers alt3_sizes/alt3_get_mesh/alt3_kstars */
+    /* dielectric.params (read_born, rest.f90:3580-3600).  born_flag != 0 adds the
+     * non-analytic term of nonanal (rest.f90:3517-3578) to every dynamical matrix, as get_freq
+     * does (rest.f90:4342-4348): D += (q.Z_a)_i (q.Z_b)_j / (q.eps.q eps0') / V / sqrt(m_a m_b)
+     * * exp(-q^2/rho^2), rho = (|g1||g2||g3|)^0.33 / 4, q -> q + 1e-10 at q = 0. */
+    int born_flag;
+    const double *epsil;       /* [3][3] row i = epsil(i,1:3) (records 2-4); NULL when born_flag == 0 */
+    const double *zeu;         /* [natoms0][3][3]: [k][i][j] = atom0(k)%charge(i,j) (record i of atom k) */
 } alt3_system;
 
 int alt3_create(const alt3_system *sys, alt3_ctx **out);
 int alt3_destroy(alt3_ctx *ctx);
 const char *alt3_last_error(void);
 int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes);
+/* Free and total memory of the current device (hipMemGetInfo), for admission by footprint.
+ * No counterpart in the reference, which does not manage device memory. */
+int alt3_device_free(long long *free_bytes, long long *total_bytes);
 
 /* Device bytes alt3_solve needs on this context for ntemps temperatures at the given V3
  * precision (an upper bound of the library's allocations; answers on host-only contexts,

@@ -175,9 +175,52 @@ def _ref_sort(r):
     return m
 
 
-def get_freq_batch(qs, model):
-    """get_freq (rest.f90:4319-4445): |eigenvalues| ascending and eigenvectors (columns)."""
+def born_params(inp):
+    """What nonanal (rest.f90:3517-3578) uses: epsil, the charges, the masses and the constants
+    it derives from the lattice (rho from g1,g2,g3 with the REAL*4 exponent 0.33, the cell
+    volume of r1,r2,r3, eps_scale); None when born_flag is 0."""
+    if not getattr(inp, "born_flag", 0):
+        return None
+    g, _ = reciprocal(inp.r123)
+    gg = (np.linalg.norm(g[0]) * np.linalg.norm(g[1]) * np.linalg.norm(g[2])) ** float(np.float32(0.33))
+    rho = gg / 4.0
+    r1, r2, r3 = inp.r123
+    om0 = abs(np.dot(r3, np.cross(r1, r2)))
+    eps_scale = 8.8541878176e-12 / 1e10 / 1.60217653e-19
+    return dict(epsil=np.asarray(inp.epsil, float), zeu=np.asarray(inp.zeu, float), masses=np.asarray(inp.masses, float),
+                rho2=rho * rho, om0=om0, eps_scale=eps_scale)
+
+
+def nonanal_batch(qs, D, b):
+    """nonanal (rest.f90:3517-3578) on a batch of dynamical matrices D[q, row, col], after
+    get_freq's shift of q = 0 to q + 1e-10 (rest.f90:4342-4346): D += zag(i) zbg(j) /
+    (qeq eps_scale) / om0 / sqrt(ma mb) exp(-q^2/rho^2), zag(i) = sum_k q_k charge_a(k,i)."""
+    qs = np.array(np.atleast_2d(qs), dtype=float)
+    zero = np.all(qs == 0.0, axis=1)
+    qs[zero] += 1.0e-10
+    E = b["epsil"]
+    qeq = (qs[:, 0] * (E[0, 0] * qs[:, 0] + E[0, 1] * qs[:, 1] + E[0, 2] * qs[:, 2])
+           + qs[:, 1] * (E[1, 0] * qs[:, 0] + E[1, 1] * qs[:, 1] + E[1, 2] * qs[:, 2])
+           + qs[:, 2] * (E[2, 0] * qs[:, 0] + E[2, 1] * qs[:, 1] + E[2, 2] * qs[:, 2]))
+    q2 = qs[:, 0] * qs[:, 0] + qs[:, 1] * qs[:, 1] + qs[:, 2] * qs[:, 2]
+    nat0 = len(b["masses"])
+    zg = np.einsum("qk,aki->qai", qs, b["zeu"])          # (nq, atom, i)
+    for na in range(nat0):
+        for nb_ in range(nat0):
+            ms = np.sqrt(b["masses"][na] * b["masses"][nb_])
+            for i in range(3):
+                for j in range(3):
+                    t = zg[:, na, i] * zg[:, nb_, j] / (qeq * b["eps_scale"]) / b["om0"] / ms * np.exp(-q2 / b["rho2"])
+                    D[:, i + 3 * na, j + 3 * nb_] += t
+    return D
+
+
+def get_freq_batch(qs, model, born=None):
+    """get_freq (rest.f90:4319-4445): |eigenvalues| ascending and eigenvectors (columns);
+    born (born_params) adds nonanal's term."""
     D = dynmat_batch(qs, *model)
+    if born is not None:
+        D = nonanal_batch(qs, D, born)
     w, v = np.linalg.eigh(D, UPLO="U")
     w = np.abs(w)  # eigh returns ascending; the reference sort is then the identity map
     re = np.abs(v.real).sum(axis=1)               # (nq, nb) per column
@@ -185,19 +228,19 @@ def get_freq_batch(qs, model):
     return w, v
 
 
-def eigen(kp_mesh, model):
+def eigen(kp_mesh, model, born=None):
     """get_frequencies with finite-difference velocities.  Returns eigenval (nb,nk),
     eigenvec (nb,nb,nk) [component, branch, k], veloc (3,nb,nk), all order='F'."""
     kp_mesh = np.asarray(kp_mesh)
     nk = len(kp_mesh)
-    w0, v0 = get_freq_batch(kp_mesh, model)
+    w0, v0 = get_freq_batch(kp_mesh, model, born)
     nb = w0.shape[1]
     vg = np.zeros((nk, nb, 3))
     for a in range(3):
         qp = kp_mesh.copy(); qp[:, a] += DQ_FD
         qm = kp_mesh.copy(); qm[:, a] -= DQ_FD
-        wp, _ = get_freq_batch(qp, model)
-        wm, _ = get_freq_batch(qm, model)
+        wp, _ = get_freq_batch(qp, model, born)
+        wm, _ = get_freq_batch(qm, model, born)
         vg[:, :, a] = (wp - wm) / 2 / DQ_FD / 2 / np.sqrt(w0) * CNST * 1e-8 * 2 * np.pi
     eival = np.zeros((nb, nk), order="F"); eivec = np.zeros((nb, nb, nk), dtype=np.complex128, order="F")
     for i in range(nk):

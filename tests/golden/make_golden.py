@@ -9,6 +9,9 @@ dumps are stored as compressed .npz next to this script.
     python tests/golden/make_golden.py iter       # si4_iter: the iterative BTE of job 3
     python tests/golden/make_golden.py self       # si4_self, si8_self: self-energy at given q
     python tests/golden/make_golden.py iso        # si4_iso, zb4_iso: isotope scattering (iso = 1)
+    python tests/golden/make_golden.py si16       # si16: the SW-Si 16^3 workload at 100/300/800 K
+    python tests/golden/make_golden.py rs8        # rs8_4: the 8-atom rock-salt-type cell at 4^3
+    python tests/golden/make_golden.py born       # zb4_born: Born charges (nonanal) on the 4-atom cell
 """
 from __future__ import annotations
 
@@ -165,7 +168,110 @@ def main_iso():
             print("wrote", name + "_iso")
 
 
+def _cg_iterations(stdout: str):
+    """Iteration counts of cg_quadratic's 'loop ended' lines (cg.f90:56), in call order."""
+    import re
+    return [int(m.group(1)) for m in re.finditer(r"CG: loop ended; iter, error, funcmin,\|Ax-b\|=\s*(\d+)", stdout)]
+
+
+def _run_probe(fc, n: int, wd: str, temps, env_extra=None):
+    refio.write_inputs(wd, fc, refio.RunParams(nc=(n, n, n), tmin=temps[0], tmax=temps[0]))
+    with open(os.path.join(wd, "probe_temps.txt"), "w") as f:
+        f.write("\n".join(str(t) for t in temps) + "\n")
+    env = dict(os.environ, REF_PROBE_ITER="0", **(env_extra or {}))
+    r = subprocess.run([PROBE], cwd=wd, check=True, capture_output=True, text=True, env=env)
+    return load_dump(os.path.join(wd, "dump")), _cg_iterations(r.stdout)
+
+
+def _config_pack(D, iters, ibz_sel, q2_sel, cm_ibz):
+    """Indexing tables, eigen data and every per-mode and per-temperature result of a probe
+    run, with |V3|^2, P1sm and P2sm on the irreducible points ibz_sel and mesh points q2_sel
+    only, and the collision-matrix rows of the irreducible points cm_ibz."""
+    nb = D["nb"]
+    out = dict(nk=np.array(D["nk"]), nibz=np.array(D["nibz"]), nb=np.array(nb), kpc_mesh=D["kpc_mesh"],
+               kpc=D["kpc"], mapibz=D["mapibz"], mapinv=D["mapinv"], mappos=D["mappos"], npos=np.array(D["npos"]),
+               kibz=D["kibz"], wibz=D["wibz"], eigenval=D["eigenval"], eigenvec=D["eigenvec"], veloc=D["veloc"],
+               op_kmatrix=D["op_kmatrix"], tet=D["tet"], star_n=D["star_n"], star_op=D["star_op"],
+               star_n0=D["star_n0"], frequency=D["frequency"], temps=np.array(D["temps"]),
+               cg_iters=np.array(iters), ibz_sel=np.asarray(ibz_sel), q2_sel=np.asarray(q2_sel))
+    for k in ("v33s8", "P1sm", "P2sm"):
+        out[k] = np.asfortranarray(D[k][np.ix_(np.asarray(ibz_sel), np.asarray(q2_sel))])
+    rows = np.concatenate([np.arange(3 * nb * i, 3 * nb * (i + 1)) for i in cm_ibz])
+    for T in D["temps"]:
+        for k, v in D[f"T{T}"].items():
+            if v is None:
+                continue
+            if k == "CM":
+                out[f"T{T}_CM_rows"] = rows
+                out[f"T{T}_CM"] = v[rows]
+            else:
+                out[f"T{T}_{k}"] = v
+    return out
+
+
+def main_si16():
+    """The SW-Si 16^3 workload (276 irreducible points) at 100, 300 and 800 K: tables, eigen
+    data, Q/tau, F_RTA, kappa_RTA, RHS, CG solution, kappa_BTE and the CG iteration counts,
+    |V3|^2 / P on three irreducible points and every eighth mesh point, collision rows of two
+    irreducible points.  About 1.5 h of one core (the V3 stage at ~5e4 elements/s)."""
+    with tempfile.TemporaryDirectory() as wd:
+        D, iters = _run_probe(models.sw_silicon(), 16, wd, TEMPS)
+        out = _config_pack(D, iters, [0, 137, D["nibz"] - 1], np.arange(0, D["nk"], 8), [1, 200])
+        np.savez_compressed(os.path.join(HERE, "ref_si16.npz"), **out)
+        print("wrote si16", iters)
+
+
+def main_rs8():
+    """The 8-atom rock-salt-type cell (24 branches, Born-Mayer-type springs, seed 0) on the
+    4^3 mesh at 300 K: everything of _config_pack, |V3|^2 and P on three irreducible points
+    and every second mesh point, the collision rows of two irreducible points."""
+    with tempfile.TemporaryDirectory() as wd:
+        D, iters = _run_probe(models.born_mayer_rocksalt(seed=0), 4, wd, (300.0,))
+        ni = D["nibz"]
+        out = _config_pack(D, iters, [0, ni // 2, ni - 1], np.arange(0, D["nk"], 2), [1, ni - 1])
+        np.savez_compressed(os.path.join(HERE, "ref_rs8_4.npz"), **out)
+        print("wrote rs8_4", iters)
+
+
+def polar_zb4():
+    """The 4-atom Ga/As cell with Born effective charges and a dielectric tensor
+    (dielectric.params with born_flag = 1).  The charges are not symmetric tensors and epsil is
+    anisotropic, so that a transposed index in the non-analytic term changes the result; the
+    charges of each Ga/As pair sum to zero."""
+    fc = models.sw_zincblende4()
+    zga = np.array([[2.07, 0.11, -0.04], [0.05, 2.02, 0.09], [-0.07, 0.03, 2.12]])
+    zeu = np.array([zga if t == 1 else -zga for t in fc.crystal.types])
+    eps = np.array([[10.9, 0.2, 0.0], [0.2, 11.3, -0.1], [0.0, -0.1, 10.5]])
+    return fc, refio.Born(1, eps, zeu)
+
+
+def main_born():
+    """nonanal (rest.f90:3517-3578) through get_frequencies on the polar 4-atom cell at 4^3:
+    eigen data, Q/tau, F_RTA and kappa_RTA/kappa_BTE at 300 K."""
+    fc, born = polar_zb4()
+    with tempfile.TemporaryDirectory() as wd:
+        refio.write_inputs(wd, fc, refio.RunParams(nc=(4, 4, 4), tmin=300.0, tmax=300.0), born=born)
+        with open(os.path.join(wd, "probe_temps.txt"), "w") as f:
+            f.write("300.0\n")
+        r = subprocess.run([PROBE], cwd=wd, check=True, capture_output=True, text=True,
+                           env=dict(os.environ, REF_PROBE_ITER="0"))
+        D = load_dump(os.path.join(wd, "dump"))
+        R = D["T300"]
+        out = dict(eigenval=D["eigenval"], eigenvec=D["eigenvec"], veloc=D["veloc"], kpc_mesh=D["kpc_mesh"],
+                   mapinv=D["mapinv"], epsil=born.epsil, zeu=born.zeu, cg_iters=np.array(_cg_iterations(r.stdout)))
+        for k in ("Q", "QN", "QU", "tautot", "FRTA", "kappaRTA", "x", "kappaBTE"):
+            out[f"T300_{k}"] = R[k]
+        np.savez_compressed(os.path.join(HERE, "ref_zb4_born.npz"), **out)
+        print("wrote zb4_born")
+
+
 def main():
+    if sys.argv[1:] == ["born"]:
+        return main_born()
+    if sys.argv[1:] == ["si16"]:
+        return main_si16()
+    if sys.argv[1:] == ["rs8"]:
+        return main_rs8()
     if sys.argv[1:] == ["iso"]:
         return main_iso()
     if sys.argv[1:] == ["iter"]:

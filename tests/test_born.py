@@ -1,0 +1,115 @@
+"""Born effective charges: the non-analytic term of nonanal (rest.f90:3517-3578), added by
+get_freq to every dynamical matrix when dielectric.params has born_flag != 0
+(rest.f90:4342-4348), read by read_born (rest.f90:3580-3600).
+
+Pinned against the reference build on a polar variant of the 4-atom Ga/As cell at 4^3
+(tests/golden/ref_zb4_born.npz, written by make_golden.py born): charges that are not
+symmetric tensors and an anisotropic epsil, so that a transposed index changes the result.
+
+CPU: dielectric.params round trip; the oracle's eigenvalues vs the reference's (1e-6, the
+vendored ZHEEV's residual level, as without charges); the term changes the eigenvalues by far
+more than that (1e-3), and transposed charge tensors fall outside the band (> 1e-5); the
+velocities, which run the term at q +- dq (1e-4: the residuals amplified by the central
+difference, as without charges).
+GPU: the device eigen solve vs the oracle (eigenvalues 1e-12, velocities 1e-6 away from Gamma)
+and the reference (1e-6); with the
+reference's eigen data loaded, Q and kappa_RTA vs the reference (1e-7), CG x and kappa_BTE
+(1e-6); with the device's own eigenvectors, kappa within the gauge spread of the degenerate
+subspaces (RTA 1 %, BTE 2 %).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_fixture
+from make_golden import polar_zb4
+from thermacond import capi, refio
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def born_ref():
+    return load_fixture("zb4_born")
+
+
+@pytest.fixture(scope="module")
+def polar(tmp_path_factory):
+    fc, born = polar_zb4()
+    d = str(tmp_path_factory.mktemp("zb4_born"))
+    refio.write_inputs(d, fc, refio.RunParams(nc=(4, 4, 4), tmin=300.0, tmax=300.0), born=born)
+    return refio.read_inputs(d), born
+
+
+def _model(inp):
+    return (inp.masses, inp.atompos, inp.iatomcell0, inp.fc2_atoms, inp.fc2_xyz, inp.fc2)
+
+
+def test_dielectric_params_round_trip(polar, born_ref):
+    inp, born = polar
+    assert inp.born_flag == 1
+    assert np.array_equal(inp.epsil, born.epsil) and np.array_equal(inp.zeu, born.zeu)
+    assert np.array_equal(born_ref["zeu"], born.zeu)
+
+
+def test_oracle_eigen_vs_reference(polar, born_ref):
+    inp, _ = polar
+    kpm = born_ref["kpc_mesh"].T
+    b = O.born_params(inp)
+    ev, vec, vel = O.eigen(kpm, _model(inp), b)
+    assert rel(ev, born_ref["eigenval"]) < 1e-6
+    # central differences of omega^2 over dq = 0.001 amplify ZHEEV's residuals (as without charges)
+    assert rel(vel, born_ref["veloc"]) < 1e-4
+    ev0, _, _ = O.eigen(kpm, _model(inp))
+    assert rel(ev0, born_ref["eigenval"]) > 1e-3       # the term is large here
+    # transposed charge tensors fall well outside the 1e-6 band
+    bt = dict(b, zeu=np.ascontiguousarray(b["zeu"].transpose(0, 2, 1)))
+    evt, _, _ = O.eigen(kpm, _model(inp), bt)
+    assert rel(evt, born_ref["eigenval"]) > 1e-5
+
+
+@pytest.mark.gpu
+def test_device_eigen_vs_oracle_and_reference(polar, born_ref):
+    print("case: Born zb4 4^3 eigen", flush=True)
+    inp, _ = polar
+    with capi.Context(inp) as c:
+        ev, vec, vel = c.eig()
+        m = c.mesh()
+    oev, _, ovel = O.eigen(m["kpc_eig"], _model(inp), O.born_params(inp))
+    assert rel(ev, oev) < 1e-12
+    gam = np.all(np.abs(m["kpc_eig"]) < 1e-12, axis=1)      # acoustic velocities at Gamma: 0/0
+    assert rel(vel[:, :, ~gam], ovel[:, :, ~gam]) < 1e-6
+    assert rel(ev, born_ref["eigenval"]) < 1e-6
+
+
+@pytest.mark.gpu
+def test_kappa_path_vs_reference(polar, born_ref):
+    print("case: Born zb4 4^3 kappa path", flush=True)
+    inp, _ = polar
+    R = born_ref
+    T = 300.0
+    with capi.Context(inp) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        c.fgr(want=False)
+        out = c.tau(T)
+        for k in ("Q", "QN", "QU", "tautot"):
+            assert rel(out[k], R["T300_" + k]) < 1e-7, k
+        kap, F, te = c.kappa_rta(T)
+        assert rel(F, R["T300_FRTA"]) < 1e-7
+        assert rel(kap, R["T300_kappaRTA"]) < 1e-7
+        c.collision(T, want=False)
+        x0 = np.array([F[i, l, a] for i in range(c.nibz) for l in range(c.nb) for a in range(3)])
+        x, it, _ = c.cg(x0)
+        assert it == int(R["cg_iters"][0])
+        assert rel(x, R["T300_x"]) < 1e-6
+        kb, _ = c.kappa_bte(T, x, te)
+        assert rel(kb, R["T300_kappaBTE"]) < 1e-6
+    with capi.Context(inp) as c:
+        o = c.solve([T], capi.V3_FP64)
+    kr = np.trace(o["kappa_rta"][0].sum(0)) / 3
+    kbb = np.trace(o["kappa_bte"][0].sum(0)) / 3
+    assert abs(kr / (np.trace(R["T300_kappaRTA"].sum(0)) / 3) - 1) < 1e-2
+    assert abs(kbb / (np.trace(R["T300_kappaBTE"].sum(0)) / 3) - 1) < 2e-2

@@ -167,3 +167,24 @@ def test_iso_zero_g_and_bad_g(si_inputs, ref4):
         assert np.array_equal(c.tau(T)["Q"], q0)
         with pytest.raises(capi.Alt3Error):
             c.set_iso(np.array([2e-4, -1e-4]))
+
+
+def test_iso_tau_on_a_sub_block(si_inputs, ref4):
+    """Q, QN and QU of an irreducible block that does not start at 0 (alt3_v3sq_ibz over
+    ibz_lo..ibz_hi) with the isotope term on: the rows of that block must equal those of the
+    whole-zone run, for the 6-branch kernel (partial sums per mesh chunk) and with the FP64
+    path.  Piso is indexed by the global irreducible point (sub.f90:8502-8504)."""
+    print("case: iso Si 4^3 tau on irreducible block 3..6 vs the whole zone", flush=True)
+    T = 300.0
+    giso = capi.giso_of(si_inputs[4])
+    for prec in (capi.V3_FP64, capi.V3_MIXED):
+        with _ctx(si_inputs[4], ref4, prec) as c:
+            c.set_iso(giso, want=False)
+            full = c.tau(T)
+            lo, hi = 3, min(6, c.nibz - 1)
+            c.v3sq(lo, hi, prec, want=False)
+            c.fgr(want=False)
+            c.set_iso(giso, want=False)
+            part = c.tau(T)
+            for k in ("Q", "QN", "QU"):
+                check(part[k], full[k][lo:hi + 1], 1e-12, f"{k} of block {lo}..{hi} (precision {prec})")

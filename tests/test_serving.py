@@ -39,6 +39,66 @@ def test_oversized_request_is_refused_on_the_host(si_inputs, monkeypatch):
     assert req.error is not None and "exceeds the HBM budget" in req.error and req.t_done > 0
 
 
+class _StubContext:
+    """Stands in for capi.Context in the admission tests (no device)."""
+    made = 0
+
+    def __init__(self, *a, **k):
+        _StubContext.made += 1
+        self.closed = False
+
+    def set_delta(self, *a):
+        pass
+
+    def solve(self, temps, precision):
+        return {"kappa_rta": np.zeros((len(temps), 6, 3, 3)), "kappa_bte": np.zeros((len(temps), 6, 3, 3)),
+                "cg_iters": np.zeros(len(temps), int), "stage_ms": np.zeros(8)}
+
+    def close(self):
+        self.closed = True
+
+
+def _bare_server(budget):
+    srv = serving.KappaServer.__new__(serving.KappaServer)
+    srv.budget, srv.use_cache, srv.slots, srv.max_bypass = budget, True, 4, 8
+    srv._cv = serving.threading.Condition()
+    srv._done, srv._pending = [], serving.collections.deque()
+    srv._cache = serving.collections.OrderedDict()
+    srv._ids = serving.itertools.count()
+    srv._in_use, srv._running, srv._bypassed = 0, 0, {}
+    return srv
+
+
+def test_worker_uses_the_admission_decision(si_inputs, monkeypatch):
+    """A request admitted without a cached context (charged its own footprint) keeps to that
+    decision even if another worker caches a context of the same crystal and mesh before this
+    one starts: it builds its own context, leaves the cache entry's busy count alone and
+    returns its footprint.  Using the new entry instead would decrement a busy count it never
+    raised, so that a context still in use could be evicted and closed."""
+    monkeypatch.setattr(serving.capi, "Context", _StubContext)
+    srv = _bare_server(1 << 40)
+    req = serving.Request("si", si_inputs[4], [300.0])
+    req.footprint = 1000
+    assert srv._try_admit(req) and req._entry is None and srv._in_use == 1000
+    other = serving._Cached(_StubContext(), 700)          # another worker finished first
+    srv._cache[req.mesh_key] = other
+    srv._in_use += other.reserve
+    srv._running = 1
+    made = _StubContext.made
+    srv._work(req)
+    assert req.error is None and not req.reused
+    assert _StubContext.made == made + 1                  # its own context
+    assert other.busy == 0 and not other.ctx.closed
+    assert srv._in_use == 700                             # only the cached context's reserve is left
+    # a request admitted on the cached context does use it and releases its busy count
+    req2 = serving.Request("si", si_inputs[4], [300.0])
+    req2.footprint = 500
+    srv._running = 1
+    assert srv._try_admit(req2) and req2._entry is other and other.busy == 1
+    srv._work(req2)
+    assert req2.reused and other.busy == 0 and srv._in_use == 700
+
+
 def _mix(si_inputs):
     S = si_inputs
     g = 0.1 * capi.THZ_TO_CM
