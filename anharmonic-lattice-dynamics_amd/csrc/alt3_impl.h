@@ -138,6 +138,9 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
+int cg_device_ws(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
+                 double *funcmin, int *iters, bool rowmajor, const float *dAf, double *work, double *red, int *state);
+int scratch_setup();   // the static scratch of alt3_cg_static on the current device (cg_static.hip)
 
 }  // namespace alt3
 
@@ -153,6 +156,7 @@ struct alt3_ctx {
     double last_ms = 0;
     long long peak_bytes = 0;
     size_t lds_max = 65536;                        // LDS per workgroup on this device
+    int device = 0;                                // the HIP device of stream and buffers
 
     // harmonic
     int nt2h = 0;                                 // FC2 terms with a home-cell first atom

@@ -1268,8 +1268,17 @@ int collision_run(alt3_ctx *ctx, double tempk, double *CM, double *RHS, bool syn
 // CG on a device matrix (dA, db); x is a device vector, overwritten.  rowmajor selects the
 // layout of dA (the collision matrix is kept row-major on the device).
 // dAf (FP32, row-major) replaces dA when given
-int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
-              double *funcmin, int *iters, bool rowmajor, const float *dAf)
+__global__ void k_zero_int(int n, int *__restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// cg_quadratic on device buffers with the work space given: work [4n] (grad, conj, aux, v0),
+// red [2*256 + 16] (dot-product partials and scalars), state [2] (converged flag, iteration;
+// cleared here by a kernel, not a memset)
+int cg_device_ws(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
+                 double *funcmin, int *iters, bool rowmajor, const float *dAf, double *work, double *red, int *state)
 {
     auto matvec = [&](const double *xin, double *yout, const int *skip) {
         if (dAf)
@@ -1281,13 +1290,10 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
         else
             hipLaunchKernelGGL(k_matvec, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, dA, xin, yout, skip);
     };
-    ALT3_TRY(ctx->d_cg.alloc((size_t)4 * n));
-    double *grad = ctx->d_cg.p, *conj = grad + n, *aux = conj + n, *v0 = aux + n;
+    double *grad = work, *conj = grad + n, *aux = conj + n, *v0 = aux + n;
     const int nblk = 256;
-    ALT3_TRY(ctx->d_red.alloc(2 * nblk + 16));
-    double *part = ctx->d_red.p, *sc = part + 2 * nblk;
-    int *state = ctx->d_flags.p + 4;   // converged flag, iteration
-    ALT3_HIP(hipMemsetAsync(state, 0, 2 * sizeof(int), ctx->stream));
+    double *part = red, *sc = part + 2 * nblk;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, ctx->stream, 2, state);
     const int gb = (n + 255) / 256;
     const double tol = 1.0 / std::pow(10.0, 2 * ndig);
     matvec(dx, aux, nullptr);
@@ -1321,6 +1327,15 @@ int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *
     }
     if (iters) *iters = iter;
     return ALT3_OK;
+}
+
+int cg_device(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
+              double *funcmin, int *iters, bool rowmajor, const float *dAf)
+{
+    ALT3_TRY(ctx->d_cg.alloc((size_t)4 * n));
+    ALT3_TRY(ctx->d_red.alloc(2 * 256 + 16));
+    return cg_device_ws(ctx, n, dA, db, dx, max_iter, ndig, funcmin, iters, rowmajor, dAf, ctx->d_cg.p, ctx->d_red.p,
+                        ctx->d_flags.p + 4);
 }
 
 // x (device, vector layout) = F_RTA of the last alt3_kappa_rta, the CG start of job 1

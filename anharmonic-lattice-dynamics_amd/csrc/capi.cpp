@@ -113,6 +113,8 @@ int alt3_create(const alt3_system *sys, alt3_ctx **out)
         ALT3_TRY(ctx->d_flags.alloc(16));
         int dev = 0, lds = 0;
         ALT3_HIP(hipGetDevice(&dev));
+        ctx->device = dev;
+        ALT3_TRY(scratch_setup());
         ALT3_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         if (lds > 0) ctx->lds_max = (size_t)lds;
         ALT3_HIP(hipEventCreate(&ctx->ev0));

@@ -47,7 +47,7 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
-           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free"]
+           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -322,6 +322,16 @@ class Context:
         Af = None if A is None else np.asfortranarray(A, dtype=np.float64)
         bf = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
         _check(lib().alt3_cg(self.h, n, _p(Af), _p(bf), _p(x), max_iter, ndig, C.byref(fm), C.byref(it)))
+        return x, it.value, fm.value
+
+    def cg_static(self, x0, A=None, b=None, max_iter: int = 100, ndig: int = 2):
+        """alt3_cg_static: alt3_cg without device allocations (static scratch, n <= 2505 with A)."""
+        x = np.array(x0, dtype=np.float64, copy=True)
+        n = len(x)
+        Af = None if A is None else np.asfortranarray(A, dtype=np.float64)
+        bf = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        it = C.c_int(); fm = C.c_double()
+        _check(lib().alt3_cg_static(self.h, n, _p(Af), _p(bf), _p(x), max_iter, ndig, C.byref(fm), C.byref(it)))
         return x, it.value, fm.value
 
     def kappa_bte(self, tempk: float, x, taueff=None):

@@ -161,6 +161,17 @@ int alt3_collision(alt3_ctx *ctx, double tempk, double *CM, double *RHS);
 int alt3_cg(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, int max_iter,
             int ndig, double *funcmin, int *iters);
 
+/* alt3_cg with no device allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  The same solve through the same kernels, with A, b, x, the CG vectors and the
+ * reduction partials staged in one file-scope static device array of 6*2^20 doubles (48 MiB)
+ * per GPU, held until the process ends: n*n + 6n + 529 doubles must fit with A given
+ * (n <= 2505), 5n + 529 with A == NULL (the context's collision matrix); larger n returns
+ * ALT3_E_ARG.  Every call on one GPU uses that GPU's array: calls that overlap on one GPU race
+ * (they need one host thread, or the caller's own serialisation, per GPU). */
+int alt3_cg_static(alt3_ctx *ctx, int n, const double *A, const double *b, double *x, int max_iter,
+                   int ndig, double *funcmin, int *iters);
+
 /* Full-BTE conductivity from the CG solution x: mode_kappa_d (sub.f90:1407-1530) with
  * veloc_avergeIBZ_invsyFBZ (:5479-5537).  taueff accumulates onto its input, like the
  * reference module array (pass the alt3_kappa_rta output). */
