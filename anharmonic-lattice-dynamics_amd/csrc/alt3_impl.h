@@ -140,7 +140,9 @@ int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and p
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
 int cg_device_ws(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
                  double *funcmin, int *iters, bool rowmajor, const float *dAf, double *work, double *red, int *state);
-int scratch_setup();   // the static scratch of alt3_cg_static on the current device (cg_static.hip)
+int scratch_setup();
+// bracket the launch of kernel class k (0: V3 contraction, 1: FGR) with the context's events
+inline void kernel_mark(alt3_ctx *ctx, int k, bool end);   // the static scratch of alt3_cg_static on the current device (cg_static.hip)
 
 }  // namespace alt3
 
@@ -153,6 +155,9 @@ struct alt3_ctx {
     alt3::V3Tables V;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // HIP events around the last |V3|^2 contraction kernel and the last FGR kernel (alt3_kernel_ms)
+    hipEvent_t ev_k[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_k_set[2] = {false, false};
     double last_ms = 0;
     long long peak_bytes = 0;
     size_t lds_max = 65536;                        // LDS per workgroup on this device
@@ -251,3 +256,9 @@ struct alt3_ctx {
     alt3::DBuf<double> d_solve_k;                  // alt3_solve: kappa_RTA, kappa_BTE per temperature
     std::vector<hipEvent_t> evpool;                // alt3_solve: stage events
 };
+
+inline void alt3::kernel_mark(alt3_ctx *ctx, int k, bool end)
+{
+    (void)hipEventRecord(ctx->ev_k[2 * k + (end ? 1 : 0)], ctx->stream);
+    if (end) ctx->ev_k_set[k] = true;
+}

@@ -119,6 +119,7 @@ int alt3_create(const alt3_system *sys, alt3_ctx **out)
         if (lds > 0) ctx->lds_max = (size_t)lds;
         ALT3_HIP(hipEventCreate(&ctx->ev0));
         ALT3_HIP(hipEventCreate(&ctx->ev1));
+        for (hipEvent_t &e : ctx->ev_k) ALT3_HIP(hipEventCreate(&e));
         ALT3_TRY(ctx->d_kpc_eig.alloc(3 * ctx->nk));
         ALT3_TRY(ctx->d_kpc.alloc(3 * ctx->nk));
         ALT3_TRY(ctx->d_mapinv.alloc(ctx->nibz));
@@ -164,6 +165,8 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_chain_e.release(); ctx->d_chain_g.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->ev_k)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -621,6 +624,23 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision, do
     }
     ctx->last_ms = total;
     if (stage_ms) std::memcpy(stage_ms, ms, sizeof(ms));
+    return ALT3_OK;
+}
+
+int alt3_kernel_ms(const alt3_ctx *ctx, double *v3_ms, double *fgr_ms)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    double *out[2] = {v3_ms, fgr_ms};
+    for (int k = 0; k < 2; k++) {
+        if (!out[k]) continue;
+        *out[k] = -1;
+        if (!ctx->ev_k_set[k]) continue;
+        ALT3_HIP(hipEventSynchronize(ctx->ev_k[2 * k + 1]));
+        float ms = 0;
+        ALT3_HIP(hipEventElapsedTime(&ms, ctx->ev_k[2 * k], ctx->ev_k[2 * k + 1]));
+        *out[k] = ms;
+    }
     return ALT3_OK;
 }
 

@@ -585,6 +585,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
         hipLaunchKernelGGL((k_fgr<double, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,            \
                            ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const double *)v, ctx->d_p1.p, \
                            ctx->d_p2.p);
+    kernel_mark(ctx, 1, false);
     if (per <= 2) {
         ALT3_FGR_LAUNCH(2)
     } else if (per <= 4) {
@@ -598,6 +599,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     }
 #undef ALT3_FGR_LAUNCH
     ALT3_HIP(hipGetLastError());
+    kernel_mark(ctx, 1, true);
     if (sync) ALT3_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_fgr = true;
     ctx->fgr_kind = ALT3_DELTA_TETRA;

@@ -419,6 +419,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync)
         ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_main<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     hipLaunchKernelGGL(k_v3_main<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ibz_lo, ndr, ctx->d_mapinv.p, \
                        ctx->d_eph.p, ctx->d_evec.p, ctx->d_eival.p, ctx->d_gp.p, c33, ctx->d_v33.p, derr)
+            kernel_mark(ctx, 0, false);
             switch (tq) {
                 case 64: LAUNCH(64); break;
                 case 32: LAUNCH(32); break;
@@ -430,6 +431,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync)
             }
 #undef LAUNCH
             ALT3_HIP(hipGetLastError());
+            kernel_mark(ctx, 0, true);
         }
     }
     if (sync) {

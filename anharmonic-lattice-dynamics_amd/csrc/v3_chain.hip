@@ -278,9 +278,11 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
     ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const double c33 = const33();
     dim3 grid((unsigned)((nk + QT - 1) / QT), (unsigned)ksub);
+    kernel_mark(ctx, 0, false);
     hipLaunchKernelGGL(k_v3_chain, grid, dim3(NW * 64), lds, ctx->stream, MeshDev::make(ctx->nc), nk, ctx->ibz_lo, ndr,
                        ks, ctx->d_mapinv.p, ephf, pg.p, PU, PV, wq, c33 * c33 / 8, out, d_err);
     ALT3_HIP(hipGetLastError());
+    kernel_mark(ctx, 0, true);
     return ALT3_OK;
 }
 

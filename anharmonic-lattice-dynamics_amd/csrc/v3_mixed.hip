@@ -197,6 +197,7 @@ int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err)
         ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_mfma<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     hipLaunchKernelGGL(k_v3_mfma<T>, grid, dim3(256), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, ndr, Kp, Np,         \
                        ctx->d_mapinv.p, ephf.p, evf.p, ctx->d_eival.p, ctx->d_gpf.p, c33, ctx->d_v33f.p, d_err)
+    kernel_mark(ctx, 0, false);
     switch (tq) {
         case 64: LAUNCH(64); break;
         case 32: LAUNCH(32); break;
@@ -204,6 +205,7 @@ int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err)
     }
 #undef LAUNCH
     ALT3_HIP(hipGetLastError());
+    kernel_mark(ctx, 0, true);
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
 }

@@ -185,10 +185,12 @@ int launch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const doub
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const double c33 = alt3::const33();
     dim3 grid((unsigned)((ctx->nk + TQ - 1) / TQ), (unsigned)ksub);
+    if (!EXT) alt3::kernel_mark(ctx, 0, false);
     hipLaunchKernelGGL((k_v3_rows<T, NB, TQ, EXT>), grid, dim3(TQ * NB), lds, ctx->stream,
                        alt3::MeshDev::make(ctx->nc), ctx->nk, ctx->ibz_lo, ndr, ctx->d_mapinv.p, eph, evec, wq, gp,
                        c33 * c33 / 8, out, d_err, ev3x, w3x, w1x);
     ALT3_HIP(hipGetLastError());
+    if (!EXT) alt3::kernel_mark(ctx, 0, true);
     return ALT3_OK;
 }
 

@@ -47,7 +47,7 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
-           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static"]
+           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static", "alt3_kernel_ms"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -356,6 +356,12 @@ class Context:
         ms = C.c_double()
         _check(lib().alt3_last_kernel_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def kernel_ms(self):
+        """HIP-event times (ms) of the last |V3|^2 contraction kernel and the last FGR kernel."""
+        v, f = C.c_double(), C.c_double()
+        _check(lib().alt3_kernel_ms(self.h, C.byref(v), C.byref(f)))
+        return v.value, f.value
 
     def footprint(self, ntemps: int, precision: int = V3_FP64) -> int:
         b = C.c_longlong()

@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the three-phonon path on one MI355X.
 
-Workload (the headline configuration): Stillinger-Weber Si, 2-atom diamond cell (a=5.43 A,
-6 branches), Gamma-centred 16x16x16 q-mesh (276 irreducible q1 x 4096 q2 x 216 branch
-triples = 244,187,136 |V3|^2 elements), tetrahedron delta integration and the full BTE by
-conjugate gradients at T = 100, 300, 800 K.  One step = the whole path on the device:
-eigen solve on the mesh, V3 over the irreducible zone, tetrahedron FGR, then per
-temperature tau, kappa_RTA, collision matrix, CG and kappa_BTE.
+Workload (the headline configuration): the 8-atom rock-salt-type cell (masses 72.63/78.97 amu,
+a = 5.6 A, 24 branches, Born-Mayer-type force constants with a seeded 5 % cubic perturbation,
+seed 0) on a Gamma-centred 12x12x12 q-mesh: 84 irreducible q1 x 1728 q2 x 13824 branch
+triples = 2,006,581,248 |V3|^2 elements, tetrahedron delta integration and the full BTE by
+conjugate gradients at T = 300 K.  One step = the whole path on the device: eigen solve on
+the mesh, V3 over the irreducible zone, tetrahedron FGR, tau, kappa_RTA, collision matrix, CG
+and kappa_BTE.
 
-value = |V3|^2 elements per second of the whole step with the mixed-precision path (FP32
-V3, scattering weights and P; FP64 from the scattering rates on); the same step with FP64 V3 is timed beside it ("fp64"), together with
-the kappa difference between the two ("mixed_vs_fp64").  The line also carries the
-V3-kernel-only rates, stage times, p50/p99 kappa-solve latency, the roofline of the
-dominant kernel (HIP-event time of k_fgr over the timed steps) and the CPU baseline (the
-reference Fortran V3 routine, built from source, on one host core, over one irreducible q1).
+value = |V3|^2 elements per second of the whole step with the mixed-precision path (V3 on
+FP32 matrix cores, FP32 tetrahedron weights and P; FP64 eigen solve, scattering rates, CG
+and kappa); the same step with FP64 V3 is timed beside it ("fp64"), together with the error
+of the mixed path against it ("mixed_vs_fp64": kappa, per-mode tau^-1 and per-element
+|V3|^2).  The line also carries stage times, p50/p99 kappa-solve latency, the roofline of the
+dominant kernel (the V3 contraction, HIP-event time over the timed steps, against the FP32
+matrix-core peak), the FGR kernel's HBM roofline, the CPU baseline (the reference Fortran V3
+routine, built from source, on one host core, over one irreducible q1 of the same cell) and,
+under "sw_si_16cubed", the Stillinger-Weber Si 16^3 step at 100/300/800 K.
 """
 from __future__ import annotations
 
@@ -26,10 +30,13 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "anharmonic-lattice-dynamics_amd"))
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_MFMA_TFLOPS = 157.3
 STAGES = ["eig", "v3", "fgr", "tau", "kappa_rta", "collision", "cg", "kappa_bte"]
 
 
@@ -45,10 +52,11 @@ def parse():
     ap.add_argument("--no-iter-bte", action="store_true", help="skip the iterative BTE (job 3) line")
     ap.add_argument("--no-iso", action="store_true", help="skip the isotope-scattering (iso = 1) line")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--workload", choices=["step", "rocksalt", "sweep", "serving", "selfenergy"], default="step",
-                    help="step: the SW-Si 16^3 headline step (default); rocksalt: the 8-atom rock-salt-type "
-                         "cell at 12^3; sweep: 64 SW-Si instances through the server; serving: a Poisson "
-                         "stream of 512 kappa requests over four cells")
+    ap.add_argument("--workload", choices=["rocksalt", "step", "sweep", "serving", "selfenergy"], default="rocksalt",
+                    help="rocksalt: the 8-atom rock-salt-type cell at 12^3, with the SW-Si 16^3 step beside it "
+                         "(default); step: the SW-Si 16^3 step alone; sweep: 64 SW-Si instances through the "
+                         "server; serving: a Poisson stream of 512 kappa requests over four cells")
+    ap.add_argument("--no-sw-si", action="store_true", help="rocksalt: leave out the SW-Si 16^3 step")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (reference for the "
@@ -72,27 +80,41 @@ def log(msg, quiet=False):
         print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_reference(inputs_dir: str, quiet: bool):
+def host_cores():
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return os.cpu_count(), usable
+
+
+def cpu_baseline_reference(inputs_dir: str, quiet: bool, what: str):
     """Reference V3 routine (calculate_w3_ibz_split_sq, compiled from the reference
-    sources into oracle/_ref/ref_probe) over IBZ q1 #1 on one host core."""
+    sources into oracle/_ref/ref_probe) over IBZ q1 #1 of the inputs in inputs_dir, on one
+    host core (the routine is serial; its cost per element does not depend on the mesh)."""
     probe = os.path.join(REPO, "oracle", "_ref", "ref_probe")
     if not os.path.exists(probe):
         return None
     env = dict(os.environ, REF_PROBE_V3_SAMPLE="1 1", OMP_NUM_THREADS="1")
-    log("cpu baseline: reference V3 on one IBZ q1 (20-45 s)", quiet)
+    log(f"cpu baseline: reference V3 on one IBZ q1 of {what}", quiet)
     r = subprocess.run([probe], cwd=inputs_dir, env=env, capture_output=True, text=True, timeout=600)
+    ncpu, usable = host_cores()
     for line in r.stdout.splitlines():
         m = re.match(r"\s*REF_PROBE_V3 elements=\s*(\d+)\s+seconds=\s*([0-9.EeDd+-]+)", line)
         if m:
             el, sec = int(m.group(1)), float(m.group(2).replace("D", "E").replace("d", "e"))
             return {"value": el / sec, "unit": "V3 elements/s", "cores": 1, "kind": "reference",
-                    "sample": f"reference calculate_w3_ibz_split_sq over 1 irreducible q1 of the same "
-                              f"16^3 SW-Si workload ({el} elements, {sec:.1f} s)"}
+                    "host_cores": ncpu, "host_cores_usable": usable,
+                    "sample": f"reference calculate_w3_ibz_split_sq over 1 irreducible q1 of {what} "
+                              f"({el} elements, {sec:.1f} s) on 1 of the box's {usable} usable host cores"}
     return None
 
 
 def timed(ctx, prec, temps, warmup, steps, label, quiet):
-    lat, stage, out = [], None, None
+    """warmup + steps whole steps (eigen solve + alt3_solve); the step is synchronous, so the
+    wall clock of each brackets its device work.  Also the HIP-event times of the V3
+    contraction kernel and the FGR kernel of every timed step (context stream)."""
+    lat, stage, out, kv, kf = [], None, None, [], []
     for it in range(warmup + steps):
         t0 = time.perf_counter()
         ctx.eig(want=False)
@@ -101,11 +123,15 @@ def timed(ctx, prec, temps, warmup, steps, label, quiet):
         if it >= warmup:
             lat.append(dt)
             stage = out["stage_ms"].copy() if stage is None else stage + out["stage_ms"]
+            v, f = ctx.kernel_ms()
+            kv.append(v)
+            kf.append(f)
         log(f"{label} step {it}: {dt * 1e3:.2f} ms  stages(ms)={[round(float(x), 3) for x in out['stage_ms']]}", quiet)
     stage = stage / steps
     lat_s = sorted(lat)
     return dict(ms=1e3 * sum(lat) / steps, p50=1e3 * lat_s[len(lat) // 2],
-                p99=1e3 * lat_s[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], stage=stage, out=out)
+                p99=1e3 * lat_s[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], stage=stage, out=out,
+                v3_kernel_ms=sum(kv) / len(kv), fgr_kernel_ms=sum(kf) / len(kf))
 
 
 def timed_iter(ctx, prec, T, quiet):
@@ -149,10 +175,10 @@ def timed_iso(ctx, prec, temps, giso, quiet):
                     "the step recomputes Piso after its eigen solve"}
 
 
-def profiled_traffic(kernel_prefix: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+def profiled_traffic(kernel_prefix: str, profile: str = "sw_si_16cubed_mixed_hbm_traffic.md"):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (calibrated against a plain copy), or None when it has no entry."""
-    path = os.path.join(REPO, "profiles", "sw_si_16cubed_mixed_hbm_traffic.md")
+    path = os.path.join(REPO, "profiles", profile)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -172,12 +198,7 @@ def run_workload(args):
     """One JSON line for a workload other than the headline step (not timed by the driver)."""
     from thermacond import capi, workloads
     say = lambda m: log(m, args.quiet)
-    if args.workload == "rocksalt":
-        out = workloads.run_rocksalt(12, [300.0], log=say)
-        res = {"metric": "3-ph V3 elements/s (whole path, mixed vs FP64)", "unit": "V3 elements/s",
-               "value": out["mixed"]["elements_per_s"], "higher_is_better": True, "n_gpus": 1, "dtype": "f64 / fp32 V3",
-               "data": "synthetic (Born-Mayer-type rock-salt force constants, seed 0)", **out}
-    elif args.workload == "sweep":
+    if args.workload == "sweep":
         out = workloads.run_sweep(64, 0, log=say)
         res = {"metric": "batched sweep: V3 elements/s over 64 SW-Si instances", "unit": "V3 elements/s",
                "value": out["elements_per_s"], "higher_is_better": True, "n_gpus": 1, "dtype": "fp32 V3 + f64",
@@ -210,20 +231,10 @@ def run_workload(args):
     print(json.dumps(res))
 
 
-def main():
-    args = parse()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; sharding over q1 across GPUs is "
-                                   "described in DESIGN.md and not launched", "n_gpus": args.gpus}))
-        sys.exit(2)
+def bench_sw_si(args, with_cpu_baseline: bool) -> dict:
+    """The SW-Si 16^3 step at the temperatures of --temps, in the precisions of --precision."""
     from thermacond import capi, models, refio
 
-    if args.workload != "step":
-        run_workload(args)
-        return
     temps = [float(t) for t in args.temps.split(",")]
     n = args.mesh
     tmp = tempfile.mkdtemp(prefix="alt3_bench_")
@@ -286,10 +297,207 @@ def main():
         res["mixed_vs_fp64"] = {"whole_step_speedup": f["ms"] / m["ms"],
                                 "v3_kernel_speedup": float(f["stage"][1] / m["stage"][1]),
                                 "kappa_rel_err_max": errs}
-    if not args.no_cpu_baseline:
-        cb = cpu_baseline_reference(tmp, args.quiet)
+    if with_cpu_baseline and not args.no_cpu_baseline:
+        cb = cpu_baseline_reference(tmp, args.quiet, f"the same {n}^3 SW-Si workload")
         if cb is not None:
             res["cpu_baseline"] = cb
+    return res
+
+
+def fc3_counts(inp):
+    """Nt3 (FC3 terms with a home-cell first atom) and N_cp (distinct cell pairs (Rj, Rk) among
+    them), the counts of the per-element flop figure 24 nb + 3 + (4 Nt3 + 6 N_cp) / nb^3."""
+    a = np.asarray(inp.fc3_atoms) - 1
+    ic0 = np.asarray(inp.iatomcell0) - 1
+    home = a[ic0[a[:, 0]] == a[:, 0]]
+    pos = np.asarray(inp.atompos)
+    rj = pos[home[:, 1]] - pos[ic0[home[:, 1]]]
+    rk = pos[home[:, 2]] - pos[ic0[home[:, 2]]]
+    ncp = len({tuple(x) for x in np.round(np.hstack([rj, rk]) * 1e6).astype(np.int64)})
+    ndr = len({tuple(x) for x in np.round((rj - rk) * 1e6).astype(np.int64)})
+    return len(home), ncp, ndr
+
+
+def gamma_acoustic_masks(ctx):
+    """Per mesh index: Gamma, and the index of -q (for q3 = -q1-q2 = Gamma)."""
+    nc = ctx.nc
+    k = np.arange(ctx.nk)
+    c2, c1, c0 = k % nc[2], (k // nc[2]) % nc[1], k // (nc[1] * nc[2])
+    neg = ((nc[2] - c2) % nc[2]) + nc[2] * (((nc[1] - c1) % nc[1]) + nc[1] * ((nc[0] - c0) % nc[0]))
+    gam = (nc[2] // 2) + nc[2] * ((nc[1] // 2) + nc[1] * (nc[0] // 2))
+    return gam, neg, (c0, c1, c2)
+
+
+def mixed_error_statement(ctx, T, nblock=4):
+    """The mixed path against the FP64 path on the same context: per-mode tau^-1 (all irreducible
+    modes but the three acoustic ones at Gamma, whose frequency is zero), and per-element
+    |V3|^2 on the first nblock irreducible points, overall and on the triples with an acoustic
+    mode within one mesh step of Gamma (the near-Gamma modes; those at Gamma itself are
+    round-off of the acoustic sum rule in both precisions and masked by the FGR as q2 and q3)."""
+    from thermacond import capi
+    tau = {}
+    for name, p in (("fp64", capi.V3_FP64), ("mixed", capi.V3_MIXED)):
+        ctx.v3sq(0, ctx.nibz - 1, p, want=False)
+        ctx.fgr(want=False)
+        tau[name] = ctx.tau(T)["tautot"]
+    gam, neg, (c0, c1, c2) = gamma_acoustic_masks(ctx)
+    mapinv = ctx.mesh()["mapinv"]
+    keep = np.ones(tau["fp64"].shape, bool)
+    keep[[i for i in range(ctx.nibz) if mapinv[i] == gam], :3] = False
+    rt = np.abs(tau["mixed"][keep] - tau["fp64"][keep]) / np.abs(tau["fp64"][keep])
+    hi = min(nblock, ctx.nibz) - 1
+    vf = ctx.v3sq(0, hi, capi.V3_FP64)
+    vm = ctx.v3sq(0, hi, capi.V3_MIXED)
+    nb = ctx.nb
+    mx = float(np.abs(vf).max())
+    d = np.abs(vm - vf)
+    # triples with an acoustic mode at Gamma (q1, q2 or q3) and within one mesh step of it
+    n = np.asarray(ctx.nc)
+    h = n // 2
+    near = (np.abs(c0 - h[0]) <= 1) & (np.abs(c1 - h[1]) <= 1) & (np.abs(c2 - h[2]) <= 1)
+    at_g = np.zeros(vf.shape, bool)
+    near_g = np.zeros(vf.shape, bool)
+    ac = np.arange(nb) < 3
+    kk = np.arange(ctx.nk)
+    A1, A2, A3 = ac[None, :, None, None], ac[None, None, :, None], ac[None, None, None, :]   # (q2, l1, l2, l3)
+    for s in range(hi + 1):
+        k1 = mapinv[s]
+        q3 = neg[k1]   # q3 = -q1-q2 is Gamma where q2 = -q1
+        if k1 == gam:
+            at_g[s] |= A1
+        elif near[k1]:
+            near_g[s] |= A1
+        at_g[s] |= (kk == gam)[:, None, None, None] & A2
+        near_g[s] |= (near & (kk != gam))[:, None, None, None] & A2
+        at_g[s] |= (kk == q3)[:, None, None, None] & A3
+        d0 = np.abs(((c0 - c0[q3] + h[0]) % n[0]) - h[0])
+        d1 = np.abs(((c1 - c1[q3] + h[1]) % n[1]) - h[1])
+        d2 = np.abs(((c2 - c2[q3] + h[2]) % n[2]) - h[2])
+        nq3 = (d0 <= 1) & (d1 <= 1) & (d2 <= 1) & (kk != q3)
+        near_g[s] |= nq3[:, None, None, None] & A3
+    near_g &= ~at_g
+    big = (np.abs(vf) > 1e-6 * mx) & ~at_g
+    rel = d[big] / np.abs(vf[big])
+    nb_ = (np.abs(vf) > 1e-6 * mx) & near_g
+    reln = d[nb_] / np.abs(vf[nb_])
+    return {"tauinv_rel_err": {"max": float(rt.max()), "p99": float(np.percentile(rt, 99)), "modes": int(rt.size)},
+            "v3sq_rel_err": {"irreducible_points": hi + 1, "elements": int(big.sum()),
+                             "max": float(rel.max()), "p99": float(np.percentile(rel, 99)),
+                             "max_abs_over_largest": float(d[~at_g].max() / mx),
+                             "selection": "elements above 1e-6 of the largest, triples with an acoustic mode at "
+                                          "Gamma left out"},
+            "v3sq_rel_err_near_gamma": {"elements": int(nb_.sum()),
+                                        "max": float(reln.max()) if reln.size else None,
+                                        "p99": float(np.percentile(reln, 99)) if reln.size else None,
+                                        "selection": "triples with an acoustic mode within one mesh step of Gamma "
+                                                     "(not at it), elements above 1e-6 of the largest"},
+            "v3sq_at_gamma_acoustic": {"elements": int(at_g.sum()),
+                                       "max_abs_over_largest": float(d[at_g].max() / mx) if at_g.any() else None,
+                                       "note": "round-off of the acoustic sum rule in both precisions; the FGR "
+                                               "masks these as q2 and q3"}}
+
+
+def bench_rocksalt(args) -> dict:
+    from thermacond import capi, models, refio, workloads
+    T = 300.0
+    n = 12
+    inp = workloads.cell_inputs("rs8", n)
+    log(f"bench: rock-salt-type 8-atom cell {n}^3, T = {T} K", args.quiet)
+    ctx = capi.Context(inp)
+    nb = ctx.nb
+    nel = ctx.nibz * ctx.nk * nb ** 3
+    sizes = {"nibz": ctx.nibz, "nk": ctx.nk, "nb": nb}
+    precs = {"both": [("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)], "mixed": [("mixed", capi.V3_MIXED)],
+             "fp64": [("fp64", capi.V3_FP64)]}.get(args.precision, [("mixed", capi.V3_MIXED), ("fp64", capi.V3_FP64)])
+    runs = {name: timed(ctx, p, [T], args.warmup, args.steps, f"rock-salt {name}", args.quiet) for name, p in precs}
+    head_name = precs[0][0]
+    head = runs[head_name]
+    err = mixed_error_statement(ctx, T) if len(runs) == 2 else None
+    ctx.close()
+    nt3, ncp, ndr = fc3_counts(inp)
+    f_elem = 24 * nb + 3 + (4 * nt3 + 6 * ncp) / nb ** 3     # SURVEY 8(d) per-element flops
+    f_exec = 8 * ndr + 16 * nb + 3                            # flops the batched form executes
+    v3_ms = head["v3_kernel_ms"]
+    tflops = f_elem * nel / (v3_ms * 1e-3) / 1e12
+    fgr_ms = head["fgr_kernel_ms"]
+    pb = 4 if head_name == "mixed" else 8
+    fgr_bytes = nel * 3 * pb
+    fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
+
+    def summary(r):
+        return {"value": nel / (r["ms"] * 1e-3), "ms_per_step": r["ms"],
+                "v3_kernel_ms": r["v3_kernel_ms"], "v3_kernel_elements_per_s": nel / (r["v3_kernel_ms"] * 1e-3),
+                "fgr_kernel_ms": r["fgr_kernel_ms"],
+                "stage_ms": {k: round(float(v), 3) for k, v in zip(STAGES, r["stage"])},
+                "latency_ms": {"p50": r["p50"], "p99": r["p99"]},
+                "kappa_300K": {"rta": kappa_iso(r["out"]["kappa_rta"][0]), "bte": kappa_iso(r["out"]["kappa_bte"][0])},
+                "cg_iterations": int(r["out"]["cg_iters"][0])}
+
+    res = {
+        "metric": "3-ph V3 elements/s (mixed vs FP64) at kappa err <=0.5%; p50/p99 kappa-solve latency",
+        "value": nel / (head["ms"] * 1e-3),
+        "unit": "V3 elements/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"mixed": "fp32 (V3 on FP32 matrix cores, tetrahedron weights, P) + f64", "fp64": "f64"}[head_name],
+        "data": "synthetic (Born-Mayer-type rock-salt force constants by finite differences, seeded 5 % cubic "
+                "perturbation, seed 0)",
+        "config": {"workload": f"rock-salt-type 8-atom cell (a = 5.6 A, masses 72.63/78.97, 24 branches), {n}^3 "
+                               f"mesh, tetrahedron FGR + CG BTE at {T} K", **sizes, "v3_elements": nel,
+                   "precision": head_name},
+        "latency_ms": {"p50": head["p50"], "p99": head["p99"]},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / PEAK_FP32_MFMA_TFLOPS,
+                     "traffic": profiled_traffic("k_v3_chain", "rs8_12cubed_mixed_hbm_traffic.md")
+                     if head_name == "mixed" else None,
+                     "kernel": "k_v3_chain" if head_name == "mixed" else "k_v3_main",
+                     "kernel_ms": v3_ms, "flops_per_element": f_elem, "executed_flops_per_element": f_exec,
+                     "counts": {"nt3": nt3, "n_cp": ncp, "ndr": ndr},
+                     "algorithmic_flops_per_launch": f_elem * nel},
+        "roofline_fgr": {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": fgr_gbs / PEAK_HBM_GBS, "kernel": "k_fgr", "kernel_ms": fgr_ms,
+                         "algorithmic_bytes_per_launch": fgr_bytes,
+                         "traffic": profiled_traffic("k_fgr", "rs8_12cubed_mixed_hbm_traffic.md")
+                         if head_name == "mixed" else None},
+    }
+    for name, r in runs.items():
+        res[name] = summary(r)
+    if len(runs) == 2:
+        m, f = runs["mixed"], runs["fp64"]
+        res["mixed_vs_fp64"] = {
+            "whole_step_speedup": f["ms"] / m["ms"], "v3_kernel_speedup": f["v3_kernel_ms"] / m["v3_kernel_ms"],
+            "kappa_rel_err": {k: abs(kappa_iso(m["out"][k][0]) / kappa_iso(f["out"][k][0]) - 1)
+                              for k in ("kappa_rta", "kappa_bte")}, **err}
+    if not args.no_cpu_baseline:
+        tmp = tempfile.mkdtemp(prefix="alt3_bench_rs8_2_")
+        refio.write_inputs(tmp, models.born_mayer_rocksalt(seed=0), refio.RunParams(nc=(2, 2, 2)))
+        cb = cpu_baseline_reference(tmp, args.quiet, "the same rock-salt-type cell (24 branches, 2^3 mesh; the "
+                                                     "routine's cost per element does not depend on the mesh)")
+        if cb is not None:
+            res["cpu_baseline"] = cb
+    return res
+
+
+def main():
+    args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; sharding over q1 across GPUs is "
+                                   "described in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.workload == "step":
+        print(json.dumps(bench_sw_si(args, True)))
+        return
+    if args.workload != "rocksalt":
+        run_workload(args)
+        return
+    res = bench_rocksalt(args)
+    if not args.no_sw_si:
+        sw = bench_sw_si(args, False)
+        res["sw_si_16cubed"] = {k: v for k, v in sw.items() if k not in ("metric", "n_gpus", "steps", "warmup",
+                                                                         "higher_is_better", "scaling", "vs_baseline")}
     print(json.dumps(res))
 
 

@@ -249,6 +249,10 @@ int alt3_solve(alt3_ctx *ctx, int ntemps, const double *temps, int precision,
 
 /* Wall time (ms, HIP events on the context stream) of the kernels of the last call. */
 int alt3_last_kernel_ms(const alt3_ctx *ctx, double *ms);
+/* HIP-event duration (ms, context stream) of the last launch of the |V3|^2 contraction kernel
+ * and of the tetrahedron FGR kernel (-1 if none ran yet), for roofline accounting.  No
+ * counterpart in the reference. */
+int alt3_kernel_ms(const alt3_ctx *ctx, double *v3_ms, double *fgr_ms);
 
 #ifdef __cplusplus
 }
