@@ -133,6 +133,10 @@ int v3_rows_points(alt3_ctx *ctx, int nq, const double *gp, const double *ev3, c
                    double *out, int *d_err);
 bool v3_rows_supported(const alt3_ctx *ctx, bool fp32);
 bool v3_chain_supported(const alt3_ctx *ctx);
+int v3_rows_inputs(alt3_ctx *ctx, int ksub, bool fp32, bool want_gpf);
+bool v3_batch_supported(const alt3_ctx *ctx);
+int v3_tile_for(alt3_ctx *ctx, bool fp32, int *tq);
+int v3_batch_launch(alt3_ctx *const *ctxs, int n, bool fp32, int tq);
 int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
                  int *d_err);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
@@ -140,9 +144,9 @@ int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and p
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
 int cg_device_ws(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
                  double *funcmin, int *iters, bool rowmajor, const float *dAf, double *work, double *red, int *state);
-int scratch_setup();
+int scratch_setup();   // the static scratch of alt3_cg_static on the current device (cg_static.hip)
 // bracket the launch of kernel class k (0: V3 contraction, 1: FGR) with the context's events
-inline void kernel_mark(alt3_ctx *ctx, int k, bool end);   // the static scratch of alt3_cg_static on the current device (cg_static.hip)
+inline void kernel_mark(alt3_ctx *ctx, int k, bool end);
 
 }  // namespace alt3
 
@@ -250,6 +254,7 @@ struct alt3_ctx {
     alt3::DBuf<double> d_qpart, d_wq, d_ev7, d_evec0;
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
     alt3::DBuf<float> d_chain_e, d_chain_g;        // operand packs of k_v3_chain (eigenvectors, G')
+    alt3::DBuf<unsigned char> d_batch;             // alt3_v3_batch: work counter and job table
     alt3::DBuf<double> d_red;                      // reduction scratch
     alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
     alt3::DBuf<double> d_solve_x, d_solve_te;      // alt3_solve: CG vector, tauinv_eff

@@ -189,14 +189,17 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
     m.coords(jc, c0, c1, c2);
     const bool in1 = m.sum_inside(a0, a1, a2, c0, c1, c2), in2 = m.diff_inside(a0, a1, a2, c0, c1, c2);
     const int np = m.sum(a0, a1, a2, c0, c1, c2), nm = m.diff(a0, a1, a2, c0, c1, c2);
-    double n3p[NT][NB], n3m[NT][NB], d2[NT][NB];
+    // the occupations at q2 in registers too for small cells; for more branches they are
+    // read per (l1, l2) (cached) instead of picked out of a register array by selects
+    constexpr bool kRegD2 = NB <= 12;
+    double n3p[NT][NB], n3m[NT][NB], d2[NT][kRegD2 ? NB : 1];
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int l = 0; l < NB; l++) {
             n3p[t][l] = dist[t * dstr + (size_t)np * NB + l] + 1;
             n3m[t][l] = dist[t * dstr + (size_t)nm * NB + l] + 1;
-            d2[t][l] = dist[t * dstr + (size_t)jc * NB + l];
+            if constexpr (kRegD2) d2[t][l] = dist[t * dstr + (size_t)jc * NB + l];
         }
     for (int l1 = 0; l1 < NB; l1++) {
         double d1[NT], q[NT], qn[NT], qu[NT];
@@ -218,9 +221,13 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
             double dd[NT];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
-                dd[t] = d2[t][0];
+                if constexpr (kRegD2) {
+                    dd[t] = d2[t][0];
 #pragma unroll
-                for (int l = 1; l < NB; l++) dd[t] = (l2 == l) ? d2[t][l] : dd[t];
+                    for (int l = 1; l < NB; l++) dd[t] = (l2 == l) ? d2[t][l] : dd[t];
+                } else {
+                    dd[t] = dist[t * dstr + (size_t)jc * NB + l2];
+                }
             }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
@@ -1076,32 +1083,38 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
                            temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr);
     ALT3_HIP(hipGetLastError());
     const int nch = (nk + 255) / 256;
-    const bool rows = nb == 6;   // k_tau_q2: occupations in registers (6 branches)
+    // k_tau_q2 (occupations at q1 +- q2 in registers): 6 branches with up to 4 temperatures per
+    // pass, 12 with 2, 24 with 1; other cells k_tau
+    const bool rows = nb == 6 || nb == 12 || nb == 24;
+    const int tpp = nb == 6 ? 4 : nb == 12 ? 2 : nb == 24 ? 1 : 4;   // temperatures per pass
     const int parts = rows ? nch : nb;   // partial sums per (s, l1)
     const size_t nblk = (size_t)ksub * nb * nb, pstr = (size_t)3 * ksub * nb * parts;
     DBuf<double> &qpart = ctx->d_qpart;
     ALT3_TRY(qpart.reserve((size_t)nt * pstr));
-    for (int t0 = 0; t0 < nt; t0 += 4) {
-        const int g = std::min(4, nt - t0);
+    for (int t0 = 0; t0 < nt; t0 += tpp) {
+        const int g = std::min(tpp, nt - t0);
         if (rows) {
             const dim3 grid((unsigned)nch, (unsigned)ksub);
-#define ALT3_TAU_Q2(G, PT, P1, P2, R1, R2)                                                                         \
-    hipLaunchKernelGGL((k_tau_q2<G, PT, 6>), grid, dim3(256), 0, ctx->stream, md, nk, ctx->ibz_lo, ctx->d_mapinv.p, \
+#define ALT3_TAU_Q2(G, PT, NB, P1, P2, R1, R2)                                                                     \
+    hipLaunchKernelGGL((k_tau_q2<G, PT, NB>), grid, dim3(256), 0, ctx->stream, md, nk, ctx->ibz_lo, ctx->d_mapinv.p, \
                        ctx->d_dist.p + t0 * dstr, dstr, P1, P2, qpart.p + t0 * pstr, pstr, (R1) + t0 * rstr,         \
                        (R2) + t0 * rstr, rstr)
+#define ALT3_TAU_Q2_ALL(PT, P1, P2, R1, R2)                                 \
+    if (nb == 24) ALT3_TAU_Q2(1, PT, 24, P1, P2, R1, R2);                    \
+    else if (nb == 12 && g == 1) ALT3_TAU_Q2(1, PT, 12, P1, P2, R1, R2);     \
+    else if (nb == 12) ALT3_TAU_Q2(2, PT, 12, P1, P2, R1, R2);               \
+    else if (g == 1) ALT3_TAU_Q2(1, PT, 6, P1, P2, R1, R2);                  \
+    else if (g == 2) ALT3_TAU_Q2(2, PT, 6, P1, P2, R1, R2);                  \
+    else if (g == 3) ALT3_TAU_Q2(3, PT, 6, P1, P2, R1, R2);                  \
+    else ALT3_TAU_Q2(4, PT, 6, P1, P2, R1, R2);
             if (ctx->p_fp32) {
                 float *r1 = ctx->d_r1f.p, *r2 = ctx->d_r2f.p;
-                if (g == 1) ALT3_TAU_Q2(1, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
-                else if (g == 2) ALT3_TAU_Q2(2, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
-                else if (g == 3) ALT3_TAU_Q2(3, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
-                else ALT3_TAU_Q2(4, float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2);
+                ALT3_TAU_Q2_ALL(float, ctx->d_p1f.p, ctx->d_p2f.p, r1, r2)
             } else {
                 double *r1 = ctx->d_r1.p, *r2 = ctx->d_r2.p;
-                if (g == 1) ALT3_TAU_Q2(1, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
-                else if (g == 2) ALT3_TAU_Q2(2, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
-                else if (g == 3) ALT3_TAU_Q2(3, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
-                else ALT3_TAU_Q2(4, double, ctx->d_p1.p, ctx->d_p2.p, r1, r2);
+                ALT3_TAU_Q2_ALL(double, ctx->d_p1.p, ctx->d_p2.p, r1, r2)
             }
+#undef ALT3_TAU_Q2_ALL
 #undef ALT3_TAU_Q2
         } else if (ctx->p_fp32) {
             auto *kern = g == 1 ? k_tau<1, float> : g == 2 ? k_tau<2, float> : g == 3 ? k_tau<3, float> : k_tau<4, float>;

@@ -144,7 +144,7 @@ int alt3_destroy(alt3_ctx *ctx)
 {
     if (!ctx) return ALT3_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    // DBuf members release in their destructors? they are PODs; release explicitly
+    // device buffers go first, while the context's stream and device state still exist
     ctx->d_fc2_rc.release(); ctx->d_fc2_amp.release(); ctx->d_fc2_rr.release();
     ctx->d_kpc_eig.release(); ctx->d_kpc.release(); ctx->d_eival.release(); ctx->d_freq.release();
     ctx->d_vel.release(); ctx->d_evec.release(); ctx->d_mappos.release(); ctx->d_born.release();
@@ -162,7 +162,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_cm.release(); ctx->d_cmf.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
-    ctx->d_chain_e.release(); ctx->d_chain_g.release();
+    ctx->d_chain_e.release(); ctx->d_chain_g.release(); ctx->d_batch.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->ev_k)
@@ -313,6 +313,86 @@ int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *
     ALT3_TRY(v3_run(ctx, ibz_lo, ibz_hi, precision, true));
     tm.stop();
     if (v33s8) ALT3_TRY(v3_copy_out(ctx, v33s8));
+    return ALT3_OK;
+}
+
+int alt3_get_v3sq(alt3_ctx *ctx, double *v33s8)
+{
+    if (!ctx || !v33s8) return fail(ALT3_E_ARG, "null argument");
+    if (ctx->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+    if (!ctx->have_v3) return fail(ALT3_E_STATE, "no |V3|^2 held: call alt3_v3sq_ibz or alt3_v3_batch first");
+    return v3_copy_out(ctx, v33s8);
+}
+
+int alt3_v3_batch(alt3_ctx *const *ctxs, int n, int precision, int tile, int *tile_used)
+{
+    if (!ctxs || n <= 0) return fail(ALT3_E_ARG, "bad argument");
+    if (precision != ALT3_V3_FP64 && precision != ALT3_V3_MIXED)
+        return fail(ALT3_E_ARG, "alt3_v3_batch: precision must be ALT3_V3_FP64 or ALT3_V3_MIXED");
+    for (int i = 0; i < n; i++) {
+        alt3_ctx *c = ctxs[i];
+        if (!c) return fail(ALT3_E_ARG, "null context");
+        if (c->host_only) return fail(ALT3_E_STATE, "host-only context has no device state");
+        if (!c->have_eig) return fail(ALT3_E_STATE, "eigenvectors missing: call alt3_eig or alt3_set_eig first");
+        if (c->nb != ctxs[0]->nb) return fail(ALT3_E_ARG, "alt3_v3_batch: every context must have the same branch count");
+        if (c->device != ctxs[0]->device) return fail(ALT3_E_ARG, "alt3_v3_batch: every context must be on one device");
+        for (int k = 0; k < i; k++)
+            if (ctxs[k] == c) return fail(ALT3_E_ARG, "alt3_v3_batch: a context appears twice");
+    }
+    if (tile_used) *tile_used = 0;
+    const bool fp32 = precision == ALT3_V3_MIXED;
+    bool packed = v3_batch_supported(ctxs[0]);
+    for (int i = 0; i < n && packed; i++) packed = v3_batch_supported(ctxs[i]);
+    if (!packed) {
+        // 24 branches (their own MFMA kernels fill the device per context) and cells without
+        // FC3 pair vectors: one launch per context, each on its own stream, then one wait each
+        for (int i = 0; i < n; i++) ALT3_TRY(v3_run(ctxs[i], 0, ctxs[i]->nibz - 1, precision, false));
+        int bad = 0;
+        for (int i = 0; i < n; i++) {
+            int herr = 0;
+            ALT3_HIP(hipMemcpyAsync(&herr, ctxs[i]->d_flags.p, sizeof(int), hipMemcpyDeviceToHost, ctxs[i]->stream));
+            ALT3_HIP(hipStreamSynchronize(ctxs[i]->stream));
+            if (herr) {
+                ctxs[i]->have_v3 = false;
+                bad++;
+            }
+        }
+        if (bad) return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
+        return ALT3_OK;
+    }
+    for (int i = 0; i < n; i++) {   // G', the per-mode scale and the FP32 copies, per context stream
+        alt3_ctx *c = ctxs[i];
+        const int ksub = c->nibz;
+        const size_t nel = (size_t)ksub * c->nb * c->nb * c->nb * c->nk;
+        c->have_v3 = c->have_fgr = c->have_tau = c->have_cm = false;
+        c->ibz_lo = 0;
+        c->ibz_hi = c->nibz - 1;
+        c->v3_prec = c->v3_req = precision;
+        c->ev_k_set[0] = false;
+        if (fp32) {
+            c->d_v33.release();
+            ALT3_TRY(c->d_v33f.alloc(nel));
+        } else {
+            c->d_v33f.release();
+            ALT3_TRY(c->d_v33.alloc(nel));
+        }
+        ALT3_HIP(hipMemsetAsync(c->d_flags.p, 0, sizeof(int), c->stream));
+        ALT3_TRY(v3_prep_launch(c, ksub));
+        ALT3_TRY(v3_rows_inputs(c, ksub, fp32, fp32));
+    }
+    for (int i = 0; i < n; i++) ALT3_HIP(hipStreamSynchronize(ctxs[i]->stream));
+    int tq = tile;
+    if (tq <= 0) ALT3_TRY(v3_tile_for(ctxs[0], fp32, &tq));
+    ALT3_TRY(v3_batch_launch(ctxs, n, fp32, tq));   // on the first context's stream, waited for
+    if (tile_used) *tile_used = tq;
+    int bad = 0;
+    for (int i = 0; i < n; i++) {
+        int herr = 0;
+        ALT3_HIP(hipMemcpy(&herr, ctxs[i]->d_flags.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (herr) bad++;
+        else ctxs[i]->have_v3 = true;
+    }
+    if (bad) return fail(ALT3_E_DEN0, "MATRIX_ELT: den=0 (a zero eigenvalue in a V3 triple)");
     return ALT3_OK;
 }
 

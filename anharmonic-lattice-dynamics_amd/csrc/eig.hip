@@ -270,7 +270,14 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, cons
 // in k_eig_solve (the norm sums are taken by the group's first lane in k_eig_solve's
 // summation order), so the results are bit-identical; a wavefront now carries up to eight
 // rotation chains at once instead of one.
-template <int G>
+// PAR (G = 64, one matrix per wavefront, cells of more than 8 branches): the sweeps in the
+// round-robin (circle) order of parallel Jacobi instead of the cyclic row order: each of the
+// m - 1 rounds of a sweep (m = nb rounded up to even) rotates m/2 disjoint pairs (p, q) at once
+// -- their angles from the same A, then A J for all of them, then J^H (A J) -- so a sweep takes
+// m - 1 dependent steps where the cyclic order takes nb (nb - 1)/2.  The rotation formulas are
+// those of the cyclic order; the eigenvalues agree to round-off, the eigenvectors inside a
+// degenerate subspace are another orthonormal basis of it.
+template <int G, bool PAR = false>
 __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, const double *__restrict__ kpc,
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
@@ -282,8 +289,12 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
     extern __shared__ double lds[];
     const int lane = threadIdx.x, g = lane / G, sub = lane % G;
     const int nn = nb * nb;
-    zd *A = reinterpret_cast<zd *>(lds) + (size_t)g * 2 * nn;   // [nb][nb] row-major
-    zd *V = A + nn;                                              // [nb][nb], V[k][col]
+    // row stride: nb, or nb + 1 in the parallel order, whose column updates have consecutive
+    // lanes on consecutive rows (a stride of nb complex doubles puts them on two LDS banks)
+    const int ld = PAR ? nb + 1 : nb;
+    auto EI = [&](int e) { return PAR ? e + e / nb : e; };       // linear index e = r nb + c -> r ld + c
+    zd *A = reinterpret_cast<zd *>(lds) + (size_t)g * 2 * nb * ld;   // [nb][ld] row-major
+    zd *V = A + nb * ld;                                             // [nb][ld], V[k][col]
     __shared__ double s_rot[M][5];                               // c, s, cos(phi), sin(phi), skip
     __shared__ double s_sum[M][2];
     __shared__ int s_ord[M][G];
@@ -317,32 +328,32 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
             re = re + term_amp[t] * cs;
             im = im + term_amp[t] * sn;
         }
-        A[e] = {re, im};
-        V[e] = {(e / nb == e % nb) ? 1.0 : 0.0, 0.0};
+        A[EI(e)] = {re, im};
+        V[EI(e)] = {(e / nb == e % nb) ? 1.0 : 0.0, 0.0};
     }
     __syncthreads();
     // 2. Hermitisation as rest.f90:514-543 (column nb untouched)
     for (int e = sub; e < nn; e += G) {
         int t = e / nb, j = e % nb;
         if (t == j) {
-            zd d = A[e];
-            if (fabs(d.i) > 9e-4 * fabs(d.r)) A[e] = {d.r, 0.0};
+            zd d = A[EI(e)];
+            if (fabs(d.i) > 9e-4 * fabs(d.r)) A[EI(e)] = {d.r, 0.0};
         }
     }
     __syncthreads();
     for (int e = sub; e < nn; e += G) {
         int t = e / nb, j = e % nb;
         if (j > t && j < nb - 1) {
-            zd a = A[t * nb + j], b = A[j * nb + t];
+            zd a = A[t * ld + j], b = A[j * ld + t];
             double mi = (a.i - b.i) / 2, mj = (a.r + b.r) / 2;
-            A[t * nb + j] = {mj, mi};
-            A[j * nb + t] = {mj, -mi};
+            A[t * ld + j] = {mj, mi};
+            A[j * ld + t] = {mj, -mi};
         }
     }
     __syncthreads();
     if (sub == 0) {
         double si, sr;
-        sum2([&](int e, double &a, double &b) { a += fabs(A[e].i); b += fabs(A[e].r); }, si, sr);
+        sum2([&](int e, double &a, double &b) { a += fabs(A[EI(e)].i); b += fabs(A[EI(e)].r); }, si, sr);
         s_sum[g][0] = si;
         s_sum[g][1] = sr;
     }
@@ -350,21 +361,113 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
     {
         bool realify = (s_sum[g][0] / nn) < (s_sum[g][1] / nn) * 1e-10;
         if (realify)
-            for (int e = sub; e < nn; e += G) A[e] = {A[e].r, 0.0};
+            for (int e = sub; e < nn; e += G) A[EI(e)] = {A[EI(e)].r, 0.0};
     }
     __syncthreads();
     if (born.on) {   // nonanal after set_dynamical_matrix (rest.f90:4348)
         double qb[3], qeq;
         born_q(kq, qb, born, qeq);
-        for (int e = sub; e < nn; e += G) A[e].r = A[e].r + nonanal_term(born, bzm, qb, qeq, nb, e);
+        for (int e = sub; e < nn; e += G) A[EI(e)].r = A[EI(e)].r + nonanal_term(born, bzm, qb, qeq, nb, e);
         __syncthreads();
     }
     for (int e = sub; e < nn; e += G) {
         int r = e / nb, c = e % nb;
-        if (r > c) A[e] = {A[c * nb + r].r, -A[c * nb + r].i};
-        if (r == c) A[e] = {A[e].r, 0.0};
+        if (r > c) A[EI(e)] = {A[c * ld + r].r, -A[c * ld + r].i};
+        if (r == c) A[EI(e)] = {A[EI(e)].r, 0.0};
     }
     __syncthreads();
+    if constexpr (PAR) {
+        static_assert(G == 64, "the parallel order runs one matrix per wavefront");
+        const int m = (nb + 1) & ~1, np = m / 2;
+        __shared__ double s_par[32][5];   // per pair: c, s, cos(phi), sin(phi), skip
+        __shared__ int s_pq[32][2];
+        for (int sweep = 0; valid && sweep < 40; sweep++) {
+            double off = 0, dia = 0;
+            for (int e = sub; e < nn; e += 64) {
+                const int r = e / nb, c = e % nb;
+                const double v2 = A[EI(e)].r * A[EI(e)].r + A[EI(e)].i * A[EI(e)].i;
+                if (r != c) off += v2; else dia += v2;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                off += __shfl_xor(off, o, 64);
+                dia += __shfl_xor(dia, o, 64);
+            }
+            off = __shfl(off, 0, 64);   // lane 0's sums: one decision for the wavefront
+            dia = __shfl(dia, 0, 64);
+            // the cyclic order's 1e-32 is below the round-off floor of a matrix with degenerate
+            // eigenvalues (a 45-degree rotation of a degenerate pair leaves |a_pq| ~ eps |a_pp|),
+            // which then runs to the sweep limit: stop at 1e-30, and flush pairs below it
+            if (!(off > 1e-30 * dia) || off == 0.0) break;
+            for (int rd = 0; rd < m - 1; rd++) {
+                if (sub < np) {
+                    const int a = sub == 0 ? rd : (rd + sub) % (m - 1);
+                    const int b = sub == 0 ? m - 1 : (rd - sub + m - 1) % (m - 1);
+                    const int p = min(a, b), qq = max(a, b);
+                    s_pq[sub][0] = p;
+                    s_pq[sub][1] = qq;
+                    double cc = 1, ss = 0, cr = 1, ci = 0, skip = 1;
+                    if (qq < nb) {   // with nb odd, the pair of the phantom index nb is idle
+                        const zd apq = A[p * ld + qq];
+                        const double mag = sqrt(apq.r * apq.r + apq.i * apq.i);
+                        const double app = A[p * ld + p].r, aqq = A[qq * ld + qq].r;
+                        if (mag <= 1e-300 || mag * mag <= 1e-30 * (app * app + aqq * aqq)) {
+                            A[p * ld + qq] = {0.0, 0.0};
+                            A[qq * ld + p] = {0.0, 0.0};
+                        } else {
+                            const double tau = (aqq - app) / (2.0 * mag);
+                            const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                            cc = 1.0 / sqrt(1.0 + t * t);
+                            ss = t * cc;
+                            cr = apq.r / mag;
+                            ci = apq.i / mag;
+                            skip = 0;
+                        }
+                    }
+                    s_par[sub][0] = cc;
+                    s_par[sub][1] = ss;
+                    s_par[sub][2] = cr;
+                    s_par[sub][3] = ci;
+                    s_par[sub][4] = skip;
+                }
+                __syncthreads();
+                for (int x = sub; x < np * nb; x += 64) {   // columns p, q of A and V: A <- A J
+                    const int kp = x / nb, k = x - kp * nb;
+                    if (s_par[kp][4] != 0.0) continue;
+                    const int p = s_pq[kp][0], qq = s_pq[kp][1];
+                    const double c = s_par[kp][0], sn = s_par[kp][1];
+                    const zd em = {s_par[kp][2], -s_par[kp][3]};
+                    zd x0 = A[k * ld + p], y0 = zmul(em, A[k * ld + qq]);
+                    A[k * ld + p] = {c * x0.r - sn * y0.r, c * x0.i - sn * y0.i};
+                    A[k * ld + qq] = {sn * x0.r + c * y0.r, sn * x0.i + c * y0.i};
+                    zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
+                    V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
+                    V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                }
+                __syncthreads();
+                for (int x = sub; x < np * nb; x += 64) {   // rows p, q of A: A <- J^H A
+                    const int kp = x / nb, k = x - kp * nb;
+                    if (s_par[kp][4] != 0.0) continue;
+                    const int p = s_pq[kp][0], qq = s_pq[kp][1];
+                    const double c = s_par[kp][0], sn = s_par[kp][1];
+                    const zd ep = {s_par[kp][2], s_par[kp][3]};
+                    zd x0 = A[p * ld + k], y0 = zmul(ep, A[qq * ld + k]);
+                    A[p * ld + k] = {c * x0.r - sn * y0.r, c * x0.i - sn * y0.i};
+                    A[qq * ld + k] = {sn * x0.r + c * y0.r, sn * x0.i + c * y0.i};
+                }
+                __syncthreads();
+                // the rotated entries are zero in exact arithmetic; the next round's angles read
+                // other entries (each pair comes once per sweep), and its updates follow a barrier
+                if (sub < np && s_par[sub][4] == 0.0) {
+                    const int p = s_pq[sub][0], qq = s_pq[sub][1];
+                    A[p * ld + qq] = {0.0, 0.0};
+                    A[qq * ld + p] = {0.0, 0.0};
+                    A[p * ld + p].i = 0.0;
+                    A[qq * ld + qq].i = 0.0;
+                }
+            }
+            __syncthreads();
+        }
+    } else {
     // 3. cyclic complex Jacobi; a finished matrix idles while the others of its wavefront go on
     bool done = !valid;
     for (int sweep = 0; sweep < 40; sweep++) {
@@ -372,7 +475,7 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
             double off, dia;
             sum2([&](int e, double &o, double &d) {
                 int r = e / nb, c = e % nb;
-                double v2 = A[e].r * A[e].r + A[e].i * A[e].i;
+                double v2 = A[EI(e)].r * A[EI(e)].r + A[EI(e)].i * A[EI(e)].i;
                 if (r != c) o += v2; else d += v2;
             }, off, dia);
             s_sum[g][0] = (!(off > 1e-32 * dia) || off == 0.0) ? 1.0 : 0.0;
@@ -383,13 +486,13 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
         for (int p = 0; p < nb - 1; p++)
             for (int qq = p + 1; qq < nb; qq++) {
                 if (sub == 0 && !done) {
-                    zd apq = A[p * nb + qq];
+                    zd apq = A[p * ld + qq];
                     double mag = sqrt(apq.r * apq.r + apq.i * apq.i);
-                    double app = A[p * nb + p].r, aqq = A[qq * nb + qq].r;
+                    double app = A[p * ld + p].r, aqq = A[qq * ld + qq].r;
                     if (mag <= 1e-300 || mag * mag <= 1e-36 * (app * app + aqq * aqq)) {
                         s_rot[g][4] = 1.0;
-                        A[p * nb + qq] = {0.0, 0.0};
-                        A[qq * nb + p] = {0.0, 0.0};
+                        A[p * ld + qq] = {0.0, 0.0};
+                        A[qq * ld + p] = {0.0, 0.0};
                     } else {
                         double tau = (aqq - app) / (2.0 * mag);
                         double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
@@ -408,29 +511,30 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
                 const zd ep = {s_rot[g][2], s_rot[g][3]};   // e^{+i phi}
                 if (act && sub < nb) {  // columns of A and V
                     const int k = sub;
-                    zd x = A[k * nb + p], y = zmul(em, A[k * nb + qq]);
-                    A[k * nb + p] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
-                    A[k * nb + qq] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
-                    zd vx = V[k * nb + p], vy = zmul(em, V[k * nb + qq]);
-                    V[k * nb + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
-                    V[k * nb + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                    zd x = A[k * ld + p], y = zmul(em, A[k * ld + qq]);
+                    A[k * ld + p] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
+                    A[k * ld + qq] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
+                    zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
+                    V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
+                    V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
                 }
                 __syncthreads();
                 if (act && sub < nb) {  // rows of A
                     const int k = sub;
-                    zd x = A[p * nb + k], y = zmul(ep, A[qq * nb + k]);
-                    A[p * nb + k] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
-                    A[qq * nb + k] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
+                    zd x = A[p * ld + k], y = zmul(ep, A[qq * ld + k]);
+                    A[p * ld + k] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
+                    A[qq * ld + k] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
                 }
                 __syncthreads();
                 if (act && sub == 0) {
-                    A[p * nb + qq] = {0.0, 0.0};
-                    A[qq * nb + p] = {0.0, 0.0};
-                    A[p * nb + p].i = 0.0;
-                    A[qq * nb + qq].i = 0.0;
+                    A[p * ld + qq] = {0.0, 0.0};
+                    A[qq * ld + p] = {0.0, 0.0};
+                    A[p * ld + p].i = 0.0;
+                    A[qq * ld + qq].i = 0.0;
                 }
                 __syncthreads();
             }
+    }
     }
     // 4. ascending order of the signed eigenvalues (zheev), ties keep index order
     if (sub == 0) {
@@ -438,9 +542,9 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
         for (int k = 0; k < nb; k++) ord[k] = k;
         for (int a = 1; a < nb; a++) {
             int x = ord[a];
-            double vx = A[x * nb + x].r;
+            double vx = A[x * ld + x].r;
             int b = a - 1;
-            while (b >= 0 && A[ord[b] * nb + ord[b]].r > vx) {
+            while (b >= 0 && A[ord[b] * ld + ord[b]].r > vx) {
                 ord[b + 1] = ord[b];
                 b--;
             }
@@ -450,14 +554,14 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
     __syncthreads();
     if (!valid || sub >= nb) return;
     const int src = s_ord[g][sub];
-    evals7[((size_t)q * nvar + var) * nb + sub] = fabs(A[src * nb + src].r);
+    evals7[((size_t)q * nvar + var) * nb + sub] = fabs(A[src * ld + src].r);
     if (var == 0) {
         // column j = sub (sorted) of V; "pure imaginary" columns (sum |Re| < 1e-3) times i
         double sre = 0;
-        for (int k = 0; k < nb; k++) sre += fabs(V[k * nb + src].r);
+        for (int k = 0; k < nb; k++) sre += fabs(V[k * ld + src].r);
         const bool rot = sre < 1e-3;
         for (int k = 0; k < nb; k++) {
-            zd v = V[k * nb + src];
+            zd v = V[k * ld + src];
             if (rot) v = {-v.i, v.r};
             size_t o = 2 * (((size_t)q * nb + sub) * nb + k);
             evec0[o] = v.r;
@@ -588,6 +692,18 @@ int eig_prepare(alt3_ctx *ctx, const alt3_system *sys)
 int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, double *evals, double *evec)
 {
     const int nb = ctx->nb;
+    if (nb > 8) {   // one matrix per wavefront, parallel (round-robin) Jacobi order
+        const size_t nmat = (size_t)np * nvar;
+        const size_t lds = (size_t)2 * nb * (nb + 1) * sizeof(zd);
+        if (lds + 4096 > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
+        const void *kern = (const void *)k_eig_group<64, true>;
+        if (lds + 4096 > 65536) ALT3_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_eig_group<64, true>), dim3((unsigned)nmat), dim3(64), lds, ctx->stream, np, nvar, nb, kq,
+                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born,
+                           ctx->d_born.p);
+        ALT3_HIP(hipGetLastError());
+        return ALT3_OK;
+    }
     const int G = nb <= 8 ? 8 : nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
     if (G < 64) {   // 64/G matrices per wavefront
         const int M = 64 / G;

@@ -46,7 +46,15 @@ constexpr int QT = 16;           // mesh points q2 per workgroup
 constexpr int NW = 8;            // waves per workgroup
 constexpr int QW = QT / NW;      // points per wave in the U and V products
 constexpr int KSMAX = 16;        // phase k-steps held in registers: ndr <= 32
-constexpr int PS = 2 * NN + 4;   // LDS stride of one point's Phi' (re, im), +4 against bank conflicts
+// LDS layouts, padded against bank conflicts (32 banks of 4 bytes):
+//   Phi'[q][part][b][c]: b stride 26 (the 16 rows b of a U operand read fall on distinct even
+//   banks, the next k-group on the odd ones), q stride 4 mod 8 (the four lane groups of a phase
+//   write on distinct banks)
+//   Ost[q][l2][l3]: l2 stride 25 (the 16 columns l2 of a V tile on distinct banks), q stride odd
+constexpr int BS = 26;
+constexpr int PP = NB * BS;          // one part (re or im) of one point
+constexpr int PS = 2 * PP + 4;       // one point
+constexpr int OS = NB * 25 + 1;      // one point of the staging area
 
 // e3 and e2 operand fragments of every mesh point from the FP32 eigenvectors evf
 // [k][l col][b comp] (complex interleaved):
@@ -102,8 +110,8 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
                                                      double c33sq8, float *__restrict__ out, int *__restrict__ err)
 {
     extern __shared__ __align__(16) float lds[];
-    float *Phi = lds;              // [QT][PS]: Re Phi'[(b,c)], then Im
-    float *Ost = Phi + QT * PS;    // [(l2,l3)][QT] scaled |V3|^2 of the current l1
+    float *Phi = lds;              // [QT][2][NB][BS]: Re Phi'[b][c], then Im
+    float *Ost = Phi + QT * PS;    // [QT][NB][25] scaled |V3|^2 (l2, l3) of the current l1
     __shared__ double w2s[QT][NB], w3s[QT][NB];
     __shared__ int k3s[QT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -154,11 +162,11 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
                     ci = mma(ai[t], b, ci);
                 }
             }
-            const int n = 16 * nt + col;
+            const int n = 16 * nt + col, bc = (n / NB) * BS + n % NB;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                Phi[(4 * grp + i) * PS + n] = cr[i];
-                Phi[(4 * grp + i) * PS + NN + n] = ci[i];
+                Phi[(4 * grp + i) * PS + bc] = cr[i];
+                Phi[(4 * grp + i) * PS + PP + bc] = ci[i];
             }
         }
         __syncthreads();
@@ -184,7 +192,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
 #pragma unroll
                 for (int mt = 0; mt < MU; mt++) {
                     const int row = 16 * mt + col, p = row >= NB ? 1 : 0, b = row - p * NB;
-                    float a = ph[(p ^ pp) * NN + b * NB + c];
+                    float a = ph[(p ^ pp) * PP + b * BS + c];
                     if (p == 0 && pp == 1) a = -a;
 #pragma unroll
                     for (int nt = 0; nt < NU; nt++) u[mt][nt] = mma(a, bu[nt], u[mt][nt]);
@@ -224,13 +232,13 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
                             const float v2 = s0 + (col < 8 ? y1 : y2);
                             const double w = w13 * w2s[q][col];
                             if (!(w <= 1.7e308)) atomicExch(err, 1);   // a zero eigenvalue in the triple
-                            Ost[(col * NB + l3) * QT + q] = v2 * (float)(c33sq8 * w * w);
+                            Ost[q * OS + col * 25 + l3] = v2 * (float)(c33sq8 * w * w);
                         }
                         if (col < 8) {   // l2 = 16 + col
                             const float v2 = s1 + y2;
                             const double w = w13 * w2s[q][16 + col];
                             if (!(w <= 1.7e308)) atomicExch(err, 1);
-                            Ost[((16 + col) * NB + l3) * QT + q] = v2 * (float)(c33sq8 * w * w);
+                            Ost[q * OS + (16 + col) * 25 + l3] = v2 * (float)(c33sq8 * w * w);
                         }
                     }
                 }
@@ -241,7 +249,8 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
         for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
             const int row = x / (QT / 4), c4 = x % (QT / 4);
             if (q0 + 4 * c4 < nk) {
-                const float4 val = *reinterpret_cast<const float4 *>(Ost + row * QT + 4 * c4);
+                const float *src = Ost + 4 * c4 * OS + (row / NB) * 25 + row % NB;
+                const float4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
                 *reinterpret_cast<float4 *>(o + (size_t)row * nk + 4 * c4) = val;
             }
         }
@@ -273,7 +282,7 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
     hipLaunchKernelGGL(k_chain_pack_g, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ksub, ndr, ks,
                        ctx->d_gp.p, pg.p);
     ALT3_HIP(hipGetLastError());
-    const size_t lds = (size_t)(QT * PS + NN * QT) * sizeof(float);
+    const size_t lds = (size_t)(QT * PS + QT * OS) * sizeof(float);
     if (lds + 2 * QT * NB * sizeof(double) + 64 > ctx->lds_max) return fail(ALT3_E_ARG, "V3 chain tile does not fit in LDS");
     ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const double c33 = const33();

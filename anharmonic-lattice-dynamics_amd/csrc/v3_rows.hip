@@ -14,6 +14,10 @@
 // written with consecutive q2 per lane.  T = double (ALT3_V3_FP64) or float (ALT3_V3_MIXED:
 // FP32 phase sum and contractions, FP64 scale); the scale uses the per-mode table
 // |lambda|^(-1/4), so 1/den = w1 w2 w3 / sqrt 8.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "alt3_impl.h"
 #include "mesh_dev.h"
 
@@ -41,13 +45,14 @@ __global__ void k_copy_real(size_t n, const double *__restrict__ in, T *__restri
 // q3 = -q1-q2 come per (s, q2) from ev3x [s][q2][NB col][NB] and w3x [s][q2][NB] (the
 // self-energy's q3 is not a mesh point); otherwise q1 is the irreducible point s and e3, w3
 // are read at the mesh point -q1-q2.
+// one tile: q1 = irreducible point s (of the block ibz_lo...), q2 = q0 .. q0 + TQ - 1
 template <typename T, int NB, int TQ, bool EXT>
-__global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, int ibz_lo, int ndr,
-                                                     const int *__restrict__ mapinv, const T *__restrict__ eph,
-                                                     const T *__restrict__ evec, const double *__restrict__ wq,
-                                                     const T *__restrict__ gp, double c33sq8, T *__restrict__ out,
-                                                     int *__restrict__ err, const T *__restrict__ ev3x,
-                                                     const double *__restrict__ w3x, const double *__restrict__ w1x)
+__device__ __forceinline__ void rows_tile(const alt3::MeshDev &m, int nk, int ibz_lo, int ndr,
+                                          const int *__restrict__ mapinv, const T *__restrict__ eph,
+                                          const T *__restrict__ evec, const double *__restrict__ wq,
+                                          const T *__restrict__ gp, double c33sq8, T *__restrict__ out,
+                                          int *__restrict__ err, const T *__restrict__ ev3x,
+                                          const double *__restrict__ w3x, const double *__restrict__ w1x, int s, int q0)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     T *E = reinterpret_cast<T *>(smem);     // [r][q] re, then im        (2 ndr TQ)
@@ -62,7 +67,6 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
     __shared__ int k3s[TQ];
     constexpr int NT = TQ * NB, NN = NB * NB;
     const int tid = threadIdx.x, q = tid % TQ, j = tid / TQ;
-    const int s = blockIdx.y, q0 = blockIdx.x * TQ;
     const int k1 = EXT ? 0 : mapinv[ibz_lo + s];
     const int n2 = q0 + q;
     const bool valid = n2 < nk;
@@ -174,6 +178,74 @@ __global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, in
 }
 
 template <typename T, int NB, int TQ, bool EXT>
+__global__ void __launch_bounds__(TQ * NB) k_v3_rows(alt3::MeshDev m, int nk, int ibz_lo, int ndr,
+                                                     const int *__restrict__ mapinv, const T *__restrict__ eph,
+                                                     const T *__restrict__ evec, const double *__restrict__ wq,
+                                                     const T *__restrict__ gp, double c33sq8, T *__restrict__ out,
+                                                     int *__restrict__ err, const T *__restrict__ ev3x,
+                                                     const double *__restrict__ w3x, const double *__restrict__ w1x)
+{
+    rows_tile<T, NB, TQ, EXT>(m, nk, ibz_lo, ndr, mapinv, eph, evec, wq, gp, c33sq8, out, err, ev3x, w3x, w1x,
+                              blockIdx.y, blockIdx.x * TQ);
+}
+
+// the arrays of one context in a batched launch (alt3_v3_batch)
+template <typename T>
+struct RowsJob {
+    alt3::MeshDev m;
+    int nk, ndr, tiles;       // tiles of q2 per irreducible point
+    long long first;          // first work item of this context in the batch
+    const int *mapinv;
+    const T *eph, *evec, *gp;
+    const double *wq;
+    T *out;
+    int *err;
+};
+
+// Shared persistent launch over the irreducible zones of several contexts of NB branches:
+// a grid sized to the device pops work items (context, q1, tile of q2) from one counter;
+// every wavefront reads the same item and leaves the loop together once the counter passes
+// the last item.  Each tile is computed exactly as k_v3_rows computes it.
+template <typename T, int NB, int TQ>
+__global__ void __launch_bounds__(TQ * NB) k_v3_rows_batch(const RowsJob<T> *__restrict__ jobs, int njobs,
+                                                           long long nitems, unsigned long long *__restrict__ counter,
+                                                           double c33sq8)
+{
+    __shared__ long long s_item;
+    __shared__ int s_job;
+    for (;;) {
+        __syncthreads();   // the previous item is done with s_item and the LDS tiles
+        if (threadIdx.x == 0) {
+            const long long it = (long long)atomicAdd(counter, 1ull);
+            int j = 0;
+            if (it < nitems) {   // last context whose first item is <= it (jobs sorted by first)
+                int lo = 0, hi = njobs - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) / 2;
+                    if (jobs[mid].first <= it) lo = mid; else hi = mid - 1;
+                }
+                j = lo;
+            }
+            s_item = it;
+            s_job = j;
+        }
+        __syncthreads();
+        const long long it = s_item;
+        if (it >= nitems) break;
+        const RowsJob<T> &J = jobs[s_job];
+        const long long local = it - J.first;
+        const int sq = (int)(local / J.tiles), tile = (int)(local - (long long)sq * J.tiles);
+        rows_tile<T, NB, TQ, false>(J.m, J.nk, 0, J.ndr, J.mapinv, J.eph, J.evec, J.wq, J.gp, c33sq8, J.out, J.err,
+                                    nullptr, nullptr, nullptr, sq, tile * TQ);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p)
+{
+    if (threadIdx.x == 0) *p = 0ull;
+}
+
+template <typename T, int NB, int TQ, bool EXT>
 int launch_rows(alt3_ctx *ctx, int ksub, const T *eph, const T *evec, const double *wq, const T *gp, T *out,
                 int *d_err, const T *ev3x, const double *w3x, const double *w1x)
 {
@@ -216,8 +288,9 @@ bool v3_rows_supported(const alt3_ctx *ctx, bool fp32)
     return ctx->V.ndr > 0 && (ctx->nb == 6 || ctx->nb == 12 || (ctx->nb == 24 && fp32));
 }
 
-// G' must be prepared (v3_prep_launch); out holds ksub*nb^3*nk values of the precision's type
-int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
+// wq = |lambda|^(-1/4) and, for the FP32 kernels, FP32 copies of the phase table, the
+// eigenvectors and (rows kernel) G' of ksub irreducible points, on the context stream
+int v3_rows_inputs(alt3_ctx *ctx, int ksub, bool fp32, bool want_gpf)
 {
     const int nk = ctx->nk, nb = ctx->nb, ndr = ctx->V.ndr;
     DBuf<double> &wq = ctx->d_wq;
@@ -225,29 +298,191 @@ int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
     hipLaunchKernelGGL(k_lam_quarter, dim3((nk * nb + 255) / 256), dim3(256), 0, ctx->stream, nk * nb, ctx->d_eival.p,
                        wq.p);
     ALT3_HIP(hipGetLastError());
+    if (!fp32) return ALT3_OK;
+    DBuf<float> &ephf = ctx->d_ephf, &evf = ctx->d_evf, &gpf = ctx->d_gpf32;
+    const size_t ne = (size_t)nk * nb * nb * 2, np = (size_t)nk * ndr * 2, ng = (size_t)ksub * ndr * nb * nb * nb * 2;
+    ALT3_TRY(ephf.reserve(np));
+    ALT3_TRY(evf.reserve(ne));
+    hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, np,
+                       ctx->d_eph.p, ephf.p);
+    hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, ne,
+                       ctx->d_evec.p, evf.p);
+    if (want_gpf) {
+        ALT3_TRY(gpf.reserve(ng));
+        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ng,
+                           ctx->d_gp.p, gpf.p);
+    }
+    ALT3_HIP(hipGetLastError());
+    return ALT3_OK;
+}
+
+// G' must be prepared (v3_prep_launch); out holds ksub*nb^3*nk values of the precision's type
+int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
+{
+    const bool chain = fp32 && v3_chain_supported(ctx);
+    ALT3_TRY(v3_rows_inputs(ctx, ksub, fp32, fp32 && !chain));
+    DBuf<double> &wq = ctx->d_wq;
     if (!fp32) {
         ALT3_TRY(dispatch_rows<double>(ctx, ksub, ctx->d_eph.p, ctx->d_evec.p, wq.p, ctx->d_gp.p, ctx->d_v33.p, d_err));
+    } else if (chain) {   // 24 branches: every contraction on the matrix cores
+        ALT3_TRY(v3_chain_run(ctx, ksub, ctx->d_evf.p, ctx->d_ephf.p, wq.p, ctx->d_v33f.p, d_err));
     } else {
-        DBuf<float> &ephf = ctx->d_ephf, &evf = ctx->d_evf, &gpf = ctx->d_gpf32;
-        const size_t ne = (size_t)nk * nb * nb * 2, np = (size_t)nk * ndr * 2, ng = (size_t)ksub * ndr * nb * nb * nb * 2;
-        ALT3_TRY(ephf.reserve(np));
-        ALT3_TRY(evf.reserve(ne));
-        if (!v3_chain_supported(ctx)) ALT3_TRY(gpf.reserve(ng));
-        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, np,
-                           ctx->d_eph.p, ephf.p);
-        hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, ctx->stream, ne,
-                           ctx->d_evec.p, evf.p);
-        if (!v3_chain_supported(ctx))
-            hipLaunchKernelGGL(k_copy_real<float>, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ng,
-                               ctx->d_gp.p, gpf.p);
-        ALT3_HIP(hipGetLastError());
-        if (v3_chain_supported(ctx))   // 24 branches: every contraction on the matrix cores
-            ALT3_TRY(v3_chain_run(ctx, ksub, evf.p, ephf.p, wq.p, ctx->d_v33f.p, d_err));
-        else
-            ALT3_TRY(dispatch_rows<float>(ctx, ksub, ephf.p, evf.p, wq.p, gpf.p, ctx->d_v33f.p, d_err));
+        ALT3_TRY(dispatch_rows<float>(ctx, ksub, ctx->d_ephf.p, ctx->d_evf.p, wq.p, ctx->d_gpf32.p, ctx->d_v33f.p, d_err));
     }
     // no wait: every buffer used here belongs to the context and outlives the kernels
     return ALT3_OK;
+}
+
+// ------------------------------------------------------------------ batched launches
+namespace {
+
+// the tile sizes the autotuner tries for a branch count (TQ * NB threads per group)
+const int *tile_candidates(int nb, int &n)
+{
+    static const int c6[] = {16, 32, 64}, c12[] = {8, 16, 32};
+    n = 3;
+    return nb == 6 ? c6 : c12;
+}
+
+size_t rows_lds(int nb, int ndr, int tq, size_t tsize)
+{
+    return (size_t)2 * ((ndr + 2 * nb * nb) * tq + (nb <= 12 ? ndr * nb * nb : 0)) * tsize;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tune;   // (fp32, nb, ndr) -> tile size
+
+template <typename T, int NB, int TQ>
+int batch_launch_t(alt3_ctx *lead, const RowsJob<T> *d_jobs, int njobs, long long nitems, int ndr_max,
+                   unsigned long long *counter)
+{
+    const void *kern = (const void *)k_v3_rows_batch<T, NB, TQ>;
+    const size_t lds = rows_lds(NB, ndr_max, TQ, sizeof(T));
+    if (lds > lead->lds_max) return fail(ALT3_E_ARG, "batched V3 tile does not fit in LDS");
+    if (lds > 65536) ALT3_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, cus = 0;
+    ALT3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, TQ * NB, lds));
+    ALT3_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lead->device));
+    const long long want = (long long)std::max(1, per_cu) * std::max(1, cus);
+    const unsigned grid = (unsigned)std::max(1ll, std::min(want, nitems));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, lead->stream, counter);
+    const double c33 = const33();
+    kernel_mark(lead, 0, false);
+    hipLaunchKernelGGL((k_v3_rows_batch<T, NB, TQ>), dim3(grid), dim3(TQ * NB), lds, lead->stream, d_jobs, njobs, nitems,
+                       counter, c33 * c33 / 8);
+    ALT3_HIP(hipGetLastError());
+    kernel_mark(lead, 0, true);
+    return ALT3_OK;
+}
+
+template <typename T>
+int batch_launch(alt3_ctx *lead, int nb, int tq, const RowsJob<T> *d_jobs, int njobs, long long nitems, int ndr_max,
+                 unsigned long long *counter)
+{
+#define ALT3_BATCH(NB, TQ) \
+    if (nb == NB && tq == TQ) return batch_launch_t<T, NB, TQ>(lead, d_jobs, njobs, nitems, ndr_max, counter);
+    ALT3_BATCH(6, 16) ALT3_BATCH(6, 32) ALT3_BATCH(6, 64)
+    ALT3_BATCH(12, 8) ALT3_BATCH(12, 16) ALT3_BATCH(12, 32)
+#undef ALT3_BATCH
+    return fail(ALT3_E_ARG, "no batched V3 kernel for this branch count and tile");
+}
+
+template <typename T>
+RowsJob<T> job_of(alt3_ctx *c, int tq, long long first, bool fp32)
+{
+    RowsJob<T> j;
+    j.m = MeshDev::make(c->nc);
+    j.nk = c->nk;
+    j.ndr = c->V.ndr;
+    j.tiles = (c->nk + tq - 1) / tq;
+    j.first = first;
+    j.mapinv = c->d_mapinv.p;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        j.eph = c->d_ephf.p;
+        j.evec = c->d_evf.p;
+        j.gp = c->d_gpf32.p;
+        j.out = c->d_v33f.p;
+    } else {
+        j.eph = c->d_eph.p;
+        j.evec = c->d_evec.p;
+        j.gp = c->d_gp.p;
+        j.out = c->d_v33.p;
+    }
+    (void)fp32;
+    j.wq = c->d_wq.p;
+    j.err = c->d_flags.p + 0;
+    return j;
+}
+
+// one batched launch over ctxs with tile tq; the job table and the counter live in the lead
+// context's batch scratch
+template <typename T>
+int batch_run(alt3_ctx *const *ctxs, int n, int tq, bool fp32)
+{
+    alt3_ctx *lead = ctxs[0];
+    std::vector<RowsJob<T>> jobs;
+    long long first = 0;
+    int ndr_max = 0;
+    for (int i = 0; i < n; i++) {
+        jobs.push_back(job_of<T>(ctxs[i], tq, first, fp32));
+        first += (long long)ctxs[i]->nibz * jobs.back().tiles;
+        ndr_max = std::max(ndr_max, ctxs[i]->V.ndr);
+    }
+    const size_t bytes = sizeof(RowsJob<T>) * jobs.size() + 64;
+    ALT3_TRY(lead->d_batch.reserve(bytes));
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(lead->d_batch.p);
+    RowsJob<T> *d_jobs = reinterpret_cast<RowsJob<T> *>(lead->d_batch.p + 64);
+    ALT3_HIP(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(RowsJob<T>) * jobs.size(), hipMemcpyHostToDevice, lead->stream));
+    ALT3_TRY(batch_launch<T>(lead, lead->nb, tq, d_jobs, n, first, ndr_max, counter));
+    // the host copy of the job table must outlive the copy
+    ALT3_HIP(hipStreamSynchronize(lead->stream));
+    return ALT3_OK;
+}
+
+}  // namespace
+
+bool v3_batch_supported(const alt3_ctx *ctx) { return ctx->V.ndr > 0 && (ctx->nb == 6 || ctx->nb == 12); }
+
+// the tile size of the batched kernel for the context's (precision, nb, ndr): the cached
+// choice, or the fastest candidate timed on this context's own zone (its inputs prepared)
+int v3_tile_for(alt3_ctx *ctx, bool fp32, int *tq)
+{
+    const auto key = std::make_tuple(fp32 ? 1 : 0, ctx->nb, ctx->V.ndr);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            *tq = it->second;
+            return ALT3_OK;
+        }
+    }
+    int nc = 0;
+    const int *cand = tile_candidates(ctx->nb, nc);
+    float best = 3.4e38f;
+    int best_tq = 0;
+    alt3_ctx *one[1] = {ctx};
+    for (int c = 0; c < nc; c++) {
+        if (rows_lds(ctx->nb, ctx->V.ndr, cand[c], fp32 ? 4 : 8) > ctx->lds_max) continue;
+        for (int rep = 0; rep < 2; rep++) {   // the first launch pays the kernel load
+            ALT3_TRY(fp32 ? batch_run<float>(one, 1, cand[c], true) : batch_run<double>(one, 1, cand[c], false));
+            float ms = 0;
+            ALT3_HIP(hipEventElapsedTime(&ms, ctx->ev_k[0], ctx->ev_k[1]));
+            if (rep == 1 && ms < best) {
+                best = ms;
+                best_tq = cand[c];
+            }
+        }
+    }
+    if (!best_tq) return fail(ALT3_E_ARG, "no V3 tile fits in LDS");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best_tq;
+    *tq = best_tq;
+    return ALT3_OK;
+}
+
+int v3_batch_launch(alt3_ctx *const *ctxs, int n, bool fp32, int tq)
+{
+    return fp32 ? batch_run<float>(ctxs, n, tq, true) : batch_run<double>(ctxs, n, tq, false);
 }
 
 // |V3|^2 [s][l1][l2][l3][q2] (FP64) for nq list points q1 with G' in gp, e3 / w3 per (s, q2)

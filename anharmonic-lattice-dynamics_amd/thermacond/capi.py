@@ -47,7 +47,8 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_kstars", "alt3_eig", "alt3_set_eig", "alt3_v3sq_ibz", "alt3_fgr", "alt3_tau", "alt3_kappa_rta",
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
-           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static", "alt3_kernel_ms"]
+           "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static", "alt3_kernel_ms",
+           "alt3_v3_batch", "alt3_get_v3sq"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -87,6 +88,18 @@ def device_free_bytes() -> int:
     f = C.c_longlong()
     _check(lib().alt3_device_free(C.byref(f), None))
     return int(f.value)
+
+
+def v3_batch(contexts: Sequence["Context"], precision: int = V3_FP64, tile: int = 0) -> int:
+    """|V3|^2 over the whole IBZ of every context in one shared persistent launch
+    (alt3_v3_batch); each context then holds what v3sq(0, nibz - 1, precision) leaves and
+    solve() reuses it.  Returns the q2 tile the launch used (0: one launch per context)."""
+    hs = (C.c_void_p * len(contexts))(*[c.h for c in contexts])
+    used = C.c_int()
+    _check(lib().alt3_v3_batch(hs, len(contexts), precision, tile, C.byref(used)))
+    for c in contexts:
+        c._last_ksub = c.nibz
+    return used.value
 
 
 def giso_of(inp: InputSet):
@@ -210,6 +223,12 @@ class Context:
         out = np.zeros((hi - lo + 1, self.nk, self.nb, self.nb, self.nb), order="F") if want else None
         _check(lib().alt3_v3sq_ibz(self.h, lo, hi, precision, _p(out)))
         self._last_ksub = hi - lo + 1
+        return out
+
+    def get_v3sq(self):
+        """The |V3|^2 the context holds (alt3_get_v3sq), shape (ksub, nk, nb, nb, nb)."""
+        out = np.zeros((self._ksub(), self.nk, self.nb, self.nb, self.nb), order="F")
+        _check(lib().alt3_get_v3sq(self.h, _p(out)))
         return out
 
     def fgr(self, want: bool = True):

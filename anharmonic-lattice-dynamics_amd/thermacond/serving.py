@@ -17,8 +17,16 @@ Cached contexts count against the budget and are evicted, least recently used fi
 an admission needs their memory.  Requests with the same key are serialised on that
 context.
 
+The |V3|^2 stage of admitted requests is batched: a worker whose context needs |V3|^2 (6 or
+12 branches; larger cells fill the device on their own) queues it with the server's V3
+batcher and waits; the batcher takes everything queued for one branch count and precision
+and runs it as one shared persistent launch (capi.v3_batch), while requests that arrive
+meanwhile queue for the next launch.  The rest of each solve (FGR, tau, kappa, CG) runs on
+the request's own stream.
+
 Every result is a pure function of its request: a context computes the same kappa whether
-it runs alone or beside others (tests/test_serving.py checks this on the device).
+it runs alone or beside others, and a batched launch computes each context's |V3|^2 with the
+same code as a lone one (tests/test_serving.py checks both on the device).
 """
 from __future__ import annotations
 
@@ -50,12 +58,15 @@ class Request:
     t_admit: float = 0.0
     t_done: float = 0.0
     footprint: int = 0
+    nb: int = 0                       # branches of the cell (from the host-only footprint context)
     reused: bool = False
     result: Optional[dict] = None
     error: Optional[str] = None
     # the admission decision: the cached context this request was admitted on (its busy count
     # raised, its reserve grown), or None when it was charged its own footprint
     _entry: Optional["_Cached"] = dataclasses.field(default=None, repr=False, compare=False)
+    # admitted with a place expected in the V3 batcher (settled once by run or unexpect)
+    _expects_v3: bool = dataclasses.field(default=False, repr=False, compare=False)
 
     @property
     def mesh_key(self) -> str:
@@ -68,12 +79,89 @@ class Request:
 
 
 class _Cached:
-    def __init__(self, ctx: capi.Context, reserve: int):
+    def __init__(self, ctx: capi.Context, reserve: int, v3_precision: Optional[int] = None):
         self.ctx = ctx
         self.reserve = reserve          # bytes counted against the budget while cached
         self.lock = threading.Lock()
         self.busy = 0
         self.last_used = time.monotonic()
+        self.v3_precision = v3_precision   # precision of the |V3|^2 the context holds
+
+
+BATCHED_BRANCHES = (6, 12)
+
+
+class V3Batcher:
+    """Shared persistent |V3|^2 launches over the contexts queued by concurrent workers."""
+
+    def __init__(self, window_s: float = 0.0):
+        """window_s > 0: once a context is queued, wait up to that long for the admitted
+        requests still on their way to the V3 stage (expect()) before launching; 0: launch
+        at once with what is queued."""
+        self._q: List[tuple] = []
+        self._cv = threading.Condition()
+        self._stop = False
+        self._expected = 0
+        self.window_s = window_s
+        self.launches = 0               # batched launches run
+        self.contexts = 0               # contexts they covered
+        self.max_batch = 0
+        self._t = threading.Thread(target=self._loop, daemon=True)
+        self._t.start()
+
+    def run(self, ctx: capi.Context, precision: int) -> None:
+        """|V3|^2 of ctx (eigen data on the device) in the next shared launch; returns when it
+        is done, raising the launch's error if it failed."""
+        done, box = threading.Event(), {}
+        with self._cv:
+            self._expected = max(0, self._expected - 1)
+            if self._stop:
+                raise RuntimeError("V3 batcher closed")
+            self._q.append((ctx, precision, done, box))
+            self._cv.notify_all()
+        done.wait()
+        if "error" in box:
+            raise box["error"]
+
+    def expect(self) -> None:
+        """An admitted request will queue its context (run) or withdraw (unexpect)."""
+        with self._cv:
+            self._expected += 1
+
+    def unexpect(self) -> None:
+        with self._cv:
+            self._expected = max(0, self._expected - 1)
+            self._cv.notify_all()
+
+    def _loop(self) -> None:
+        while True:
+            with self._cv:
+                while not self._q and not self._stop:
+                    self._cv.wait()
+                if not self._q:
+                    return
+                t_end = time.monotonic() + self.window_s
+                while self._expected > 0 and not self._stop and time.monotonic() < t_end:
+                    self._cv.wait(timeout=max(1e-4, t_end - time.monotonic()))
+                key = (self._q[0][0].nb, self._q[0][1])
+                group = [e for e in self._q if (e[0].nb, e[1]) == key]
+                self._q = [e for e in self._q if (e[0].nb, e[1]) != key]
+            try:
+                capi.v3_batch([e[0] for e in group], key[1])
+            except Exception as ex:   # each waiter sees it; its own solve then decides
+                for e in group:
+                    e[3]["error"] = ex
+            self.launches += 1
+            self.contexts += len(group)
+            self.max_batch = max(self.max_batch, len(group))
+            for e in group:
+                e[2].set()
+
+    def close(self) -> None:
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        self._t.join()
 
 
 def device_free_bytes() -> int:
@@ -85,9 +173,10 @@ class KappaServer:
     """Continuous-batching scheduler over alt3 contexts (see the module docstring)."""
 
     def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, max_bypass: int = 8,
-                 cache: bool = True):
+                 cache: bool = True, batch_v3: bool = True, batch_window_s: float = 0.0):
         capi.lib()                      # load the library once, before any worker thread
         self.slots = slots
+        self.batcher = V3Batcher(batch_window_s) if batch_v3 else None
         self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes())
         self.max_bypass = max_bypass
         self.use_cache = cache
@@ -137,6 +226,8 @@ class KappaServer:
         self._sched.join()
         for t in self._threads:
             t.join()
+        if self.batcher is not None:
+            self.batcher.close()
         for c in self._cache.values():
             c.ctx.close()
         self._cache.clear()
@@ -150,6 +241,7 @@ class KappaServer:
     # ------------------------------------------------------------------ internals
     def _estimate(self, req: Request) -> int:
         with capi.Context(req.inputs, nc=req.nc, host_only=True) as h:
+            req.nb = h.nb
             return h.footprint(len(req.temps), req.precision)
 
     def _evict_for(self, need: int) -> bool:
@@ -202,6 +294,9 @@ class KappaServer:
                     self._cv.wait(timeout=0.5)
                     continue
                 admitted.t_admit = time.perf_counter()
+                if self.batcher is not None and admitted.nb in BATCHED_BRANCHES:
+                    admitted._expects_v3 = True
+                    self.batcher.expect()
                 self._running += 1
                 self.max_concurrent = max(self.max_concurrent, self._running)
                 t = threading.Thread(target=self._work, args=(admitted,), daemon=True)
@@ -224,19 +319,33 @@ class KappaServer:
                 req.reused = True
             with lock:
                 ctx.set_delta(req.delta, req.sigma)
+                held = ctx_entry is not None and ctx_entry.v3_precision == req.precision
+                if req._expects_v3 and not held:
+                    if ctx_entry is None:
+                        ctx.eig(want=False)     # a cached context keeps its eigen data
+                    req._expects_v3 = False     # run() settles the expectation
+                    try:
+                        self.batcher.run(ctx, req.precision)
+                    except capi.Alt3Error:
+                        pass    # the solve below recomputes what the launch did not leave
                 out = ctx.solve([float(t) for t in req.temps], req.precision)
+                if ctx_entry is not None:
+                    ctx_entry.v3_precision = req.precision
             req.result = dict(kappa_rta=out["kappa_rta"], kappa_bte=out["kappa_bte"], cg_iters=out["cg_iters"],
                               stage_ms=out["stage_ms"])
         except Exception as e:  # the request fails, the server keeps serving
             req.error = f"{type(e).__name__}: {e}"
         finally:
+            if req._expects_v3:
+                req._expects_v3 = False
+                self.batcher.unexpect()
             with self._cv:
                 if ctx_entry is not None:
                     ctx_entry.busy -= 1
                     ctx_entry.last_used = time.monotonic()
                 elif own_ctx is not None:
                     if self.use_cache and req.error is None and req.mesh_key not in self._cache:
-                        self._cache[req.mesh_key] = _Cached(own_ctx, req.footprint)
+                        self._cache[req.mesh_key] = _Cached(own_ctx, req.footprint, req.precision)
                     else:
                         own_ctx.close()
                         self._in_use -= req.footprint

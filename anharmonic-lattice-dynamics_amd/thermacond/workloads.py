@@ -16,8 +16,12 @@ import numpy as np
 from . import capi, models, refio, serving
 
 CELLS = ("sw-si", "zb4", "rs8", "rs16")
-# largest even mesh per cell in the request stream (16-atom cells beyond 6^3 need > 40 GB)
-MESH_CAP = {"sw-si": 16, "zb4": 12, "rs8": 8, "rs16": 6}
+# largest even mesh (at most 16^3) per cell in the request stream whose device footprint
+# (alt3_footprint, one temperature) fits one 288 GB device: mixed V3 -- SW-Si 16^3 3.5 GB,
+# zb4 16^3 38.5 GB, rs8 16^3 119 GB, rs16 10^3 185 GB (12^3: 481 GB); FP64 -- rs8 16^3
+# 233 GB, rs16 8^3 108 GB (10^3: 348 GB)
+MESH_CAP = {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 10}
+MESH_CAP_FP64 = {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 8}
 
 
 _fc_done: Dict[tuple, models.ForceConstants] = {}
@@ -88,45 +92,122 @@ def run_rocksalt(n: int = 12, temps: Sequence[float] = (300.0,), log=print) -> d
 
 
 # ---------------------------------------------------------------------------- batched sweep
-def sweep_requests(count: int = 64, seed: int = 0) -> List[serving.Request]:
-    """SW-Si instances: lattice constant ~ U[5.4, 5.9] A per instance, mesh from
-    {8,10,12,14}^3, temperatures from {100,200,300,500,800} K (all five per instance)."""
+SWEEP_MESHES = (8, 10, 12, 14)
+SWEEP_TEMPS = (100.0, 200.0, 300.0, 500.0, 800.0)
+
+
+def sweep_draw(seed: int):
+    """Instance `seed` of the SW-Si sweep: lattice constant ~ U[5.4, 5.9] A (rounded to
+    1e-4 A) and mesh from {8,10,12,14}^3, drawn from numpy's default_rng(seed)."""
     rng = np.random.default_rng(seed)
-    draws = [(round(float(rng.uniform(5.4, 5.9)), 4), int(rng.choice([8, 10, 12, 14]))) for _ in range(count)]
-    for a, _ in draws:
-        if ("sw-si", a, 0) not in _fc_done:
-            _fc_done[("sw-si", a, 0)] = models.sw_silicon(a, batched=True)
-    reqs = []
-    for a, n in draws:
-        reqs.append(serving.Request(f"sw-si-a{a}", cell_inputs("sw-si", n, a), [100.0, 200.0, 300.0, 500.0, 800.0],
-                                    precision=capi.V3_MIXED))
-    return reqs
+    return round(float(rng.uniform(5.4, 5.9)), 4), int(rng.choice(SWEEP_MESHES))
 
 
-def run_sweep(count: int = 64, seed: int = 0, slots: int = 4, log=print) -> dict:
-    t0 = time.perf_counter()
-    reqs = sweep_requests(count, seed)
-    t_gen = time.perf_counter() - t0
-    log(f"sweep: {count} instances generated in {t_gen:.1f} s")
-    nel = 0
-    with serving.KappaServer(slots=slots) as srv:
+def sweep_fc_pack(seeds: Sequence[int], fcs: Sequence[models.ForceConstants]) -> dict:
+    """Force constants of sweep instances as flat arrays (npz): per instance its seed, a,
+    mesh, atom list and FC2/FC3 term lists (indices int8, values float64)."""
+    out = {"seeds": np.asarray(seeds, np.int64)}
+    for s, fc in zip(seeds, fcs):
+        a, n = sweep_draw(s)
+        out[f"s{s}_a_n"] = np.array([a, n], np.float64)
+        out[f"s{s}_atompos"] = fc.atompos
+        out[f"s{s}_iatomcell0"] = fc.iatomcell0.astype(np.int8)
+        out[f"s{s}_iatomcell"] = fc.iatomcell.astype(np.int8)
+        out[f"s{s}_fc2_idx"] = fc.fc2[:, :4].astype(np.int8)
+        out[f"s{s}_fc2_val"] = fc.fc2[:, 4]
+        out[f"s{s}_fc3_idx"] = fc.fc3[:, :6].astype(np.int8)
+        out[f"s{s}_fc3_val"] = fc.fc3[:, 6]
+    return out
+
+
+def sweep_fc_unpack(z, seed: int) -> models.ForceConstants:
+    a, _ = z[f"s{seed}_a_n"]
+    return models.ForceConstants(
+        models.diamond(float(a)), z[f"s{seed}_atompos"], z[f"s{seed}_iatomcell0"].astype(np.int64),
+        z[f"s{seed}_iatomcell"].astype(np.int64),
+        np.column_stack([z[f"s{seed}_fc2_idx"].astype(np.float64), z[f"s{seed}_fc2_val"]]),
+        np.column_stack([z[f"s{seed}_fc3_idx"].astype(np.float64), z[f"s{seed}_fc3_val"]]))
+
+
+def sweep_instances(count: int = 64, seed: int = 0, fc_file: str = "") -> List[tuple]:
+    """(seed, a, n, InputSet) of instances seed .. seed+count-1.  Force constants come from
+    fc_file (sweep_fc_pack) where it holds the instance, else from models.sw_silicon(a)
+    (about 7 s each on one core)."""
+    z = np.load(fc_file) if fc_file and os.path.exists(fc_file) else None
+    have = set(int(s) for s in z["seeds"]) if z is not None else set()
+    out = []
+    for s in range(seed, seed + count):
+        a, n = sweep_draw(s)
+        key = ("sw-si", a, 0)
+        if key not in _fc_done:
+            _fc_done[key] = sweep_fc_unpack(z, s) if s in have else models.sw_silicon(a, batched=True)
+        out.append((s, a, n, cell_inputs("sw-si", n, a)))
+    return out
+
+
+def sweep_requests(count: int = 64, seed: int = 0, fc_file: str = "") -> List[serving.Request]:
+    """SW-Si instances seed .. seed+count-1 (sweep_draw), all five temperatures of
+    SWEEP_TEMPS each, mixed V3."""
+    return [serving.Request(f"sw-si-s{s}-a{a}-{n}", inp, list(SWEEP_TEMPS), precision=capi.V3_MIXED)
+            for s, a, n, inp in sweep_instances(count, seed, fc_file)]
+
+
+def solve_packed(insts: Sequence[refio.InputSet], temps: Sequence[float], precision: int = capi.V3_MIXED,
+                 tile: int = 0, log=None) -> dict:
+    """Every instance on the device at once: eigen solves, then |V3|^2 of all instances in
+    one shared persistent launch (capi.v3_batch), then per instance the rest of the path
+    (alt3_solve reuses the |V3|^2 it holds).  Returns per-instance results, the tile used and
+    the phase times."""
+    ctxs = [capi.Context(inp) for inp in insts]
+    try:
         t0 = time.perf_counter()
-        for r in reqs:
-            srv.submit(r)
-        srv.wait(reqs)
-        span = time.perf_counter() - t0
-    for r in reqs:
-        if r.error:
-            raise RuntimeError(r.error)
-        with capi.Context(r.inputs, host_only=True) as h:
+        for c in ctxs:
+            c.eig(want=False)
+        t1 = time.perf_counter()
+        tile_used = capi.v3_batch(ctxs, precision, tile)
+        t2 = time.perf_counter()
+        res = [c.solve(list(temps), precision) for c in ctxs]
+        t3 = time.perf_counter()
+    finally:
+        for c in ctxs:
+            c.close()
+    if log:
+        log(f"packed: eig {1e3 * (t1 - t0):.1f} ms, V3 batch {1e3 * (t2 - t1):.1f} ms (tile {tile_used}), "
+            f"rest {1e3 * (t3 - t2):.1f} ms")
+    return {"results": res, "tile": tile_used, "eig_s": t1 - t0, "v3_s": t2 - t1, "rest_s": t3 - t2,
+            "span_s": t3 - t0}
+
+
+def run_sweep(count: int = 64, seed: int = 0, fc_file: str = "", log=print) -> dict:
+    """The sweep packed: all instances resident, one shared V3 launch (tile autotuned, the
+    tuning run done once before the timed pass), then each instance's FGR, tau, RTA and CG."""
+    t0 = time.perf_counter()
+    insts = sweep_instances(count, seed, fc_file)
+    t_gen = time.perf_counter() - t0
+    log(f"sweep: {count} instances ready in {t_gen:.1f} s")
+    inps = [x[3] for x in insts]
+    solve_packed(inps[:1], SWEEP_TEMPS[:1])     # kernel load + tile autotune, untimed
+    r = solve_packed(inps, SWEEP_TEMPS, log=log)
+    nel = 0
+    for inp in inps:
+        with capi.Context(inp, host_only=True) as h:
             nel += h.nibz * h.nk * h.nb ** 3
-    s = serving.latency_summary(reqs)
-    return {"instances": count, "span_s": span, "v3_elements": nel, "elements_per_s": nel / span,
-            "solves_per_hour": s["solves_per_hour"], "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
-            "kappa_300K": {"rta_min": min(kiso(r.result["kappa_rta"][2]) for r in reqs),
-                           "rta_max": max(kiso(r.result["kappa_rta"][2]) for r in reqs),
-                           "bte_min": min(kiso(r.result["kappa_bte"][2]) for r in reqs),
-                           "bte_max": max(kiso(r.result["kappa_bte"][2]) for r in reqs)},
+    res = r["results"]
+    # the same instances one at a time (eig, per-context V3 launch, rest), for comparison
+    lone_s = 0.0
+    for inp in inps:
+        with capi.Context(inp) as c:
+            t0 = time.perf_counter()
+            c.solve(list(SWEEP_TEMPS), capi.V3_MIXED)
+            lone_s += time.perf_counter() - t0
+    log(f"sweep: packed {r['span_s']:.2f} s, one at a time {lone_s:.2f} s")
+    return {"instances": count, "span_s": r["span_s"], "lone_span_s": lone_s, "v3_batch_s": r["v3_s"], "tile": r["tile"],
+            "v3_elements": nel, "elements_per_s": nel / r["span_s"], "v3_batch_elements_per_s": nel / r["v3_s"],
+            "solves_per_hour": count / r["span_s"] * 3600,
+            "kappa_300K": {"rta_min": min(kiso(o["kappa_rta"][2]) for o in res),
+                           "rta_max": max(kiso(o["kappa_rta"][2]) for o in res),
+                           "bte_min": min(kiso(o["kappa_bte"][2]) for o in res),
+                           "bte_max": max(kiso(o["kappa_bte"][2]) for o in res)},
             "generation_s": t_gen}
 
 
@@ -176,13 +257,14 @@ def run_selfenergy(n: int = 30, tempk: float = 300.0, eta: float = 30.0, log=pri
 
 # ---------------------------------------------------------------------------- serving stream
 def serving_requests(count: int, seed: int, precision: int) -> List[serving.Request]:
-    """Cells uniform over CELLS, mesh an even size from 6^3 up to the cell's cap (4^3 where
-    the cap is below 6), T ~ U[100, 800] K, one temperature per request."""
+    """Cells uniform over CELLS, mesh an even size from 6^3 up to the cell's cap for the
+    precision (MESH_CAP, MESH_CAP_FP64), T ~ U[100, 800] K, one temperature per request."""
     rng = np.random.default_rng(seed)
+    caps = MESH_CAP if precision == capi.V3_MIXED else MESH_CAP_FP64
     reqs = []
     for _ in range(count):
         cell = CELLS[int(rng.integers(len(CELLS)))]
-        sizes = [m for m in range(6, MESH_CAP[cell] + 1, 2)] or [4]
+        sizes = [m for m in range(6, caps[cell] + 1, 2)]
         n = int(rng.choice(sizes))
         T = round(float(rng.uniform(100, 800)), 1)
         reqs.append(serving.Request(cell, cell_inputs(cell, n), [T], precision=precision))
@@ -207,9 +289,11 @@ def run_serving(count: int = 512, seed: int = 0, slots: int = 4, precision: int 
     with serving.KappaServer(slots=slots) as srv:
         serving.replay(srv, reqs, arrivals)
         conc = srv.max_concurrent
+        b = srv.batcher
     s = serving.latency_summary(reqs)
     s.update({"arrival_rate_per_s": rate, "service_ms_alone": service * 1e3, "slots": slots,
-              "max_concurrent": conc, "setup_s": time.perf_counter() - t0 - s["span_s"]})
+              "max_concurrent": conc, "setup_s": time.perf_counter() - t0 - s["span_s"],
+              "v3_shared_launches": b.launches, "v3_contexts_batched": b.contexts, "v3_max_batch": b.max_batch})
     errs = [r.error for r in reqs if r.error]
     if errs:
         s["errors"] = errs[:3]

@@ -29,9 +29,11 @@ extern "C" {
 /* V3 arithmetic */
 #define ALT3_V3_FP64_REF 0   /* per-element sum over all FC3 terms (matrix_elt_simplified) */
 #define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
-#define ALT3_V3_MIXED 2      /* batched phase-GEMM on FP32 MFMA, FP32 contractions, FP64 scale;
-                                cells whose 16-point MFMA tile exceeds the LDS (nb > ~28) run
-                                the FP64 path instead */
+#define ALT3_V3_MIXED 2      /* FP32 phase sum and contractions, FP64 scale: 24 branches run
+                                all three products on FP32 MFMA (k_v3_chain), 6 and 12 branches
+                                on FP32 VALU (k_v3_rows), other counts the phase GEMM on FP32
+                                MFMA and the contractions on VALU (k_v3_mfma); cells whose
+                                16-point MFMA tile exceeds the LDS (nb > ~28) run the FP64 path */
 
 /* delta-function integration of the scattering rates */
 #define ALT3_DELTA_TETRA 0   /* linear tetrahedra, weight_tet (tetrahedron.f90:188-384) */
@@ -116,6 +118,25 @@ int alt3_set_eig(alt3_ctx *ctx, const double *eigenval, const double *eigenvec, 
  * q3 = -q1-q2: calculate_w3_ibz_split_sq (sub.f90:1050-1233).  Kept on the device for
  * alt3_fgr; v33s8 (optional) receives Fortran v33s8(ksub,nk,nb,nb,nb), FP64. */
 int alt3_v3sq_ibz(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int precision, double *v33s8);
+
+/* alt3_v3sq_ibz over the whole IBZ of n contexts (same branch count, same device) in one
+ * shared persistent launch: a grid sized to the device pops work items (context, q1, tile of
+ * q2) from one counter, so small meshes fill the chip together.  The work item is the
+ * reference's IBZ block of calculate_w3_ibz_split_sq (kappa.f90:420-431, sub.f90:1050-1233)
+ * cut into q2 tiles; each context then holds its |V3|^2 exactly as after its own
+ * alt3_v3sq_ibz(0, nibz-1) call (bit-identical: the tile is computed by the same code) and
+ * alt3_solve reuses it.  precision: ALT3_V3_FP64 or ALT3_V3_MIXED.  tile: q2 points per work
+ * item (6 branches: 16, 32 or 64; 12 branches: 8, 16 or 32), or 0 for the autotuned choice
+ * (the fastest candidate timed once on the first context, cached per precision, branch count
+ * and FC3 pair-vector count for the life of the process); tile_used (optional) receives it.
+ * Other branch counts run one launch per context (tile_used = 0).  No counterpart in the
+ * reference, which runs one IBZ block per MPI rank. */
+int alt3_v3_batch(alt3_ctx *const *ctxs, int n, int precision, int tile, int *tile_used);
+
+/* Copy out the |V3|^2 the context holds (from alt3_v3sq_ibz or alt3_v3_batch), in the layout
+ * of alt3_v3sq_ibz's v33s8, without recomputing it: the v33s8 array calculate_w3_ibz_split_sq
+ * leaves in memory (sub.f90:1050-1233). */
+int alt3_get_v3sq(alt3_ctx *ctx, double *v33s8);
 
 /* Tetrahedron-integrated Fermi golden rule: calculate_FGR_mpi2 (sub.f90:7199-7499,
  * weight_tet tetrahedron.f90:188-384).  Uses the |V3|^2 of the last alt3_v3sq_ibz call

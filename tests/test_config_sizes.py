@@ -1,0 +1,165 @@
+"""Parity at the sizes of the benchmark workloads, against the reference THERMACOND build:
+
+* the 8-atom rock-salt-type cell (24 branches, the headline cell) at 4^3 (ref_rs8_4.npz:
+  all tables, eigen data, |V3|^2 and P on three irreducible points and every second mesh
+  point, Q/tau/F_RTA/kappa_RTA, two collision-matrix row blocks, RHS, CG and kappa_BTE at
+  300 K).  The headline mesh itself (12^3, 2e9 elements) needs more than the container's memory
+  in the reference's five nk*nibz*nb^3 arrays; its mixed path is held against the FP64 path on
+  the device (bench.py) and its cell against the reference here.
+* the SW-Si 16^3 workload (ref_si16.npz: tables, eigen data, |V3|^2 and P on three irreducible
+  points and every eighth mesh point, every per-mode and per-temperature result at 100, 300 and
+  800 K and the CG iteration counts).
+
+CPU: mesh and symmetry tables bit-exact (host-only contexts); the oracle's |V3|^2 on the
+reference eigen data for one irreducible point of the rock-salt cell (1e-10).
+GPU (reference eigen data loaded): |V3|^2 (batched FP64, 1e-8), P1sm/P2sm 1e-8; Q, QN, QU,
+tau, F_RTA, kappa_RTA 1e-7; collision rows and RHS 1e-7; the CG iteration count equal; CG x
+and kappa_BTE 1e-6 (rock-salt) / 1e-5 (SW-Si kappa_BTE through alt3_solve); the mixed path's
+kappa within 1e-5 of the FP64 path's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, inputs_for, load_fixture
+from thermacond import capi
+
+
+def rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def check(a, b, tol, what):
+    r = rel(a, b)
+    assert r < tol, f"{what}: max rel diff {r:.3e} >= {tol:.1e}"
+
+
+def _have(name):
+    return os.path.exists(os.path.join(GOLDEN, f"ref_{name}.npz"))
+
+
+@pytest.fixture(scope="module")
+def rs8ref():
+    return load_fixture("rs8_4")
+
+
+@pytest.fixture(scope="module")
+def rs8inp(tmp_path_factory):
+    from thermacond import models
+    return inputs_for(models.born_mayer_rocksalt(seed=0), 4, tmp_path_factory, "rs8cfg")
+
+
+@pytest.fixture(scope="module")
+def si16ref():
+    if not _have("si16"):
+        pytest.skip("ref_si16.npz not written (make_golden.py si16)")
+    return load_fixture("si16")
+
+
+def _tables(inp, R):
+    with capi.Context(inp, host_only=True) as h:
+        m = h.mesh()
+        assert h.nibz == int(R["nibz"]) and h.nk == int(R["nk"])
+        assert np.array_equal(m["mapibz"] + 1, R["mapibz"]) and np.array_equal(m["mapinv"] + 1, R["mapinv"])
+        assert np.array_equal(m["wibz"], R["wibz"])
+        assert np.array_equal(m["tet"] + 1, R["tet"].T)
+        assert np.abs(m["kpc_eig"] - R["kpc_mesh"].T).max() < 1e-14
+        for i in range(h.nibz):
+            ops, n0 = h.kstars(i)
+            k = int(R["star_n"][i])
+            assert len(ops) == k and np.array_equal(ops + 1, R["star_op"][:k, i]) and np.array_equal(n0 + 1, R["star_n0"][:k, i])
+
+
+def test_rs8_tables_bit_exact(rs8inp, rs8ref):
+    _tables(rs8inp, rs8ref)
+
+
+def test_si16_tables_bit_exact(si_inputs, si16ref):
+    _tables(si_inputs[16], si16ref)
+
+
+def test_rs8_oracle_v3_vs_reference(rs8inp, rs8ref):
+    """The oracle's |V3|^2 restatement on 24 branches: one irreducible point, the reference's
+    own eigenvectors, every mesh point q2 of the fixture."""
+    R, inp = rs8ref, rs8inp
+    nc = np.array(inp.params.nc)
+    g, _ = O.reciprocal(inp.r123)
+    kp, _ = O.make_kp_reg_tet(nc, g)
+    mapinv = (R["mapinv"] - 1).astype(np.int32)
+    s = int(R["ibz_sel"][0])
+    v = O.v3sq(s, s, mapinv, kp, nc, g, R["eigenval"], R["eigenvec"], inp.masses, inp.atompos, inp.iatomcell0,
+               inp.fc3_atoms, inp.fc3_xyz, inp.fc3)
+    check(v[0][R["q2_sel"]], R["v33s8"][0], 1e-10, "oracle v33s8 (rock-salt)")
+
+
+@pytest.mark.gpu
+def test_rs8_path_vs_reference(rs8inp, rs8ref):
+    print("case: rock-salt 4^3 path vs reference (reference eigen data)", flush=True)
+    R = rs8ref
+    T = 300.0
+    t = "T300_"
+    with capi.Context(rs8inp) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        v = c.v3sq(0, c.nibz - 1, capi.V3_FP64)
+        check(v[np.ix_(R["ibz_sel"], R["q2_sel"])], R["v33s8"], 1e-8, "v33s8")
+        P1, P2 = c.fgr()
+        check(P1[np.ix_(R["ibz_sel"], R["q2_sel"])], R["P1sm"], 1e-8, "P1sm")
+        check(P2[np.ix_(R["ibz_sel"], R["q2_sel"])], R["P2sm"], 1e-8, "P2sm")
+        out = c.tau(T)
+        for k in ("Q", "QN", "QU", "tautot"):
+            check(out[k], R[t + k], 1e-7, k)
+        kap, F, te = c.kappa_rta(T)
+        check(F, R[t + "FRTA"], 1e-7, "F_RTA")
+        check(kap, R[t + "kappaRTA"], 1e-7, "kappa_RTA")
+        CM, RHS = c.collision(T)
+        check(CM[R[t + "CM_rows"]], R[t + "CM"], 1e-7, "collision rows")
+        check(RHS, R[t + "RHS"], 1e-7, "RHS")
+        x0 = np.array([F[i, l, a] for i in range(c.nibz) for l in range(c.nb) for a in range(3)])
+        x, it, _ = c.cg(x0)
+        assert it == int(R["cg_iters"][0])
+        check(x, R[t + "x"], 1e-6, "CG x")
+        kb, _ = c.kappa_bte(T, x, te)
+        check(kb, R[t + "kappaBTE"], 1e-6, "kappa_BTE")
+        om = c.solve([T], capi.V3_MIXED)
+    for k, ref in (("kappa_rta", R[t + "kappaRTA"]), ("kappa_bte", R[t + "kappaBTE"])):
+        a, b = np.trace(om[k][0].sum(0)) / 3, np.trace(ref.sum(0)) / 3
+        assert abs(a / b - 1) < 1e-5, (k, a, b)
+
+
+@pytest.mark.gpu
+def test_si16_path_vs_reference(si_inputs, si16ref):
+    print("case: SW-Si 16^3 path vs reference (reference eigen data), 100/300/800 K", flush=True)
+    R = si16ref
+    temps = [float(T) for T in R["temps"]]
+    with capi.Context(si_inputs[16]) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        v = c.v3sq(0, c.nibz - 1, capi.V3_FP64)
+        check(v[np.ix_(R["ibz_sel"], R["q2_sel"])], R["v33s8"], 1e-8, "v33s8")
+        del v
+        P1, P2 = c.fgr()
+        check(P1[np.ix_(R["ibz_sel"], R["q2_sel"])], R["P1sm"], 1e-8, "P1sm")
+        check(P2[np.ix_(R["ibz_sel"], R["q2_sel"])], R["P2sm"], 1e-8, "P2sm")
+        del P1, P2
+        for T in R["temps"]:
+            t = f"T{T}_"
+            out = c.tau(float(T))
+            for k in ("Q", "QN", "QU", "tautot"):
+                check(out[k], R[t + k], 1e-7, f"{t}{k}")
+            kap, F, te = c.kappa_rta(float(T))
+            check(kap, R[t + "kappaRTA"], 1e-7, f"{t}kappa_RTA")
+            CM, RHS = c.collision(float(T))
+            check(CM[R[t + "CM_rows"]], R[t + "CM"], 1e-7, f"{t}collision rows")
+            check(RHS, R[t + "RHS"], 1e-7, f"{t}RHS")
+        o64 = c.solve(temps, capi.V3_FP64)
+        om = c.solve(temps, capi.V3_MIXED)
+    for i, T in enumerate(R["temps"]):
+        t = f"T{T}_"
+        check(o64["kappa_rta"][i], R[t + "kappaRTA"], 1e-7, f"{t}solve kappa_RTA")
+        check(o64["kappa_bte"][i], R[t + "kappaBTE"], 1e-5, f"{t}solve kappa_BTE")
+        assert int(o64["cg_iters"][i]) == int(R["cg_iters"][i]), (T, o64["cg_iters"][i], R["cg_iters"][i])
+        for k in ("kappa_rta", "kappa_bte"):
+            a, b = np.trace(om[k][i].sum(0)) / 3, np.trace(o64[k][i].sum(0)) / 3
+            assert abs(a / b - 1) < 1e-5, (T, k, a, b)

@@ -1,17 +1,23 @@
 """The kappa-solve server (thermacond/serving.py): footprint admission and continuous batching.
 
 CPU: footprints come from host-only contexts and grow with the mesh, the branches and the
-number of temperatures; a request larger than the budget is refused without touching the device.
+number of temperatures; a request larger than the budget is refused without touching the
+device; the request stream draws meshes up to the lifted caps (16^3 for every cell whose
+footprint fits, 10^3 for the 16-atom cell).
 GPU: a mixed stream of uneven requests (meshes 4^3-8^3, one to three temperatures, FP64 and
 mixed V3, tetrahedron and Gaussian delta) run concurrently through the server gives, for
 every request, the kappa the same request gives alone on a fresh context (1e-9 relative;
 the FGR's LDS atomics reorder sums at the 1e-16 level), the device held more than one
-request at a time, and requests with a cached context reused it.
+request at a time, and requests with a cached context reused it; requests queued together
+share one persistent |V3|^2 launch and keep their lone results; the four cells (SW-Si, the
+4-atom zincblende-type, the 8- and 16-atom rock-salt-type cells) at meshes up to 16^3 in
+one stream, each result equal to its lone solve (1e-9).
 """
 import numpy as np
 import pytest
 
-from thermacond import capi, serving
+from conftest import inputs_for
+from thermacond import capi, serving, workloads
 
 
 def test_footprint_grows_with_work(si_inputs):
@@ -66,6 +72,7 @@ def _bare_server(budget):
     srv._cache = serving.collections.OrderedDict()
     srv._ids = serving.itertools.count()
     srv._in_use, srv._running, srv._bypassed = 0, 0, {}
+    srv.batcher = None
     return srv
 
 
@@ -145,3 +152,92 @@ def test_footprint_bounds_device_allocations(si_inputs):
         est = c.footprint(3, capi.V3_FP64)
         c.solve([100.0, 300.0, 800.0], capi.V3_FP64)
         assert c.device_bytes() <= est
+
+
+def test_stream_draws_up_to_the_lifted_caps(monkeypatch):
+    monkeypatch.setattr(workloads, "cell_inputs", lambda cell, n: (cell, n))
+    monkeypatch.setattr(serving, "Request", lambda key, inp, temps, precision: inp)
+    got = workloads.serving_requests(400, 0, capi.V3_MIXED)
+    top = {c: max(n for cc, n in got if cc == c) for c in workloads.CELLS}
+    assert top == {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 10}
+    got = workloads.serving_requests(400, 0, capi.V3_FP64)
+    assert max(n for c, n in got if c == "rs16") == 8 and max(n for c, n in got if c == "rs8") == 16
+
+
+@pytest.mark.gpu
+def test_shared_v3_launch_keeps_lone_results(si_inputs, tmp_path_factory):
+    print("case: serving, six requests in one shared V3 launch", flush=True)
+    fc = si_inputs["fc"]
+    inps = [inputs_for(fc, n, tmp_path_factory, f"sv{n}") for n in (4, 6, 8, 10)]
+
+    def reqs():
+        return [serving.Request("si", inp, [300.0], precision=p) for inp in inps[:3] for p in (capi.V3_MIXED,)] + \
+               [serving.Request("si", inps[3], [200.0, 500.0], precision=capi.V3_MIXED),
+                serving.Request("si", inps[1], [300.0], precision=capi.V3_FP64),
+                serving.Request("si", inps[2], [300.0], precision=capi.V3_FP64)]
+    alone = []
+    with serving.KappaServer(slots=1, cache=False, batch_v3=False) as srv:
+        for r in reqs():
+            srv.submit(r)
+            srv.wait([r])
+            assert r.error is None, r.error
+            alone.append(r)
+    mix = reqs()
+    with serving.KappaServer(slots=6, cache=False, batch_window_s=5.0) as srv:
+        for r in mix:
+            srv.submit(r)
+        srv.wait(mix)
+        b = srv.batcher
+    assert b.contexts == len(mix) and b.launches < len(mix) and b.max_batch >= 2, (b.launches, b.max_batch)
+    for a, m in zip(alone, mix):
+        assert m.error is None, m.error
+        for k in ("kappa_rta", "kappa_bte"):
+            d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
+            assert d < 1e-12, (m.mesh_key, k, d)
+        assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
+
+
+def _four_cells(si_inputs, cells, tmp_path_factory):
+    S = si_inputs
+    I = lambda name, n: inputs_for(cells[name], n, tmp_path_factory, f"mix_{name}")
+    M, F = capi.V3_MIXED, capi.V3_FP64
+    zb16, zb8, rs16m, rs6, sc8, sc4 = I("zb4", 16), I("zb4", 8), I("rs8", 16), I("rs8", 6), I("rs16", 8), I("rs16", 4)
+    return lambda: [serving.Request("si", S[16], [300.0], precision=M),
+                    serving.Request("zb4", zb16, [250.0], precision=M),
+                    serving.Request("rs8", rs16m, [300.0], precision=M),
+                    serving.Request("rs16", sc8, [400.0], precision=M),
+                    serving.Request("si", S[16], [100.0, 800.0], precision=M),
+                    serving.Request("zb4", zb8, [700.0], precision=F),
+                    serving.Request("rs8", rs6, [150.0], precision=F),
+                    serving.Request("rs16", sc4, [600.0], precision=F),
+                    serving.Request("si", S[8], [500.0], precision=F)]
+
+
+@pytest.mark.gpu
+def test_four_cell_mix_at_full_meshes(si_inputs, cells, tmp_path_factory):
+    """SW-Si 16^3, the 4-atom cell at 16^3 and 8^3, the 8-atom cell at 16^3 (119 GB mixed)
+    and 6^3, the 16-atom cell at 8^3 (60 GB) and 4^3, through one server: every result
+    equals the request alone on a fresh context."""
+    make = _four_cells(si_inputs, cells, tmp_path_factory)
+    print("case: four-cell serving mix up to 16^3, alone", flush=True)
+    alone = []
+    with serving.KappaServer(slots=1, cache=False) as srv:
+        for r in make():
+            print(f"  alone {r.mesh_key} {r.temps} precision {r.precision}", flush=True)
+            srv.submit(r)
+            srv.wait([r])
+            assert r.error is None, r.error
+            alone.append(r)
+    print("case: four-cell serving mix up to 16^3, in the mix", flush=True)
+    mix = make()
+    with serving.KappaServer(slots=4, cache=True) as srv:
+        serving.replay(srv, mix, serving.poisson_arrivals(len(mix), rate_per_s=20.0, seed=11))
+        conc = srv.max_concurrent
+    for a, m in zip(alone, mix):
+        assert m.error is None, m.error
+        for k in ("kappa_rta", "kappa_bte"):
+            d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
+            assert d < 1e-9, (m.mesh_key, m.temps, k, d)
+        assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
+    assert conc > 1
+    print("four-cell mix:", serving.latency_summary(mix), flush=True)
