@@ -18,29 +18,37 @@
 namespace {
 
 // SSORT of the corner energies (descending, first maximum wins) and the reference's map
-// uns from sorted position to corner (tetrahedron.f90:248-300)
+// uns from sorted position to corner (tetrahedron.f90:248-300).  The sorted values do not
+// depend on how they are sorted (the energies are never NaN or -0), so a compare-exchange
+// network on registers replaces the selection sort; uns is then built from the sorted values
+// exactly as the reference builds it.  Every index below is a compile-time constant once the
+// loops are unrolled, so nothing goes to per-lane scratch memory.
+__device__ __forceinline__ void cx_desc(float &a, float &b)
+{
+    const float hi = fmaxf(a, b), lo = fminf(a, b);
+    a = hi;
+    b = lo;
+}
+
 __device__ __forceinline__ void tet_sort(const float *w, float *en, int *uns)
 {
     en[0] = w[0];
     en[1] = w[1];
     en[2] = w[2];
     en[3] = w[3];
-    // SSORT: descending, first maximum wins
-    for (int i = 0; i < 3; i++) {
-        int m = i;
-        for (int k = i + 1; k < 4; k++)
-            if (en[k] > en[m]) m = k;
-        if (m != i) {
-            float t = en[i];
-            en[i] = en[m];
-            en[m] = t;
-        }
-    }
+    cx_desc(en[0], en[1]);
+    cx_desc(en[2], en[3]);
+    cx_desc(en[0], en[2]);
+    cx_desc(en[1], en[3]);
+    cx_desc(en[1], en[2]);
     if (en[0] != en[1] && en[1] != en[2] && en[2] != en[3]) {
+#pragma unroll
         for (int k = 0; k < 4; k++) uns[k] = (en[k] == w[0]) ? 1 : (en[k] == w[1]) ? 2 : (en[k] == w[2]) ? 3 : 4;
     } else {
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             uns[k] = (en[k] == w[0]) ? 1 : (en[k] == w[1]) ? 2 : (en[k] == w[2]) ? 3 : 4;
+#pragma unroll
             for (int l = 0; l < k; l++) {
                 if (uns[l] == uns[k] && uns[k] == 1) {
                     if (en[k] == w[1]) uns[k] = 2;
@@ -54,6 +62,20 @@ __device__ __forceinline__ void tet_sort(const float *w, float *en, int *uns)
                 }
             }
         }
+    }
+}
+
+// c[uns[k] - 1] = cc[k] for k = 3, 2, 1, 0 in that order (a later k overwrites), as selects
+template <typename T>
+__device__ __forceinline__ void unsort(const int *uns, const T *cc, T *c)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        T v = 0;
+#pragma unroll
+        for (int k = 3; k >= 0; k--)
+            if (uns[k] - 1 == i) v = cc[k];
+        c[i] = v;
     }
 }
 
@@ -120,12 +142,7 @@ __device__ void weight_tet1(const float *w, double e, double *c)
         cc[1] = (d4 * d4 * d4) / (double)(e41 * e42 * (e43 * e43));
         cc[0] = ((e - E3) / (double)e43 + (e - E2) / (double)e42 + (e - E1) / (double)e41) * (d4 * d4) / (double)(e41 * e42 * e43);
     }
-    double out[4] = {0, 0, 0, 0};
-    for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
-    c[0] = out[0];
-    c[1] = out[1];
-    c[2] = out[2];
-    c[3] = out[3];
+    unsort(uns, cc, c);
 }
 
 // The same weights in FP32 arithmetic, for the mixed path (whose |V3|^2 and P are FP32): the
@@ -186,16 +203,25 @@ __device__ __forceinline__ void weight_tet1f(const float *w, double e, float *c)
         cc[1] = h * (u4 * q43);
         cc[0] = (d3 * q43 + d2 * q42 + d1 * q41) * h;
     }
-    float out[4] = {0, 0, 0, 0};
-    for (int k = 3; k >= 0; k--) out[uns[k] - 1] = cc[k];
-    c[0] = out[0];
-    c[1] = out[1];
-    c[2] = out[2];
-    c[3] = out[3];
+    unsort(uns, cc, c);
 }
 
-// corners (0..7 = bits g1,g2,g3 of the cube) of the 6 tetrahedra of make_kp_reg_tet
-__constant__ int kTc[6][4] = {{0, 1, 2, 5}, {1, 2, 3, 5}, {0, 2, 4, 5}, {2, 3, 5, 7}, {2, 4, 5, 6}, {2, 5, 6, 7}};
+// corners (0..7 = bits g1,g2,g3 of the cube) of the 6 tetrahedra of make_kp_reg_tet:
+// {0,1,2,5} {1,2,3,5} {0,2,4,5} {2,3,5,7} {2,4,5,6} {2,5,6,7}; column y of the table packed
+// as 3 bits per tetrahedron, so that a runtime tetrahedron index needs no memory load
+constexpr unsigned tc_col(int y)
+{
+    constexpr int t[6][4] = {{0, 1, 2, 5}, {1, 2, 3, 5}, {0, 2, 4, 5}, {2, 3, 5, 7}, {2, 4, 5, 6}, {2, 5, 6, 7}};
+    unsigned v = 0;
+    for (int l = 0; l < 6; l++) v |= (unsigned)t[l][y] << (3 * l);
+    return v;
+}
+__device__ __forceinline__ int tet_corner(int l, int y)
+{
+    constexpr unsigned c0 = tc_col(0), c1 = tc_col(1), c2 = tc_col(2), c3 = tc_col(3);
+    const unsigned col = y == 0 ? c0 : y == 1 ? c1 : y == 2 ? c2 : c3;
+    return (int)((col >> (3 * l)) & 7u);
+}
 
 // a[j] for a register array and a runtime index, as a chain of selects
 template <int N>
@@ -229,7 +255,7 @@ constexpr int kFgrList = 3 * kFgrCase;
 __device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return !(e < (double)lo || e > (double)hi); }
 
 template <typename VT, int kFgrPer>   // kFgrPer: mesh points (and cubes) per thread
-__global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
+__global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4) ? 6 : 4) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
                                                      const double *__restrict__ freqT,
                                                      const VT *__restrict__ v33, VT *__restrict__ P1,
@@ -333,26 +359,46 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
     VT *P = proc == 0 ? P1 : P2;
     // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list of
     // their weight_tet case (e below the second-lowest corner, below the third, or above), so
-    // that the lanes of a wave take the same branch of the weight formula
+    // that the lanes of a wave take the same branch of the weight formula.  Called by every
+    // lane of the group with the same j; the list slots are claimed once per wave and case
+    // (one LDS atomic per wave instead of one per tetrahedron).
     auto push = [&](int j, double e) {
-        if (omega_in(e, pick(cmn, j), pick(cmx, j))) {
-            const int k = tid + j * kFgrThreads;
+        const int k = tid + j * kFgrThreads;
+        const bool cube_in = k < nk && omega_in(e, pick(cmn, j), pick(cmx, j));
+        float ce[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (cube_in) {
             int c0, c1, c2;
             m.coords(k, c0, c1, c2);
             const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
                       d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
-            const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
-                                 en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
-                                 en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
+            ce[0] = en[m.index(c0, c1, c2)];
+            ce[1] = en[m.index(d0, c1, c2)];
+            ce[2] = en[m.index(c0, d1, c2)];
+            ce[3] = en[m.index(d0, d1, c2)];
+            ce[4] = en[m.index(c0, c1, d2)];
+            ce[5] = en[m.index(d0, c1, d2)];
+            ce[6] = en[m.index(c0, d1, d2)];
+            ce[7] = en[m.index(d0, d1, d2)];
+        }
 #pragma unroll
-            for (int l = 0; l < 6; l++) {
-                const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
-                const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
-                if (omega_in(e, fminf(s1, s3), fmaxf(s2, s4))) {
-                    const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
-                    const int cs = (e <= (double)fminf(m1, m2)) ? 0 : (e <= (double)fmaxf(m1, m2)) ? 1 : 2;
-                    const int x = atomicAdd(&s_cnt[cs], 1);
-                    if (x < kFgrCase) s_list[cs * kFgrCase + x] = (unsigned short)((k << 3) | l);
+        for (int l = 0; l < 6; l++) {
+            const float a = ce[tet_corner(l, 0)], b = ce[tet_corner(l, 1)], c = ce[tet_corner(l, 2)],
+                        d = ce[tet_corner(l, 3)];
+            const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
+            const bool act = cube_in && omega_in(e, fminf(s1, s3), fmaxf(s2, s4));
+            const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
+            const int cs = (e <= (double)fminf(m1, m2)) ? 0 : (e <= (double)fmaxf(m1, m2)) ? 1 : 2;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const bool mine = act && cs == q;
+                const unsigned long long mask = __ballot(mine);
+                if (mask) {   // the same for every lane of the wave
+                    const int leader = __ffsll((long long)mask) - 1;
+                    int base = 0;
+                    if ((int)__lane_id() == leader) base = atomicAdd(&s_cnt[q], __popcll(mask));
+                    base = __shfl(base, leader, 64);
+                    const int x = base + __popcll(mask & ((1ull << __lane_id()) - 1ull));
+                    if (mine && x < kFgrCase) s_list[q * kFgrCase + x] = (unsigned short)((k << 3) | l);
                 }
             }
         }
@@ -367,7 +413,7 @@ __global__ void __launch_bounds__(kFgrThreads) k_fgr(alt3::MeshDev m, int nk, in
             int v[4];
 #pragma unroll
             for (int y = 0; y < 4; y++) {
-                const int p = kTc[l][y];
+                const int p = tet_corner(l, y);
                 int e0 = c0 + (p & 1), e1 = c1 + ((p >> 1) & 1), e2 = c2 + ((p >> 2) & 1);
                 if (e0 == m.n0) e0 = 0;
                 if (e1 == m.n1) e1 = 0;
@@ -561,8 +607,9 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 9 * kFgrThreads)
-        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 4608");
+    // the list codes (cube << 3 | tetrahedron) are 16 bits wide: nk <= 8192 (a 20^3 mesh)
+    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 16 * kFgrThreads)
+        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 8192");
     ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     ALT3_TRY(ctx->d_freqT.reserve((size_t)nk * nb));
@@ -594,8 +641,12 @@ int fgr_run(alt3_ctx *ctx, bool sync)
         ALT3_FGR_LAUNCH(6)
     } else if (per <= 8) {
         ALT3_FGR_LAUNCH(8)
-    } else {
+    } else if (per <= 9) {
         ALT3_FGR_LAUNCH(9)
+    } else if (per <= 12) {
+        ALT3_FGR_LAUNCH(12)
+    } else {
+        ALT3_FGR_LAUNCH(16)
     }
 #undef ALT3_FGR_LAUNCH
     ALT3_HIP(hipGetLastError());
@@ -671,7 +722,8 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
             }
             if (!omega_in(omega1, lo, hi)) continue;
             for (int l = 0; l < 6; l++) {
-                const float a = ce[kTc[l][0]], b = ce[kTc[l][1]], c = ce[kTc[l][2]], d = ce[kTc[l][3]];
+                const float a = ce[tet_corner(l, 0)], b = ce[tet_corner(l, 1)], c = ce[tet_corner(l, 2)],
+                            d = ce[tet_corner(l, 3)];
                 const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
                 if (omega_in(omega1, fminf(s1, s3), fmaxf(s2, s4))) {
                     const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
@@ -692,7 +744,7 @@ __global__ void __launch_bounds__(kIsoThreads) k_piso(MeshDev m, int nk, int nb,
             m.coords(k, a0, a1, a2);
             int v[4];
             for (int y = 0; y < 4; y++) {
-                const int p = kTc[l][y];
+                const int p = tet_corner(l, y);
                 int b0 = a0 + (p & 1), b1 = a1 + ((p >> 1) & 1), b2 = a2 + ((p >> 2) & 1);
                 if (b0 == m.n0) b0 = 0;
                 if (b1 == m.n1) b1 = 0;
