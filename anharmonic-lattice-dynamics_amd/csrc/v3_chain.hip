@@ -102,7 +102,22 @@ __global__ void k_chain_pack_g(int ksub, int ndr, int ks, const double *__restri
     Gpk[x] = v;
 }
 
-// one workgroup of 8 waves per (q1 = s, tile of 16 q2), looping over l1
+constexpr int CH = 6;                // chunks of QT points per workgroup
+constexpr int TPW = (NTP + NW - 1) / NW;   // phase N-tiles per wave (5: waves 0-3, 4: waves 4-7)
+constexpr int NDRMAX = 2 * KSMAX;    // cell differences held in the LDS phase rows
+// phase rows of the tile in LDS as [part][r][q], q fastest with row stride ELD = 16 mod 32: the
+// 16 points (lanes) of an A-operand read fall on consecutive banks and the next r (the other
+// half-wave's lane group) 16 banks further
+constexpr int ELD = (CH * QT + 15) / 32 * 32 + 16;
+static_assert(ELD >= CH * QT && ELD % 32 == 16, "phase row stride");
+
+// One workgroup of 8 waves per (q1 = s, tile of CH x 16 mesh points q2), looping over l1.  The
+// phase B operands of (s, l1) -- G', the largest operand -- are read from memory once per l1
+// into the registers of the waves that use them and serve all CH chunks of 16 points (with one
+// chunk per workgroup, as before, the phase GEMM waited on these loads for every 16 points).
+// Per chunk: phase GEMM into LDS, barrier, U and V per point with |V3|^2 into the staging
+// area, barrier, rows out (overlapping the next chunk's phase GEMM, which writes only Phi).
+template <int KS>
 __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, int ibz_lo, int ndr, int ks,
                                                      const int *__restrict__ mapinv, const float *__restrict__ ephf,
                                                      const float *__restrict__ Gpk, const float *__restrict__ PU,
@@ -110,149 +125,176 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
                                                      double c33sq8, float *__restrict__ out, int *__restrict__ err)
 {
     extern __shared__ __align__(16) float lds[];
-    float *Phi = lds;              // [QT][2][NB][BS]: Re Phi'[b][c], then Im
-    float *Ost = Phi + QT * PS;    // [QT][NB][25] scaled |V3|^2 (l2, l3) of the current l1
+    float *Phi = lds;                  // [QT][2][NB][BS]: Re Phi'[b][c], then Im
+    float *Ost = Phi + QT * PS;        // [QT][NB][25] scaled |V3|^2 (l2, l3) of the current l1
+    float *Es = Ost + QT * OS;         // [2][NDRMAX][ELD] phase rows e^{i q2.dR_r} of the tile
     __shared__ double w2s[QT][NB], w3s[QT][NB];
-    __shared__ int k3s[QT];
+    __shared__ int k3s[CH * QT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int s = blockIdx.y, q0 = blockIdx.x * QT;
+    const int s = blockIdx.y, q00 = blockIdx.x * (CH * QT);
+    const int nch = min(CH, (nk - q00 + QT - 1) / QT);   // chunks with at least one mesh point
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
     m.coords(k1, a0, a1, a2);
-    if (tid < QT) {
+    if (tid < CH * QT) {
         int b0, b1, b2;
-        m.coords(min(q0 + tid, nk - 1), b0, b1, b2);
+        m.coords(min(q00 + tid, nk - 1), b0, b1, b2);
         k3s[tid] = m.neg_sum(a0, a1, a2, b0, b1, b2);
     }
-    // phase A operands for the whole kernel: point q = lane%16, k = 4t + lane/16;
-    // ar: rows of Re Phi' ([Re E, -Im E]), ai: rows of Im Phi' ([Im E, Re E])
-    float ar[KSMAX], ai[KSMAX];
-    {
-        const int n2c = min(q0 + (lane & 15), nk - 1);
-#pragma unroll
-        for (int t = 0; t < KSMAX; t++) {
-            const int kk = 4 * t + (lane >> 4);
-            ar[t] = ai[t] = 0.0f;
-            if (t < ks && kk < 2 * ndr) {
-                const int r = kk < ndr ? kk : kk - ndr;
-                const float er = ephf[2 * ((size_t)n2c * ndr + r)], ei = ephf[2 * ((size_t)n2c * ndr + r) + 1];
-                ar[t] = kk < ndr ? er : -ei;
-                ai[t] = kk < ndr ? ei : er;
-            }
-        }
+    for (int x = tid; x < CH * QT * ndr; x += NW * 64) {
+        const int q = x / ndr, r = x - q * ndr;
+        const float *e = ephf + 2 * ((size_t)min(q00 + q, nk - 1) * ndr + r);
+        Es[r * ELD + q] = e[0];
+        Es[(NDRMAX + r) * ELD + q] = e[1];
     }
-    __syncthreads();
-    for (int x = tid; x < QT * NB; x += NW * 64) {
-        const int q = x / NB, l = x % NB;
-        w2s[q][l] = wq[(size_t)min(q0 + q, nk - 1) * NB + l];
-        w3s[q][l] = wq[(size_t)k3s[q] * NB + l];
-    }
-    __syncthreads();
     const int col = lane & 15, grp = lane >> 4;
+    // phase B operands of this wave's N-tiles nt = wave + 8 j
+    float gb[TPW][KS];
+    auto load_g = [&](int l1) {
+        const float *G = Gpk + (size_t)(s * NB + l1) * ks * NTP * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < TPW; j++)
+#pragma unroll
+            for (int t = 0; t < KS; t++) {
+                const int nt = wave + NW * j;
+                gb[j][t] = (nt < NTP && t < ks) ? G[((size_t)t * NTP + nt) * 64] : 0.0f;
+            }
+    };
+    __syncthreads();
     for (int l1 = 0; l1 < NB; l1++) {
-        // ---- phase GEMM: N-tiles nt = wave, wave + 8, ... of the 36
-        const float *G = Gpk + (size_t)(s * NB + l1) * ks * NTP * 64;
-        for (int nt = wave; nt < NTP; nt += NW) {
-            f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int t = 0; t < KSMAX; t++) {
-                if (t < ks) {
-                    const float b = G[((size_t)t * NTP + nt) * 64 + lane];
-                    cr = mma(ar[t], b, cr);
-                    ci = mma(ai[t], b, ci);
-                }
-            }
-            const int n = 16 * nt + col, bc = (n / NB) * BS + n % NB;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                Phi[(4 * grp + i) * PS + bc] = cr[i];
-                Phi[(4 * grp + i) * PS + PP + bc] = ci[i];
-            }
-        }
-        __syncthreads();
-        // ---- U and V, one point at a time per wave
         const double w1 = wq[(size_t)k1 * NB + l1];
-        for (int jq = 0; jq < QW; jq++) {
-            const int q = wave * QW + jq;
-            const int n2c = min(q0 + q, nk - 1);
-            const float *ph = Phi + q * PS;
-            const float *pu = PU + (size_t)k3s[q] * FU * 64 + lane;
-            const float *pv = PV + (size_t)n2c * FV * 64 + lane;
-            f4 u[MU][NU];
+        load_g(l1);
+        for (int ch = 0; ch < nch; ch++) {
+            const int q0 = q00 + ch * QT;
+            // ---- phase GEMM of the chunk: rows of point q = lane%16, k = 4t + lane/16;
+            // ar: rows of Re Phi' ([Re E, -Im E]), ai: rows of Im Phi' ([Im E, Re E])
+            float ar[KS], ai[KS];
+            {
+                const float *e = Es + ch * QT + col;
 #pragma unroll
-            for (int a = 0; a < MU; a++)
-#pragma unroll
-                for (int b = 0; b < NU; b++) u[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int t = 0; t < KU; t++) {
-                float bu[NU];
-#pragma unroll
-                for (int nt = 0; nt < NU; nt++) bu[nt] = pu[(t * NU + nt) * 64];
-                const int kk = 4 * t + grp, pp = kk >= NB ? 1 : 0, c = kk - pp * NB;
-#pragma unroll
-                for (int mt = 0; mt < MU; mt++) {
-                    const int row = 16 * mt + col, p = row >= NB ? 1 : 0, b = row - p * NB;
-                    float a = ph[(p ^ pp) * PP + b * BS + c];
-                    if (p == 0 && pp == 1) a = -a;
-#pragma unroll
-                    for (int nt = 0; nt < NU; nt++) u[mt][nt] = mma(a, bu[nt], u[mt][nt]);
-                }
-            }
-            f4 v[NU][NV];
-#pragma unroll
-            for (int a = 0; a < NU; a++)
-#pragma unroll
-                for (int b = 0; b < NV; b++) v[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int mt = 0; mt < MU; mt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    float bv[NV];
-#pragma unroll
-                    for (int nv = 0; nv < NV; nv++) bv[nv] = pv[((mt * 4 + i) * NV + nv) * 64];
-#pragma unroll
-                    for (int mv = 0; mv < NU; mv++)
-#pragma unroll
-                        for (int nv = 0; nv < NV; nv++) v[mv][nv] = mma(u[mt][mv][i], bv[nv], v[mv][nv]);
-                }
-            // |V|^2: element i of V tile (mv, nv) at lane l is column n = 16 nv + l%16 of row
-            // l3 = 16 mv + 4(l/16) + i; n < 24 is Re V(l2 = n), n >= 24 is Im V(l2 = n - 24), found
-            // at column (n + 8) % 16 of the next tile(s): lane l ^ 8
-#pragma unroll
-            for (int mv = 0; mv < NU; mv++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int l3 = 16 * mv + 4 * grp + i;
-                    const float s0 = v[mv][0][i] * v[mv][0][i], s1 = v[mv][1][i] * v[mv][1][i],
-                                s2 = v[mv][2][i] * v[mv][2][i];
-                    const float y1 = __shfl_xor(s1, 8, 64), y2 = __shfl_xor(s2, 8, 64);
-                    if (l3 < NB) {
-                        const double w13 = w1 * w3s[q][l3];
-                        {   // l2 = col
-                            const float v2 = s0 + (col < 8 ? y1 : y2);
-                            const double w = w13 * w2s[q][col];
-                            if (!(w <= 1.7e308)) atomicExch(err, 1);   // a zero eigenvalue in the triple
-                            Ost[q * OS + col * 25 + l3] = v2 * (float)(c33sq8 * w * w);
-                        }
-                        if (col < 8) {   // l2 = 16 + col
-                            const float v2 = s1 + y2;
-                            const double w = w13 * w2s[q][16 + col];
-                            if (!(w <= 1.7e308)) atomicExch(err, 1);
-                            Ost[q * OS + (16 + col) * 25 + l3] = v2 * (float)(c33sq8 * w * w);
-                        }
+                for (int t = 0; t < KS; t++) {
+                    const int kk = 4 * t + grp;
+                    ar[t] = ai[t] = 0.0f;
+                    if (t < ks && kk < 2 * ndr) {
+                        const int r = kk < ndr ? kk : kk - ndr;
+                        const float er = e[r * ELD], ei = e[(NDRMAX + r) * ELD];
+                        ar[t] = kk < ndr ? er : -ei;
+                        ai[t] = kk < ndr ? ei : er;
                     }
                 }
-        }
-        __syncthreads();
-        // ---- rows (l1, l2, l3) of the tile: 16 consecutive q2 each, as float4
-        float *o = out + (size_t)(s * NB + l1) * NN * nk + q0;
-        for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
-            const int row = x / (QT / 4), c4 = x % (QT / 4);
-            if (q0 + 4 * c4 < nk) {
-                const float *src = Ost + 4 * c4 * OS + (row / NB) * 25 + row % NB;
-                const float4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
-                *reinterpret_cast<float4 *>(o + (size_t)row * nk + 4 * c4) = val;
             }
+#pragma unroll
+            for (int j = 0; j < TPW; j++) {
+                const int nt = wave + NW * j;
+                if (nt < NTP) {
+                    f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int t = 0; t < KS; t++) {
+                        if (t < ks) {
+                            cr = mma(ar[t], gb[j][t], cr);
+                            ci = mma(ai[t], gb[j][t], ci);
+                        }
+                    }
+                    const int n = 16 * nt + col, bc = (n / NB) * BS + n % NB;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        Phi[(4 * grp + i) * PS + bc] = cr[i];
+                        Phi[(4 * grp + i) * PS + PP + bc] = ci[i];
+                    }
+                }
+            }
+            for (int x = tid; x < QT * NB; x += NW * 64) {
+                const int q = x / NB, l = x % NB;
+                w2s[q][l] = wq[(size_t)min(q0 + q, nk - 1) * NB + l];
+                w3s[q][l] = wq[(size_t)k3s[ch * QT + q] * NB + l];
+            }
+            __syncthreads();
+            // ---- U and V, one point at a time per wave
+            for (int jq = 0; jq < QW; jq++) {
+                const int q = wave * QW + jq;
+                const int n2c = min(q0 + q, nk - 1);
+                const float *ph = Phi + q * PS;
+                const float *pu = PU + (size_t)k3s[ch * QT + q] * FU * 64 + lane;
+                const float *pv = PV + (size_t)n2c * FV * 64 + lane;
+                f4 u[MU][NU];
+#pragma unroll
+                for (int a = 0; a < MU; a++)
+#pragma unroll
+                    for (int b = 0; b < NU; b++) u[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int t = 0; t < KU; t++) {
+                    float bu[NU];
+#pragma unroll
+                    for (int nt = 0; nt < NU; nt++) bu[nt] = pu[(t * NU + nt) * 64];
+                    const int kk = 4 * t + grp, pp = kk >= NB ? 1 : 0, c = kk - pp * NB;
+#pragma unroll
+                    for (int mt = 0; mt < MU; mt++) {
+                        const int row = 16 * mt + col, p = row >= NB ? 1 : 0, b = row - p * NB;
+                        float a = ph[(p ^ pp) * PP + b * BS + c];
+                        if (p == 0 && pp == 1) a = -a;
+#pragma unroll
+                        for (int nt = 0; nt < NU; nt++) u[mt][nt] = mma(a, bu[nt], u[mt][nt]);
+                    }
+                }
+                f4 v[NU][NV];
+#pragma unroll
+                for (int a = 0; a < NU; a++)
+#pragma unroll
+                    for (int b = 0; b < NV; b++) v[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int mt = 0; mt < MU; mt++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        float bv[NV];
+#pragma unroll
+                        for (int nv = 0; nv < NV; nv++) bv[nv] = pv[((mt * 4 + i) * NV + nv) * 64];
+#pragma unroll
+                        for (int mv = 0; mv < NU; mv++)
+#pragma unroll
+                            for (int nv = 0; nv < NV; nv++) v[mv][nv] = mma(u[mt][mv][i], bv[nv], v[mv][nv]);
+                    }
+                // |V|^2: element i of V tile (mv, nv) at lane l is column n = 16 nv + l%16 of row
+                // l3 = 16 mv + 4(l/16) + i; n < 24 is Re V(l2 = n), n >= 24 is Im V(l2 = n - 24),
+                // found at column (n + 8) % 16 of the next tile(s): lane l ^ 8
+#pragma unroll
+                for (int mv = 0; mv < NU; mv++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int l3 = 16 * mv + 4 * grp + i;
+                        const float s0 = v[mv][0][i] * v[mv][0][i], s1 = v[mv][1][i] * v[mv][1][i],
+                                    s2 = v[mv][2][i] * v[mv][2][i];
+                        const float y1 = __shfl_xor(s1, 8, 64), y2 = __shfl_xor(s2, 8, 64);
+                        if (l3 < NB) {
+                            const double w13 = w1 * w3s[q][l3];
+                            {   // l2 = col
+                                const float v2 = s0 + (col < 8 ? y1 : y2);
+                                const double w = w13 * w2s[q][col];
+                                if (!(w <= 1.7e308)) atomicExch(err, 1);   // a zero eigenvalue in the triple
+                                Ost[q * OS + col * 25 + l3] = v2 * (float)(c33sq8 * w * w);
+                            }
+                            if (col < 8) {   // l2 = 16 + col
+                                const float v2 = s1 + y2;
+                                const double w = w13 * w2s[q][16 + col];
+                                if (!(w <= 1.7e308)) atomicExch(err, 1);
+                                Ost[q * OS + (16 + col) * 25 + l3] = v2 * (float)(c33sq8 * w * w);
+                            }
+                        }
+                    }
+            }
+            __syncthreads();
+            // ---- rows (l1, l2, l3) of the chunk: 16 consecutive q2 each, as float4
+            float *o = out + (size_t)(s * NB + l1) * NN * nk + q0;
+            for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
+                const int row = x / (QT / 4), c4 = x % (QT / 4);
+                if (q0 + 4 * c4 < nk) {
+                    const float *src = Ost + 4 * c4 * OS + (row / NB) * 25 + row % NB;
+                    const float4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
+                    *reinterpret_cast<float4 *>(o + (size_t)row * nk + 4 * c4) = val;
+                }
+            }
+            // the next chunk's phase GEMM writes only Phi (read by nobody now) and the w tables
+            // (read by nobody now); its U and V write Ost after the barrier that follows it,
+            // when these reads are done
         }
     }
 }
@@ -282,14 +324,23 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
     hipLaunchKernelGGL(k_chain_pack_g, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ksub, ndr, ks,
                        ctx->d_gp.p, pg.p);
     ALT3_HIP(hipGetLastError());
-    const size_t lds = (size_t)(QT * PS + QT * OS) * sizeof(float);
-    if (lds + 2 * QT * NB * sizeof(double) + 64 > ctx->lds_max) return fail(ALT3_E_ARG, "V3 chain tile does not fit in LDS");
-    ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const size_t lds = (size_t)(QT * PS + QT * OS + 2 * NDRMAX * ELD) * sizeof(float);
+    if (lds + 2 * QT * NB * sizeof(double) + CH * QT * sizeof(int) + 64 > ctx->lds_max)
+        return fail(ALT3_E_ARG, "V3 chain tile does not fit in LDS");
     const double c33 = const33();
-    dim3 grid((unsigned)((nk + QT - 1) / QT), (unsigned)ksub);
-    kernel_mark(ctx, 0, false);
-    hipLaunchKernelGGL(k_v3_chain, grid, dim3(NW * 64), lds, ctx->stream, MeshDev::make(ctx->nc), nk, ctx->ibz_lo, ndr,
-                       ks, ctx->d_mapinv.p, ephf, pg.p, PU, PV, wq, c33 * c33 / 8, out, d_err);
+    dim3 grid((unsigned)((nk + CH * QT - 1) / (CH * QT)), (unsigned)ksub);
+#define ALT3_CHAIN(KS)                                                                                                \
+    {                                                                                                                 \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_chain<KS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        kernel_mark(ctx, 0, false);                                                                                   \
+        hipLaunchKernelGGL(k_v3_chain<KS>, grid, dim3(NW * 64), lds, ctx->stream, MeshDev::make(ctx->nc), nk,          \
+                           ctx->ibz_lo, ndr, ks, ctx->d_mapinv.p, ephf, pg.p, PU, PV, wq, c33 * c33 / 8, out, d_err); \
+    }
+    if (ks <= 8) ALT3_CHAIN(8)
+    else if (ks <= 12) ALT3_CHAIN(12)
+    else if (ks <= 14) ALT3_CHAIN(14)
+    else ALT3_CHAIN(16)
+#undef ALT3_CHAIN
     ALT3_HIP(hipGetLastError());
     kernel_mark(ctx, 0, true);
     return ALT3_OK;
