@@ -213,7 +213,10 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
     if (precision == ALT3_V3_MIXED) {
         const long long Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb * nb + 15) / 16) * 16;
         b += nel * 4 + ni * nb * Kp * Np * 4 + nk * nb * nb * 8 + nk * ndr * 8;
-        if (nb == 24) b += nk * 60 * 64 * 4 + ni * nb * (Kp / 4) * 36 * 64 * 4;                // k_v3_chain packs
+        if (nb == 24) {                                                                         // k_v3_chain packs
+            const long long k0 = Kp / 4, ks = k0 <= 8 ? 8 : k0 <= 12 ? 12 : k0 <= 14 ? 14 : 16;
+            b += nk * 60 * 64 * 4 + ni * nb * ks * 36 * 64 * 4;
+        }
     } else {
         b += nel * 8;
     }

@@ -102,14 +102,8 @@ __global__ void k_chain_pack_g(int ksub, int ndr, int ks, const double *__restri
     Gpk[x] = v;
 }
 
-constexpr int CH = 6;                // chunks of QT points per workgroup
+constexpr int CH = 4;                // chunks of QT points per workgroup
 constexpr int TPW = (NTP + NW - 1) / NW;   // phase N-tiles per wave (5: waves 0-3, 4: waves 4-7)
-constexpr int NDRMAX = 2 * KSMAX;    // cell differences held in the LDS phase rows
-// phase rows of the tile in LDS as [part][r][q], q fastest with row stride ELD = 16 mod 32: the
-// 16 points (lanes) of an A-operand read fall on consecutive banks and the next r (the other
-// half-wave's lane group) 16 banks further
-constexpr int ELD = (CH * QT + 15) / 32 * 32 + 16;
-static_assert(ELD >= CH * QT && ELD % 32 == 16, "phase row stride");
 
 // One workgroup of 8 waves per (q1 = s, tile of CH x 16 mesh points q2), looping over l1.  The
 // phase B operands of (s, l1) -- G', the largest operand -- are read from memory once per l1
@@ -127,7 +121,8 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
     extern __shared__ __align__(16) float lds[];
     float *Phi = lds;                  // [QT][2][NB][BS]: Re Phi'[b][c], then Im
     float *Ost = Phi + QT * PS;        // [QT][NB][25] scaled |V3|^2 (l2, l3) of the current l1
-    float *Es = Ost + QT * OS;         // [2][NDRMAX][ELD] phase rows e^{i q2.dR_r} of the tile
+    float *Ao = Ost + QT * OS;         // [CH][KS][64] phase A operands (rows of Re Phi'), per lane
+    float *Ai = Ao + CH * KS * 64;     // [CH][KS][64] (rows of Im Phi')
     __shared__ double w2s[QT][NB], w3s[QT][NB];
     __shared__ int k3s[CH * QT];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -141,23 +136,34 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
         m.coords(min(q00 + tid, nk - 1), b0, b1, b2);
         k3s[tid] = m.neg_sum(a0, a1, a2, b0, b1, b2);
     }
-    for (int x = tid; x < CH * QT * ndr; x += NW * 64) {
-        const int q = x / ndr, r = x - q * ndr;
-        const float *e = ephf + 2 * ((size_t)min(q00 + q, nk - 1) * ndr + r);
-        Es[r * ELD + q] = e[0];
-        Es[(NDRMAX + r) * ELD + q] = e[1];
+    // phase A operands of every chunk in the lane order of the MFMA: lane (col, grp) of k-step t
+    // holds row q = col, k = 4t + grp of [Re E, -Im E] (Ao) and [Im E, Re E] (Ai)
+    for (int x = tid; x < CH * KS * 64; x += NW * 64) {
+        const int ln = x & 63, ct = x >> 6, t = ct % KS, ch = ct / KS;
+        const int kk = 4 * t + (ln >> 4);
+        float vr = 0.0f, vi = 0.0f;
+        if (kk < 2 * ndr) {
+            const int r = kk < ndr ? kk : kk - ndr;
+            const float *e = ephf + 2 * ((size_t)min(q00 + ch * QT + (ln & 15), nk - 1) * ndr + r);
+            vr = kk < ndr ? e[0] : -e[1];
+            vi = kk < ndr ? e[1] : e[0];
+        }
+        Ao[x] = vr;
+        Ai[x] = vi;
     }
     const int col = lane & 15, grp = lane >> 4;
     // phase B operands of this wave's N-tiles nt = wave + 8 j
     float gb[TPW][KS];
+    // (the packs hold KS k-steps, zero beyond 2 ndr: no guards, so that the MFMA chains below
+    // are straight-line code; a wave without a fifth tile reads the last tile again, unused)
     auto load_g = [&](int l1) {
-        const float *G = Gpk + (size_t)(s * NB + l1) * ks * NTP * 64 + lane;
+        const float *G = Gpk + (size_t)(s * NB + l1) * KS * NTP * 64 + lane;
 #pragma unroll
         for (int j = 0; j < TPW; j++)
 #pragma unroll
             for (int t = 0; t < KS; t++) {
-                const int nt = wave + NW * j;
-                gb[j][t] = (nt < NTP && t < ks) ? G[((size_t)t * NTP + nt) * 64] : 0.0f;
+                const int nt = min(wave + NW * j, NTP - 1);
+                gb[j][t] = G[((size_t)t * NTP + nt) * 64];
             }
     };
     __syncthreads();
@@ -169,31 +175,20 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain(alt3::MeshDev m, int nk, i
             // ---- phase GEMM of the chunk: rows of point q = lane%16, k = 4t + lane/16;
             // ar: rows of Re Phi' ([Re E, -Im E]), ai: rows of Im Phi' ([Im E, Re E])
             float ar[KS], ai[KS];
-            {
-                const float *e = Es + ch * QT + col;
 #pragma unroll
-                for (int t = 0; t < KS; t++) {
-                    const int kk = 4 * t + grp;
-                    ar[t] = ai[t] = 0.0f;
-                    if (t < ks && kk < 2 * ndr) {
-                        const int r = kk < ndr ? kk : kk - ndr;
-                        const float er = e[r * ELD], ei = e[(NDRMAX + r) * ELD];
-                        ar[t] = kk < ndr ? er : -ei;
-                        ai[t] = kk < ndr ? ei : er;
-                    }
-                }
+            for (int t = 0; t < KS; t++) {
+                ar[t] = Ao[(ch * KS + t) * 64 + lane];
+                ai[t] = Ai[(ch * KS + t) * 64 + lane];
             }
 #pragma unroll
             for (int j = 0; j < TPW; j++) {
                 const int nt = wave + NW * j;
-                if (nt < NTP) {
+                if (j < TPW - 1 || nt < NTP) {   // the same for every lane of the wave
                     f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int t = 0; t < KS; t++) {
-                        if (t < ks) {
-                            cr = mma(ar[t], gb[j][t], cr);
-                            ci = mma(ai[t], gb[j][t], ci);
-                        }
+                        cr = mma(ar[t], gb[j][t], cr);
+                        ci = mma(ai[t], gb[j][t], ci);
                     }
                     const int n = 16 * nt + col, bc = (n / NB) * BS + n % NB;
 #pragma unroll
@@ -314,7 +309,9 @@ bool v3_chain_supported(const alt3_ctx *ctx)
 int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
                  int *d_err)
 {
-    const int nk = ctx->nk, ndr = ctx->V.ndr, ks = (2 * ndr + 3) / 4;
+    const int nk = ctx->nk, ndr = ctx->V.ndr, ks0 = (2 * ndr + 3) / 4;
+    // k-steps of the phase GEMM rounded up to a kernel instance; the packs are zero beyond 2 ndr
+    const int ks = ks0 <= 8 ? 8 : ks0 <= 12 ? 12 : ks0 <= 14 ? 14 : 16;
     DBuf<float> &pe = ctx->d_chain_e, &pg = ctx->d_chain_g;
     const size_t ne = (size_t)nk * (FU + FV) * 64, ng = (size_t)ksub * NB * ks * NTP * 64;
     ALT3_TRY(pe.reserve(ne));
@@ -324,7 +321,7 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
     hipLaunchKernelGGL(k_chain_pack_g, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, ctx->stream, ksub, ndr, ks,
                        ctx->d_gp.p, pg.p);
     ALT3_HIP(hipGetLastError());
-    const size_t lds = (size_t)(QT * PS + QT * OS + 2 * NDRMAX * ELD) * sizeof(float);
+    const size_t lds = (size_t)(QT * PS + QT * OS + 2 * CH * ks * 64) * sizeof(float);
     if (lds + 2 * QT * NB * sizeof(double) + CH * QT * sizeof(int) + 64 > ctx->lds_max)
         return fail(ALT3_E_ARG, "V3 chain tile does not fit in LDS");
     const double c33 = const33();
@@ -336,9 +333,9 @@ int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, c
         hipLaunchKernelGGL(k_v3_chain<KS>, grid, dim3(NW * 64), lds, ctx->stream, MeshDev::make(ctx->nc), nk,          \
                            ctx->ibz_lo, ndr, ks, ctx->d_mapinv.p, ephf, pg.p, PU, PV, wq, c33 * c33 / 8, out, d_err); \
     }
-    if (ks <= 8) ALT3_CHAIN(8)
-    else if (ks <= 12) ALT3_CHAIN(12)
-    else if (ks <= 14) ALT3_CHAIN(14)
+    if (ks == 8) ALT3_CHAIN(8)
+    else if (ks == 12) ALT3_CHAIN(12)
+    else if (ks == 14) ALT3_CHAIN(14)
     else ALT3_CHAIN(16)
 #undef ALT3_CHAIN
     ALT3_HIP(hipGetLastError());
