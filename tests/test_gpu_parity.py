@@ -177,7 +177,8 @@ def test_solve_own_eigenvectors_vs_reference(si_inputs, ref4, ref8):
 
 
 def test_mixed_v3_vs_fp64(si_inputs):
-    """ALT3_V3_MIXED (FP32 MFMA phase GEMM) against the FP64 path on the same eigenvectors:
+    """ALT3_V3_MIXED (6 branches: FP32 phase sum and contractions on the vector ALUs, FP64
+    scale) against the FP64 path on the same eigenvectors:
     |V3|^2 within 1e-5 of the largest element, kappa within 0.5 %."""
     print("case: mixed vs fp64 8^3", flush=True)
     with capi.Context(si_inputs[8]) as c:
