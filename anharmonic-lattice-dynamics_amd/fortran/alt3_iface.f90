@@ -60,6 +60,11 @@ module alt3_iface
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx, eigenval, eigenvec, veloc
      end function alt3_eig
+     ! load eigen data computed elsewhere (same layouts as alt3_eig)
+     integer(c_int) function alt3_set_eig(ctx, eigenval, eigenvec, veloc) bind(C, name='alt3_set_eig')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, eigenval, eigenvec, veloc
+     end function alt3_set_eig
 
      integer(c_int) function alt3_v3sq_ibz(ctx, ibz_lo, ibz_hi, precision, v33s8) bind(C, name='alt3_v3sq_ibz')
        import :: c_int, c_ptr

@@ -14,8 +14,13 @@
 !  calculate_tauRTA_iso, CollisionMatrix_iso, update_iso; kappa.f90:522-543, :574-578,
 !  :710-716, :1401-1407).
 !
-!  usage: alt3_kappa [fp64 | mixed | fp64ref] [--host-only]
+!  usage: alt3_kappa [fp64 | mixed | fp64ref] [--host-only] [--eig FILE]
 !    --host-only builds the mesh and symmetry tables only (no GPU) and prints their sizes.
+!    --eig FILE  takes the eigen data from FILE instead of the device eigen solve (alt3_set_eig):
+!                one stream of eigenval(nb,nk) real(8), eigenvec(nb,nb,nk) complex(8) and
+!                veloc(3,nb,nk) real(8), the arrays get_frequencies + subst_eivecs leave in
+!                module eigen (kappa.f90:283-300), so that the rest of the path and the output
+!                files can be set beside a reference run on the same eigenvectors.
 program alt3_kappa
   use iso_c_binding
   use alt3_iface
@@ -62,18 +67,30 @@ program alt3_kappa
   real(c_double), target :: fmin
   real(c_double) :: tempk
   character(len=32) :: arg
+  character(len=4096) :: eig_file
+  complex(c_double_complex), allocatable, target :: eigenvec(:, :, :)
   logical :: host_only
-  integer :: i, l, a, ia, it
+  integer :: i, l, a, ia, it, u
 
   prec = ALT3_V3_FP64
   host_only = .false.
-  do i = 1, command_argument_count()
+  eig_file = ''
+  i = 0
+  do while (i < command_argument_count())
+     i = i + 1
      call get_command_argument(i, arg)
      select case (trim(arg))
      case ('fp64'); prec = ALT3_V3_FP64
      case ('mixed'); prec = ALT3_V3_MIXED
      case ('fp64ref'); prec = ALT3_V3_FP64_REF
      case ('--host-only'); host_only = .true.
+     case ('--eig')
+        i = i + 1
+        call get_command_argument(i, eig_file)
+        if (len_trim(eig_file) == 0) then
+           write(*, *) 'alt3_kappa: --eig needs a file'
+           stop 2
+        end if
      case default
         write(*, *) 'alt3_kappa: unknown argument ', trim(arg)
         stop 2
@@ -125,8 +142,18 @@ program alt3_kappa
   n = nibz * nb * 3
   allocate(x(n))
 
-  ! get_frequencies (kappa.f90:283); eigenvectors stay on the device
-  call alt3_check(alt3_eig(ctx, c_loc(eigenval), c_null_ptr, c_loc(veloc)), 'alt3_eig')
+  if (len_trim(eig_file) > 0) then
+     ! eigen data of another solver (e.g. the reference's get_frequencies + subst_eivecs)
+     allocate(eigenvec(nb, nb, nk))
+     open(newunit=u, file=trim(eig_file), access='stream', form='unformatted', status='old')
+     read(u) eigenval, eigenvec, veloc
+     close(u)
+     call alt3_check(alt3_set_eig(ctx, c_loc(eigenval), c_loc(eigenvec), c_loc(veloc)), 'alt3_set_eig')
+     deallocate(eigenvec)
+  else
+     ! get_frequencies (kappa.f90:283); eigenvectors stay on the device
+     call alt3_check(alt3_eig(ctx, c_loc(eigenval), c_null_ptr, c_loc(veloc)), 'alt3_eig')
+  end if
   call alt3_check(alt3_get_mesh(ctx, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(mapinv), c_loc(kibz), &
        c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr), 'alt3_get_mesh')
   tempk = tmin                                     ! kappa.f90:565, :2649

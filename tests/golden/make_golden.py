@@ -12,6 +12,7 @@ dumps are stored as compressed .npz next to this script.
     python tests/golden/make_golden.py si16       # si16: the SW-Si 16^3 workload at 100/300/800 K
     python tests/golden/make_golden.py rs8        # rs8_4: the 8-atom rock-salt-type cell at 4^3
     python tests/golden/make_golden.py born       # zb4_born: Born charges (nonanal) on the 4-atom cell
+    python tests/golden/make_golden.py writekappa # write_kappa_si4/: the reference's own output files
 """
 from __future__ import annotations
 
@@ -265,7 +266,31 @@ def main_born():
         print("wrote zb4_born")
 
 
+def main_writekappa():
+    """The files write_kappa writes (sub.f90:1818-1916) in the probe's job-1 sequence on the
+    SW-Si 4^3 inputs at 100, 300 and 800 K: tau_RTA / kappa_RTA after mode_kappa and
+    tau_RTAandeff-cg / kappa_direct-cg after cg_quadratic and mode_kappa_d, one set per
+    temperature, kept as text.  eig.bin holds the probe's eigen data (after subst_eivecs) as one
+    stream, eigenval(nb,nk) real(8), eigenvec(nb,nb,nk) complex(8), veloc(3,nb,nk) real(8), the
+    form the Fortran driver's --eig option reads."""
+    import shutil
+    out = os.path.join(HERE, "write_kappa_si4")
+    os.makedirs(out, exist_ok=True)
+    with tempfile.TemporaryDirectory() as wd:
+        _, D = run_reference(4, wd)
+        for T in TEMPS:
+            t = f"T{int(round(T))}"
+            for stem in ("tau_RTA", "kappa_RTA", "tau_RTAandeff-cg", "kappa_direct-cg"):
+                shutil.copy(os.path.join(wd, f"{stem}_{t}.dat"), os.path.join(out, f"{stem}_{t}.dat"))
+        with open(os.path.join(out, "eig.bin"), "wb") as f:
+            for k in ("eigenval", "eigenvec", "veloc"):
+                f.write(np.asfortranarray(D[k]).tobytes(order="F"))
+    print("wrote write_kappa_si4")
+
+
 def main():
+    if sys.argv[1:] == ["writekappa"]:
+        return main_writekappa()
     if sys.argv[1:] == ["born"]:
         return main_born()
     if sys.argv[1:] == ["si16"]:

@@ -170,3 +170,110 @@ def test_driver_iso_matches_binding(cells, tmp_path):
     kr = np.trace(r["kappa_rta"][0].sum(0)) / 3
     assert abs(_tot_kap(tmp_path / "kappa_RTA.dat") / kr - 1) < 1e-9
     assert abs(np.trace(r0["kappa_rta"][0].sum(0)) / 3 / kr - 1) > 1e-4
+
+
+# ---------------------------------------------------------------- against the reference's files
+WK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "write_kappa_si4")
+
+
+def _numbers(path):
+    """Every numeric token of a write_kappa file, line by line, with its printed significant
+    digits (header lines are skipped)."""
+    out = []
+    for line in open(path):
+        toks = line.split()
+        row = []
+        for t in toks:
+            s = t.replace("D", "E")
+            try:
+                v = float(s)
+            except ValueError:
+                continue
+            mant = s.split("E")[0].lstrip("+-").replace(".", "").lstrip("0")
+            row.append((v, max(len(mant), 1)))
+        if row and len(row) == len(toks):
+            out.append(row)
+    return out
+
+
+def _compare_files(ours, ref, what):
+    """Column by column: each value within 1e-8 of its column's largest magnitude, or within
+    one unit of the last digit the format prints (the files carry 4 to 8 digits, and values a
+    hair apart can round to neighbouring last digits); NaN where the reference writes NaN (its
+    1/tauinv_eff of a mode without scattering)."""
+    A, B = _numbers(ours), _numbers(ref)
+    assert len(A) == len(B), (what, len(A), len(B))
+    cols = {}
+    for ra, rb in zip(A, B):
+        assert len(ra) == len(rb), (what, ra, rb)
+        for j, (vb, _) in enumerate(rb):
+            if np.isfinite(vb):
+                cols[(len(rb), j)] = max(cols.get((len(rb), j), 0.0), abs(vb))
+    for i, (ra, rb) in enumerate(zip(A, B)):
+        for j, ((va, _), (vb, sig)) in enumerate(zip(ra, rb)):
+            if np.isnan(vb):
+                assert np.isnan(va), (what, i, j, va, vb)
+                continue
+            step = 10 ** (np.floor(np.log10(abs(vb))) - sig + 1) if vb != 0 else 0.0
+            tol = max(1e-8 * cols.get((len(rb), j), 0.0), 1.0000001 * step)
+            assert abs(va - vb) <= tol, (what, i, j, va, vb, tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("T", [100, 300, 800])
+def test_driver_files_equal_reference_write_kappa(si_inputs, tmp_path, T):
+    """The driver with the reference's eigen data (--eig) against the files the reference's own
+    write_kappa wrote in the same job-1 sequence (SW-Si 4^3): kappa_RTA, tau_RTA,
+    kappa_direct-cg and tau_RTAandeff-cg, column by column."""
+    print(f"case: driver files vs reference write_kappa, SW-Si 4^3, {T} K", flush=True)
+    from thermacond import models, refio
+    refio.write_inputs(str(tmp_path), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), tmin=float(T), tmax=float(T)))
+    _run(tmp_path, "fp64", "--eig", os.path.join(WK, "eig.bin"))
+    for ours, ref in (("kappa_RTA.dat", f"kappa_RTA_T{T}.dat"), ("tau_RTA.dat", f"tau_RTA_T{T}.dat"),
+                      ("kappa_direct-cg.dat", f"kappa_direct-cg_T{T}.dat"),
+                      ("tau_RTAandeff-cg.dat", f"tau_RTAandeff-cg_T{T}.dat")):
+        _compare_files(os.path.join(tmp_path, ours), os.path.join(WK, ref), ours)
+    # tot_kap is list-directed (full precision)
+    for ours, ref in (("kappa_RTA.dat", f"kappa_RTA_T{T}.dat"), ("kappa_direct-cg.dat", f"kappa_direct-cg_T{T}.dat")):
+        a, b = _tot_kap(os.path.join(tmp_path, ours)), _tot_kap(os.path.join(WK, ref))
+        assert abs(a / b - 1) < (1e-8 if "RTA" in ours else 1e-6), (ours, a, b)
+
+
+def test_reference_files_parse(si_inputs):
+    """The fixture files hold what the comparison reads: nb mode lines and the sum line in the
+    kappa files, nibz*nb mode lines of 15 columns in the tau files."""
+    with capi.Context(si_inputs[4], host_only=True) as h:
+        nibz, nb = h.nibz, h.nb
+    for T in (100, 300, 800):
+        for stem in ("kappa_RTA", "kappa_direct-cg"):
+            k = _numbers(os.path.join(WK, f"{stem}_T{T}.dat"))
+            assert len(k) == nb + 1 and all(len(r) == 7 for r in k[:nb]) and len(k[nb]) == 3
+        for stem in ("tau_RTA", "tau_RTAandeff-cg"):
+            t = _numbers(os.path.join(WK, f"{stem}_T{T}.dat"))
+            assert len(t) == nibz * nb and all(len(r) == 15 for r in t)
+
+
+JOB1_REF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "alt3_job1_ref")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("T", [100, 300, 800])
+def test_reference_job1_over_the_adapters(si_inputs, tmp_path, T):
+    """kappa.f90's job-1 sequence with the reference's own readers, eigen solver, Qandtau,
+    mode_kappa, cal_RHS_IBZ, mode_kappa_d and write_kappa, and the hot path through the drop-in
+    adapters (fortran/alt3_thermacond.f90, compiled against the reference's modules), against the
+    reference's own files on the same inputs (SW-Si 4^3), column by column."""
+    if not os.path.exists(JOB1_REF):
+        pytest.skip("oracle/_ref/alt3_job1_ref is built only where the reference sources are")
+    print(f"case: reference job 1 over the adapters, SW-Si 4^3, {T} K", flush=True)
+    from thermacond import refio
+    refio.write_inputs(str(tmp_path), si_inputs["fc"], refio.RunParams(nc=(4, 4, 4), tmin=float(T), tmax=float(T)))
+    r = subprocess.run([JOB1_REF, "fp64"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "alt3_job1_ref: done" in r.stdout, r.stdout[-3000:] + r.stderr[-3000:]
+    for ours, ref in (("kappa_RTA.dat", f"kappa_RTA_T{T}.dat"), ("tau_RTA.dat", f"tau_RTA_T{T}.dat"),
+                      ("kappa_direct-cg.dat", f"kappa_direct-cg_T{T}.dat"),
+                      ("tau_RTAandeff-cg.dat", f"tau_RTAandeff-cg_T{T}.dat")):
+        _compare_files(os.path.join(tmp_path, ours), os.path.join(WK, ref), ours)
+    for ours, ref in (("kappa_RTA.dat", f"kappa_RTA_T{T}.dat"), ("kappa_direct-cg.dat", f"kappa_direct-cg_T{T}.dat")):
+        a, b = _tot_kap(os.path.join(tmp_path, ours)), _tot_kap(os.path.join(WK, ref))
+        assert abs(a / b - 1) < (1e-8 if "RTA" in ours else 1e-6), (ours, a, b)
