@@ -231,26 +231,28 @@ program alt3_kappa
 
   if (job == 4) then
      ! the collision system for a later job 5: CM-1D.dat row by row, RHS_IBZ.dat, and DS.dat =
-     ! F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:2577-2625)
+     ! F_RTA flattened (i, l, alpha), alpha fastest (kappa.f90:2577-2625).  One value per line as
+     ! there, but with 17 significant digits (the reference's list-directed output keeps fewer),
+     ! so that job 5's list-directed reads (read_CM_1D_mpi, read_RHS, read_DS) get every bit back
      allocate(cm(n, n), rhs(n))
      call alt3_check(alt3_collision(ctx, tempk, c_loc(cm), c_loc(rhs)), 'alt3_collision')
      open(10, file='CM-1D.dat', status='unknown')
      do i = 1, n
         do ia = 1, n
-           write(10, *) cm(i, ia)
+           write(10, '(es25.17e3)') cm(i, ia)
         end do
      end do
      close(10)
      open(10, file='RHS_IBZ.dat', status='unknown')
      do i = 1, n
-        write(10, *) rhs(i)
+        write(10, '(es25.17e3)') rhs(i)
      end do
      close(10)
      open(10, file='DS.dat', status='unknown')
      do i = 1, nibz
         do l = 1, nb
            do a = 1, 3
-              write(10, *) frta(i, l, a)
+              write(10, '(es25.17e3)') frta(i, l, a)
            end do
         end do
      end do
