@@ -255,6 +255,7 @@ struct alt3_ctx {
     alt3::DBuf<float> d_ephf, d_evf, d_gpf32;
     alt3::DBuf<float> d_chain_e, d_chain_g;        // operand packs of k_v3_chain (eigenvectors, G')
     alt3::DBuf<unsigned char> d_batch;             // alt3_v3_batch: work counter and job table
+    alt3::DBuf<float> d_fgr_big;                   // k_fgr_big: per-workgroup energies, ranges, weight sums
     alt3::DBuf<double> d_red;                      // reduction scratch
     alt3::DBuf<double> d_freqT;                    // frequencies branch-major [nb][nk] (FGR)
     alt3::DBuf<double> d_solve_x, d_solve_te;      // alt3_solve: CG vector, tauinv_eff

@@ -162,7 +162,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_cm.release(); ctx->d_cmf.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
-    ctx->d_chain_e.release(); ctx->d_chain_g.release(); ctx->d_batch.release();
+    ctx->d_chain_e.release(); ctx->d_chain_g.release(); ctx->d_batch.release(); ctx->d_fgr_big.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->ev_k)
@@ -221,6 +221,7 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
         b += nel * 8;
     }
     b += 2 * nel * (precision == ALT3_V3_MIXED ? 4 : 8);                                       // P1sm, P2sm
+    if (nk > 8192) b += 1024 * 20 * nk;          // k_fgr_big scratch: 5 floats per mesh point per workgroup
     const long long rb = precision == ALT3_V3_MIXED ? 4 : 8;                                  // R in the precision of P
     b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * rb + 3 * ni * nb * nb * 8);  // tau pass
     b += n * n * 8 + n * 8 * 8 + ni * nb * 9 * 8 * 2;                                           // CM, RHS, CG, kappa

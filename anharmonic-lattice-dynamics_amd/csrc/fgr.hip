@@ -10,6 +10,8 @@
 // The corner energies and weights keep the reference's REAL*4 types (energies rounded to
 // float, differences in float, mixed float/double formulae) so ties and the degenerate-gap
 // clamp behave identically; FMA contraction is disabled in this file for the same reason.
+#include <algorithm>
+
 #include "alt3_impl.h"
 #include "mesh_dev.h"
 
@@ -511,6 +513,181 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
     }
 }
 
+// Meshes whose corner energies and weight sums do not fit in LDS (nk > 8192, e.g. 30^3): the
+// same work items and arithmetic as k_fgr, with the per-item arrays -- corner energies,
+// cube energy ranges and weight sums -- in a slice of global scratch owned by the
+// workgroup.  The grid is persistent (one workgroup per scratch slice), each workgroup
+// taking items blockIdx.x, blockIdx.x + gridDim.x, ...; the weight sums are global FP64
+// atomics (L2), the tetrahedron lists stay in LDS with 32-bit codes and are filled chunk by
+// chunk when one pass over the mesh overflows them.
+constexpr int kBigCase = 4096;
+template <typename VT>
+__global__ void __launch_bounds__(kFgrThreads) k_fgr_big(alt3::MeshDev m, int nk, int nb, int ibz_lo, int nitems,
+                                                         const int *__restrict__ mapinv,
+                                                         const double *__restrict__ freq,
+                                                         const double *__restrict__ freqT,
+                                                         const VT *__restrict__ v33, VT *__restrict__ P1,
+                                                         VT *__restrict__ P2, float *__restrict__ scratch)
+{
+    __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
+    __shared__ int s_cnt[3];
+    __shared__ unsigned s_list[3 * kBigCase];
+    float *en = scratch + (size_t)blockIdx.x * 5 * nk;            // [nk]   (5 nk floats per slice)
+    float2 *cr = reinterpret_cast<float2 *>(en + nk);              // [nk] cube (min, max)
+    double *W = reinterpret_cast<double *>(en + 3 * nk);           // [nk]  (nk even: 8-byte aligned)
+    constexpr bool kSym = sizeof(VT) == sizeof(float);
+    const int tid = threadIdx.x, gam = m.gamma();
+    const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int proc = item & 1;
+        int r = item >> 1;
+        const int l3 = r % nb;
+        r /= nb;
+        const int l2 = r % nb;
+        const int s = r / nb;
+        if (kSym && proc == 1 && l2 > l3) continue;   // the same for the whole group
+        const bool mirror = kSym && proc == 1 && l2 < l3;
+        const int k1 = mapinv[ibz_lo + s];
+        int a0, a1, a2;
+        m.coords(k1, a0, a1, a2);
+        float lo = 3.4e38f, hi = -3.4e38f;
+        for (int w = tid; w < nk; w += kFgrThreads) {
+            int c0, c1, c2;
+            m.coords(w, c0, c1, c2);
+            const int p = proc == 0 ? m.sum(a0, a1, a2, c0, c1, c2) : m.diff(a0, a1, a2, c0, c1, c2);
+            const double e = proc == 0 ? -freqT[(size_t)l2 * nk + w] + freqT[(size_t)l3 * nk + p]
+                                       : freqT[(size_t)l2 * nk + w] + freqT[(size_t)l3 * nk + p];
+            const float ef = (float)e;
+            en[w] = ef;
+            W[w] = 0.0;
+            lo = fminf(lo, ef);
+            hi = fmaxf(hi, ef);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, off, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            s_lo[tid >> 6] = lo;
+            s_hi[tid >> 6] = hi;
+        }
+        __syncthreads();   // en written by every thread; the range partials
+        for (int w = 0; w < kFgrThreads / 64; w++) {
+            lo = fminf(lo, s_lo[w]);
+            hi = fmaxf(hi, s_hi[w]);
+        }
+        for (int k = tid; k < nk; k += kFgrThreads) {
+            int c0, c1, c2;
+            m.coords(k, c0, c1, c2);
+            const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                      d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+            const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
+                                 en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
+                                 en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
+            float mn = ce[0], mx = ce[0];
+#pragma unroll
+            for (int q = 1; q < 8; q++) {
+                mn = fminf(mn, ce[q]);
+                mx = fmaxf(mx, ce[q]);
+            }
+            cr[k] = make_float2(mn, mx);
+        }
+        __syncthreads();   // cube ranges complete
+        VT *P = proc == 0 ? P1 : P2;
+        for (int l1 = 0; l1 < nb; l1++) {
+            const double omega1 = freq[(size_t)k1 * nb + l1];
+            const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+            const size_t basem = ((((size_t)s * nb + l1) * nb + l3) * nb + l2) * (size_t)nk;
+            if (omega_in(omega1, lo, hi)) {
+                // chunks of cubes whose tetrahedra always fit the lists (6 per cube)
+                constexpr int kChunk = (kBigCase / 6) / kFgrThreads * kFgrThreads;
+                for (int k0 = 0; k0 < nk; k0 += kChunk) {
+                    if (tid < 3) s_cnt[tid] = 0;
+                    __syncthreads();
+                    for (int k = k0 + tid; k < min(nk, k0 + kChunk); k += kFgrThreads) {
+                        const float2 rg = cr[k];
+                        if (!omega_in(omega1, rg.x, rg.y)) continue;
+                        int c0, c1, c2;
+                        m.coords(k, c0, c1, c2);
+                        const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
+                                  d2 = (c2 + 1 == m.n2) ? 0 : c2 + 1;
+                        const float ce[8] = {en[m.index(c0, c1, c2)], en[m.index(d0, c1, c2)], en[m.index(c0, d1, c2)],
+                                             en[m.index(d0, d1, c2)], en[m.index(c0, c1, d2)], en[m.index(d0, c1, d2)],
+                                             en[m.index(c0, d1, d2)], en[m.index(d0, d1, d2)]};
+#pragma unroll
+                        for (int l = 0; l < 6; l++) {
+                            const float a = ce[tet_corner(l, 0)], b = ce[tet_corner(l, 1)], c = ce[tet_corner(l, 2)],
+                                        d = ce[tet_corner(l, 3)];
+                            const float s1 = fminf(a, b), s2 = fmaxf(a, b), s3 = fminf(c, d), s4 = fmaxf(c, d);
+                            if (omega_in(omega1, fminf(s1, s3), fmaxf(s2, s4))) {
+                                const float m1 = fmaxf(s1, s3), m2 = fminf(s2, s4);
+                                const int cs = (omega1 <= (double)fminf(m1, m2)) ? 0
+                                               : (omega1 <= (double)fmaxf(m1, m2)) ? 1 : 2;
+                                const int x = atomicAdd(&s_cnt[cs], 1);
+                                s_list[cs * kBigCase + x] = ((unsigned)k << 3) | (unsigned)l;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const int n0 = s_cnt[0], n1 = s_cnt[1], n2 = s_cnt[2];
+                    for (int x = tid; x < n0 + n1 + n2; x += kFgrThreads) {
+                        const int at = x < n0 ? x : x < n0 + n1 ? kBigCase + x - n0 : 2 * kBigCase + x - n0 - n1;
+                        const unsigned code = s_list[at];
+                        const int kk = (int)(code >> 3), l = (int)(code & 7u);
+                        int c0, c1, c2;
+                        m.coords(kk, c0, c1, c2);
+                        int v[4];
+#pragma unroll
+                        for (int y = 0; y < 4; y++) {
+                            const int pc = tet_corner(l, y);
+                            int e0 = c0 + (pc & 1), e1 = c1 + ((pc >> 1) & 1), e2 = c2 + ((pc >> 2) & 1);
+                            if (e0 == m.n0) e0 = 0;
+                            if (e1 == m.n1) e1 = 0;
+                            if (e2 == m.n2) e2 = 0;
+                            v[y] = m.index(e0, e1, e2);
+                        }
+                        const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
+                        if constexpr (sizeof(VT) == sizeof(float)) {
+                            float c[4];
+                            weight_tet1f(w4, omega1, c);
+                            for (int y = 0; y < 4; y++)
+                                if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)(c[y] / 6.0f));
+                        } else {
+                            double c[4];
+                            weight_tet1(w4, omega1, c);
+                            for (int y = 0; y < 4; y++)
+                                if (c[y] != 0.0) atomicAdd(&W[v[y]], c[y] / 6.0);
+                        }
+                    }
+                    __syncthreads();   // the lists are reused by the next chunk
+                }
+            }
+            // P(q_w) = W(q_w) F(q_w) (and the mirrored row W(q1-q_w) F'(q_w)), W cleared after
+            // every thread has read it
+            for (int w = tid; w < nk; w += kFgrThreads) {
+                int c0, c1, c2;
+                m.coords(w, c0, c1, c2);
+                const int vo = proc ? m.neg(c0, c1, c2) : w;
+                const int pq = proc == 0 ? m.sum(a0, a1, a2, c0, c1, c2) : m.diff(a0, a1, a2, c0, c1, c2);
+                const bool vz = (l2 < 3 && w == gam) || (l3 < 3 && pq == gam);
+                double F = (double)v33[base + vo] * tpi2 / nk;
+                if (vz) F = 0.0;
+                P[base + w] = (VT)(W[w] * F);
+                if (mirror) {
+                    const int pm = m.diff(a0, a1, a2, c0, c1, c2);
+                    const bool vzm = (l3 < 3 && w == gam) || (l2 < 3 && pm == gam);
+                    double Fm = (double)v33[basem + vo] * tpi2 / nk;
+                    if (vzm) Fm = 0.0;
+                    P[basem + w] = (VT)(W[pm] * Fm);
+                }
+            }
+            __syncthreads();
+            for (int w = tid; w < nk; w += kFgrThreads) W[w] = 0.0;
+            __syncthreads();
+        }
+    }
+}
+
 // Gaussian-smeared variant (delta_g, extratools.f90:239-246): the same corner energies and
 // masks as calculate_FGR_mpi2, but delta(w1 - e) -> exp(-x^2/2s^2)/(sqrt(2pi) s) at the mesh
 // point itself:  P_sm(q1,q2) = F(q2) delta_g(w(q1,l1) - e(q2), sigma).  FP64 throughout.
@@ -607,14 +784,39 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
     size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
-    // the list codes (cube << 3 | tetrahedron) are 16 bits wide: nk <= 8192 (a 20^3 mesh)
-    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 16 * kFgrThreads)
-        return fail(ALT3_E_ARG, "tetrahedron kernel holds the mesh in LDS: nk <= 8192");
     ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     ALT3_TRY(ctx->d_freqT.reserve((size_t)nk * nb));
     hipLaunchKernelGGL(k_freq_t, dim3((unsigned)((nk * nb + 255) / 256)), dim3(256), 0, ctx->stream, nk, nb, ctx->d_freq.p,
                        ctx->d_freqT.p);
+    // the LDS kernel's list codes (cube << 3 | tetrahedron) are 16 bits wide: nk <= 8192 (20^3);
+    // larger meshes take the global-scratch kernel
+    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 16 * kFgrThreads) {
+        if ((unsigned long long)nk * 8 > (1ull << 31)) return fail(ALT3_E_ARG, "mesh too large for the FGR codes");
+        const long long nitems = 2ll * ksub * nb * nb;
+        const void *kern = mixed ? (const void *)k_fgr_big<float> : (const void *)k_fgr_big<double>;
+        int per_cu = 0, cus = 0;
+        ALT3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kFgrThreads, 0));
+        ALT3_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const long long grid = std::max(1ll, std::min(nitems, (long long)std::max(1, per_cu) * std::max(1, cus)));
+        ALT3_TRY(ctx->d_fgr_big.reserve((size_t)grid * 5 * nk));
+        kernel_mark(ctx, 1, false);
+        if (mixed)
+            hipLaunchKernelGGL(k_fgr_big<float>, dim3((unsigned)grid), dim3(kFgrThreads), 0, ctx->stream, md, nk, nb,
+                               ctx->ibz_lo, (int)nitems, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p,
+                               ctx->d_v33f.p, ctx->d_p1f.p, ctx->d_p2f.p, ctx->d_fgr_big.p);
+        else
+            hipLaunchKernelGGL(k_fgr_big<double>, dim3((unsigned)grid), dim3(kFgrThreads), 0, ctx->stream, md, nk, nb,
+                               ctx->ibz_lo, (int)nitems, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p,
+                               ctx->d_v33.p, ctx->d_p1.p, ctx->d_p2.p, ctx->d_fgr_big.p);
+        ALT3_HIP(hipGetLastError());
+        kernel_mark(ctx, 1, true);
+        if (sync) ALT3_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->have_fgr = true;
+        ctx->fgr_kind = ALT3_DELTA_TETRA;
+        ctx->have_tau = ctx->have_cm = false;
+        return ALT3_OK;
+    }
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
