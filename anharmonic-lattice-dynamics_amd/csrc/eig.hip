@@ -16,7 +16,7 @@
 
 namespace {
 
-struct zd {
+struct alignas(16) zd {
     double r, i;
 };
 __device__ __forceinline__ zd zmul(zd a, zd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
