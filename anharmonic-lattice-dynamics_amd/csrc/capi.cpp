@@ -20,6 +20,7 @@ int eig_run(alt3_ctx *ctx);
 int v3_prepare(alt3_ctx *ctx);
 int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync);
 int v3_copy_out(alt3_ctx *ctx, double *host);
+bool v3_mixed_fits(const alt3_ctx *ctx);   // v3_mixed.hip
 template <typename T>
 int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host);
 int fgr_run(alt3_ctx *ctx, bool sync);
@@ -204,6 +205,8 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
 {
     if (!ctx || !bytes || ntemps < 1) return fail(ALT3_E_ARG, "bad argument");
     if (precision < ALT3_V3_FP64_REF || precision > ALT3_V3_MIXED) return fail(ALT3_E_ARG, "unknown precision");
+    // cells whose mixed tile does not fit in LDS run the FP64 path (v3_run): their footprint too
+    if (precision == ALT3_V3_MIXED && !v3_mixed_fits(ctx)) precision = ALT3_V3_FP64;
     const long long nk = ctx->nk, nb = ctx->nb, ni = ctx->nibz, ndr = ctx->V.ndr, nt = std::min(ntemps, 4);
     const long long nel = ni * nb * nb * nb * nk, n = ni * nb * 3;
     long long b = 0;

@@ -163,7 +163,11 @@ static size_t mixed_lds(const alt3_ctx *ctx, int t)
     return (size_t)t * (Kp + Np + 2 * nb2) * sizeof(float);
 }
 
-bool v3_mixed_fits(const alt3_ctx *ctx) { return mixed_lds(ctx, 16) <= ctx->lds_max; }
+// host-only contexts have no device to ask: they assume the 160 KB of a gfx950 workgroup
+bool v3_mixed_fits(const alt3_ctx *ctx)
+{
+    return mixed_lds(ctx, 16) <= (ctx->host_only ? (size_t)163840 : ctx->lds_max);
+}
 
 int v3_mixed_run(alt3_ctx *ctx, int ksub, int *d_err)
 {

@@ -18,9 +18,9 @@ from . import capi, models, refio, serving
 CELLS = ("sw-si", "zb4", "rs8", "rs16")
 # largest even mesh (at most 16^3) per cell in the request stream whose device footprint
 # (alt3_footprint, one temperature) fits one 288 GB device: mixed V3 -- SW-Si 16^3 3.5 GB,
-# zb4 16^3 38.5 GB, rs8 16^3 119 GB, rs16 10^3 185 GB (12^3: 481 GB); FP64 -- rs8 16^3
-# 233 GB, rs16 8^3 108 GB (10^3: 348 GB)
-MESH_CAP = {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 10}
+# zb4 16^3 38.5 GB, rs8 16^3 119 GB, rs16 8^3 108 GB (48 branches run the FP64 path, 10^3:
+# 348 GB); FP64 -- rs8 16^3 233 GB, rs16 8^3 108 GB
+MESH_CAP = {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 8}
 MESH_CAP_FP64 = {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 8}
 
 

@@ -3,7 +3,8 @@
 CPU: footprints come from host-only contexts and grow with the mesh, the branches and the
 number of temperatures; a request larger than the budget is refused without touching the
 device; the request stream draws meshes up to the lifted caps (16^3 for every cell whose
-footprint fits, 10^3 for the 16-atom cell).
+footprint fits, 8^3 for the 16-atom cell, whose 48 branches run the FP64 path: 348 GB at
+10^3); a mixed request on that cell is charged the FP64 footprint.
 GPU: a mixed stream of uneven requests (meshes 4^3-8^3, one to three temperatures, FP64 and
 mixed V3, tetrahedron and Gaussian delta) run concurrently through the server gives, for
 every request, the kappa the same request gives alone on a fresh context (1e-9 relative;
@@ -154,12 +155,21 @@ def test_footprint_bounds_device_allocations(si_inputs):
         assert c.device_bytes() <= est
 
 
+def test_mixed_footprint_of_a_48_branch_cell_is_the_fp64_one(cells, tmp_path_factory):
+    inp = inputs_for(cells["rs16"], 2, tmp_path_factory, "fp_rs16")
+    with capi.Context(inp, host_only=True) as h:
+        assert h.footprint(1, capi.V3_MIXED) == h.footprint(1, capi.V3_FP64)
+    inp = inputs_for(cells["rs8"], 8, tmp_path_factory, "fp_rs8")
+    with capi.Context(inp, host_only=True) as h:   # 24 branches: FP32 |V3|^2 and P
+        assert h.footprint(1, capi.V3_MIXED) < 0.6 * h.footprint(1, capi.V3_FP64)
+
+
 def test_stream_draws_up_to_the_lifted_caps(monkeypatch):
     monkeypatch.setattr(workloads, "cell_inputs", lambda cell, n: (cell, n))
     monkeypatch.setattr(serving, "Request", lambda key, inp, temps, precision: inp)
     got = workloads.serving_requests(400, 0, capi.V3_MIXED)
     top = {c: max(n for cc, n in got if cc == c) for c in workloads.CELLS}
-    assert top == {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 10}
+    assert top == {"sw-si": 16, "zb4": 16, "rs8": 16, "rs16": 8}
     got = workloads.serving_requests(400, 0, capi.V3_FP64)
     assert max(n for c, n in got if c == "rs16") == 8 and max(n for c, n in got if c == "rs8") == 16
 
