@@ -139,6 +139,10 @@ int v3_tile_for(alt3_ctx *ctx, bool fp32, int *tq);
 int v3_batch_launch(alt3_ctx *const *ctxs, int n, bool fp32, int tq);
 int v3_chain_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
                  int *d_err);
+bool v3_chain16_supported(const alt3_ctx *ctx);
+size_t v3_chain16_pack_floats(int nk, int ksub);
+int v3_chain16_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
+                   int *d_err);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
 int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
@@ -198,6 +202,7 @@ struct alt3_ctx {
 
     // FGR
     int delta_kind = ALT3_DELTA_TETRA;             // used by alt3_solve
+    int v3_mma = ALT3_MMA_F16X3;                   // matrix-core arithmetic of the 24-branch mixed V3
     double delta_sigma = 0;                        // Gaussian width, cm^-1
     int fgr_kind = ALT3_DELTA_TETRA;               // what d_p1/d_p2 hold
     double fgr_sigma = 0;

@@ -216,9 +216,10 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
     if (precision == ALT3_V3_MIXED) {
         const long long Kp = ((2 * ndr + 3) / 4) * 4, Np = ((2 * nb * nb + 15) / 16) * 16;
         b += nel * 4 + ni * nb * Kp * Np * 4 + nk * nb * nb * 8 + nk * ndr * 8;
-        if (nb == 24) {                                                                         // k_v3_chain packs
+        if (nb == 24) {                          // k_v3_chain packs (FP32) or k_v3_chain16's, the larger
             const long long k0 = Kp / 4, ks = k0 <= 8 ? 8 : k0 <= 12 ? 12 : k0 <= 14 ? 14 : 16;
-            b += nk * 60 * 64 * 4 + ni * nb * ks * 36 * 64 * 4;
+            b += std::max(nk * 60 * 64 * 4 + ni * nb * ks * 36 * 64 * 4,
+                          (long long)alt3::v3_chain16_pack_floats((int)nk, (int)ni) * 4);
         }
     } else {
         b += nel * 8;
@@ -456,6 +457,15 @@ int alt3_set_delta(alt3_ctx *ctx, int kind, double sigma)
     if (kind == ALT3_DELTA_GAUSS && !(sigma > 0)) return fail(ALT3_E_ARG, "Gaussian width must be positive");
     ctx->delta_kind = kind;
     ctx->delta_sigma = sigma;
+    return ALT3_OK;
+}
+
+int alt3_set_v3_mma(alt3_ctx *ctx, int kind)
+{
+    if (!ctx) return fail(ALT3_E_ARG, "null context");
+    if (kind != ALT3_MMA_F16X3 && kind != ALT3_MMA_F32) return fail(ALT3_E_ARG, "unknown matrix-core kind");
+    if (kind != ctx->v3_mma) ctx->have_v3 = ctx->have_fgr = ctx->have_tau = ctx->have_cm = false;
+    ctx->v3_mma = kind;
     return ALT3_OK;
 }
 

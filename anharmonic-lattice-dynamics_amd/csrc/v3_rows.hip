@@ -325,7 +325,10 @@ int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err)
     if (!fp32) {
         ALT3_TRY(dispatch_rows<double>(ctx, ksub, ctx->d_eph.p, ctx->d_evec.p, wq.p, ctx->d_gp.p, ctx->d_v33.p, d_err));
     } else if (chain) {   // 24 branches: every contraction on the matrix cores
-        ALT3_TRY(v3_chain_run(ctx, ksub, ctx->d_evf.p, ctx->d_ephf.p, wq.p, ctx->d_v33f.p, d_err));
+        if (ctx->v3_mma == ALT3_MMA_F16X3 && v3_chain16_supported(ctx))
+            ALT3_TRY(v3_chain16_run(ctx, ksub, ctx->d_evf.p, ctx->d_ephf.p, wq.p, ctx->d_v33f.p, d_err));
+        else
+            ALT3_TRY(v3_chain_run(ctx, ksub, ctx->d_evf.p, ctx->d_ephf.p, wq.p, ctx->d_v33f.p, d_err));
     } else {
         ALT3_TRY(dispatch_rows<float>(ctx, ksub, ctx->d_ephf.p, ctx->d_evf.p, wq.p, ctx->d_gpf32.p, ctx->d_v33f.p, d_err));
     }

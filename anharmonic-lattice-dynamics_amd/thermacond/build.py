@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libalt3.so")
 OBJ_DIR = os.path.join(PKG_DIR, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["host_tables.cpp", "isotopes.cpp", "capi.cpp", "eig.hip", "v3.hip", "v3_mixed.hip", "v3_rows.hip", "v3_chain.hip", "fgr.hip", "bte.hip",
+SOURCES = ["host_tables.cpp", "isotopes.cpp", "capi.cpp", "eig.hip", "v3.hip", "v3_mixed.hip", "v3_rows.hip", "v3_chain.hip", "v3_chain16.hip", "fgr.hip", "bte.hip",
            "self.hip", "cg_static.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + INCLUDE, "-I" + CSRC]

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libalt3.so")
 
 V3_FP64_REF, V3_FP64, V3_MIXED = 0, 1, 2
 DELTA_TETRA, DELTA_GAUSS = 0, 1
+MMA_F16X3, MMA_F32 = 0, 1
 THZ_TO_CM = 33.35640951981521  # 1 THz in cm^-1
 
 ERRORS = {-1: "ALT3_E_ARG", -2: "ALT3_E_HIP", -3: "ALT3_E_MESH", -4: "ALT3_E_STATE", -5: "ALT3_E_DEN0",
@@ -48,7 +49,7 @@ EXPORTS = ["alt3_create", "alt3_destroy", "alt3_last_error", "alt3_device_bytes"
            "alt3_collision", "alt3_cg", "alt3_kappa_bte", "alt3_solve", "alt3_last_kernel_ms", "alt3_fgr_gauss",
            "alt3_set_delta", "alt3_footprint", "alt3_update", "alt3_kappa_iter", "alt3_self_w3", "alt3_self_w4", "alt3_set_fgr",
            "alt3_set_iso", "alt3_isotope_g", "alt3_device_free", "alt3_cg_static", "alt3_kernel_ms",
-           "alt3_v3_batch", "alt3_get_v3sq"]
+           "alt3_v3_batch", "alt3_get_v3sq", "alt3_set_v3_mma"]
 
 # read_params (rest.f90:36,44): max_iter = 150, update_mix = 0.4; the job-3 stop normk < 0.0001
 # (kappa.f90:1476).  Both constants are REAL*4 literals.
@@ -274,6 +275,11 @@ class Context:
     def set_delta(self, kind: int, sigma: float = 0.0):
         _check(lib().alt3_set_delta(self.h, kind, C.c_double(sigma)))
 
+    def set_v3_mma(self, kind: int):
+        """Matrix-core arithmetic of the mixed V3 on 24-branch cells: MMA_F16X3 (split FP16,
+        the default) or MMA_F32."""
+        _check(lib().alt3_set_v3_mma(self.h, kind))
+
     def tau(self, tempk: float):
         ksub = self._ksub()
         outs = [np.zeros((ksub, self.nb), order="F") for _ in range(6)]
@@ -366,6 +372,7 @@ class Context:
         t = np.ascontiguousarray(temps, dtype=np.float64)
         kr_f = np.zeros(nt * self.nb * 9); kb_f = np.zeros_like(kr_f)
         _check(lib().alt3_solve(self.h, nt, _p(t), precision, _p(kr_f), _p(kb_f), _p(its), _p(ms)))
+        self._last_ksub = self.nibz     # the solve leaves the whole zone's arrays in the context
         for i in range(nt):
             kr[i] = kr_f[i * self.nb * 9:(i + 1) * self.nb * 9].reshape((self.nb, 3, 3), order="F")
             kb[i] = kb_f[i * self.nb * 9:(i + 1) * self.nb * 9].reshape((self.nb, 3, 3), order="F")

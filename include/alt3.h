@@ -35,6 +35,12 @@ extern "C" {
                                 MFMA and the contractions on VALU (k_v3_mfma); cells whose
                                 16-point MFMA tile exceeds the LDS (nb > ~28) run the FP64 path */
 
+/* matrix-core arithmetic of the mixed V3 on 24-branch cells (alt3_set_v3_mma) */
+#define ALT3_MMA_F16X3 0     /* FP16 MFMA on split operands: x = hi + lo in two halves, three
+                                products hi.hi + hi.lo + lo.hi per product, FP32 accumulation
+                                (k_v3_chain16) */
+#define ALT3_MMA_F32 1       /* FP32 MFMA (k_v3_chain) */
+
 /* delta-function integration of the scattering rates */
 #define ALT3_DELTA_TETRA 0   /* linear tetrahedra, weight_tet (tetrahedron.f90:188-384) */
 #define ALT3_DELTA_GAUSS 1   /* Gaussian delta_g (extratools.f90:239-246) at the mesh points */
@@ -159,6 +165,11 @@ int alt3_set_fgr(alt3_ctx *ctx, int ibz_lo, int ibz_hi, const double *P1sm, cons
 /* Delta integration used by alt3_solve: ALT3_DELTA_TETRA (default) or ALT3_DELTA_GAUSS with
  * width sigma (cm^-1). */
 int alt3_set_delta(alt3_ctx *ctx, int kind, double sigma);
+
+/* Matrix-core arithmetic of the mixed-precision V3 on 24-branch cells: ALT3_MMA_F16X3 (default)
+ * or ALT3_MMA_F32.  No reference counterpart: the reference computes V3 in FP64
+ * (matrix_elt_simplified, THERMACOND/sub.f90:5888-6011); both kinds are its batched form. */
+int alt3_set_v3_mma(alt3_ctx *ctx, int kind);
 /* Occupations and scattering rates at tempk: distribution (sub.f90:1955-1980),
  * calculate_tauRTA_mpi2 (:7953-8194), Qandtau (:8198-8305).  Outputs Fortran (ksub,nb):
  * Q, QN, QU, tauinv_N, tauinv_U, tauinv_tot (1/s).  Any may be NULL. */
