@@ -203,11 +203,19 @@ __global__ void k_c16_pack_g(int ksub, int ndr, const double *__restrict__ gp, c
     o[64] = lo;
 }
 
+// lane l ^ 8 of the row of 16 lanes (DPP row_ror:8), for the Im V columns of the V tiles
+__device__ __forceinline__ float xor8(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x128, 0xf, 0xf, false));
+}
+
 // One workgroup of 8 waves per (q1 = s, tile of CH x 16 mesh points q2), looping over l1, as
 // k_v3_chain: the G' operands of (s, l1) in the registers of the waves that use them for all CH
 // chunks; per chunk the phase GEMM into LDS (as split halves), barrier, U and V per point with
-// |V3|^2 into the staging area, barrier, rows out.
-__global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk, int ibz_lo, int ndr,
+// |V3|^2 into the staging area, barrier, rows out.  Work items (tile, s) are dealt to the XCDs in
+// contiguous ranges, tile-major: the workgroups resident on one XCD at a time hold one or two
+// tiles of points and read their eigenvector operands (20 KB a point) from that XCD's L2.
+__global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk, int ibz_lo, int ksub, int ndr,
                                                        const int *__restrict__ mapinv, const float *__restrict__ ephf,
                                                        const h8 *__restrict__ Gh, const h8 *__restrict__ EU,
                                                        const h8 *__restrict__ EV, const double *__restrict__ wq,
@@ -218,14 +226,20 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
     _Float16 *Phi = reinterpret_cast<_Float16 *>(lds_raw);              // [QT][PSTR]
     float *Ost = reinterpret_cast<float *>(Phi + QT * PSTR);             // [QT][OS]
     h8 *Aph = reinterpret_cast<h8 *>(Ost + QT * OS);                     // [CH][KT][4][64]
-    __shared__ double w2s[QT][NB], w3s[QT][NB];
+    __shared__ float w2s[QT][NB], w3s[QT][NB];     // |lambda|^(-1/2) at q2 and q3 (FP32)
     __shared__ int k3s[CH * QT];
+    __shared__ int bad;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int s = blockIdx.y, q00 = blockIdx.x * (CH * QT);
+    // blocks go round-robin to the 8 XCDs: XCD x runs items x*per .. x*per + per - 1 in order
+    const int ntile = (nk + CH * QT - 1) / (CH * QT), nitem = ntile * ksub, per = (nitem + 7) / 8;
+    const int item = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (item >= nitem) return;
+    const int s = item % ksub, q00 = (item / ksub) * (CH * QT);
     const int nch = min(CH, (nk - q00 + QT - 1) / QT);
     const int k1 = mapinv[ibz_lo + s];
     int a0, a1, a2;
     m.coords(k1, a0, a1, a2);
+    if (tid == 0) bad = 0;
     if (tid < CH * QT) {
         int b0, b1, b2;
         m.coords(min(q00 + tid, nk - 1), b0, b1, b2);
@@ -275,12 +289,39 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                 gl[j][t] = G[((size_t)(t * NTP + nt) * 2 + 1) * 64];
             }
     };
+    // e3 operands of one point loaded one step ahead of their use (before the barrier for the
+    // wave's first point of a chunk, during V for the next); e2 operands where V starts (the
+    // compiler issues them among U's products)
+    h8 puh[KT][2], pul[KT][2], pvh[KT][3], pvl[KT][3];
+    auto load_u = [&](int ch, int q) {
+        const h8 *eu = EU + (size_t)k3s[ch * QT + q] * EUF * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < KT; t++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                puh[t][nt] = eu[((t * 2 + nt) * 2) * 64];
+                pul[t][nt] = eu[((t * 2 + nt) * 2 + 1) * 64];
+            }
+    };
+    auto load_v = [&](int ch, int q) {
+        const h8 *ev = EV + (size_t)min(q00 + ch * QT + q, nk - 1) * EVF * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < KT; t++)
+#pragma unroll
+            for (int nv = 0; nv < 3; nv++) {
+                pvh[t][nv] = ev[((t * 3 + nv) * 2) * 64];
+                pvl[t][nv] = ev[((t * 3 + nv) * 2 + 1) * 64];
+            }
+    };
     __syncthreads();
     for (int l1 = 0; l1 < NB; l1++) {
+        load_g(l1);
         const double w1 = wq[(size_t)k1 * NB + l1];
         const float sg = g_scale(bound + s * NB + l1);
-        const double cs = c33sq8 * (64.0 / ((double)sg * (double)sg));   // (sg / 8)^-2: the scales
-        load_g(l1);
+        // const33^2/8 |l1|^(-1/2) (sg / 8)^-2 (the scales come out); a zero eigenvalue in a
+        // triple (w = inf) is flagged from the tables, every triple of the tile being computed
+        const float cs = (float)(c33sq8 * (64.0 / ((double)sg * (double)sg)) * w1 * w1);
+        if (!(w1 <= 1.7e308) && tid == 0) bad = 1;
         for (int ch = 0; ch < nch; ch++) {
             const int q0 = q00 + ch * QT;
             // ---- phase GEMM of the chunk: D[(b,c)][q] at lane q + 16 (m/4), m = 4 grp + i
@@ -323,19 +364,19 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     *reinterpret_cast<h4 *>(dst + 3 * PLANE) = il;
                 }
             }
+            load_u(ch, wave * QW);
             for (int x = tid; x < QT * NB; x += NW * 64) {
                 const int q = x / NB, l = x % NB;
-                w2s[q][l] = wq[(size_t)min(q0 + q, nk - 1) * NB + l];
-                w3s[q][l] = wq[(size_t)k3s[ch * QT + q] * NB + l];
+                const double a = wq[(size_t)min(q0 + q, nk - 1) * NB + l], b = wq[(size_t)k3s[ch * QT + q] * NB + l];
+                if (!(a <= 1.7e308) || !(b <= 1.7e308)) bad = 1;
+                w2s[q][l] = (float)(a * a);
+                w3s[q][l] = (float)(b * b);
             }
             __syncthreads();
             // ---- U and V, one point at a time per wave
             for (int jq = 0; jq < QW; jq++) {
                 const int q = wave * QW + jq;
-                const int n2c = min(q0 + q, nk - 1);
                 const _Float16 *ph = Phi + q * PSTR;
-                const h8 *eu = EU + (size_t)k3s[ch * QT + q] * EUF * 64 + lane;
-                const h8 *ev = EV + (size_t)n2c * EVF * 64 + lane;
                 f4 u[3][2];
 #pragma unroll
                 for (int a = 0; a < 3; a++)
@@ -343,12 +384,6 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     for (int b = 0; b < 2; b++) u[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < KT; t++) {
-                    h8 bh[2], bl[2];
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++) {
-                        bh[nt] = eu[((t * 2 + nt) * 2) * 64];
-                        bl[nt] = eu[((t * 2 + nt) * 2 + 1) * 64];
-                    }
                     // block kb = 4t + grp of k = (p',c): p' = kb / 3, c = 8 (kb % 3) .. + 7
                     const int kb = 4 * t + grp, kbc = kb < 6 ? kb : 5, pp = kbc / 3, c0 = 8 * (kbc % 3);
 #pragma unroll
@@ -367,10 +402,12 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                             al = flip_sign(al, mask);
                         }
 #pragma unroll
-                        for (int nt = 0; nt < 2; nt++) u[mt][nt] = mma3(ah, al, bh[nt], bl[nt], u[mt][nt]);
+                        for (int nt = 0; nt < 2; nt++) u[mt][nt] = mma3(ah, al, puh[t][nt], pul[t][nt], u[mt][nt]);
                     }
                 }
+                if (jq + 1 < QW) load_u(ch, q + 1);
                 // V: the U accumulators, scaled by 2^-3 and split, are the A operands
+                load_v(ch, q);
                 f4 v[2][3];
 #pragma unroll
                 for (int a = 0; a < 2; a++)
@@ -378,12 +415,6 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     for (int b = 0; b < 3; b++) v[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < KT; t++) {
-                    h8 bh[3], bl[3];
-#pragma unroll
-                    for (int nv = 0; nv < 3; nv++) {
-                        bh[nv] = ev[((t * 3 + nv) * 2) * 64];
-                        bl[nv] = ev[((t * 3 + nv) * 2 + 1) * 64];
-                    }
 #pragma unroll
                     for (int mv = 0; mv < 2; mv++) {
                         h8 ah, al;
@@ -398,7 +429,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                             al[j] = b;
                         }
 #pragma unroll
-                        for (int nv = 0; nv < 3; nv++) v[mv][nv] = mma3(ah, al, bh[nv], bl[nv], v[mv][nv]);
+                        for (int nv = 0; nv < 3; nv++) v[mv][nv] = mma3(ah, al, pvh[t][nv], pvl[t][nv], v[mv][nv]);
                     }
                 }
                 // |V|^2: element i of V tile (mv, nv) at lane l is column n = 16 nv + l%16 of row
@@ -411,21 +442,12 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                         const int l3 = 16 * mv + 4 * grp + i;
                         const float s0 = v[mv][0][i] * v[mv][0][i], s1 = v[mv][1][i] * v[mv][1][i],
                                     s2 = v[mv][2][i] * v[mv][2][i];
-                        const float y1 = __shfl_xor(s1, 8, 64), y2 = __shfl_xor(s2, 8, 64);
+                        const float y1 = xor8(s1), y2 = xor8(s2);
                         if (l3 < NB) {
-                            const double w13 = w1 * w3s[q][l3];
-                            {   // l2 = col
-                                const float v2 = s0 + (col < 8 ? y1 : y2);
-                                const double w = w13 * w2s[q][col];
-                                if (!(w <= 1.7e308)) atomicExch(err, 1);   // a zero eigenvalue in the triple
-                                Ost[q * OS + col * 25 + l3] = v2 * (float)(cs * w * w);
-                            }
-                            if (col < 8) {   // l2 = 16 + col
-                                const float v2 = s1 + y2;
-                                const double w = w13 * w2s[q][16 + col];
-                                if (!(w <= 1.7e308)) atomicExch(err, 1);
-                                Ost[q * OS + (16 + col) * 25 + l3] = v2 * (float)(cs * w * w);
-                            }
+                            const float c13 = cs * w3s[q][l3];
+                            Ost[q * OS + col * 25 + l3] = (s0 + (col < 8 ? y1 : y2)) * (c13 * w2s[q][col]);   // l2 = col
+                            if (col < 8)   // l2 = 16 + col
+                                Ost[q * OS + (16 + col) * 25 + l3] = (s1 + y2) * (c13 * w2s[q][16 + col]);
                         }
                     }
             }
@@ -442,6 +464,8 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
             }
         }
     }
+    __syncthreads();
+    if (tid == 0 && bad) atomicExch(err, 1);   // a zero eigenvalue in a triple (den = 0)
 }
 
 }  // namespace
@@ -487,11 +511,12 @@ int v3_chain16_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf,
     if (lds + 2 * QT * NB * sizeof(double) + CH * QT * sizeof(int) + 64 > ctx->lds_max)
         return fail(ALT3_E_ARG, "FP16 split chain tile does not fit in LDS");
     const double c33 = const33();
-    dim3 grid((unsigned)((nk + CH * QT - 1) / (CH * QT)), (unsigned)ksub);
+    const int nitem = ((nk + CH * QT - 1) / (CH * QT)) * ksub;
+    dim3 grid((unsigned)(8 * ((nitem + 7) / 8)));
     ALT3_HIP(hipFuncSetAttribute((const void *)k_v3_chain16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     kernel_mark(ctx, 0, false);
     hipLaunchKernelGGL(k_v3_chain16, grid, dim3(NW * 64), lds, ctx->stream, MeshDev::make(ctx->nc), nk, ctx->ibz_lo,
-                       ndr, ctx->d_mapinv.p, ephf, Gh, EU, EV, wq, c33 * c33 / 8, bound, out, d_err);
+                       ksub, ndr, ctx->d_mapinv.p, ephf, Gh, EU, EV, wq, c33 * c33 / 8, bound, out, d_err);
     ALT3_HIP(hipGetLastError());
     kernel_mark(ctx, 0, true);
     return ALT3_OK;
