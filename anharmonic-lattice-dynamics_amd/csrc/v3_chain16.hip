@@ -23,12 +23,13 @@
 // n + 16 (m / 4), element m % 4.  A and B see the same k order, so each operand below is packed
 // in one order of its own choosing (stated with it) and its partner in the same order.
 //
-// Range.  FP16 holds magnitudes up to 65504.  G' of each (q1, l1) is scaled by its own
-// sg = 2^(14 - e), where 2^e bounds max_{b,c} sum_r (|Re G'_r[b,c]| + |Im G'_r[b,c]|) over that
-// (q1, l1) (k_c16_bound), so |Phi'| <= 2^14; U is bounded by |Phi' row|_2 <= sqrt(24) 2^14 and is
-// scaled by 2^-3 before it is split; phases and eigenvector components are at most 1.  The
-// scales are powers of two and come out of |V|^2 exactly.  Halves below 2^-14 (subnormal) carry
-// at least 2^-24 absolute, 2^-38 of the row's bound.  The scale is per (q1, l1) because a row can
+// Range.  FP16 holds magnitudes up to 65504, normal down to 2^-14, subnormal in steps of 2^-24.
+// The eigenvector operands e2, e3 are packed times 2^4 (components up to 16, so that the halves
+// of small components stay well above the subnormal steps); G' of each (q1, l1) is scaled by its
+// own sg = 2^(7 - e), where 2^e bounds max_{b,c} sum_r (|Re G'_r[b,c]| + |Im G'_r[b,c]|) over
+// that (q1, l1) (k_c16_bound), so |Phi'| <= 2^7 (phases are at most 1), and the U products,
+// bounded by |Phi' row|_2 |16 e3 col|_2 <= sqrt(24) 2^11, are split as they come.  The scales
+// are powers of two and come out of |V|^2 exactly (V carries 2^8 sg).  The scale is per (q1, l1) because a row can
 // be far below the others: at q1 = Gamma the acoustic l1 give G' at the round-off of the sum rule
 // (1e-16 of the optical rows), which one global scale would flush to zero, leaving Q = 0 for
 // those modes where the FP64 and FP32 paths keep a round-off value.
@@ -80,9 +81,10 @@ constexpr int OS = NB * 25 + 1;              // one point of the FP32 staging ar
 
 // e3 and e2 operands of every mesh point from the FP32 eigenvectors evf [k][l col][b comp]
 // (complex interleaved), as halves hi then lo, 16 bytes per lane:
-//   EU[k][t][nt][hl][lane]: B_U[k = (p',c)][l3 = 16 nt + lane%16]; lane group g of step t holds
+//   EU[k][t][nt][hl][lane]: 16 B_U[k = (p',c)][l3 = 16 nt + lane%16]; lane group g of step t holds
 //     the block kb = 4t + g: p' = kb / 3, c = 8 (kb % 3) + j (zero for kb >= 6 and l3 >= 24)
-//   EV[k][t][nv][hl][lane]: B_V[k = (p,b)][(p'',l2) = 16 nv + lane%16]; half j of lane group g
+//   EV[k][t][nv][hl][lane]: 16 B_V[k = (p,b)][(p'',l2)], column lane%16 of tile nv: Re V(l2 = 0..15)
+//     (nv = 0), Im V(l2 = 0..15) (nv = 1), Re V(16..23) | Im V(16..23) (nv = 2); half j of lane group g
 //     holds row (p,b) = 4g + j (t = 0, j < 4), 16 + 4g + j - 4 (t = 0, j >= 4), 32 + 4g + j
 //     (t = 1, j < 4) -- the rows the U accumulators hold -- and zero for t = 1, j >= 4
 constexpr int EUF = KT * 2 * 2;      // 16-byte operand slots per lane and point (EU)
@@ -108,7 +110,7 @@ __global__ void k_c16_pack_e(int nk, const float *__restrict__ evf, h8 *__restri
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             float v = 0.0f;
-            if (kb < 6 && l3 < NB) v = e[2 * (l3 * NB + 8 * (kb % 3) + j) + kb / 3];
+            if (kb < 6 && l3 < NB) v = e[2 * (l3 * NB + 8 * (kb % 3) + j) + kb / 3] * 16.0f;
             _Float16 a, b;
             split(v, a, b);
             hi[j] = a;
@@ -119,7 +121,8 @@ __global__ void k_c16_pack_e(int nk, const float *__restrict__ evf, h8 *__restri
         o[64] = lo;
     } else {
         const int fr = frag - KT * 2, t = fr / 3, nv = fr % 3;
-        const int n = 16 * nv + (lane & 15), p2 = n / NB, l2 = n % NB;
+        // columns: Re V(l2 = 0..15), Im V(l2 = 0..15), then Re V(16..23) | Im V(16..23)
+        const int cl = lane & 15, p2 = nv < 2 ? nv : (cl >= 8 ? 1 : 0), l2 = nv < 2 ? cl : 16 + (cl & 7);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int row = ev_row(t, g, j);
@@ -127,7 +130,7 @@ __global__ void k_c16_pack_e(int nk, const float *__restrict__ evf, h8 *__restri
             if (row >= 0) {
                 const int p = row / NB, b = row % NB;
                 const float er = e[2 * (l2 * NB + b)], ei = e[2 * (l2 * NB + b) + 1];
-                v = (p == p2) ? er : (p == 0 ? ei : -ei);
+                v = ((p == p2) ? er : (p == 0 ? ei : -ei)) * 16.0f;
             }
             _Float16 a, b;
             split(v, a, b);
@@ -167,7 +170,7 @@ __device__ __forceinline__ float g_scale(const unsigned *bound)
     if (!(b > 0.0f)) return 1.0f;
     int e;
     frexpf(b, &e);   // b < 2^e
-    return ldexpf(1.0f, 14 - e);
+    return ldexpf(1.0f, 7 - e);
 }
 
 // phase A operands from G' [s][r][l1][(b,c)] (FP64 complex), scaled by sg of (s, l1):
@@ -318,9 +321,9 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
         load_g(l1);
         const double w1 = wq[(size_t)k1 * NB + l1];
         const float sg = g_scale(bound + s * NB + l1);
-        // const33^2/8 |l1|^(-1/2) (sg / 8)^-2 (the scales come out); a zero eigenvalue in a
+        // const33^2/8 |l1|^(-1/2) (2^8 sg)^-2 (the scales come out); a zero eigenvalue in a
         // triple (w = inf) is flagged from the tables, every triple of the tile being computed
-        const float cs = (float)(c33sq8 * (64.0 / ((double)sg * (double)sg)) * w1 * w1);
+        const float cs = (float)(c33sq8 / (65536.0 * (double)sg * (double)sg) * w1 * w1);
         if (!(w1 <= 1.7e308) && tid == 0) bad = 1;
         for (int ch = 0; ch < nch; ch++) {
             const int q0 = q00 + ch * QT;
@@ -389,24 +392,19 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
 #pragma unroll
                     for (int mt = 0; mt < 3; mt++) {
                         const int row = 16 * mt + col, p = row >= NB ? 1 : 0, b = row - p * NB;
-                        // Re Phi' on the diagonal blocks, -Im Phi' above, Im Phi' below
+                        // Re Phi' on the diagonal blocks, -Im Phi' above, Im Phi' below; the lanes
+                        // of the zero blocks (kb >= 6) read finite halves of block 5, which meet
+                        // zero e3 operands
                         const _Float16 *src = ph + (p == pp ? 0 : 2 * PLANE) + b * NB + c0;
-                        h8 ah = *reinterpret_cast<const h8 *>(src);
-                        h8 al = *reinterpret_cast<const h8 *>(src + PLANE);
-                        const unsigned mask = (kb >= 6) ? 0u : (p == 0 && pp == 1) ? 0x80008000u : 0u;
-                        if (kb >= 6) {
-                            ah = h8{};
-                            al = h8{};
-                        } else {
-                            ah = flip_sign(ah, mask);
-                            al = flip_sign(al, mask);
-                        }
+                        const unsigned mask = (p == 0 && pp == 1) ? 0x80008000u : 0u;
+                        const h8 ah = flip_sign(*reinterpret_cast<const h8 *>(src), mask);
+                        const h8 al = flip_sign(*reinterpret_cast<const h8 *>(src + PLANE), mask);
 #pragma unroll
                         for (int nt = 0; nt < 2; nt++) u[mt][nt] = mma3(ah, al, puh[t][nt], pul[t][nt], u[mt][nt]);
                     }
                 }
                 if (jq + 1 < QW) load_u(ch, q + 1);
-                // V: the U accumulators, scaled by 2^-3 and split, are the A operands
+                // V: the U accumulators, split, are the A operands
                 load_v(ch, q);
                 f4 v[2][3];
 #pragma unroll
@@ -424,7 +422,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                             if (t == 0) xv = j < 4 ? u[0][mv][j] : u[1][mv][j - 4];
                             else xv = j < 4 ? u[2][mv][j] : 0.0f;
                             _Float16 a, b;
-                            split(xv * 0.125f, a, b);
+                            split(xv, a, b);
                             ah[j] = a;
                             al[j] = b;
                         }
@@ -432,22 +430,27 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                         for (int nv = 0; nv < 3; nv++) v[mv][nv] = mma3(ah, al, pvh[t][nv], pvl[t][nv], v[mv][nv]);
                     }
                 }
-                // |V|^2: element i of V tile (mv, nv) at lane l is column n = 16 nv + l%16 of row
-                // l3 = 16 mv + 4(l/16) + i; n < 24 is Re V(l2 = n), n >= 24 is Im V(l2 = n - 24),
-                // found at column (n + 8) % 16 of the next tile(s): lane l ^ 8
+                // |V|^2: element i of V tile (mv, nv) at lane l is row l3 = 16 mv + 4(l/16) + i; tiles
+                // 0 and 1 hold Re and Im V of l2 = l%16, tile 2 Re V(16 + l%16) in lanes l%16 < 8
+                // and the matching Im V in lane l ^ 8
+                const float wa = w2s[q][col], wb = w2s[q][16 + (col & 7)];
+                float w3r[2][4];
+#pragma unroll
+                for (int mv = 0; mv < 2; mv++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) w3r[mv][i] = cs * w3s[q][min(16 * mv + 4 * grp + i, NB - 1)];
 #pragma unroll
                 for (int mv = 0; mv < 2; mv++)
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const int l3 = 16 * mv + 4 * grp + i;
-                        const float s0 = v[mv][0][i] * v[mv][0][i], s1 = v[mv][1][i] * v[mv][1][i],
-                                    s2 = v[mv][2][i] * v[mv][2][i];
-                        const float y1 = xor8(s1), y2 = xor8(s2);
+                        const float sa = fmaf(v[mv][1][i], v[mv][1][i], v[mv][0][i] * v[mv][0][i]);
+                        const float s2 = v[mv][2][i] * v[mv][2][i];
+                        const float sb = s2 + xor8(s2);
                         if (l3 < NB) {
-                            const float c13 = cs * w3s[q][l3];
-                            Ost[q * OS + col * 25 + l3] = (s0 + (col < 8 ? y1 : y2)) * (c13 * w2s[q][col]);   // l2 = col
-                            if (col < 8)   // l2 = 16 + col
-                                Ost[q * OS + (16 + col) * 25 + l3] = (s1 + y2) * (c13 * w2s[q][16 + col]);
+                            Ost[q * OS + col * 25 + l3] = sa * (w3r[mv][i] * wa);   // l2 = col
+                            if (col < 8)                                               // l2 = 16 + col
+                                Ost[q * OS + (16 + col) * 25 + l3] = sb * (w3r[mv][i] * wb);
                         }
                     }
             }
