@@ -55,6 +55,18 @@ __device__ __forceinline__ void split(float x, _Float16 &hi, _Float16 &lo)
     hi = (_Float16)x;
     lo = (_Float16)(x - (float)hi);
 }
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+// two values at once: one packed conversion for the two hi halves, one for the two lo halves
+__device__ __forceinline__ void split2(float a, float b, h2 &hi, h2 &lo)
+{
+    const f2 x = {a, b};
+    hi = __builtin_convertvector(x, h2);
+    lo = __builtin_convertvector(x - __builtin_convertvector(hi, f2), h2);
+}
+__device__ __forceinline__ h4 cat(h2 a, h2 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3); }
+__device__ __forceinline__ h8 cat(h4 a, h4 b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
 __device__ __forceinline__ h8 flip_sign(h8 x, unsigned mask)
 {
     u4 u = __builtin_bit_cast(u4, x);
@@ -72,9 +84,10 @@ constexpr int QW = QT / NW;      // points per wave in the U and V products
 constexpr int CH = 4;            // chunks of QT points per workgroup
 constexpr int TPW = (NTP + NW - 1) / NW;   // phase M-tiles per wave (5: waves 0-3, 4: waves 4-7)
 constexpr int KSPL = 32;         // 2 ndr <= 64: ndr <= 32
-// LDS: Phi'[q][plane][b][c] halves, planes Re hi, Re lo, Im hi, Im lo; rows of 24 halves, a
-// point every 1160 dwords (8 mod 64: the 16 points a phase write spreads over the lanes fall on
-// distinct banks)
+// LDS: Phi'[q][plane][b][c] halves, planes Re hi, Im hi, Re lo, Im lo (the Im plane 32 banks from
+// the Re one: a U operand tile that reads Re rows b = 16..23 and Im rows b = 0..7 hits distinct
+// banks); rows of 24 halves, a point every 1160 dwords (8 mod 64: the 16 points a phase write
+// spreads over the lanes fall on distinct banks)
 constexpr int PLANE = NB * NB;               // halves per plane
 constexpr int PSTR = 4 * PLANE + 16;         // halves per point
 constexpr int OS = NB * 25 + 1;              // one point of the FP32 staging area
@@ -229,7 +242,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
     _Float16 *Phi = reinterpret_cast<_Float16 *>(lds_raw);              // [QT][PSTR]
     float *Ost = reinterpret_cast<float *>(Phi + QT * PSTR);             // [QT][OS]
     h8 *Aph = reinterpret_cast<h8 *>(Ost + QT * OS);                     // [CH][KT][4][64]
-    __shared__ float w2s[QT][NB], w3s[QT][NB];     // |lambda|^(-1/2) at q2 and q3 (FP32)
+    __shared__ float w2s[QT][NB], w3s[QT][NB];     // |lambda|^(-1/2) at q2, and at q3 times cs (FP32)
     __shared__ int k3s[CH * QT];
     __shared__ int bad;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -349,22 +362,15 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     }
                     // (b,c) = 16 nt + 4 grp + i: four consecutive c of one row b
                     _Float16 *dst = Phi + col * PSTR + 16 * nt + 4 * grp;
-                    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-                    h4 rh, rl, ih, il;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        _Float16 a, b;
-                        split(cr[i], a, b);
-                        rh[i] = a;
-                        rl[i] = b;
-                        split(ci[i], a, b);
-                        ih[i] = a;
-                        il[i] = b;
-                    }
-                    *reinterpret_cast<h4 *>(dst) = rh;
-                    *reinterpret_cast<h4 *>(dst + PLANE) = rl;
-                    *reinterpret_cast<h4 *>(dst + 2 * PLANE) = ih;
-                    *reinterpret_cast<h4 *>(dst + 3 * PLANE) = il;
+                    h2 rh0, rl0, rh1, rl1, ih0, il0, ih1, il1;
+                    split2(cr[0], cr[1], rh0, rl0);
+                    split2(cr[2], cr[3], rh1, rl1);
+                    split2(ci[0], ci[1], ih0, il0);
+                    split2(ci[2], ci[3], ih1, il1);
+                    *reinterpret_cast<h4 *>(dst) = cat(rh0, rh1);
+                    *reinterpret_cast<h4 *>(dst + PLANE) = cat(ih0, ih1);
+                    *reinterpret_cast<h4 *>(dst + 2 * PLANE) = cat(rl0, rl1);
+                    *reinterpret_cast<h4 *>(dst + 3 * PLANE) = cat(il0, il1);
                 }
             }
             load_u(ch, wave * QW);
@@ -373,7 +379,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                 const double a = wq[(size_t)min(q0 + q, nk - 1) * NB + l], b = wq[(size_t)k3s[ch * QT + q] * NB + l];
                 if (!(a <= 1.7e308) || !(b <= 1.7e308)) bad = 1;
                 w2s[q][l] = (float)(a * a);
-                w3s[q][l] = (float)(b * b);
+                w3s[q][l] = cs * (float)(b * b);   // with the l1 factor
             }
             __syncthreads();
             // ---- U and V, one point at a time per wave
@@ -395,10 +401,10 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                         // Re Phi' on the diagonal blocks, -Im Phi' above, Im Phi' below; the lanes
                         // of the zero blocks (kb >= 6) read finite halves of block 5, which meet
                         // zero e3 operands
-                        const _Float16 *src = ph + (p == pp ? 0 : 2 * PLANE) + b * NB + c0;
+                        const _Float16 *src = ph + (p == pp ? 0 : PLANE) + b * NB + c0;
                         const unsigned mask = (p == 0 && pp == 1) ? 0x80008000u : 0u;
                         const h8 ah = flip_sign(*reinterpret_cast<const h8 *>(src), mask);
-                        const h8 al = flip_sign(*reinterpret_cast<const h8 *>(src + PLANE), mask);
+                        const h8 al = flip_sign(*reinterpret_cast<const h8 *>(src + 2 * PLANE), mask);
 #pragma unroll
                         for (int nt = 0; nt < 2; nt++) u[mt][nt] = mma3(ah, al, puh[t][nt], pul[t][nt], u[mt][nt]);
                     }
@@ -415,17 +421,17 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                 for (int t = 0; t < KT; t++) {
 #pragma unroll
                     for (int mv = 0; mv < 2; mv++) {
-                        h8 ah, al;
-#pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            float xv;
-                            if (t == 0) xv = j < 4 ? u[0][mv][j] : u[1][mv][j - 4];
-                            else xv = j < 4 ? u[2][mv][j] : 0.0f;
-                            _Float16 a, b;
-                            split(xv, a, b);
-                            ah[j] = a;
-                            al[j] = b;
+                        const f4 x0 = t == 0 ? u[0][mv] : u[2][mv];
+                        h2 h0, l0, h1, l1, h2_, l2_, h3, l3_;
+                        split2(x0[0], x0[1], h0, l0);
+                        split2(x0[2], x0[3], h1, l1);
+                        if (t == 0) {
+                            split2(u[1][mv][0], u[1][mv][1], h2_, l2_);
+                            split2(u[1][mv][2], u[1][mv][3], h3, l3_);
+                        } else {
+                            h2_ = l2_ = h3 = l3_ = h2{};
                         }
+                        const h8 ah = cat(cat(h0, h1), cat(h2_, h3)), al = cat(cat(l0, l1), cat(l2_, l3_));
 #pragma unroll
                         for (int nv = 0; nv < 3; nv++) v[mv][nv] = mma3(ah, al, pvh[t][nv], pvl[t][nv], v[mv][nv]);
                     }
@@ -438,7 +444,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
 #pragma unroll
                 for (int mv = 0; mv < 2; mv++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) w3r[mv][i] = cs * w3s[q][min(16 * mv + 4 * grp + i, NB - 1)];
+                    for (int i = 0; i < 4; i++) w3r[mv][i] = w3s[q][min(16 * mv + 4 * grp + i, NB - 1)];
 #pragma unroll
                 for (int mv = 0; mv < 2; mv++)
 #pragma unroll
