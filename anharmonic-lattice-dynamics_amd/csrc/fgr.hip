@@ -225,16 +225,6 @@ __device__ __forceinline__ int tet_corner(int l, int y)
     return (int)((col >> (3 * l)) & 7u);
 }
 
-// a[j] for a register array and a runtime index, as a chain of selects
-template <int N>
-__device__ __forceinline__ float pick(const float (&a)[N], int j)
-{
-    float r = a[0];
-#pragma unroll
-    for (int i = 1; i < N; i++) r = (j == i) ? a[i] : r;
-    return r;
-}
-
 // One workgroup per (s, l2, l3, process).  The corner energies e(q2) depend on l2, l3 and
 // the process but not on l1, so they are built once in LDS (with the energy range of every
 // cube kept in registers) and reused for the nb values omega(q1, l1); per l1 the LDS weight
@@ -251,12 +241,15 @@ __global__ void k_freq_t(int nk, int nb, const double *__restrict__ freq, double
 }
 
 constexpr int kFgrCase = 4096;     // LDS list per weight_tet case; one chunk of cubes (<= 6*512) always fits
+constexpr int kFgrMaxNb = 64;      // branches (bits of the per-cube l1 masks)
 constexpr int kFgrList = 3 * kFgrCase;
 
 // e inside [lo, hi] as the reference tests it (no weight outside, tetrahedron.f90:343)
 __device__ __forceinline__ bool omega_in(double e, float lo, float hi) { return !(e < (double)lo || e > (double)hi); }
 
-template <typename VT, int kFgrPer>   // kFgrPer: mesh points (and cubes) per thread
+// kFgrPer: mesh points (and cubes) per thread; MT: the per-cube l1 masks (32 bits for up to 32
+// branches, 64 for up to 64)
+template <typename VT, int kFgrPer, typename MT>
 __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4) ? 6 : 4) k_fgr(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
                                                      const double *__restrict__ freqT,
@@ -266,8 +259,10 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
     extern __shared__ double lds[];
     double *W = lds;               // [nk]
     float *en = reinterpret_cast<float *>(W + nk);     // [nk]
+    unsigned short *s_cube = reinterpret_cast<unsigned short *>(en + nk);   // [nk] cubes active for this l1
     __shared__ float s_lo[kFgrThreads / 64], s_hi[kFgrThreads / 64];
-    __shared__ int s_cnt[3];
+    __shared__ int s_cnt[3], s_ncube;
+    __shared__ double s_om[kFgrMaxNb];           // omega(q1, l1)
     __shared__ unsigned short s_list[kFgrList];  // [case][kFgrCase] of (cube << 3 | tetrahedron), nk <= 8192
     // The groups of one q1 read the same |V3|^2 rows (both processes, and the decay mirror the
     // (l3, l2) rows of the coalescence groups).  Workgroups are dealt to the eight XCDs round
@@ -359,14 +354,24 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
     }
     const double tpi2 = (2 * alt3::PI) * (2 * alt3::PI);
     VT *P = proc == 0 ? P1 : P2;
-    // tetrahedra of cube j (k = tid + j*T) whose energy range contains e go to the LDS list of
-    // their weight_tet case (e below the second-lowest corner, below the third, or above), so
-    // that the lanes of a wave take the same branch of the weight formula.  Called by every
-    // lane of the group with the same j; the list slots are claimed once per wave and case
-    // (one LDS atomic per wave instead of one per tetrahedron).
-    auto push = [&](int j, double e) {
-        const int k = tid + j * kFgrThreads;
-        const bool cube_in = k < nk && omega_in(e, pick(cmn, j), pick(cmx, j));
+    // the branches l1 whose omega(q1, l1) falls in each cube's energy range, once per item
+    // (bit l1 of cm[j] for cube k = tid + j*T), so that each l1 pass visits only its cubes
+    if (tid < nb) s_om[tid] = freq[(size_t)k1 * nb + tid];
+    __syncthreads();
+    MT cm[kFgrPer];
+#pragma unroll
+    for (int j = 0; j < kFgrPer; j++) {
+        cm[j] = 0;
+        if (tid + j * kFgrThreads < nk)
+            for (int l = 0; l < nb; l++)
+                if (omega_in(s_om[l], cmn[j], cmx[j])) cm[j] |= (MT)1 << l;
+    }
+    // tetrahedra of cube k whose energy range contains e go to the LDS list of their
+    // weight_tet case (e below the second-lowest corner, below the third, or above), so that
+    // the lanes of a wave take the same branch of the weight formula.  Called by every lane of
+    // the group (cube_in false where a lane has no cube); the list slots are claimed once per
+    // wave and case (one LDS atomic per wave instead of one per tetrahedron).
+    auto push = [&](int k, bool cube_in, double e) {
         float ce[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (cube_in) {
             int c0, c1, c2;
@@ -464,20 +469,37 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             // every lane busy.  The lists normally hold all of them; if one overflows, the
             // cubes are taken again chunk by chunk (at most 6 entries per cube and thread).
             if (tid < 3) s_cnt[tid] = 0;
+            if (tid == 0) s_ncube = 0;
             __syncthreads();
+            // the cubes of this l1, compacted (wave-aggregated appends)
+#pragma unroll
+            for (int j = 0; j < kFgrPer; j++) {
+                const int k = tid + j * kFgrThreads;
+                const bool act = k < nk && (cm[j] >> l1 & 1);
+                const unsigned long long mask = __ballot(act);
+                if (mask) {   // the same for every lane of the wave
+                    const int leader = __ffsll((long long)mask) - 1;
+                    int base = 0;
+                    if ((int)__lane_id() == leader) base = atomicAdd(&s_ncube, __popcll(mask));
+                    base = __shfl(base, leader, 64);
+                    if (act) s_cube[base + __popcll(mask & ((1ull << __lane_id()) - 1ull))] = (unsigned short)k;
+                }
+            }
+            __syncthreads();
+            const int ncube = s_ncube;
 #pragma unroll 1
-            for (int j = 0; j * kFgrThreads < nk; j++) push(j, omega1);
+            for (int x0 = 0; x0 < ncube; x0 += kFgrThreads) push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
             __syncthreads();
             const int n0 = s_cnt[0], n1 = s_cnt[1], n2 = s_cnt[2];
             if (n0 <= kFgrCase && n1 <= kFgrCase && n2 <= kFgrCase) {
                 process(n0, n1, n2, omega1);
-            } else {
+            } else {   // a list overflowed: the cubes again, T at a time (at most 6T entries a case)
 #pragma unroll 1
-                for (int j = 0; j * kFgrThreads < nk; j++) {
+                for (int x0 = 0; x0 < ncube; x0 += kFgrThreads) {
                     __syncthreads();
                     if (tid < 3) s_cnt[tid] = 0;
                     __syncthreads();
-                    push(j, omega1);
+                    push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
                     __syncthreads();
                     process(s_cnt[0], s_cnt[1], s_cnt[2], omega1);
                 }
@@ -783,7 +805,8 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     const bool mixed = ctx->v3_prec == ALT3_V3_MIXED;
-    size_t lds = (size_t)nk * (sizeof(double) + sizeof(float));
+    // W, corner energies, active-cube list
+    size_t lds = (size_t)nk * (sizeof(double) + sizeof(float) + sizeof(unsigned short));
     ALT3_TRY(alloc_p(ctx, nel));
     const MeshDev md = MeshDev::make(ctx->nc);
     ALT3_TRY(ctx->d_freqT.reserve((size_t)nk * nb));
@@ -791,7 +814,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
                        ctx->d_freqT.p);
     // the LDS kernel's list codes (cube << 3 | tetrahedron) are 16 bits wide: nk <= 8192 (20^3);
     // larger meshes take the global-scratch kernel
-    if (lds + kFgrList * sizeof(unsigned short) + 256 > ctx->lds_max || nk > 16 * kFgrThreads) {
+    if (lds + kFgrList * sizeof(unsigned short) + 1024 > ctx->lds_max || nk > 16 * kFgrThreads || nb > kFgrMaxNb) {
         if ((unsigned long long)nk * 8 > (1ull << 31)) return fail(ALT3_E_ARG, "mesh too large for the FGR codes");
         const long long nitems = 2ll * ksub * nb * nb;
         const void *kern = mixed ? (const void *)k_fgr_big<float> : (const void *)k_fgr_big<double>;
@@ -820,20 +843,26 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
-    const bool big = lds + kFgrList * sizeof(unsigned short) + 256 > 65536;   // static + dynamic LDS
-#define ALT3_FGR_LAUNCH(PER)                                                                                     \
+    const bool big = lds + kFgrList * sizeof(unsigned short) + 1024 > 65536;   // static + dynamic LDS
+#define ALT3_FGR_LAUNCH_MT(PER, MT)                                                                              \
     if (big) {                                                                                                   \
-        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<float, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<double, PER>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<float, PER, MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        ALT3_HIP(hipFuncSetAttribute((const void *)k_fgr<double, PER, MT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     }                                                                                                            \
     if (mixed)                                                                                                   \
-        hipLaunchKernelGGL((k_fgr<float, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
+        hipLaunchKernelGGL((k_fgr<float, PER, MT>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
                            ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const float *)v, ctx->d_p1f.p,         \
                            ctx->d_p2f.p);                                                                        \
     else                                                                                                         \
-        hipLaunchKernelGGL((k_fgr<double, PER>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,            \
+        hipLaunchKernelGGL((k_fgr<double, PER, MT>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,        \
                            ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const double *)v, ctx->d_p1.p, \
                            ctx->d_p2.p);
+#define ALT3_FGR_LAUNCH(PER)                                                                                     \
+    if (nb <= 32) {                                                                                              \
+        ALT3_FGR_LAUNCH_MT(PER, unsigned)                                                                        \
+    } else {                                                                                                     \
+        ALT3_FGR_LAUNCH_MT(PER, unsigned long long)                                                              \
+    }
     kernel_mark(ctx, 1, false);
     if (per <= 2) {
         ALT3_FGR_LAUNCH(2)
@@ -851,6 +880,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
         ALT3_FGR_LAUNCH(16)
     }
 #undef ALT3_FGR_LAUNCH
+#undef ALT3_FGR_LAUNCH_MT
     ALT3_HIP(hipGetLastError());
     kernel_mark(ctx, 1, true);
     if (sync) ALT3_HIP(hipStreamSynchronize(ctx->stream));
