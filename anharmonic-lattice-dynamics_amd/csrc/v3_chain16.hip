@@ -90,7 +90,11 @@ constexpr int KSPL = 32;         // 2 ndr <= 64: ndr <= 32
 // spreads over the lanes fall on distinct banks)
 constexpr int PLANE = NB * NB;               // halves per plane
 constexpr int PSTR = 4 * PLANE + 16;         // halves per point
-constexpr int OS = NB * 25 + 1;              // one point of the FP32 staging area
+// FP32 staging area of |V3|^2 [q][l2][l3]: l2 stride 27, a point every 650 floats (the V-tile
+// writes and the row reads of the output hit at most 3 and 2 lanes per bank with 32 banks, 2
+// with 64)
+constexpr int LS = 27;
+constexpr int OS = NB * LS + 2;
 
 // e3 and e2 operands of every mesh point from the FP32 eigenvectors evf [k][l col][b comp]
 // (complex interleaved), as halves hi then lo, 16 bytes per lane:
@@ -454,9 +458,9 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                         const float s2 = v[mv][2][i] * v[mv][2][i];
                         const float sb = s2 + xor8(s2);
                         if (l3 < NB) {
-                            Ost[q * OS + col * 25 + l3] = sa * (w3r[mv][i] * wa);   // l2 = col
+                            Ost[q * OS + col * LS + l3] = sa * (w3r[mv][i] * wa);   // l2 = col
                             if (col < 8)                                               // l2 = 16 + col
-                                Ost[q * OS + (16 + col) * 25 + l3] = sb * (w3r[mv][i] * wb);
+                                Ost[q * OS + (16 + col) * LS + l3] = sb * (w3r[mv][i] * wb);
                         }
                     }
             }
@@ -466,7 +470,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
             for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
                 const int row = x / (QT / 4), c4 = x % (QT / 4);
                 if (q0 + 4 * c4 < nk) {
-                    const float *src = Ost + 4 * c4 * OS + (row / NB) * 25 + row % NB;
+                    const float *src = Ost + 4 * c4 * OS + (row / NB) * LS + row % NB;
                     const float4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
                     *reinterpret_cast<float4 *>(o + (size_t)row * nk + 4 * c4) = val;
                 }
