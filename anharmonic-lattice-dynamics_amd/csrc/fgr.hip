@@ -441,12 +441,27 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             }
         }
     };
+    // P = W F with F = |V3|^2 (2 pi)^2 / nk
+    const double fscale = tpi2 / nk;
     for (int l1 = 0; l1 < nb; l1++) {
-        const double omega1 = freq[(size_t)k1 * nb + l1];
+        const double omega1 = s_om[l1];
         const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
+        const size_t basem = ((((size_t)s * nb + l1) * nb + l3) * nb + l2) * (size_t)nk;   // mirrored row
+        if (!omega_in(omega1, lo, hi)) {
+            // no tetrahedron of this item reaches omega1: W = 0, so P = 0 without reading
+            // |V3|^2 (uniform over the group)
+#pragma unroll
+            for (int j = 0; j < kFgrPer; j++) {
+                const int w = tid + j * kFgrThreads;
+                if (w < nk) {
+                    P[base + w] = (VT)0;
+                    if (mirror) P[basem + w] = (VT)0;
+                }
+            }
+            continue;
+        }
         // |V3|^2 of this l1 (at q2 = q_w for coalescence, q2 = -q_w for decay), requested
         // before the weight phase so that the load latency overlaps it
-        const size_t basem = ((((size_t)s * nb + l1) * nb + l3) * nb + l2) * (size_t)nk;   // mirrored row
         VT fv[kFgrPer], fvm[kFgrPer];
 #pragma unroll
         for (int j = 0; j < kFgrPer; j++) {
@@ -463,7 +478,7 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
                 if (mirror) fvm[j] = v33[basem + vo];
             }
         }
-        if (omega_in(omega1, lo, hi)) {
+        {
             // Only tetrahedra whose energy range contains omega1 have weights, and they are
             // few: they are compacted into LDS lists so that the weight formula runs with
             // every lane busy.  The lists normally hold all of them; if one overflows, the
@@ -512,16 +527,20 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
         for (int j = 0; j < kFgrPer; j++) {
             const int w = tid + j * kFgrThreads;
             if (w < nk) {
-                double F = (double)fv[j] * tpi2 / nk;
-                if (vzero >> j & 1u) F = 0.0;
-                P[base + w] = (VT)(W[w] * F);
+                if constexpr (sizeof(VT) == sizeof(float)) {   // FP32 P: the product in FP32
+                    const float F = (vzero >> j & 1u) ? 0.0f : fv[j] * (float)fscale;
+                    P[base + w] = (float)W[w] * F;
+                } else {
+                    double F = (double)fv[j] * tpi2 / nk;
+                    if (vzero >> j & 1u) F = 0.0;
+                    P[base + w] = (VT)(W[w] * F);
+                }
                 if (mirror) {
                     int c0, c1, c2;
                     m.coords(w, c0, c1, c2);
                     const int p = m.diff(a0, a1, a2, c0, c1, c2);
-                    double Fm = (double)fvm[j] * tpi2 / nk;
-                    if (vzerom >> j & 1u) Fm = 0.0;
-                    P[basem + w] = (VT)(W[p] * Fm);
+                    const float Fm = (vzerom >> j & 1u) ? 0.0f : (float)fvm[j] * (float)fscale;
+                    P[basem + w] = (VT)((float)W[p] * Fm);
                 }
             }
         }
