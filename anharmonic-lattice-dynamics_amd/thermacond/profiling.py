@@ -21,7 +21,17 @@ from typing import Dict, Tuple
 
 
 def kernel_name(n: str) -> str:
-    """Demangled name without namespaces' noise and argument list (template arguments kept)."""
+    """Demangled name without namespaces' noise and argument list (template arguments kept).
+    A name the profiler left mangled (its demangler gives up on _Float16 vector arguments) is
+    reduced to the function's own name: _ZN12_GLOBAL__N_1<len><name>..."""
+    if n.startswith("_ZN"):
+        rest = n[3:]
+        if rest.startswith("12_GLOBAL__N_1"):
+            rest = rest[len("12_GLOBAL__N_1"):]
+        digits = len(rest) - len(rest.lstrip("0123456789"))
+        if digits:
+            ln = int(rest[:digits])
+            return rest[digits:digits + ln]
     n = n.replace("(anonymous namespace)::", "").replace("alt3::", "").replace("void ", "")
     depth, out = 0, ""
     for ch in n:

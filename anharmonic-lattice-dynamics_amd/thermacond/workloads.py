@@ -272,27 +272,44 @@ def serving_requests(count: int, seed: int, precision: int) -> List[serving.Requ
 
 
 def run_serving(count: int = 512, seed: int = 0, slots: int = 4, precision: int = capi.V3_MIXED,
-                load: float = 0.8, probe: int = 32, log=print) -> dict:
-    """Poisson arrivals at `load` times the rate one slot serves alone: the service time is
-    measured first on `probe` requests of the same distribution run one at a time."""
+                load: float = 0.8, calib: int = 64, log=print) -> dict:
+    """Poisson arrivals at `load` times the server's capacity.  The capacity is measured first,
+    closed loop, on the same server: `calib` requests of the same distribution (another seed)
+    submitted at once, completions per second over their span.  The main stream then runs on
+    that server (its context cache as the calibration left it), so that the offered rate is
+    `load` of what the server sustains, and the latencies are those of a server at that load
+    rather than of a growing queue."""
     t0 = time.perf_counter()
-    probe_reqs = serving_requests(probe, seed + 1000, precision)
-    with serving.KappaServer(slots=1) as srv:
-        for r in probe_reqs:
-            srv.submit(r)
-            srv.wait([r])
-    service = float(np.mean([r.t_done - r.t_admit for r in probe_reqs]))
-    rate = load / service
-    log(f"serving: mean service time alone {service * 1e3:.1f} ms -> arrival rate {rate:.1f}/s")
+    cal_reqs = serving_requests(calib, seed + 1000, precision)
     reqs = serving_requests(count, seed, precision)
-    arrivals = serving.poisson_arrivals(count, rate, seed)
+    t_gen = time.perf_counter() - t0
+    log(f"serving: {calib} + {count} requests ready in {t_gen:.1f} s")
     with serving.KappaServer(slots=slots) as srv:
-        serving.replay(srv, reqs, arrivals)
+        t1 = time.perf_counter()
+        for r in cal_reqs:
+            srv.submit(r)
+        srv.wait(cal_reqs)
+        cal = serving.latency_summary(cal_reqs)
+        capacity = cal["completed"] / cal["span_s"] if cal["span_s"] > 0 else 0.0
+        rate = load * capacity
+        log(f"serving: capacity {capacity:.2f} requests/s closed loop ({calib} requests, {cal['span_s']:.1f} s) "
+            f"-> Poisson rate {rate:.2f}/s")
+        arrivals = serving.poisson_arrivals(count, rate, seed)
+        done = [0]
+
+        def progress(req):
+            done[0] += 1
+            if done[0] % 64 == 0:
+                log(f"serving: {done[0]}/{count} submitted at {time.perf_counter() - t1:.0f} s")
+
+        serving.replay(srv, reqs, arrivals, on_submit=progress)
         conc = srv.max_concurrent
         b = srv.batcher
     s = serving.latency_summary(reqs)
-    s.update({"arrival_rate_per_s": rate, "service_ms_alone": service * 1e3, "slots": slots,
-              "max_concurrent": conc, "setup_s": time.perf_counter() - t0 - s["span_s"],
+    busy = sum(float(np.sum(r.result["stage_ms"])) for r in reqs if r.result) * 1e-3
+    s.update({"arrival_rate_per_s": rate, "capacity_per_s": capacity, "load": load, "calibration": cal,
+              "slots": slots, "max_concurrent": conc, "generation_s": t_gen,
+              "device_stage_s_over_span": busy / s["span_s"] if s["span_s"] > 0 else None,
               "v3_shared_launches": b.launches, "v3_contexts_batched": b.contexts, "v3_max_batch": b.max_batch})
     errs = [r.error for r in reqs if r.error]
     if errs:
