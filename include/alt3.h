@@ -29,10 +29,11 @@ extern "C" {
 /* V3 arithmetic */
 #define ALT3_V3_FP64_REF 0   /* per-element sum over all FC3 terms (matrix_elt_simplified) */
 #define ALT3_V3_FP64 1       /* batched phase-GEMM + contraction, FP64 */
-#define ALT3_V3_MIXED 2      /* FP32 phase sum and contractions, FP64 scale: 24 branches run
-                                all three products on FP32 MFMA (k_v3_chain), 6 and 12 branches
-                                on FP32 VALU (k_v3_rows), other counts the phase GEMM on FP32
-                                MFMA and the contractions on VALU (k_v3_mfma); cells whose
+#define ALT3_V3_MIXED 2      /* phase sum and contractions with FP32 accumulation, FP64 scale:
+                                24 branches run all three products on matrix cores (split FP16,
+                                k_v3_chain16, or FP32, k_v3_chain: alt3_set_v3_mma), 6 and 12
+                                branches on FP32 VALU (k_v3_rows), other counts the phase GEMM on
+                                FP32 MFMA and the contractions on VALU (k_v3_mfma); cells whose
                                 16-point MFMA tile exceeds the LDS (nb > ~28) run the FP64 path */
 
 /* matrix-core arithmetic of the mixed V3 on 24-branch cells (alt3_set_v3_mma) */
