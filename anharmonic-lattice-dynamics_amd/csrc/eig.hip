@@ -206,9 +206,11 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, cons
                         zd x = A[k * nb + p], y = zmul(em, A[k * nb + qq]);
                         A[k * nb + p] = {c * x.r - s * y.r, c * x.i - s * y.i};
                         A[k * nb + qq] = {s * x.r + c * y.r, s * x.i + c * y.i};
-                        zd vx = V[k * nb + p], vy = zmul(em, V[k * nb + qq]);
-                        V[k * nb + p] = {c * vx.r - s * vy.r, c * vx.i - s * vy.i};
-                        V[k * nb + qq] = {s * vx.r + c * vy.r, s * vx.i + c * vy.i};
+                        if (var == 0) {   // the displaced matrices give eigenvalues only
+                            zd vx = V[k * nb + p], vy = zmul(em, V[k * nb + qq]);
+                            V[k * nb + p] = {c * vx.r - s * vy.r, c * vx.i - s * vy.i};
+                            V[k * nb + qq] = {s * vx.r + c * vy.r, s * vx.i + c * vy.i};
+                        }
                     }
                     __syncthreads();
                     if (lane < nb) {  // rows of A
@@ -439,9 +441,11 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
                     zd x0 = A[k * ld + p], y0 = zmul(em, A[k * ld + qq]);
                     A[k * ld + p] = {c * x0.r - sn * y0.r, c * x0.i - sn * y0.i};
                     A[k * ld + qq] = {sn * x0.r + c * y0.r, sn * x0.i + c * y0.i};
-                    zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
-                    V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
-                    V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                    if (var == 0) {   // the displaced matrices give eigenvalues only
+                        zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
+                        V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
+                        V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                    }
                 }
                 __syncthreads();
                 for (int x = sub; x < np * nb; x += 64) {   // rows p, q of A: A <- J^H A
@@ -514,9 +518,11 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
                     zd x = A[k * ld + p], y = zmul(em, A[k * ld + qq]);
                     A[k * ld + p] = {c * x.r - sn * y.r, c * x.i - sn * y.i};
                     A[k * ld + qq] = {sn * x.r + c * y.r, sn * x.i + c * y.i};
-                    zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
-                    V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
-                    V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                    if (var == 0) {   // the displaced matrices give eigenvalues only
+                        zd vx = V[k * ld + p], vy = zmul(em, V[k * ld + qq]);
+                        V[k * ld + p] = {c * vx.r - sn * vy.r, c * vx.i - sn * vy.i};
+                        V[k * ld + qq] = {sn * vx.r + c * vy.r, sn * vx.i + c * vy.i};
+                    }
                 }
                 __syncthreads();
                 if (act && sub < nb) {  // rows of A
