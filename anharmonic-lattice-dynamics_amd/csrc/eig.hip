@@ -279,13 +279,20 @@ __global__ void __launch_bounds__(64) k_eig_solve(int nk, int nvar, int nb, cons
 // m - 1 dependent steps where the cyclic order takes nb (nb - 1)/2.  The rotation formulas are
 // those of the cyclic order; the eigenvalues agree to round-off, the eigenvectors inside a
 // degenerate subspace are another orthonormal basis of it.
+// Variants var0 .. var0 + nv - 1 of every point (nvar: the layout of evals7).  warm (PAR, the
+// displaced variants only): the eigenvectors of the central solve, evec0 as written by the
+// variant-0 launch; the displaced matrix is first turned into W^H D(q +- dq) W with W those
+// eigenvectors (nearly diagonal: its off-diagonal entries are O(dq)), so that the Jacobi
+// sweeps start close to convergence.  A unitary similarity leaves the eigenvalues, which is
+// all a displaced solve gives.
 template <int G, bool PAR = false>
 __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, const double *__restrict__ kpc,
                                                   const int *__restrict__ ent_ptr,
                                                   const double *__restrict__ term_amp,
                                                   const double *__restrict__ term_rr, double dq,
                                                   double *__restrict__ evals7, double *__restrict__ evec0,
-                                                  alt3::BornDev born, const double *__restrict__ bzm)
+                                                  alt3::BornDev born, const double *__restrict__ bzm, int var0 = 0,
+                                                  int nv = 0, const double *__restrict__ warm = nullptr)
 {
     constexpr int M = 64 / G;
     extern __shared__ double lds[];
@@ -300,9 +307,10 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
     __shared__ double s_rot[M][5];                               // c, s, cos(phi), sin(phi), skip
     __shared__ double s_sum[M][2];
     __shared__ int s_ord[M][G];
+    if (nv <= 0) nv = nvar;
     const int mat = blockIdx.x * M + g;
-    const bool valid = mat < nk * nvar;
-    const int q = valid ? mat / nvar : 0, var = valid ? mat % nvar : 0;
+    const bool valid = mat < nk * nv;
+    const int q = valid ? mat / nv : 0, var = valid ? var0 + mat % nv : 0;
     // norm sum of k_eig_solve: lane l of 64 sums entries l, l+64, ..., then the lane sums in order
     auto sum2 = [&](auto f, double &s0, double &s1) {
         s0 = 0;
@@ -380,6 +388,71 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
     __syncthreads();
     if constexpr (PAR) {
         static_assert(G == 64, "the parallel order runs one matrix per wavefront");
+        if (warm != nullptr && var > 0) {   // A <- W^H A W (uniform over the wavefront)
+            constexpr int EPL = 16;         // entries per lane, nb <= 32
+            for (int e = sub; e < nn; e += 64) {   // W[k][c] = component k of eigenvector c, into V
+                const int r = e / nb, c = e % nb;
+                const double *w = warm + 2 * (((size_t)q * nb + c) * nb + r);
+                V[r * ld + c] = {w[0], w[1]};
+            }
+            __syncthreads();
+            zd t[EPL];
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {   // T = A W
+                const int e = sub + 64 * i;
+                t[i] = {0.0, 0.0};
+                if (e < nn) {
+                    const int r = e / nb, c = e % nb;
+                    double ar = 0, ai = 0;
+                    for (int k = 0; k < nb; k++) {
+                        const zd a = A[r * ld + k], w = V[k * ld + c];
+                        ar += a.r * w.r - a.i * w.i;
+                        ai += a.r * w.i + a.i * w.r;
+                    }
+                    t[i] = {ar, ai};
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {
+                const int e = sub + 64 * i;
+                if (e < nn) A[EI(e)] = t[i];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {   // W^H T
+                const int e = sub + 64 * i;
+                if (e < nn) {
+                    const int r = e / nb, c = e % nb;
+                    double ar = 0, ai = 0;
+                    for (int k = 0; k < nb; k++) {
+                        const zd w = V[k * ld + r], a = A[k * ld + c];
+                        ar += w.r * a.r + w.i * a.i;
+                        ai += w.r * a.i - w.i * a.r;
+                    }
+                    t[i] = {ar, ai};
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < EPL; i++) {
+                const int e = sub + 64 * i;
+                if (e < nn) A[EI(e)] = t[i];
+            }
+            __syncthreads();
+            for (int e = sub; e < nn; e += 64) {   // Hermitian to the last bit: the upper triangle's mean
+                const int r = e / nb, c = e % nb;
+                if (r < c) {
+                    const zd a = A[r * ld + c], b = A[c * ld + r];
+                    const zd mid = {0.5 * (a.r + b.r), 0.5 * (a.i - b.i)};
+                    A[r * ld + c] = mid;
+                    A[c * ld + r] = {mid.r, -mid.i};
+                } else if (r == c) {
+                    A[r * ld + c].i = 0.0;
+                }
+            }
+            __syncthreads();
+        }
         const int m = (nb + 1) & ~1, np = m / 2;
         __shared__ double s_par[32][5];   // per pair: c, s, cos(phi), sin(phi), skip
         __shared__ int s_pq[32][2];
@@ -704,9 +777,18 @@ int eig_launch(alt3_ctx *ctx, int np, int nvar, const double *kq, double dq, dou
         if (lds + 4096 > ctx->lds_max) return fail(ALT3_E_ARG, "eigen solve: matrix pair does not fit in LDS");
         const void *kern = (const void *)k_eig_group<64, true>;
         if (lds + 4096 > 65536) ALT3_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_eig_group<64, true>), dim3((unsigned)nmat), dim3(64), lds, ctx->stream, np, nvar, nb, kq,
-                           ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born,
-                           ctx->d_born.p);
+        if (nvar == 1 || nb > 32) {
+            hipLaunchKernelGGL((k_eig_group<64, true>), dim3((unsigned)nmat), dim3(64), lds, ctx->stream, np, nvar, nb,
+                               kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born,
+                               ctx->d_born.p, 0, nvar, nullptr);
+        } else {   // the central solves, then the displaced ones started from their eigenvectors
+            hipLaunchKernelGGL((k_eig_group<64, true>), dim3((unsigned)np), dim3(64), lds, ctx->stream, np, nvar, nb,
+                               kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq, evals, evec, ctx->born,
+                               ctx->d_born.p, 0, 1, nullptr);
+            hipLaunchKernelGGL((k_eig_group<64, true>), dim3((unsigned)(np * (size_t)(nvar - 1))), dim3(64), lds,
+                               ctx->stream, np, nvar, nb, kq, ctx->d_fc2_rc.p, ctx->d_fc2_amp.p, ctx->d_fc2_rr.p, dq,
+                               evals, evec, ctx->born, ctx->d_born.p, 1, nvar - 1, evec);
+        }
         ALT3_HIP(hipGetLastError());
         return ALT3_OK;
     }
