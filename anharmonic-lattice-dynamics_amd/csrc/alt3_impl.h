@@ -144,6 +144,7 @@ size_t v3_chain16_pack_floats(int nk, int ksub);
 int v3_chain16_run(alt3_ctx *ctx, int ksub, const float *evf, const float *ephf, const double *wq, float *out,
                    int *d_err);
 int v3_rows_run(alt3_ctx *ctx, int ksub, bool fp32, int *d_err);
+int prow_all(alt3_ctx *ctx);   // mark every P row as stored (writers that fill all rows)
 int piso_run(alt3_ctx *ctx);   // P_iso_split into d_piso when giso is set and piso_ok is not
 int cg_start_launch(alt3_ctx *ctx, double *x, double *te);
 int cg_device_ws(alt3_ctx *ctx, int n, const double *dA, const double *db, double *dx, int max_iter, int ndig,
@@ -209,6 +210,12 @@ struct alt3_ctx {
     alt3::DBuf<double> d_p1, d_p2;                 // [ksub][nb][nb][nb][nk]
     alt3::DBuf<float> d_p1f, d_p2f;                // same in FP32 (mixed path: P = W |V3|^2 with FP32 |V3|^2)
     bool p_fp32 = false;                           // which pair holds the current P
+    // P rows held: [process][ksub][nb][nb][nb] (l1, l2, l3 of one (s, process)); 0 = the row is
+    // zero and not stored (its memory is stale), 1 = stored.  The tetrahedron FGR leaves rows
+    // whose l1 no tetrahedron reaches unstored; every reader of P goes through this map.
+    alt3::DBuf<unsigned char> d_prow;
+    alt3::DBuf<double> d_pzero;                    // a line of zeros the tau loads of unstored rows read
+    alt3::DBuf<unsigned long long> d_prowb;        // the same packed per (s, l1, l2): P1 bits 0-31, P2 bits 32-63
     bool have_fgr = false;
 
     // per temperature: one tau pass fills slot t of each array for every temperature it

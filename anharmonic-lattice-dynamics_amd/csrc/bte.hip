@@ -105,7 +105,9 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
                                              const int *__restrict__ mapinv, const double *__restrict__ dist,
                                              size_t dstr, const PT *__restrict__ P1sm,
                                              const PT *__restrict__ P2sm, double *__restrict__ qpart, size_t qstr,
-                                             PT *__restrict__ R1, PT *__restrict__ R2, size_t rstr)
+                                             PT *__restrict__ R1, PT *__restrict__ R2, size_t rstr,
+                                             const unsigned char *__restrict__ rm1,
+                                             const unsigned char *__restrict__ rm2)
 {
     __shared__ double sh[256];
     const int l2 = blockIdx.x % nb, sl = blockIdx.x / nb, s = sl / nb, l1 = sl % nb;
@@ -136,8 +138,10 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
         }
 #pragma unroll 3
         for (int l3 = 0; l3 < nb; l3++) {
-            const size_t x = row + (size_t)l3 * nk + j;
-            const double P1 = (double)P1sm[x], P2 = (double)P2sm[x];
+            const size_t x = row + (size_t)l3 * nk + j, rid = ((size_t)sl * nb + l2) * nb + l3;
+            // rows the row map marks zero are not stored (rm NULL: all stored)
+            const double P1 = (!rm1 || rm1[rid]) ? (double)P1sm[x] : 0.0;
+            const double P2 = (!rm2 || rm2[rid]) ? (double)P2sm[x] : 0.0;
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const double p1 = P1 * (f1[t] * (dist[t * dstr + (size_t)np * nb + l3] + 1));
@@ -167,6 +171,27 @@ __global__ void __launch_bounds__(256) k_tau(alt3::MeshDev m, int nk, int nb, in
     }
 }
 
+__global__ void k_fill_zero(int n, double *__restrict__ a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) a[x] = 0.0;
+}
+
+// the P row map packed per (s, l1, l2): bit l3 for the P1 row, bit 32 + l3 for the P2 row
+// (cells of up to 32 branches)
+__global__ void k_prow_pack(size_t n, int nb, size_t p2, const unsigned char *__restrict__ rm,
+                            unsigned long long *__restrict__ out)
+{
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // (s, l1, l2)
+    if (x >= n) return;
+    unsigned long long w = 0;
+    for (int l3 = 0; l3 < nb; l3++) {
+        if (rm[x * nb + l3]) w |= 1ull << l3;
+        if (rm[p2 + x * nb + l3]) w |= 1ull << (32 + l3);
+    }
+    out[x] = w;
+}
+
 // one workgroup per (s, chunk of 256 mesh points q2), every (l1, l2, l3) inside, for cells of
 // NB branches: the occupations at q1+q2 and q1-q2 (the gathers of k_tau) are read once per
 // thread into registers, so the inner loops only stream the P rows (12 independent loads per
@@ -176,7 +201,9 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
                                                 const double *__restrict__ dist, size_t dstr,
                                                 const PT *__restrict__ P1sm, const PT *__restrict__ P2sm,
                                                 double *__restrict__ qpart, size_t qstr, PT *__restrict__ R1,
-                                                PT *__restrict__ R2, size_t rstr)
+                                                PT *__restrict__ R2, size_t rstr,
+                                                const unsigned long long *__restrict__ rmb,
+                                                const PT *__restrict__ pzero)
 {
     __shared__ double sh[256];
     const int s = blockIdx.y, nch = gridDim.x, ch = blockIdx.x;
@@ -210,12 +237,30 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
         }
 #pragma unroll 1
         for (int l2 = 0; l2 < NB; l2++) {
-            const size_t row = (((size_t)s * NB + l1) * NB + l2) * NB * (size_t)nk + jc;   // (s,l1,l2,l3=0)
+            const size_t rid = (((size_t)s * NB + l1) * NB + l2) * NB;                   // row map (s,l1,l2,l3=0)
+            const size_t row = rid * (size_t)nk + jc;                                       // (s,l1,l2,l3=0)
+            // rows the row map marks zero are not stored (rm NULL: all stored): bit l3 of b1, b2,
+            // the same for the whole group; their loads go to one cached line of zeros (pzero),
+            // so they cost no HBM traffic and no branch
             PT P1[NB], P2[NB];
+            if constexpr (sizeof(PT) == sizeof(float)) {   // the FP64 FGR stores every row
+                unsigned b1 = (1u << NB) - 1, b2 = b1;
+                if (rmb) {   // the packed map: P1 rows in the low word, P2 rows in the high one
+                    const unsigned long long w = rmb[((size_t)s * NB + l1) * NB + l2];
+                    b1 = (unsigned)w;
+                    b2 = (unsigned)(w >> 32);
+                }
 #pragma unroll
-            for (int l3 = 0; l3 < NB; l3++) {
-                P1[l3] = P1sm[row + (size_t)l3 * nk];
-                P2[l3] = P2sm[row + (size_t)l3 * nk];
+                for (int l3 = 0; l3 < NB; l3++) {   // an unstored row reads the zero line instead
+                    P1[l3] = *((b1 >> l3 & 1u) ? P1sm + row + (size_t)l3 * nk : pzero);
+                    P2[l3] = *((b2 >> l3 & 1u) ? P2sm + row + (size_t)l3 * nk : pzero);
+                }
+            } else {
+#pragma unroll
+                for (int l3 = 0; l3 < NB; l3++) {
+                    P1[l3] = P1sm[row + (size_t)l3 * nk];
+                    P2[l3] = P2sm[row + (size_t)l3 * nk];
+                }
             }
             // select the occupations of branch l2 without dynamic register indexing
             double dd[NT];
@@ -637,7 +682,9 @@ template <typename PT>
 __global__ void __launch_bounds__(256) k_tau_s(alt3::MeshDev m, int nk, int nb, int ibz_lo,
                                                const int *__restrict__ mapinv, const double *__restrict__ dist,
                                                const PT *__restrict__ P1sm, const PT *__restrict__ P2sm,
-                                               PT *__restrict__ S1, PT *__restrict__ S2)
+                                               PT *__restrict__ S1, PT *__restrict__ S2,
+                                               const unsigned char *__restrict__ rm1,
+                                               const unsigned char *__restrict__ rm2)
 {
     const int s = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nk) return;
@@ -653,9 +700,10 @@ __global__ void __launch_bounds__(256) k_tau_s(alt3::MeshDev m, int nk, int nb, 
             double s1 = 0, s2 = 0;
             for (int l2 = 0; l2 < nb; l2++) {
                 const double d2 = dist[(size_t)j * nb + l2];
-                const size_t x = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk + j;
-                s1 += (double)P1sm[x] * ((d1 * d2) * n3p);
-                s2 += (double)P2sm[x] * ((d1 * (d2 + 1)) * n3m);
+                const size_t rid = (((size_t)s * nb + l1) * nb + l2) * nb + l3, x = rid * (size_t)nk + j;
+                // rows the row map marks zero are not stored (rm NULL: all stored)
+                if (!rm1 || rm1[rid]) s1 += (double)P1sm[x] * ((d1 * d2) * n3p);
+                if (!rm2 || rm2[rid]) s2 += (double)P2sm[x] * ((d1 * (d2 + 1)) * n3m);
             }
             const size_t o = (((size_t)s * nb + l1) * nb + l3) * (size_t)nk + j;
             S1[o] = (PT)s1;
@@ -1083,6 +1131,21 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
                            temp_cm(temps[t]), ctx->classical, ctx->d_dist.p + t * dstr, derr);
     ALT3_HIP(hipGetLastError());
     const int nch = (nk + 255) / 256;
+    // the P row map (rows of l1 no tetrahedron reaches are zero and not stored); packed per
+    // (s, l1, l2) for k_tau_q2
+    const unsigned char *rm1 = ctx->d_prow.p, *rm2 = rm1 ? rm1 + ctx->d_prow.n / 2 : nullptr;
+    const unsigned long long *rmb = nullptr;
+    if (ctx->d_pzero.n == 0) {   // one line of zeros for the loads of unstored rows
+        ALT3_TRY(ctx->d_pzero.alloc(64));
+        hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, ctx->stream, 64, ctx->d_pzero.p);
+    }
+    if (rm1 && nb <= 32) {
+        const size_t n = (size_t)ksub * nb * nb;
+        ALT3_TRY(ctx->d_prowb.reserve(n));
+        hipLaunchKernelGGL(k_prow_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n, nb,
+                           ctx->d_prow.n / 2, rm1, ctx->d_prowb.p);
+        rmb = ctx->d_prowb.p;
+    }
     // k_tau_q2 (occupations at q1 +- q2 in registers): 6 branches with up to 4 temperatures per
     // pass, 12 with 2, 24 with 1; other cells k_tau
     const bool rows = nb == 6 || nb == 12 || nb == 24;
@@ -1098,7 +1161,7 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
 #define ALT3_TAU_Q2(G, PT, NB, P1, P2, R1, R2)                                                                     \
     hipLaunchKernelGGL((k_tau_q2<G, PT, NB>), grid, dim3(256), 0, ctx->stream, md, nk, ctx->ibz_lo, ctx->d_mapinv.p, \
                        ctx->d_dist.p + t0 * dstr, dstr, P1, P2, qpart.p + t0 * pstr, pstr, (R1) + t0 * rstr,         \
-                       (R2) + t0 * rstr, rstr)
+                       (R2) + t0 * rstr, rstr, rmb, (const PT *)ctx->d_pzero.p)
 #define ALT3_TAU_Q2_ALL(PT, P1, P2, R1, R2)                                 \
     if (nb == 24) ALT3_TAU_Q2(1, PT, 24, P1, P2, R1, R2);                    \
     else if (nb == 12 && g == 1) ALT3_TAU_Q2(1, PT, 12, P1, P2, R1, R2);     \
@@ -1120,12 +1183,14 @@ int tau_multi_run(alt3_ctx *ctx, const double *temps, int nt, bool sync)
             auto *kern = g == 1 ? k_tau<1, float> : g == 2 ? k_tau<2, float> : g == 3 ? k_tau<3, float> : k_tau<4, float>;
             hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1f.p, ctx->d_p2f.p,
-                               qpart.p + t0 * pstr, pstr, ctx->d_r1f.p + t0 * rstr, ctx->d_r2f.p + t0 * rstr, rstr);
+                               qpart.p + t0 * pstr, pstr, ctx->d_r1f.p + t0 * rstr, ctx->d_r2f.p + t0 * rstr, rstr,
+                               rm1, rm2);
         } else {
             auto *kern = g == 1 ? k_tau<1, double> : g == 2 ? k_tau<2, double> : g == 3 ? k_tau<3, double> : k_tau<4, double>;
             hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo,
                                ctx->d_mapinv.p, ctx->d_dist.p + t0 * dstr, dstr, ctx->d_p1.p, ctx->d_p2.p,
-                               qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr);
+                               qpart.p + t0 * pstr, pstr, ctx->d_r1.p + t0 * rstr, ctx->d_r2.p + t0 * rstr, rstr,
+                               rm1, rm2);
         }
         ALT3_HIP(hipGetLastError());
     }
@@ -1428,8 +1493,9 @@ static int update_build(alt3_ctx *ctx, double tempk, const char *who)
         int rc = S1.alloc(rstr);
         if (rc == ALT3_OK) rc = S2.alloc(rstr);
         if (rc != ALT3_OK) return rc;
+        const unsigned char *rm1 = ctx->d_prow.p, *rm2 = rm1 ? rm1 + ctx->d_prow.n / 2 : nullptr;
         hipLaunchKernelGGL(k_tau_s<PT>, gs, dim3(256), 0, ctx->stream, md, nk, nb, ctx->ibz_lo, ctx->d_mapinv.p,
-                           ctx->dist(), P1, P2, S1.p, S2.p);
+                           ctx->dist(), P1, P2, S1.p, S2.p, rm1, rm2);
         if (lds > 65536) {
             hipError_t e = hipFuncSetAttribute((const void *)k_update_c<PT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return fail(ALT3_E_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));

@@ -22,7 +22,7 @@ int v3_run(alt3_ctx *ctx, int ibz_lo, int ibz_hi, int prec, bool sync);
 int v3_copy_out(alt3_ctx *ctx, double *host);
 bool v3_mixed_fits(const alt3_ctx *ctx);   // v3_mixed.hip
 template <typename T>
-int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host);
+int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host, const unsigned char *rmap = nullptr);
 int fgr_run(alt3_ctx *ctx, bool sync);
 int fgr_gauss_run(alt3_ctx *ctx, double sigma);
 int upload_bte_tables(alt3_ctx *ctx);
@@ -44,12 +44,14 @@ int kappa_iter_run(alt3_ctx *ctx, double tempk, int max_iter, double mix, double
 namespace {
 int copy_p(alt3_ctx *ctx, double *P1sm, double *P2sm)
 {
+    const size_t p2 = ctx->d_prow.n / 2;   // rows not stored come out as zeros
+    const unsigned char *m1 = ctx->d_prow.p, *m2 = m1 ? m1 + p2 : nullptr;
     if (ctx->p_fp32) {
-        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1f.p, P1sm));
-        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2f.p, P2sm));
+        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1f.p, P1sm, m1));
+        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2f.p, P2sm, m2));
     } else {
-        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm));
-        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm));
+        if (P1sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p1.p, P1sm, m1));
+        if (P2sm) ALT3_TRY(copy_fortran5(ctx, ctx->d_p2.p, P2sm, m2));
     }
     return ALT3_OK;
 }
@@ -163,7 +165,7 @@ int alt3_destroy(alt3_ctx *ctx)
     ctx->d_cm.release(); ctx->d_cmf.release(); ctx->d_rhs.release(); ctx->d_cg.release(); ctx->d_red.release();
     ctx->d_flags.release(); ctx->d_negs.release(); ctx->d_qpart.release(); ctx->d_wq.release();
     ctx->d_ev7.release(); ctx->d_evec0.release(); ctx->d_ephf.release(); ctx->d_evf.release(); ctx->d_gpf32.release();
-    ctx->d_chain_e.release(); ctx->d_chain_g.release(); ctx->d_batch.release(); ctx->d_fgr_big.release();
+    ctx->d_chain_e.release(); ctx->d_chain_g.release(); ctx->d_batch.release(); ctx->d_fgr_big.release(); ctx->d_prow.release(); ctx->d_prowb.release(); ctx->d_pzero.release();
     ctx->d_s1.release(); ctx->d_s2.release(); ctx->d_s1f.release(); ctx->d_s2f.release(); ctx->d_upd.release(); ctx->d_iter.release();
     for (hipEvent_t e : ctx->evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->ev_k)
@@ -196,7 +198,7 @@ int alt3_device_bytes(const alt3_ctx *ctx, long long *bytes)
     b += ctx->d_s1.bytes() + ctx->d_s2.bytes() + ctx->d_s1f.bytes() + ctx->d_s2f.bytes() + ctx->d_upd.bytes();
     b += ctx->d_qpart.bytes() + ctx->d_wq.bytes() + ctx->d_ev7.bytes() + ctx->d_evec0.bytes();
     b += ctx->d_ephf.bytes() + ctx->d_evf.bytes() + ctx->d_gpf32.bytes();
-    b += ctx->d_chain_e.bytes() + ctx->d_chain_g.bytes();
+    b += ctx->d_chain_e.bytes() + ctx->d_chain_g.bytes() + ctx->d_prow.bytes() + ctx->d_prowb.bytes();
     *bytes = b;
     return ALT3_OK;
 }
@@ -224,7 +226,7 @@ int alt3_footprint(const alt3_ctx *ctx, int ntemps, int precision, long long *by
     } else {
         b += nel * 8;
     }
-    b += 2 * nel * (precision == ALT3_V3_MIXED ? 4 : 8);                                       // P1sm, P2sm
+    b += 2 * nel * (precision == ALT3_V3_MIXED ? 4 : 8) + 2 * nel / nk;                        // P1sm, P2sm, row map
     if (nk > 8192) b += 1024 * 20 * nk;          // k_fgr_big scratch: 5 floats per mesh point per workgroup
     const long long rb = precision == ALT3_V3_MIXED ? 4 : 8;                                  // R in the precision of P
     b += nt * (nk * nb * 8 + 3 * ni * nb * 8 + 2 * ni * nb * nb * nk * rb + 3 * ni * nb * nb * 8);  // tau pass
@@ -432,8 +434,10 @@ int alt3_set_fgr(alt3_ctx *ctx, int ibz_lo, int ibz_hi, const double *P1sm, cons
     ctx->d_p2f.release();
     ALT3_TRY(ctx->d_p1.alloc(nel));
     ALT3_TRY(ctx->d_p2.alloc(nel));
+    ALT3_TRY(ctx->d_prow.alloc(2 * nel / ctx->nk));
     ALT3_TRY(load_fortran5(ctx, ksub, P1sm, ctx->d_p1.p));
     ALT3_TRY(load_fortran5(ctx, ksub, P2sm, ctx->d_p2.p));
+    ALT3_TRY(prow_all(ctx));
     ctx->have_fgr = true;
     ctx->fgr_kind = ALT3_DELTA_TETRA;
     return ALT3_OK;

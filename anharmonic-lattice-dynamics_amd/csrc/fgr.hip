@@ -254,7 +254,8 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
                                                      const int *__restrict__ mapinv, const double *__restrict__ freq,
                                                      const double *__restrict__ freqT,
                                                      const VT *__restrict__ v33, VT *__restrict__ P1,
-                                                     VT *__restrict__ P2)
+                                                     VT *__restrict__ P2, unsigned char *__restrict__ prow,
+                                                     size_t prow2)
 {
     extern __shared__ double lds[];
     double *W = lds;               // [nk]
@@ -447,18 +448,29 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
         const double omega1 = s_om[l1];
         const size_t base = ((((size_t)s * nb + l1) * nb + l2) * nb + l3) * (size_t)nk;
         const size_t basem = ((((size_t)s * nb + l1) * nb + l3) * nb + l2) * (size_t)nk;   // mirrored row
+        unsigned char *rmap = prow + (proc ? prow2 : 0);
         if (!omega_in(omega1, lo, hi)) {
             // no tetrahedron of this item reaches omega1: W = 0, so P = 0 without reading
-            // |V3|^2 (uniform over the group)
+            // |V3|^2 (uniform over the group).  FP32 P: the row is marked zero in the row map
+            // and not written; FP64 P: written as zeros (its readers take every row)
+            if constexpr (kSym) {
+                if (tid == 0) {
+                    rmap[base / nk] = 0;
+                    if (mirror) rmap[basem / nk] = 0;
+                }
+            } else {
+                if (tid == 0) rmap[base / nk] = 1;
 #pragma unroll
-            for (int j = 0; j < kFgrPer; j++) {
-                const int w = tid + j * kFgrThreads;
-                if (w < nk) {
-                    P[base + w] = (VT)0;
-                    if (mirror) P[basem + w] = (VT)0;
+                for (int j = 0; j < kFgrPer; j++) {
+                    const int w = tid + j * kFgrThreads;
+                    if (w < nk) P[base + w] = (VT)0;
                 }
             }
             continue;
+        }
+        if (tid == 0) {
+            rmap[base / nk] = 1;
+            if (mirror) rmap[basem / nk] = 1;
         }
         // |V3|^2 of this l1 (at q2 = q_w for coalescence, q2 = -q_w for decay), requested
         // before the weight phase so that the load latency overlaps it
@@ -777,8 +789,15 @@ __global__ void __launch_bounds__(256) k_fgr_gauss(alt3::MeshDev m, int nk, int 
 namespace alt3 {
 
 // P in the precision of |V3|^2: FP32 for the mixed path, FP64 otherwise
+__global__ void k_fill_u8(size_t n, unsigned char v, unsigned char *__restrict__ a)
+{
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) a[x] = v;
+}
+
 static int alloc_p(alt3_ctx *ctx, size_t nel)
 {
+    ALT3_TRY(ctx->d_prow.alloc(2 * nel / ctx->nk));
     ctx->p_fp32 = ctx->v3_prec == ALT3_V3_MIXED;
     if (ctx->p_fp32) {
         ctx->d_p1.release();
@@ -794,6 +813,18 @@ static int alloc_p(alt3_ctx *ctx, size_t nel)
     return ALT3_OK;
 }
 
+// every P row stored (writers that fill whole arrays: the Gaussian FGR, the global-scratch
+// tetrahedron FGR, P loaded from the caller)
+int prow_all(alt3_ctx *ctx)
+{
+    const size_t n = ctx->d_prow.n;
+    if (n == 0) return ALT3_OK;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n, (unsigned char)1,
+                       ctx->d_prow.p);
+    ALT3_HIP(hipGetLastError());
+    return ALT3_OK;
+}
+
 int fgr_gauss_run(alt3_ctx *ctx, double sigma)
 {
     if (!ctx->have_v3) return fail(ALT3_E_STATE, "alt3_fgr_gauss needs alt3_v3sq_ibz first");
@@ -801,6 +832,7 @@ int fgr_gauss_run(alt3_ctx *ctx, double sigma)
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     const size_t nel = (size_t)ksub * nb * nb * nb * nk;
     ALT3_TRY(alloc_p(ctx, nel));
+    ALT3_TRY(prow_all(ctx));
     const MeshDev md = MeshDev::make(ctx->nc);
     dim3 grid((unsigned)((nk + 255) / 256), (unsigned)(nb * nb * nb), (unsigned)ksub);
     if (ctx->p_fp32)
@@ -842,6 +874,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
         ALT3_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         const long long grid = std::max(1ll, std::min(nitems, (long long)std::max(1, per_cu) * std::max(1, cus)));
         ALT3_TRY(ctx->d_fgr_big.reserve((size_t)grid * 5 * nk));
+        ALT3_TRY(prow_all(ctx));   // this kernel writes every row
         kernel_mark(ctx, 1, false);
         if (mixed)
             hipLaunchKernelGGL(k_fgr_big<float>, dim3((unsigned)grid), dim3(kFgrThreads), 0, ctx->stream, md, nk, nb,
@@ -861,6 +894,7 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     }
     dim3 grid((unsigned)(2 * (size_t)ksub * nb * nb));
     const int per = (nk + kFgrThreads - 1) / kFgrThreads;
+    const size_t prow2 = (size_t)ksub * nb * nb * nb;   // the P2 rows of the row map
     const void *v = mixed ? (const void *)ctx->d_v33f.p : (const void *)ctx->d_v33.p;
     const bool big = lds + kFgrList * sizeof(unsigned short) + 1024 > 65536;   // static + dynamic LDS
 #define ALT3_FGR_LAUNCH_MT(PER, MT)                                                                              \
@@ -871,11 +905,11 @@ int fgr_run(alt3_ctx *ctx, bool sync)
     if (mixed)                                                                                                   \
         hipLaunchKernelGGL((k_fgr<float, PER, MT>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb, ctx->ibz_lo, \
                            ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const float *)v, ctx->d_p1f.p,         \
-                           ctx->d_p2f.p);                                                                        \
+                           ctx->d_p2f.p, ctx->d_prow.p, prow2);                                                  \
     else                                                                                                         \
         hipLaunchKernelGGL((k_fgr<double, PER, MT>), grid, dim3(kFgrThreads), lds, ctx->stream, md, nk, nb,        \
                            ctx->ibz_lo, ctx->d_mapinv.p, ctx->d_freq.p, ctx->d_freqT.p, (const double *)v, ctx->d_p1.p, \
-                           ctx->d_p2.p);
+                           ctx->d_p2.p, ctx->d_prow.p, prow2);
 #define ALT3_FGR_LAUNCH(PER)                                                                                     \
     if (nb <= 32) {                                                                                              \
         ALT3_FGR_LAUNCH_MT(PER, unsigned)                                                                        \

@@ -246,7 +246,9 @@ __global__ void __launch_bounds__(256) k_v3_main(alt3::MeshDims md, int nk, int 
 
 // [s][l1][l2][l3][n2] -> Fortran v33s8(ksub,nk,nb,nb,nb)
 template <typename T>
-__global__ void k_to_fortran5(int ksub, int nk, int nb, const T *__restrict__ in, double *__restrict__ out)
+// rmap (or NULL): the P row map, 0 for rows that are zero and not stored
+__global__ void k_to_fortran5(int ksub, int nk, int nb, const T *__restrict__ in, double *__restrict__ out,
+                              const unsigned char *__restrict__ rmap = nullptr)
 {
     size_t tot = (size_t)ksub * nk * nb * nb * nb;
     size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,7 +259,8 @@ __global__ void k_to_fortran5(int ksub, int nk, int nb, const T *__restrict__ in
     int l1 = x % nb; x /= nb;
     int l2 = x % nb; x /= nb;
     int l3 = (int)x;
-    out[o] = (double)in[((((size_t)s * nb + l1) * nb + l2) * nb + l3) * nk + n2];
+    const size_t row = (((size_t)s * nb + l1) * nb + l2) * nb + l3;
+    out[o] = (rmap && !rmap[row]) ? 0.0 : (double)in[row * nk + n2];
 }
 
 // Fortran (ksub,nk,nb,nb,nb) -> [s][l1][l2][l3][n2] (the inverse of k_to_fortran5)
@@ -463,20 +466,20 @@ int v3_copy_out(alt3_ctx *ctx, double *host)
 }
 
 template <typename T>
-int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host)
+int copy_fortran5(alt3_ctx *ctx, const T *dev, double *host, const unsigned char *rmap)
 {
     const int nk = ctx->nk, nb = ctx->nb, ksub = ctx->ibz_hi - ctx->ibz_lo + 1;
     size_t tot = (size_t)ksub * nk * nb * nb * nb;
     DBuf<double> tmp;
     ALT3_TRY(tmp.alloc(tot));
-    hipLaunchKernelGGL(k_to_fortran5<T>, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, ksub, nk, nb, dev, tmp.p);
+    hipLaunchKernelGGL(k_to_fortran5<T>, dim3((tot + 255) / 256), dim3(256), 0, ctx->stream, ksub, nk, nb, dev, tmp.p, rmap);
     ALT3_HIP(hipGetLastError());
     ALT3_HIP(hipMemcpyAsync(host, tmp.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
 }
 
-template int copy_fortran5<double>(alt3_ctx *, const double *, double *);
+template int copy_fortran5<double>(alt3_ctx *, const double *, double *, const unsigned char *);
 
 // host Fortran (ksub,nk,nb,nb,nb) into a device array of the kernels' layout
 int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev)
@@ -492,6 +495,6 @@ int load_fortran5(alt3_ctx *ctx, int ksub, const double *host, double *dev)
     ALT3_HIP(hipStreamSynchronize(ctx->stream));
     return ALT3_OK;
 }
-template int copy_fortran5<float>(alt3_ctx *, const float *, double *);
+template int copy_fortran5<float>(alt3_ctx *, const float *, double *, const unsigned char *);
 
 }  // namespace alt3

@@ -11,7 +11,9 @@ GPU, on the same eigenvectors:
 * the range handling of the FP16 form: with the cubic force constants scaled by 2^20 or
   2^-30 (G' would overflow or underflow FP16 unscaled) |V3|^2 is the unscaled result times
   2^40 or 2^-60, bit for bit, since the power-of-two scales of G' absorb the factor exactly
-  (at 2^-60 but for elements that leave FP32's normal range in the final product).
+  (at 2^-60 but for elements that leave FP32's normal range in the final product);
+* the mixed path's P1sm/P2sm, whose zero rows the FGR does not store (the P row map), copied
+  out equal to the FP64 path's (1e-5 of the largest element, zero rows exactly zero).
 """
 import dataclasses
 
@@ -101,3 +103,25 @@ def test_f16x3_scale_invariance(cells, tmp_path_factory):
         big = np.abs(ref) > 1e-20 * mx
         assert np.array_equal(out[e][big], ref[big]), e
         assert np.abs(out[e] - ref).max() <= 1e-25 * mx, e
+
+
+@pytest.mark.gpu
+def test_mixed_p_rows_vs_fp64(cells, tmp_path_factory):
+    """The FP32 FGR leaves the rows of branches l1 outside an item's energy range unstored (the
+    P row map); copied out they must read as zeros, and every row must match the FP64 path's."""
+    print("case: rock-salt 4^3 P1sm/P2sm mixed (row map) vs FP64", flush=True)
+    inp = inputs_for(cells["rs8"], 4, tmp_path_factory, "rs8prow")
+    with capi.Context(inp) as c:
+        c.eig(want=False)
+        c.v3sq(0, c.nibz - 1, capi.V3_FP64, want=False)
+        P1d, P2d = c.fgr()
+        c.v3sq(0, c.nibz - 1, capi.V3_MIXED, want=False)
+        P1m, P2m = c.fgr()
+    for a, b, name in ((P1m, P1d, "P1sm"), (P2m, P2d, "P2sm")):
+        assert np.isfinite(a).all(), name
+        mx = np.abs(b).max()
+        assert np.abs(a - b).max() < 1e-5 * mx, (name, np.abs(a - b).max() / mx)
+        # rows (s, l1, l2, l3) that are zero on the FP64 path are zero on the mixed path
+        zd = np.all(b == 0, axis=1)
+        assert np.all(a.transpose(0, 2, 3, 4, 1)[zd] == 0), name
+        print(f"  {name}: max diff {np.abs(a - b).max() / mx:.2e} of the largest, {zd.mean():.0%} zero rows")
