@@ -82,14 +82,16 @@ constexpr int QT = 16;           // mesh points q2 per chunk
 constexpr int NW = 8;            // waves per workgroup
 constexpr int QW = QT / NW;      // points per wave in the U and V products
 constexpr int CH = 4;            // chunks of QT points per workgroup
+static_assert(QT * NB <= NW * 64, "one weight pair per thread");
 constexpr int TPW = (NTP + NW - 1) / NW;   // phase M-tiles per wave (5: waves 0-3, 4: waves 4-7)
 constexpr int KSPL = 32;         // 2 ndr <= 64: ndr <= 32
 // LDS: Phi'[q][plane][b][c] halves, planes Re hi, Im hi, Re lo, Im lo (the Im plane 32 banks from
 // the Re one: a U operand tile that reads Re rows b = 16..23 and Im rows b = 0..7 hits distinct
-// banks); rows of 24 halves, a point every 1160 dwords (8 mod 64: the 16 points a phase write
-// spreads over the lanes fall on distinct banks)
+// banks); rows of 24 halves, a point every 1156 dwords (4 mod 32, 16-byte aligned for the U
+// operand reads: the 16 points of a phase write, two dwords each, fall on two lanes per bank
+// pair at most)
 constexpr int PLANE = NB * NB;               // halves per plane
-constexpr int PSTR = 4 * PLANE + 16;         // halves per point
+constexpr int PSTR = 4 * PLANE + 8;          // halves per point
 // FP32 staging area of |V3|^2 [q][l2][l3]: l2 stride 27, a point every 650 floats (the V-tile
 // writes and the row reads of the output hit at most 3 and 2 lanes per bank with 32 banks, 2
 // with 64)
@@ -323,6 +325,32 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                 pul[t][nt] = eu[((t * 2 + nt) * 2 + 1) * 64];
             }
     };
+    // wq at q2 and q3 of this thread's (q, l) of the next chunk, loaded during U and V of the
+    // current one and stored as the weights once its epilogue has read them
+    double wn2 = 0.0, wn3 = 0.0;
+    auto load_w = [&](int ch) {
+        if (tid < QT * NB) {
+            const int q = tid / NB, l = tid % NB;
+            wn2 = wq[(size_t)min(q00 + ch * QT + q, nk - 1) * NB + l];
+            wn3 = wq[(size_t)k3s[ch * QT + q] * NB + l];
+        }
+    };
+    auto store_w = [&](float cs) {
+        if (tid < QT * NB) {
+            const int q = tid / NB, l = tid % NB;
+            if (!(wn2 <= 1.7e308) || !(wn3 <= 1.7e308)) bad = 1;
+            w2s[q][l] = (float)(wn2 * wn2);
+            w3s[q][l] = cs * (float)(wn3 * wn3);   // with the l1 factor
+        }
+    };
+    // const33^2/8 |l1|^(-1/2) (2^8 sg)^-2 (the scales come out); a zero eigenvalue in a triple
+    // (w = inf) is flagged from the tables, every triple of the tile being computed
+    auto cs_of = [&](int l1) {
+        const double w1 = wq[(size_t)k1 * NB + l1];
+        const float sg = g_scale(bound + s * NB + l1);
+        if (!(w1 <= 1.7e308)) bad = 1;
+        return (float)(c33sq8 / (65536.0 * (double)sg * (double)sg) * w1 * w1);
+    };
     auto load_v = [&](int ch, int q) {
         const h8 *ev = EV + (size_t)min(q00 + ch * QT + q, nk - 1) * EVF * 64 + lane;
 #pragma unroll
@@ -334,15 +362,13 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
             }
     };
     __syncthreads();
+    load_w(0);
+    store_w(cs_of(0));
     for (int l1 = 0; l1 < NB; l1++) {
         load_g(l1);
-        const double w1 = wq[(size_t)k1 * NB + l1];
-        const float sg = g_scale(bound + s * NB + l1);
-        // const33^2/8 |l1|^(-1/2) (2^8 sg)^-2 (the scales come out); a zero eigenvalue in a
-        // triple (w = inf) is flagged from the tables, every triple of the tile being computed
-        const float cs = (float)(c33sq8 / (65536.0 * (double)sg * (double)sg) * w1 * w1);
-        if (!(w1 <= 1.7e308) && tid == 0) bad = 1;
         for (int ch = 0; ch < nch; ch++) {
+            // the chunk after this one
+            const int nl1 = ch + 1 < nch ? l1 : l1 + 1, nc = ch + 1 < nch ? ch + 1 : 0;
             const int q0 = q00 + ch * QT;
             // ---- phase GEMM of the chunk: D[(b,c)][q] at lane q + 16 (m/4), m = 4 grp + i
             h8 erh[KT], erl[KT], eih[KT], eil[KT];
@@ -378,14 +404,8 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                 }
             }
             load_u(ch, wave * QW);
-            for (int x = tid; x < QT * NB; x += NW * 64) {
-                const int q = x / NB, l = x % NB;
-                const double a = wq[(size_t)min(q0 + q, nk - 1) * NB + l], b = wq[(size_t)k3s[ch * QT + q] * NB + l];
-                if (!(a <= 1.7e308) || !(b <= 1.7e308)) bad = 1;
-                w2s[q][l] = (float)(a * a);
-                w3s[q][l] = cs * (float)(b * b);   // with the l1 factor
-            }
             __syncthreads();
+            if (nl1 < NB) load_w(nc);
             // ---- U and V, one point at a time per wave
             for (int jq = 0; jq < QW; jq++) {
                 const int q = wave * QW + jq;
@@ -465,6 +485,7 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     }
             }
             __syncthreads();
+            if (nl1 < NB) store_w(cs_of(nl1));
             // ---- rows (l1, l2, l3) of the chunk: 16 consecutive q2 each, as float4
             float *o = out + (size_t)(s * NB + l1) * NN * nk + q0;
             for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
