@@ -486,14 +486,16 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
             }
             __syncthreads();
             if (nl1 < NB) store_w(cs_of(nl1));
-            // ---- rows (l1, l2, l3) of the chunk: 16 consecutive q2 each, as float4
+            // ---- rows (l1, l2, l3) of the chunk: 16 consecutive q2 each, as float4, stored
+            // non-temporal: the rows are not read again by this kernel, and cached they would
+            // push the eigenvector operands the workgroups of the XCD share out of its L2
             float *o = out + (size_t)(s * NB + l1) * NN * nk + q0;
             for (int x = tid; x < NN * (QT / 4); x += NW * 64) {
                 const int row = x / (QT / 4), c4 = x % (QT / 4);
                 if (q0 + 4 * c4 < nk) {
                     const float *src = Ost + 4 * c4 * OS + (row / NB) * LS + row % NB;
-                    const float4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
-                    *reinterpret_cast<float4 *>(o + (size_t)row * nk + 4 * c4) = val;
+                    const f4 val = {src[0], src[OS], src[2 * OS], src[3 * OS]};
+                    __builtin_nontemporal_store(val, reinterpret_cast<f4 *>(o + (size_t)row * nk + 4 * c4));
                 }
             }
         }
