@@ -105,9 +105,10 @@ constexpr int OS = NB * LS + 2;
 //   EV[k][t][nv][hl][lane]: 16 B_V[k = (p,b)][(p'',l2)], column lane%16 of tile nv: Re V(l2 = 0..15)
 //     (nv = 0), Im V(l2 = 0..15) (nv = 1), Re V(16..23) | Im V(16..23) (nv = 2); half j of lane group g
 //     holds row (p,b) = 4g + j (t = 0, j < 4), 16 + 4g + j - 4 (t = 0, j >= 4), 32 + 4g + j
-//     (t = 1, j < 4) -- the rows the U accumulators hold -- and zero for t = 1, j >= 4
+//     (t = 1, j < 4) -- the rows the U accumulators hold; the halves j >= 4 of step t = 1 are
+//     zero and not stored: that step's slots hold 8 bytes per lane, after the six of step 0
 constexpr int EUF = KT * 2 * 2;      // 16-byte operand slots per lane and point (EU)
-constexpr int EVF = KT * 3 * 2;      // (EV)
+constexpr int EVF = 3 * 2 + 3;       // (EV: six slots of step 0, six 8-byte slots of step 1)
 
 __device__ __forceinline__ int ev_row(int t, int g, int j)
 {
@@ -156,9 +157,15 @@ __global__ void k_c16_pack_e(int nk, const float *__restrict__ evf, h8 *__restri
             hi[j] = a;
             lo[j] = b;
         }
-        h8 *o = EV + ((size_t)k * EVF + (t * 3 + nv) * 2) * 64 + lane;
-        o[0] = hi;
-        o[64] = lo;
+        if (t == 0) {
+            h8 *o = EV + ((size_t)k * EVF + nv * 2) * 64 + lane;
+            o[0] = hi;
+            o[64] = lo;
+        } else {
+            h4 *o = reinterpret_cast<h4 *>(EV + ((size_t)k * EVF + 6) * 64) + nv * 2 * 64 + lane;
+            o[0] = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
+            o[64] = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
+        }
     }
 }
 
@@ -352,14 +359,15 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
         return (float)(c33sq8 / (65536.0 * (double)sg * (double)sg) * w1 * w1);
     };
     auto load_v = [&](int ch, int q) {
-        const h8 *ev = EV + (size_t)min(q00 + ch * QT + q, nk - 1) * EVF * 64 + lane;
+        const h8 *ev = EV + (size_t)min(q00 + ch * QT + q, nk - 1) * EVF * 64;
+        const h4 *ev1 = reinterpret_cast<const h4 *>(ev + 6 * 64) + lane;
 #pragma unroll
-        for (int t = 0; t < KT; t++)
-#pragma unroll
-            for (int nv = 0; nv < 3; nv++) {
-                pvh[t][nv] = ev[((t * 3 + nv) * 2) * 64];
-                pvl[t][nv] = ev[((t * 3 + nv) * 2 + 1) * 64];
-            }
+        for (int nv = 0; nv < 3; nv++) {
+            pvh[0][nv] = ev[(nv * 2) * 64 + lane];
+            pvl[0][nv] = ev[(nv * 2 + 1) * 64 + lane];
+            pvh[1][nv] = cat(ev1[(nv * 2) * 64], h4{});
+            pvl[1][nv] = cat(ev1[(nv * 2 + 1) * 64], h4{});
+        }
     };
     __syncthreads();
     load_w(0);
