@@ -100,14 +100,17 @@ constexpr int OS = NB * LS + 2;
 
 // e3 and e2 operands of every mesh point from the FP32 eigenvectors evf [k][l col][b comp]
 // (complex interleaved), as halves hi then lo, 16 bytes per lane:
-//   EU[k][t][nt][hl][lane]: 16 B_U[k = (p',c)][l3 = 16 nt + lane%16]; lane group g of step t holds
-//     the block kb = 4t + g: p' = kb / 3, c = 8 (kb % 3) + j (zero for kb >= 6 and l3 >= 24)
+//   EU[k][t][nt][hl][lane]: 16 B_U[k = (p',c)][l3 = 16 nt + lane%16] (zero for l3 >= 24), row
+//     k = 24 p' + c; half j of lane group g holds row 8g + j in step t = 0 and row 32 + 4g + j
+//     (p' = 1, c = 8 + 4g + j) in step t = 1, j < 4; the halves j >= 4 of step 1 are zero and
+//     not stored: that step's slots hold 8 bytes per lane, after the four of step 0
 //   EV[k][t][nv][hl][lane]: 16 B_V[k = (p,b)][(p'',l2)], column lane%16 of tile nv: Re V(l2 = 0..15)
 //     (nv = 0), Im V(l2 = 0..15) (nv = 1), Re V(16..23) | Im V(16..23) (nv = 2); half j of lane group g
 //     holds row (p,b) = 4g + j (t = 0, j < 4), 16 + 4g + j - 4 (t = 0, j >= 4), 32 + 4g + j
 //     (t = 1, j < 4) -- the rows the U accumulators hold; the halves j >= 4 of step t = 1 are
 //     zero and not stored: that step's slots hold 8 bytes per lane, after the six of step 0
-constexpr int EUF = KT * 2 * 2;      // 16-byte operand slots per lane and point (EU)
+constexpr int EUF = 2 * 2 + 2;       // 16-byte operand slots per lane and point (EU: four of
+                                     // step 0, four 8-byte slots of step 1)
 constexpr int EVF = 3 * 2 + 3;       // (EV: six slots of step 0, six 8-byte slots of step 1)
 
 __device__ __forceinline__ int ev_row(int t, int g, int j)
@@ -125,20 +128,26 @@ __global__ void k_c16_pack_e(int nk, const float *__restrict__ evf, h8 *__restri
     const float *e = evf + (size_t)k * NB * NB * 2;   // [l][b] complex
     h8 hi, lo;
     if (frag < KT * 2) {
-        const int t = frag / 2, nt = frag % 2;
-        const int kb = 4 * t + g, l3 = 16 * nt + (lane & 15);
+        const int t = frag / 2, nt = frag % 2, l3 = 16 * nt + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
+            const int row = t == 0 ? 8 * g + j : (j < 4 ? 32 + 4 * g + j : -1);
             float v = 0.0f;
-            if (kb < 6 && l3 < NB) v = e[2 * (l3 * NB + 8 * (kb % 3) + j) + kb / 3] * 16.0f;
+            if (row >= 0 && l3 < NB) v = e[2 * (l3 * NB + row % NB) + row / NB] * 16.0f;
             _Float16 a, b;
             split(v, a, b);
             hi[j] = a;
             lo[j] = b;
         }
-        h8 *o = EU + ((size_t)k * EUF + (t * 2 + nt) * 2) * 64 + lane;
-        o[0] = hi;
-        o[64] = lo;
+        if (t == 0) {
+            h8 *o = EU + ((size_t)k * EUF + nt * 2) * 64 + lane;
+            o[0] = hi;
+            o[64] = lo;
+        } else {
+            h4 *o = reinterpret_cast<h4 *>(EU + ((size_t)k * EUF + 4) * 64) + nt * 2 * 64 + lane;
+            o[0] = __builtin_shufflevector(hi, hi, 0, 1, 2, 3);
+            o[64] = __builtin_shufflevector(lo, lo, 0, 1, 2, 3);
+        }
     } else {
         const int fr = frag - KT * 2, t = fr / 3, nv = fr % 3;
         // columns: Re V(l2 = 0..15), Im V(l2 = 0..15), then Re V(16..23) | Im V(16..23)
@@ -323,14 +332,15 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
     // compiler issues them among U's products)
     h8 puh[KT][2], pul[KT][2], pvh[KT][3], pvl[KT][3];
     auto load_u = [&](int ch, int q) {
-        const h8 *eu = EU + (size_t)k3s[ch * QT + q] * EUF * 64 + lane;
+        const h8 *eu = EU + (size_t)k3s[ch * QT + q] * EUF * 64;
+        const h4 *eu1 = reinterpret_cast<const h4 *>(eu + 4 * 64) + lane;
 #pragma unroll
-        for (int t = 0; t < KT; t++)
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++) {
-                puh[t][nt] = eu[((t * 2 + nt) * 2) * 64];
-                pul[t][nt] = eu[((t * 2 + nt) * 2 + 1) * 64];
-            }
+        for (int nt = 0; nt < 2; nt++) {
+            puh[0][nt] = eu[(nt * 2) * 64 + lane];
+            pul[0][nt] = eu[(nt * 2 + 1) * 64 + lane];
+            puh[1][nt] = cat(eu1[(nt * 2) * 64], h4{});
+            pul[1][nt] = cat(eu1[(nt * 2 + 1) * 64], h4{});
+        }
     };
     // wq at q2 and q3 of this thread's (q, l) of the next chunk, loaded during U and V of the
     // current one and stored as the weights once its epilogue has read them
@@ -425,18 +435,26 @@ __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk,
                     for (int b = 0; b < 2; b++) u[a][b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < KT; t++) {
-                    // block kb = 4t + grp of k = (p',c): p' = kb / 3, c = 8 (kb % 3) .. + 7
-                    const int kb = 4 * t + grp, kbc = kb < 6 ? kb : 5, pp = kbc / 3, c0 = 8 * (kbc % 3);
+                    // k = (p',c) of lane group grp: 8 grp .. + 7 (t = 0: p' = grp / 3, c0 = 8 (grp % 3)),
+                    // 32 + 4 grp .. + 3 (t = 1: p' = 1, c0 = 8 + 4 grp; the upper four halves meet
+                    // zero e3 operands and are zero here too)
+                    const int pp = t == 0 ? grp / 3 : 1, c0 = t == 0 ? 8 * (grp % 3) : 8 + 4 * grp;
 #pragma unroll
                     for (int mt = 0; mt < 3; mt++) {
                         const int row = 16 * mt + col, p = row >= NB ? 1 : 0, b = row - p * NB;
-                        // Re Phi' on the diagonal blocks, -Im Phi' above, Im Phi' below; the lanes
-                        // of the zero blocks (kb >= 6) read finite halves of block 5, which meet
-                        // zero e3 operands
+                        // Re Phi' on the diagonal blocks, -Im Phi' above, Im Phi' below
                         const _Float16 *src = ph + (p == pp ? 0 : PLANE) + b * NB + c0;
                         const unsigned mask = (p == 0 && pp == 1) ? 0x80008000u : 0u;
-                        const h8 ah = flip_sign(*reinterpret_cast<const h8 *>(src), mask);
-                        const h8 al = flip_sign(*reinterpret_cast<const h8 *>(src + 2 * PLANE), mask);
+                        h8 ah, al;
+                        if (t == 0) {
+                            ah = *reinterpret_cast<const h8 *>(src);
+                            al = *reinterpret_cast<const h8 *>(src + 2 * PLANE);
+                        } else {
+                            ah = cat(*reinterpret_cast<const h4 *>(src), h4{});
+                            al = cat(*reinterpret_cast<const h4 *>(src + 2 * PLANE), h4{});
+                        }
+                        ah = flip_sign(ah, mask);
+                        al = flip_sign(al, mask);
 #pragma unroll
                         for (int nt = 0; nt < 2; nt++) u[mt][nt] = mma3(ah, al, puh[t][nt], pul[t][nt], u[mt][nt]);
                     }
