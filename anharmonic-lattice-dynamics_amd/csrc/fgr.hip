@@ -325,11 +325,19 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
         s_lo[tid >> 6] = lo;
         s_hi[tid >> 6] = hi;
     }
+    if (tid < nb) s_om[tid] = freq[(size_t)k1 * nb + tid];
+    // list counters: cleared here and at the end of every l1 pass that uses them
+    if (tid < 3) s_cnt[tid] = 0;
+    if (tid == 0) s_ncube = 0;
     __syncthreads();
     for (int w = 0; w < kFgrThreads / 64; w++) {
         lo = fminf(lo, s_lo[w]);
         hi = fmaxf(hi, s_hi[w]);
     }
+    // most items reach no omega(q1, l1) at all (their rows are all zero): they skip the cube
+    // ranges and masks below, and every l1 pass takes its zero-row path (uniform over the group)
+    bool any = false;
+    for (int l = 0; l < nb; l++) any |= omega_in(s_om[l], lo, hi);
     // energy range of cube k = tid + j*T; corners c + {0,e1,e2,e1+e2,e3,e3+e1,e3+e2,e3+e1+e2}
     // (pt1..pt8 of make_kp_reg_tet), tetrahedra (1,2,3,6) (2,3,4,6) (1,3,5,6) (3,4,6,8)
     // (3,5,6,7) (3,6,7,8)
@@ -339,7 +347,7 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
         const int k = tid + j * kFgrThreads;
         cmn[j] = 3.4e38f;
         cmx[j] = -3.4e38f;
-        if (k < nk) {
+        if (any && k < nk) {
             int c0, c1, c2;
             m.coords(k, c0, c1, c2);
             const int d0 = (c0 + 1 == m.n0) ? 0 : c0 + 1, d1 = (c1 + 1 == m.n1) ? 0 : c1 + 1,
@@ -357,13 +365,11 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
     VT *P = proc == 0 ? P1 : P2;
     // the branches l1 whose omega(q1, l1) falls in each cube's energy range, once per item
     // (bit l1 of cm[j] for cube k = tid + j*T), so that each l1 pass visits only its cubes
-    if (tid < nb) s_om[tid] = freq[(size_t)k1 * nb + tid];
-    __syncthreads();
     MT cm[kFgrPer];
 #pragma unroll
     for (int j = 0; j < kFgrPer; j++) {
         cm[j] = 0;
-        if (tid + j * kFgrThreads < nk)
+        if (any && tid + j * kFgrThreads < nk)
             for (int l = 0; l < nb; l++)
                 if (omega_in(s_om[l], cmn[j], cmx[j])) cm[j] |= (MT)1 << l;
     }
@@ -429,11 +435,11 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
                 v[y] = m.index(e0, e1, e2);
             }
             const float w4[4] = {en[v[0]], en[v[1]], en[v[2]], en[v[3]]};
-            if constexpr (sizeof(VT) == sizeof(float)) {
+            if constexpr (sizeof(VT) == sizeof(float)) {   // W in units of 1/6 (the 1/6 is in F)
                 float c[4];
                 weight_tet1f(w4, e, c);
                 for (int y = 0; y < 4; y++)
-                    if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)(c[y] / 6.0f));
+                    if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)c[y]);
             } else {
                 double c[4];
                 weight_tet1(w4, e, c);
@@ -495,9 +501,6 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             // few: they are compacted into LDS lists so that the weight formula runs with
             // every lane busy.  The lists normally hold all of them; if one overflows, the
             // cubes are taken again chunk by chunk (at most 6 entries per cube and thread).
-            if (tid < 3) s_cnt[tid] = 0;
-            if (tid == 0) s_ncube = 0;
-            __syncthreads();
             // the cubes of this l1, compacted (wave-aggregated appends)
 #pragma unroll
             for (int j = 0; j < kFgrPer; j++) {
@@ -514,8 +517,10 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             }
             __syncthreads();
             const int ncube = s_ncube;
+            // the waves whose lanes hold no cube skip the pushes (their ballots would be empty)
 #pragma unroll 1
-            for (int x0 = 0; x0 < ncube; x0 += kFgrThreads) push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
+            for (int x0 = 0; x0 < ncube; x0 += kFgrThreads)
+                if (x0 + (tid & ~63) < ncube) push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
             __syncthreads();
             const int n0 = s_cnt[0], n1 = s_cnt[1], n2 = s_cnt[2];
             if (n0 <= kFgrCase && n1 <= kFgrCase && n2 <= kFgrCase) {
@@ -526,7 +531,7 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
                     __syncthreads();
                     if (tid < 3) s_cnt[tid] = 0;
                     __syncthreads();
-                    push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
+                    if (x0 + (tid & ~63) < ncube) push(x0 + tid < ncube ? s_cube[x0 + tid] : 0, x0 + tid < ncube, omega1);
                     __syncthreads();
                     process(s_cnt[0], s_cnt[1], s_cnt[2], omega1);
                 }
@@ -534,13 +539,14 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             __syncthreads();
         }
         // P(q_w) = W(q_w) F(q_w) (and the mirrored row W(q1-q_w) F'(q_w)); the weight sums are
-        // cleared once every thread has read them
+        // cleared once every thread has read them (at once where no thread reads another's)
+        const float fs6 = (float)(fscale / 6.0);
 #pragma unroll
         for (int j = 0; j < kFgrPer; j++) {
             const int w = tid + j * kFgrThreads;
             if (w < nk) {
                 if constexpr (sizeof(VT) == sizeof(float)) {   // FP32 P: the product in FP32
-                    const float F = (vzero >> j & 1u) ? 0.0f : fv[j] * (float)fscale;
+                    const float F = (vzero >> j & 1u) ? 0.0f : fv[j] * fs6;
                     P[base + w] = (float)W[w] * F;
                 } else {
                     double F = (double)fv[j] * tpi2 / nk;
@@ -551,17 +557,23 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
                     int c0, c1, c2;
                     m.coords(w, c0, c1, c2);
                     const int p = m.diff(a0, a1, a2, c0, c1, c2);
-                    const float Fm = (vzerom >> j & 1u) ? 0.0f : (float)fvm[j] * (float)fscale;
+                    const float Fm = (vzerom >> j & 1u) ? 0.0f : (float)fvm[j] * fs6;
                     P[basem + w] = (VT)((float)W[p] * Fm);
+                } else {
+                    W[w] = 0.0;
                 }
             }
         }
-        if (mirror) __syncthreads();   // uniform over the group
+        if (mirror) {   // uniform over the group
+            __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kFgrPer; j++) {
-            const int w = tid + j * kFgrThreads;
-            if (w < nk) W[w] = 0.0;
+            for (int j = 0; j < kFgrPer; j++) {
+                const int w = tid + j * kFgrThreads;
+                if (w < nk) W[w] = 0.0;
+            }
         }
+        if (tid < 3) s_cnt[tid] = 0;
+        if (tid == 0) s_ncube = 0;
         __syncthreads();
     }
 }

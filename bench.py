@@ -193,6 +193,49 @@ def profiled_traffic(kernel_prefix: str, profile: str = "sw_si_16cubed_mixed_hbm
     return None
 
 
+def profiled_counter(kernel_prefix: str, counter: str, profile: str):
+    """Per-launch value of a shader counter from a committed rocprofv3 --pmc summary (a
+    markdown table with one column per counter), or None."""
+    path = os.path.join(REPO, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    cols = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if line.startswith("| kernel |"):
+            cols = cells
+        elif cols and line.startswith("| `") and counter in cols and cells[0].strip("`").split("<")[0] == kernel_prefix:
+            return float(cells[cols.index(counter)])
+    return None
+
+
+# VALU issue rate of the chip: a wave64 FP32/FP64 vector instruction occupies a 16-lane SIMD
+# for 4 cycles; 256 CUs x 4 SIMDs at 2.4 GHz
+PEAK_VALU_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def fgr_roofline(nel: int, pb: int, fgr_ms: float, traffic_profile, sq_profile) -> dict:
+    """The tetrahedron FGR kernel against HBM on the algorithmic bytes (read |V3|^2, write
+    P1sm + P2sm: 3 pb bytes per element), with what it actually moves and issues beside it."""
+    fgr_bytes = nel * 3 * pb
+    fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
+    traffic = profiled_traffic("k_fgr", traffic_profile) if traffic_profile else None
+    valu = profiled_counter("k_fgr", "SQ_INSTS_VALU", sq_profile) if sq_profile else None
+    r = {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": fgr_gbs / PEAK_HBM_GBS,
+         "traffic": traffic, "kernel": "k_fgr", "kernel_ms": fgr_ms, "algorithmic_bytes_per_launch": fgr_bytes}
+    if traffic is not None:
+        r["measured_hbm_gbs"] = traffic / (fgr_ms * 1e-3) / 1e9
+    if valu is not None:
+        r["valu_issue"] = {"wave_insts_per_launch": valu, "peak_wave_insts_per_s": PEAK_VALU_WAVE_INSTS,
+                           "frac": valu / (fgr_ms * 1e-3) / PEAK_VALU_WAVE_INSTS}
+    if pb == 4:
+        r["note"] = ("achieved counts the full |V3|^2 and P arrays; the FP32 path skips the rows whose l1 lies "
+                     "outside an item's energy range (P row map), so the kernel moves 'traffic' bytes and is "
+                     "bound by its vector ALU work (tetrahedron weights, list pushes, LDS atomics): "
+                     "valu_issue.frac is SQ_INSTS_VALU over the chip's wave64 VALU issue rate")
+    return r
+
+
 def kappa_iso(k):
     return float(k.sum(0).trace() / 3)
 
@@ -276,8 +319,7 @@ def bench_sw_si(args, with_cpu_baseline: bool) -> dict:
     # dominant kernel of the step: the tetrahedron FGR kernel (one launch per step)
     v3_bytes = 4 if head_name == "mixed" else 8   # |V3|^2 and P1sm/P2sm share the precision
     fgr_ms = float(head["stage"][2])
-    fgr_bytes = nel * (v3_bytes + 2 * v3_bytes)  # read |V3|^2, write P1sm + P2sm
-    fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
+    prof = n == 16 and head_name == "mixed"
     res = {
         "metric": "3-ph V3 elements/s (mixed vs FP64) at kappa err <=0.5%; p50/p99 kappa-solve latency",
         "value": nel / (head["ms"] * 1e-3),
@@ -289,11 +331,8 @@ def bench_sw_si(args, with_cpu_baseline: bool) -> dict:
         "config": {"workload": f"SW-Si 2-atom diamond, {n}^3 mesh, tetrahedron FGR + CG BTE at {temps} K",
                    **sizes, "v3_elements": nel, "precision": head_name},
         "latency_ms": {"p50": head["p50"], "p99": head["p99"]},
-        "roofline": {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": fgr_gbs / PEAK_HBM_GBS,
-                     "traffic": profiled_traffic("k_fgr") if (n == 16 and head_name == "mixed") else None,
-                     "kernel": "k_fgr",
-                     "algorithmic_bytes_per_launch": fgr_bytes, "kernel_ms": fgr_ms},
+        "roofline": fgr_roofline(nel, v3_bytes, fgr_ms, "sw_si_16cubed_mixed_hbm_traffic.md" if prof else None,
+                                 "sw_si_16cubed_mixed_sq_counters.md" if prof else None),
     }
     for name, r in runs.items():
         res[name] = summary(r)
@@ -432,8 +471,6 @@ def bench_rocksalt(args) -> dict:
     tflops = f_elem * nel / (v3_ms * 1e-3) / 1e12
     fgr_ms = head["fgr_kernel_ms"]
     pb = 4 if head_name == "mixed" else 8
-    fgr_bytes = nel * 3 * pb
-    fgr_gbs = fgr_bytes / (fgr_ms * 1e-3) / 1e9
     if head_name == "mixed":
         # k_v3_chain16: FP16 MFMA on split (hi + lo) operands, three products per product.  Per
         # (q1, l1, tile of 16 points): 36 phase M-tiles x 2 k-steps x 3 x (Re, Im), per point U
@@ -461,11 +498,8 @@ def bench_rocksalt(args) -> dict:
                    "traffic": None, "kernel": "k_v3_main", "kernel_ms": v3_ms, "flops_per_element": f_elem,
                    "executed_flops_per_element": f_exec, "counts": {"nt3": nt3, "n_cp": ncp, "ndr": ndr},
                    "algorithmic_flops_per_launch": f_elem * nel}
-    fgr_roof = {"bound": "hbm", "achieved": fgr_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": fgr_gbs / PEAK_HBM_GBS, "kernel": "k_fgr", "kernel_ms": fgr_ms,
-                "algorithmic_bytes_per_launch": fgr_bytes,
-                "traffic": profiled_traffic("k_fgr", "rs8_12cubed_mixed_hbm_traffic.md")
-                if head_name == "mixed" else None}
+    fgr_roof = fgr_roofline(nel, pb, fgr_ms, "rs8_12cubed_mixed_hbm_traffic.md" if head_name == "mixed" else None,
+                            "rs8_12cubed_mixed_sq_counters.md" if head_name == "mixed" else None)
 
     def summary(r):
         return {"value": nel / (r["ms"] * 1e-3), "ms_per_step": r["ms"],
