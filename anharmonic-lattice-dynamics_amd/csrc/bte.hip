@@ -235,8 +235,30 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
             d1[t] = dist[t * dstr + (size_t)k1 * NB + l1];
             q[t] = qn[t] = qu[t] = 0;
         }
+        // the branches l2 with a stored P row (FP32 path with a row map; every l2 otherwise):
+        // the others (44 % of the (q1, l1, l2) blocks on rock-salt 12^3) have R1 = R2 = 0 and add
+        // nothing, so the loop below visits only these (the sums skip only additions of zero)
+        unsigned live = (1u << NB) - 1;
+        if constexpr (sizeof(PT) == sizeof(float)) {
+            if (rmb) {
+                live = 0;
+                for (int l2 = 0; l2 < NB; l2++)
+                    if (rmb[((size_t)s * NB + l1) * NB + l2]) live |= 1u << l2;
+                if (valid) {
+                    for (unsigned z = ~live & ((1u << NB) - 1); z; z &= z - 1) {
+                        const size_t o = (((size_t)s * NB + l1) * NB + __builtin_ctz(z)) * (size_t)nk + j;
+#pragma unroll
+                        for (int t = 0; t < NT; t++) {
+                            R1[t * rstr + o] = (PT)0;
+                            R2[t * rstr + o] = (PT)0;
+                        }
+                    }
+                }
+            }
+        }
 #pragma unroll 1
-        for (int l2 = 0; l2 < NB; l2++) {
+        for (unsigned lv = live; lv; lv &= lv - 1) {
+            const int l2 = __builtin_ctz(lv);
             const size_t rid = (((size_t)s * NB + l1) * NB + l2) * NB;                   // row map (s,l1,l2,l3=0)
             const size_t row = rid * (size_t)nk + jc;                                       // (s,l1,l2,l3=0)
             // rows the row map marks zero are not stored (rm NULL: all stored): bit l3 of b1, b2,
