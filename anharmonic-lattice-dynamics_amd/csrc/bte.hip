@@ -265,6 +265,7 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
             // the same for the whole group; their loads go to one cached line of zeros (pzero),
             // so they cost no HBM traffic and no branch
             PT P1[NB], P2[NB];
+            unsigned bb = (1u << NB) - 1;   // l3 with a stored P1 or P2 row
             if constexpr (sizeof(PT) == sizeof(float)) {   // the FP64 FGR stores every row
                 unsigned b1 = (1u << NB) - 1, b2 = b1;
                 if (rmb) {   // the packed map: P1 rows in the low word, P2 rows in the high one
@@ -272,6 +273,7 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
                     b1 = (unsigned)w;
                     b2 = (unsigned)(w >> 32);
                 }
+                bb = b1 | b2;
 #pragma unroll
                 for (int l3 = 0; l3 < NB; l3++) {   // an unstored row reads the zero line instead
                     P1[l3] = *((b1 >> l3 & 1u) ? P1sm + row + (size_t)l3 * nk : pzero);
@@ -300,15 +302,21 @@ __global__ void __launch_bounds__(256) k_tau_q2(alt3::MeshDev m, int nk, int ibz
             for (int t = 0; t < NT; t++) {
                 const double f1 = d1[t] * dd[t], f2 = d1[t] * (dd[t] + 1);
                 double r1 = 0, r2 = 0;
+                // chunks of eight l3 without a stored row add only zeros and are skipped (uniform)
 #pragma unroll
-                for (int l3 = 0; l3 < NB; l3++) {
-                    const double p1 = (double)P1[l3] * (f1 * n3p[t][l3]);
-                    const double p2 = (double)P2[l3] * (f2 * n3m[t][l3]);
-                    q[t] = q[t] + (p1 + 0.5 * p2);
-                    if (in1) qn[t] += p1; else qu[t] += p1;
-                    if (in2) qn[t] += 0.5 * p2; else qu[t] += 0.5 * p2;
-                    r1 += p1;
-                    r2 += p2;
+                for (int c = 0; c < NB; c += 8) {
+                    if (bb >> c & 0xffu) {
+#pragma unroll
+                        for (int l3 = c; l3 < NB && l3 < c + 8; l3++) {
+                            const double p1 = (double)P1[l3] * (f1 * n3p[t][l3]);
+                            const double p2 = (double)P2[l3] * (f2 * n3m[t][l3]);
+                            q[t] = q[t] + (p1 + 0.5 * p2);
+                            if (in1) qn[t] += p1; else qu[t] += p1;
+                            if (in2) qn[t] += 0.5 * p2; else qu[t] += 0.5 * p2;
+                            r1 += p1;
+                            r2 += p2;
+                        }
+                    }
                 }
                 if (valid) {
                     const size_t o = (((size_t)s * NB + l1) * NB + l2) * (size_t)nk + j;
