@@ -438,8 +438,8 @@ __global__ void __launch_bounds__(kFgrThreads, (sizeof(VT) == 4 && kFgrPer <= 4)
             if constexpr (sizeof(VT) == sizeof(float)) {   // W in units of 1/6 (the 1/6 is in F)
                 float c[4];
                 weight_tet1f(w4, e, c);
-                for (int y = 0; y < 4; y++)
-                    if (c[y] != 0.0f) atomicAdd(&W[v[y]], (double)c[y]);
+                // a zero weight adds +-0, which leaves W as it is: no test before the atomics
+                for (int y = 0; y < 4; y++) atomicAdd(&W[v[y]], (double)c[y]);
             } else {
                 double c[4];
                 weight_tet1(w4, e, c);
