@@ -252,7 +252,7 @@ __device__ __forceinline__ float xor8(float x)
 // chunks; per chunk the phase GEMM into LDS (as split halves), barrier, U and V per point with
 // |V3|^2 into the staging area, barrier, rows out.  Work items (tile, s) are dealt to the XCDs in
 // contiguous ranges, tile-major: the workgroups resident on one XCD at a time hold one or two
-// tiles of points and read their eigenvector operands (20 KB a point) from that XCD's L2.
+// tiles of points and read their eigenvector operands (15 KB a point) from that XCD's L2.
 __global__ void __launch_bounds__(NW * 64) k_v3_chain16(alt3::MeshDev m, int nk, int ibz_lo, int ksub, int ndr,
                                                        const int *__restrict__ mapinv, const float *__restrict__ ephf,
                                                        const h8 *__restrict__ Gh, const h8 *__restrict__ EU,
