@@ -110,25 +110,22 @@ __device__ void weight_tet1(const float *w, double e, double *c)
         cc[1] = (d1 * d1 * d1) / (double)(e21 * (e31 * e31) * e41);
         cc[0] = (d1 * d1 * d1) / (double)(e21 * e31 * (e41 * e41));
     } else if (E2 <= e && e <= E3) {
-        // the reference's quotients, each denominator (a REAL*4 product) inverted once: ten
-        // FP64 reciprocals instead of twenty divisions; the quotients are rounded to REAL*4
-        const double r31 = 1.0 / (double)(e31 * e31), r41 = 1.0 / (double)(e41 * e41);
-        const double r32 = 1.0 / (double)(e32 * e32), r42 = 1.0 / (double)(e42 * e42);
-        float c11 = (float)((E3 - e) * r31);
-        float c12 = (float)((E4 - e) * r41);
-        float c21 = (float)((E3 - e) * r32);
-        float c22 = (float)((E4 - e) * r42);
-        float c31 = (float)((e - E2) * r32);
-        float c32 = (float)((e - E1) * r31);
-        float c41 = (float)((e - E2) * r42);
-        float c42 = (float)((e - E1) * r41);
+        // the reference's quotients as divisions, rounded to REAL*4 (a reciprocal times the
+        // numerator differs by an ulp of FP64 now and then, and near a degenerate corner pair that
+        // ulp can move the REAL*4 rounding of a quotient whose terms nearly cancel in P)
+        float c11 = (float)((E3 - e) / (double)(e31 * e31));
+        float c12 = (float)((E4 - e) / (double)(e41 * e41));
+        float c21 = (float)((E3 - e) / (double)(e32 * e32));
+        float c22 = (float)((E4 - e) / (double)(e42 * e42));
+        float c31 = (float)((e - E2) / (double)(e32 * e32));
+        float c32 = (float)((e - E1) / (double)(e31 * e31));
+        float c41 = (float)((e - E2) / (double)(e42 * e42));
+        float c42 = (float)((e - E1) / (double)(e41 * e41));
         const double a32 = (E3 - e) * (e - E2), a41 = (E4 - e) * (e - E1), a31 = (E3 - e) * (e - E1), a42 = (E4 - e) * (e - E2);
-        const double i1 = 1.0 / (double)(e42 * e32), i2 = 1.0 / (double)(e41 * e42), i3 = 1.0 / (double)(e32 * e41);
-        const double i4 = 1.0 / (double)(e42 * e31), i5 = 1.0 / (double)(e31 * e32), i6 = 1.0 / (double)(e31 * e41);
-        float b11 = (float)(a32 * i1 + a41 * i2 + a31 * i3);
-        float b12 = (float)(a41 * i4 + a42 * i1 + a31 * i5);
-        float b21 = (float)(a32 * i4 + a42 * i2 + a31 * i6);
-        float b22 = (float)(a32 * i5 + a41 * i6 + a42 * i3);
+        float b11 = (float)(a32 / (double)(e42 * e32) + a41 / (double)(e41 * e42) + a31 / (double)(e32 * e41));
+        float b12 = (float)(a41 / (double)(e42 * e31) + a42 / (double)(e42 * e32) + a31 / (double)(e31 * e32));
+        float b21 = (float)(a32 / (double)(e42 * e31) + a42 / (double)(e42 * e41) + a31 / (double)(e31 * e41));
+        float b22 = (float)(a32 / (double)(e32 * e31) + a41 / (double)(e41 * e31) + a42 / (double)(e32 * e41));
         float b31 = b21;
         float b32 = b11;
         float b41 = b22;

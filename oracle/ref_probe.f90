@@ -198,6 +198,42 @@ program ref_probe
     write(*,'(a,i0,a,es16.8)') 'REF_PROBE_V3 elements=', nv3_split, ' seconds=', dble(c1-c0)/dble(crate)
     stop
  endif
+ ! block mode: REF_PROBE_SUBSET="lo hi" runs V3, the FGR and calculate_tauRTA_mpi2 on that IBZ
+ ! block only, as one rank of kappa.f90 does with its ksubset (kappa.f90:420-431), dumps
+ ! v33s8, P1sm, P2sm and per temperature the block's Qvaluempi/_N/_U (ksub x ndyn) as
+ ! T<t>_Qsub, T<t>_QNsub, T<t>_QUsub, and stops
+ call get_environment_variable('REF_PROBE_SUBSET', envs, status=ios_)
+ if (ios_ == 0 .and. len_trim(envs) > 0) then
+    read(envs,*) ksubset(1), ksubset(2)
+    ksub = ksubset(2) - ksubset(1) + 1
+    nv3_split = nkc*ksub*ndyn**3
+    call allocate_v33sq(nv3_split)
+    allocate(v33s8(ksub,nkc,ndyn,ndyn,ndyn))
+    call calculate_w3_ibz_split_sq(ksubset,nv3_split,ndyn,nkc,kpc,eigenval,eigenvec,nv3o)
+    call dumpr('v33s8', v33s8, size(v33s8))
+    allocate(P1smpi(ksub,nkc,ndyn,ndyn,ndyn), P2smpi(ksub,nkc,ndyn,ndyn,ndyn))
+    P1smpi = 0; P2smpi = 0
+    call calculate_FGR_mpi2(ksubset,ndyn,nv3o)
+    call dumpr('P1sm', P1smpi, size(P1smpi))
+    call dumpr('P2sm', P2smpi, size(P2smpi))
+    call dumpr('frequency', frequency, nkc*ndyn)
+    allocate(Qvaluempi(ksub,ndyn), Qvalue_Nmpi(ksub,ndyn), Qvalue_Umpi(ksub,ndyn))
+    open(78, file='probe_temps.txt', status='old')
+    do
+       read(78,*,iostat=ios_) tempk
+       if (ios_ /= 0) exit
+       write(tag,'(a,i0)') 'T', nint(tempk)
+       call allocate_FGR(ksub,nkc,ndyn,ndyn,ndyn)
+       call distribution(nkc,ndyn,tempk)
+       call calculate_tauRTA_mpi2(ksubset,ndyn,tempk)
+       call dumpr(trim(tag)//'_Qsub', Qvaluempi, ksub*ndyn)
+       call dumpr(trim(tag)//'_QNsub', Qvalue_Nmpi, ksub*ndyn)
+       call dumpr(trim(tag)//'_QUsub', Qvalue_Umpi, ksub*ndyn)
+    enddo
+    close(78)
+    write(*,'(a,i0,a,i0)') 'REF_PROBE_SUBSET done ', ksubset(1), ' ', ksubset(2)
+    stop
+ endif
  ksub = nibz
  nv3_split = nkc*ksub*ndyn**3
  call allocate_v33sq(nv3_split)

@@ -10,6 +10,7 @@ dumps are stored as compressed .npz next to this script.
     python tests/golden/make_golden.py self       # si4_self, si8_self: self-energy at given q
     python tests/golden/make_golden.py iso        # si4_iso, zb4_iso: isotope scattering (iso = 1)
     python tests/golden/make_golden.py si16       # si16: the SW-Si 16^3 workload at 100/300/800 K
+    python tests/golden/make_golden.py si16blocks # si16_blocks: three irreducible blocks of it
     python tests/golden/make_golden.py rs8        # rs8_4: the 8-atom rock-salt-type cell at 4^3
     python tests/golden/make_golden.py born       # zb4_born: Born charges (nonanal) on the 4-atom cell
     python tests/golden/make_golden.py writekappa # write_kappa_si4/: the reference's own output files
@@ -222,6 +223,47 @@ def main_si16():
         print("wrote si16", iters)
 
 
+def main_si16blocks():
+    """The SW-Si 16^3 workload (276 irreducible points) on three one-point irreducible blocks,
+    q1 = 1, 138 and 276 (1-based), each run as one rank of kappa.f90 runs its ksubset
+    (REF_PROBE_SUBSET): the tables and eigen data, |V3|^2 and P on every eighth mesh point, and
+    the blocks' Q, QN and QU at 100, 300 and 800 K.  About a minute of one core per block (the
+    three blocks run side by side)."""
+    import concurrent.futures as cf
+    blocks = (1, 138, 276)
+
+    def run(q):
+        wd = tempfile.mkdtemp(prefix=f"si16b{q}_")
+        refio.write_inputs(wd, models.sw_silicon(), refio.RunParams(nc=(16, 16, 16), tmin=300.0, tmax=300.0))
+        with open(os.path.join(wd, "probe_temps.txt"), "w") as f:
+            f.write("\n".join(str(t) for t in TEMPS) + "\n")
+        env = dict(os.environ, REF_PROBE_SUBSET=f"{q} {q}", REF_PROBE_ITER="0")
+        subprocess.run([PROBE], cwd=wd, check=True, capture_output=True, text=True, env=env)
+        return wd, load_dump(os.path.join(wd, "dump"), ksub=1)
+
+    with cf.ThreadPoolExecutor(len(blocks)) as ex:
+        runs = list(ex.map(run, blocks))
+    D = runs[0][1]
+    q2_sel = np.arange(0, D["nk"], 8)
+    out = dict(nk=np.array(D["nk"]), nibz=np.array(D["nibz"]), nb=np.array(D["nb"]), kpc_mesh=D["kpc_mesh"],
+               kpc=D["kpc"], mapibz=D["mapibz"], mapinv=D["mapinv"], mappos=D["mappos"], npos=np.array(D["npos"]),
+               kibz=D["kibz"], wibz=D["wibz"], eigenval=D["eigenval"], eigenvec=D["eigenvec"], veloc=D["veloc"],
+               op_kmatrix=D["op_kmatrix"], tet=D["tet"], star_n=D["star_n"], star_op=D["star_op"],
+               star_n0=D["star_n0"], frequency=D["frequency"], temps=np.array(TEMPS),
+               ibz_sel=np.array(blocks) - 1, q2_sel=q2_sel)
+    for k in ("v33s8", "P1sm", "P2sm"):
+        out[k] = np.concatenate([np.asfortranarray(Db[k][:, q2_sel]) for _, Db in runs], axis=0)
+    for T in TEMPS:
+        t = f"T{int(T)}"
+        for k in ("Q", "QN", "QU"):
+            out[f"{t}_{k}"] = np.concatenate([Db[t][k + "sub"] for _, Db in runs], axis=0)
+    for wd, _ in runs:
+        import shutil
+        shutil.rmtree(wd, ignore_errors=True)
+    np.savez_compressed(os.path.join(HERE, "ref_si16_blocks.npz"), **out)
+    print("wrote si16_blocks", {k: v.shape for k, v in out.items() if k in ("v33s8", "T300_Q")})
+
+
 def main_rs8():
     """The 8-atom rock-salt-type cell (24 branches, Born-Mayer-type springs, seed 0) on the
     4^3 mesh at 300 K: everything of _config_pack, |V3|^2 and P on three irreducible points
@@ -295,6 +337,8 @@ def main():
         return main_born()
     if sys.argv[1:] == ["si16"]:
         return main_si16()
+    if sys.argv[1:] == ["si16blocks"]:
+        return main_si16blocks()
     if sys.argv[1:] == ["rs8"]:
         return main_rs8()
     if sys.argv[1:] == ["iso"]:

@@ -7,9 +7,10 @@ import os
 import numpy as np
 
 
-def load_dump(d: str) -> dict:
+def load_dump(d: str, ksub: int | None = None) -> dict:
     """Arrays in the reference's column-major layouts (numpy order='F'); indices 1-based as
-    dumped (mapibz, mapinv, tet, star_n0, mappos)."""
+    dumped (mapibz, mapinv, tet, star_n0, mappos).  ksub: the size of the IBZ block of a
+    REF_PROBE_SUBSET run (|V3|^2, P and the block's Q rows T<t>_Qsub/_QNsub/_QUsub)."""
     with open(os.path.join(d, "dims.txt")) as f:
         nk, nibz, nb, nat0, nt2, nt3, nops, npos = (int(x) for x in f.read().split())
     out = dict(nk=nk, nibz=nibz, nb=nb, natoms0=nat0, nt2=nt2, nt3=nt3, nops=nops, npos=npos)
@@ -37,15 +38,20 @@ def load_dump(d: str) -> dict:
     out["star_n"] = r("star_n", (nibz,), np.int32)
     out["star_op"] = r("star_op", (48, nibz), np.int32)
     out["star_n0"] = r("star_n0", (48, nibz), np.int32)
-    out["v33s8"] = r("v33s8", (nibz, nk, nb, nb, nb))
-    out["P1sm"] = r("P1sm", (nibz, nk, nb, nb, nb))
-    out["P2sm"] = r("P2sm", (nibz, nk, nb, nb, nb))
+    nblk = nibz if ksub is None else ksub
+    out["v33s8"] = r("v33s8", (nblk, nk, nb, nb, nb))
+    out["P1sm"] = r("P1sm", (nblk, nk, nb, nb, nb))
+    out["P2sm"] = r("P2sm", (nblk, nk, nb, nb, nb))
     out["frequency"] = r("frequency", (nk, nb))
     temps = sorted({int(f.split("_")[0][1:]) for f in os.listdir(d) if f.startswith("T") and f.endswith(".bin")})
     out["temps"] = temps
     N = nibz * nb * 3
     for T in temps:
         t = f"T{T}"
+        if ksub is not None:
+            out[t] = dict(Qsub=r(f"{t}_Qsub", (ksub, nb)), QNsub=r(f"{t}_QNsub", (ksub, nb)),
+                          QUsub=r(f"{t}_QUsub", (ksub, nb)))
+            continue
         out[t] = dict(
             dist=r(f"{t}_dist", (nk, nb)), Q=r(f"{t}_Q", (nibz, nb)), QN=r(f"{t}_QN", (nibz, nb)),
             QU=r(f"{t}_QU", (nibz, nb)), tauN=r(f"{t}_tauN", (nibz, nb)), tauU=r(f"{t}_tauU", (nibz, nb)),

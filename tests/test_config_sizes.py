@@ -8,7 +8,10 @@
   the device (bench.py) and its cell against the reference here.
 * the SW-Si 16^3 workload (ref_si16.npz: tables, eigen data, |V3|^2 and P on three irreducible
   points and every eighth mesh point, every per-mode and per-temperature result at 100, 300 and
-  800 K and the CG iteration counts).
+  800 K and the CG iteration counts); and three one-point irreducible blocks of it
+  (ref_si16_blocks.npz: the reference run as one rank runs its ksubset, q1 = 1, 138, 276:
+  tables, eigen data, |V3|^2 and P on every eighth mesh point, the blocks' Q, QN, QU at 100,
+  300 and 800 K).
 
 CPU: mesh and symmetry tables bit-exact (host-only contexts); the oracle's |V3|^2 on the
 reference eigen data for one irreducible point of the rock-salt cell (1e-10).
@@ -59,6 +62,13 @@ def si16ref():
     return load_fixture("si16")
 
 
+@pytest.fixture(scope="module")
+def si16blocks():
+    if not _have("si16_blocks"):
+        pytest.skip("ref_si16_blocks.npz not written (make_golden.py si16blocks)")
+    return load_fixture("si16_blocks")
+
+
 def _tables(inp, R):
     with capi.Context(inp, host_only=True) as h:
         m = h.mesh()
@@ -79,6 +89,10 @@ def test_rs8_tables_bit_exact(rs8inp, rs8ref):
 
 def test_si16_tables_bit_exact(si_inputs, si16ref):
     _tables(si_inputs[16], si16ref)
+
+
+def test_si16_blocks_tables_bit_exact(si_inputs, si16blocks):
+    _tables(si_inputs[16], si16blocks)
 
 
 def test_rs8_oracle_v3_vs_reference(rs8inp, rs8ref):
@@ -163,3 +177,43 @@ def test_si16_path_vs_reference(si_inputs, si16ref):
         for k in ("kappa_rta", "kappa_bte"):
             a, b = np.trace(om[k][i].sum(0)) / 3, np.trace(o64[k][i].sum(0)) / 3
             assert abs(a / b - 1) < 1e-5, (T, k, a, b)
+
+
+def _l1_kept(mapinv, q, nb, gamma):
+    """Branches l1 compared at irreducible point q: all but the acoustic ones when q is Gamma.
+    There |V3|^2 of an acoustic l1 is the round-off of the acoustic sum rule divided by the
+    round-off frequency w(Gamma, l1) ~ 1e-6, and the reference's FGR masks acoustic Gamma modes
+    only as q2 and q3 (sub.f90:7349-7356), so its P and Q rows for them carry that round-off:
+    the batched V3 sums in another order and gets a round-off of the same size but not the
+    same value (1 % apart on the element of (Gamma, Gamma, 0, 4, 5)).  The |V3|^2 check, at
+    1e-8 of the largest element, keeps them."""
+    l1 = np.arange(nb)
+    return l1 >= 3 if int(mapinv[q]) - 1 == gamma else np.ones(nb, bool)
+
+
+@pytest.mark.gpu
+def test_si16_blocks_vs_reference(si_inputs, si16blocks):
+    """Each one-point irreducible block of SW-Si 16^3 through the device path with the reference
+    eigen data, as a rank of kappa.f90 would run it: |V3|^2 (1e-8 of the largest element) and
+    P (1e-8) on the fixture's mesh points, Q, QN, QU of the block (1e-7) at 100, 300 and 800 K;
+    at q1 = Gamma (the last block) without the acoustic l1 rows (_l1_kept)."""
+    print("case: SW-Si 16^3 one-point IBZ blocks vs reference (reference eigen data)", flush=True)
+    R = si16blocks
+    sel = R["q2_sel"]
+    nb = int(R["nb"])
+    gamma = 8 + 16 * (8 + 16 * 8)
+    with capi.Context(si_inputs[16]) as c:
+        c.set_eig(R["eigenval"], R["eigenvec"], R["veloc"])
+        for i, q in enumerate(R["ibz_sel"]):
+            q = int(q)
+            keep = _l1_kept(R["mapinv"], q, nb, gamma)
+            v = c.v3sq(q, q, capi.V3_FP64)
+            check(v[0][sel], R["v33s8"][i], 1e-8, f"v33s8 q1={q}")
+            P1, P2 = c.fgr()
+            check(P1[0][sel][:, keep], R["P1sm"][i][:, keep], 1e-8, f"P1sm q1={q}")
+            check(P2[0][sel][:, keep], R["P2sm"][i][:, keep], 1e-8, f"P2sm q1={q}")
+            for T in R["temps"]:
+                out = c.tau(float(T))
+                t = f"T{int(T)}_"
+                for k in ("Q", "QN", "QU"):
+                    check(out[k][0][keep], R[t + k][i][keep], 1e-7, f"{t}{k} q1={q}")

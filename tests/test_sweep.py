@@ -8,8 +8,10 @@ CPU: the pack holds seeds 0..63 with the draws of sweep_draw, all four meshes oc
 instance 0 regenerated by models.sw_silicon equals the pack bit for bit.
 GPU: every instance of the packed sweep (all 64 resident, |V3|^2 of all of them in one
 persistent launch, autotuned tile) gives kappa_RTA, kappa_BTE and CG iteration counts equal
-to its lone solve (relative 1e-12; the tile is computed by the same code, so the results
-are bit-identical in practice); two 8^3 instances through the oracle chain (eigen 1e-10,
+to its lone solve (the tile is computed by the same code; the FP32 FGR adds tetrahedron
+weights into double W with atomics whose order varies from run to run, a last-bit change of
+W flips the float rounding of a P element now and then, and two lone solves of one instance
+differ by up to 4e-11 in kappa_RTA and 3e-10 in kappa_BTE: relative 1e-9 and 1e-8); two 8^3 instances through the oracle chain (eigen 1e-10,
 |V3|^2 of two q1 1e-8, P 1e-8, kappa_RTA 1e-8, kappa_BTE 1e-5) and their packed mixed kappa
 within 1e-5 of FP64; |V3|^2 after a batched launch equal to the lone path's bit for bit
 (and P1/P2 to 1e-12, the FGR's atomics order) for every tile candidate, FP64 and mixed, 6
@@ -75,8 +77,8 @@ def test_sweep_packed_equals_lone(sweep):
     for (s, a, n, inp), o in zip(sweep, r["results"]):
         with capi.Context(inp) as c:
             lone = c.solve(TEMPS, capi.V3_MIXED)
-        for k in ("kappa_rta", "kappa_bte"):
-            assert rel(o[k], lone[k]) <= 1e-12, (s, a, n, k, rel(o[k], lone[k]))
+        for k, tol in (("kappa_rta", 1e-9), ("kappa_bte", 1e-8)):
+            assert rel(o[k], lone[k]) <= tol, (s, a, n, k, rel(o[k], lone[k]))
         assert np.array_equal(o["cg_iters"], lone["cg_iters"]), (s, o["cg_iters"], lone["cg_iters"])
 
 
