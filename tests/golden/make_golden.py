@@ -215,7 +215,7 @@ def main_si16():
     """The SW-Si 16^3 workload (276 irreducible points) at 100, 300 and 800 K: tables, eigen
     data, Q/tau, F_RTA, kappa_RTA, RHS, CG solution, kappa_BTE and the CG iteration counts,
     |V3|^2 / P on three irreducible points and every eighth mesh point, collision rows of two
-    irreducible points.  About 1.5 h of one core (the V3 stage at ~5e4 elements/s)."""
+    irreducible points.  About four hours of one core (the V3 stage alone 2.9 h)."""
     with tempfile.TemporaryDirectory() as wd:
         D, iters = _run_probe(models.sw_silicon(), 16, wd, TEMPS)
         out = _config_pack(D, iters, [0, 137, D["nibz"] - 1], np.arange(0, D["nk"], 8), [1, 200])
