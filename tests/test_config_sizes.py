@@ -18,7 +18,8 @@ reference eigen data for one irreducible point of the rock-salt cell (1e-10).
 GPU (reference eigen data loaded): |V3|^2 (batched FP64, 1e-8), P1sm/P2sm 1e-8; Q, QN, QU,
 tau, F_RTA, kappa_RTA 1e-7; collision rows and RHS 1e-7; the CG iteration count equal; CG x
 and kappa_BTE 1e-6 (rock-salt) / 1e-5 (SW-Si kappa_BTE through alt3_solve); the mixed path's
-kappa within 1e-5 of the FP64 path's.
+kappa within 1e-5 of the FP64 path's.  SW-Si 16^3 leaves the acoustic branches of q1 = Gamma
+out of the P and per-mode checks (_l1_kept).
 """
 import os
 
