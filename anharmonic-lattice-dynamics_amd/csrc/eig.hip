@@ -650,36 +650,67 @@ __global__ void __launch_bounds__(64) k_eig_group(int nk, int nvar, int nb, cons
 }
 
 // finitedif_vel velocities + get_frequencies' second sort
-__global__ void k_eig_finish(int nk, int nb, double dq, double cnst, const double *__restrict__ evals7,
-                             const double *__restrict__ evec0, double *__restrict__ eival,
-                             double *__restrict__ evec, double *__restrict__ vel, double *__restrict__ freq)
+// One wavefront per mesh point q, lane j holding mode j (nb <= 64): velocities by the central
+// differences of the displaced eigenvalues, the eigenvalues sorted ascending as the reference's
+// exchange sort orders them (extratools.f90:154-175), and the eigenvectors permuted to match.
+// The exchange sort's step i moves the running minimum of positions i..nb-1 into position i:
+// each position j > i holding a value strictly below everything in i..j-1 receives that
+// running minimum (the first position of it), and position i receives the minimum of the
+// whole range (first position).  Each step is one prefix-minimum scan across the lanes, so
+// the permutation, ties included, is the serial sort's.
+__global__ void __launch_bounds__(256) k_eig_finish(int nk, int nb, double dq, double cnst,
+                                                    const double *__restrict__ evals7,
+                                                    const double *__restrict__ evec0, double *__restrict__ eival,
+                                                    double *__restrict__ evec, double *__restrict__ vel,
+                                                    double *__restrict__ freq)
 {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nk) return;
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nk) return;   // uniform over the wavefront
     const double *e = evals7 + (size_t)q * kDq * nb;
-    int mp[64];
-    for (int i = 0; i < nb; i++) mp[i] = i;
-    for (int i = 0; i < nb; i++)  // sort (extratools.f90:154-175)
-        for (int j = i + 1; j < nb; j++)
-            if (e[mp[i]] > e[mp[j]]) {
-                int t = mp[i];
-                mp[i] = mp[j];
-                mp[j] = t;
-            }
-    for (int l = 0; l < nb; l++) {
-        double e0 = e[l];
+    const bool act = lane < nb;
+    const double e0 = act ? e[lane] : 0.0;
+    if (act)
         for (int a = 0; a < 3; a++) {
-            double ep = e[(1 + 2 * a) * nb + l], em = e[(2 + 2 * a) * nb + l];
-            vel[((size_t)q * nb + l) * 3 + a] = (ep - em) / 2 / dq / 2 / sqrt(e0) * cnst * 1e-8 * 2 * alt3::PI;
+            const double ep = e[(1 + 2 * a) * nb + lane], em = e[(2 + 2 * a) * nb + lane];
+            vel[((size_t)q * nb + lane) * 3 + a] = (ep - em) / 2 / dq / 2 / sqrt(e0) * cnst * 1e-8 * 2 * alt3::PI;
+        }
+    double v = e0;   // value at position lane
+    int id = lane;   // its mode before sorting
+    for (int i = 0; i < nb - 1; i++) {
+        // inclusive scan over positions i..lane of (value, id), the earlier one kept on ties
+        const bool in = lane >= i && act;
+        double sv = in ? v : INFINITY;
+        int sid = in ? id : -1;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double ov = __shfl_up(sv, off, 64);
+            const int oid = __shfl_up(sid, off, 64);
+            if (lane >= off && !(sv < ov)) {
+                sv = ov;
+                sid = oid;
+            }
+        }
+        const double xv = __shfl_up(sv, 1, 64);   // exclusive: positions i..lane-1
+        const int xid = __shfl_up(sid, 1, 64);
+        const double mv = __shfl(sv, nb - 1, 64);   // the whole range
+        const int mid = __shfl(sid, nb - 1, 64);
+        if (act && lane > i && v < xv) {
+            v = xv;
+            id = xid;
+        } else if (lane == i) {
+            v = mv;
+            id = mid;
         }
     }
+    if (act) {
+        eival[(size_t)q * nb + lane] = v;
+        freq[(size_t)q * nb + lane] = sqrt(v) * cnst;
+    }
     for (int j = 0; j < nb; j++) {
-        double ev = e[mp[j]];
-        eival[(size_t)q * nb + j] = ev;
-        freq[(size_t)q * nb + j] = sqrt(ev) * cnst;
-        const double *src = evec0 + 2 * ((size_t)q * nb + mp[j]) * nb;
-        double *dst = evec + 2 * ((size_t)q * nb + j) * nb;
-        for (int k = 0; k < 2 * nb; k++) dst[k] = src[k];
+        const int src = __shfl(id, j, 64);
+        const double *s = evec0 + 2 * ((size_t)q * nb + src) * nb;
+        double *d = evec + 2 * ((size_t)q * nb + j) * nb;
+        for (int k = lane; k < 2 * nb; k += 64) d[k] = s[k];
     }
 }
 
@@ -839,7 +870,7 @@ int eig_run(alt3_ctx *ctx)
     const double dq = (double)0.001f;
     ALT3_TRY(eig_launch(ctx, nk, kDq, ctx->d_kpc_eig.p, dq, ev7.p, evec0.p));
     ALT3_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_eig_finish, dim3((nk + 63) / 64), dim3(64), 0, ctx->stream, nk, nb, dq, cnst(), ev7.p,
+    hipLaunchKernelGGL(k_eig_finish, dim3((nk + 3) / 4), dim3(256), 0, ctx->stream, nk, nb, dq, cnst(), ev7.p,
                        evec0.p, ctx->d_eival.p, ctx->d_evec.p, ctx->d_vel.p, ctx->d_freq.p);
     ALT3_HIP(hipGetLastError());
     // subst_eivecs
