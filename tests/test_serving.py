@@ -7,18 +7,28 @@ footprint fits, 8^3 for the 16-atom cell, whose 48 branches run the FP64 path: 3
 10^3); a mixed request on that cell is charged the FP64 footprint.
 GPU: a mixed stream of uneven requests (meshes 4^3-8^3, one to three temperatures, FP64 and
 mixed V3, tetrahedron and Gaussian delta) run concurrently through the server gives, for
-every request, the kappa the same request gives alone on a fresh context (1e-9 relative;
-the FGR's LDS atomics reorder sums at the 1e-16 level), the device held more than one
+every request, the kappa the same request gives alone on a fresh context (lone_tol: 1e-9
+relative, 1e-8 for a mixed request's kappa_BTE), the device held more than one
 request at a time, and requests with a cached context reused it; requests queued together
 share one persistent |V3|^2 launch and keep their lone results; the four cells (SW-Si, the
 4-atom zincblende-type, the 8- and 16-atom rock-salt-type cells) at meshes up to 16^3 in
-one stream, each result equal to its lone solve (1e-9).
+one stream, each result equal to its lone solve (lone_tol).
 """
 import numpy as np
 import pytest
 
 from conftest import inputs_for
 from thermacond import capi, serving, workloads
+
+
+def lone_tol(req, k):
+    """Relative tolerance of a request's kappa against its lone solve.  The FGR adds tetrahedron
+    weights into FP64 sums with atomics whose order varies from run to run (1e-16); on the mixed
+    path a last-bit change of a sum now and then flips the float rounding of a P element, and two
+    lone solves of one mixed request differ by up to ~4e-11 (kappa_RTA) and ~3e-10 (kappa_BTE)."""
+    if req.precision == capi.V3_FP64:
+        return 1e-9
+    return 1e-9 if k == "kappa_rta" else 1e-8
 
 
 def test_footprint_grows_with_work(si_inputs):
@@ -138,7 +148,7 @@ def test_alone_equals_in_the_mix(si_inputs):
         assert m.error is None, m.error
         for k in ("kappa_rta", "kappa_bte"):
             d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
-            assert d < 1e-9, (m.mesh_key, m.temps, k, d)
+            assert d < lone_tol(m, k), (m.mesh_key, m.temps, k, d)
         assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
     assert conc > 1
     assert any(m.reused for m in mix)
@@ -202,8 +212,9 @@ def test_shared_v3_launch_keeps_lone_results(si_inputs, tmp_path_factory):
     for a, m in zip(alone, mix):
         assert m.error is None, m.error
         for k in ("kappa_rta", "kappa_bte"):
+            tol = 1e-12 if m.precision == capi.V3_FP64 else lone_tol(m, k)
             d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
-            assert d < 1e-12, (m.mesh_key, k, d)
+            assert d < tol, (m.mesh_key, k, d)
         assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
 
 
@@ -247,7 +258,7 @@ def test_four_cell_mix_at_full_meshes(si_inputs, cells, tmp_path_factory):
         assert m.error is None, m.error
         for k in ("kappa_rta", "kappa_bte"):
             d = np.max(np.abs(m.result[k] - a.result[k])) / np.max(np.abs(a.result[k]))
-            assert d < 1e-9, (m.mesh_key, m.temps, k, d)
+            assert d < lone_tol(m, k), (m.mesh_key, m.temps, k, d)
         assert list(m.result["cg_iters"]) == list(a.result["cg_iters"])
     assert conc > 1
     print("four-cell mix:", serving.latency_summary(mix), flush=True)
